@@ -1,0 +1,190 @@
+/*
+ * redondito.h -- C-ABI of the MI355X-native pseudospectral transform path.
+ *
+ * Every function returns an int status (RD_OK = 0, negative on error) and
+ * takes plain pointers and sizes.  Field arrays are DEVICE pointers (HBM);
+ * calls are ordered on the plan's HIP stream (rd_plan_set_stream) and return
+ * without waiting unless the name says otherwise; rd_plan_sync waits.
+ *
+ * Layouts are the reference's (one rank holds everything):
+ *   spectral a(lm, k): complex FP64 interleaved (re, im), lm fastest, lm in
+ *     the SHTns m-major order LM(l,m) = m(2 lmax + 1 - m)/2 + l, with a
+ *     plane stride of nlm complex values;
+ *   physical f(theta, phi, k): real FP64, theta fastest, nlat*nphi per plane.
+ * Each entry point names the reference routine it replaces (paths relative
+ * to the reference tree).
+ */
+#ifndef REDONDITO_H
+#define REDONDITO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RD_OK 0
+#define RD_ERR_ARG (-1)     /* bad argument or size                        */
+#define RD_ERR_HIP (-2)     /* a HIP runtime call failed                   */
+#define RD_ERR_TABLE (-3)   /* FC-Gram table missing or of the wrong size  */
+#define RD_ERR_NOMEM (-4)   /* device allocation failed                    */
+#define RD_ERR_LU (-5)      /* singular boundary matrix in setup           */
+#define RD_ERR_STATE (-6)   /* call needs a setup that was not done        */
+
+/* Precision forms of rd_nonlinear_uxw. */
+#define RD_PREC_FP64 0      /* plain FP64 path (MFMA F64 Legendre)          */
+#define RD_PREC_FP32 1      /* Legendre contractions on FP32 MFMA           */
+#define RD_PREC_BF16X3 2    /* BF16 3-term split, FP32 accumulation         */
+
+typedef struct rd_plan rd_plan;
+
+/* Human-readable text for a status code. */
+const char *rd_error_string(int code);
+
+/* Reads A{C}-{d}.dat and Q{d}.dat (Fortran stream FP64, column major) from
+ * tdir and forms dir = A.Q^T (C x d, column major) on the host.
+ * Replaces load_dirichlet_tables, src/sphp/fcgram_mod.f90:186-263. */
+int rd_load_dirichlet_tables(const char *tdir, int C, int d, double *dir);
+
+/* Creates the plan: Gauss-Legendre grid, associated-Legendre tables (FP64,
+ * built on the device), phi and radial FFT twiddles, the FC-Gram operator
+ * dir (C x d column major, may be NULL when C = 0), the radial grid
+ * z_k = rint + k dz and kz, and the device workspace.  nz counts the C
+ * continuation points as the reference does (nr = nz - C physical shells).
+ * Replaces fcgram_create_plan (src/sphp/fcgram_mod.f90:49-150) with the two
+ * fftp3d_create_plan calls (src/sphp/sphp_sca.F90:70-144; device part
+ * src/sphp-gpu/sphp_sca.F90:478-625) and the grid part of box_init
+ * (src/pseudo/pseudospec_mod.f90:126-236). */
+int rd_plan_create(rd_plan **plan, int lmax, int nlat, int nphi, int nz, int C, int d,
+                   double dz, double rint, const double *dir);
+
+/* Replaces fcgram_destroy_plan (src/sphp/fcgram_mod.f90:153-182). */
+int rd_plan_destroy(rd_plan *plan);
+
+/* Orders the plan's work on an existing HIP stream (NULL: the plan's own). */
+int rd_plan_set_stream(rd_plan *plan, void *hip_stream);
+
+/* Waits for all work queued on the plan's stream (the reference's
+ * device_sync, src/memmgt/gdevice_mod.f90). */
+int rd_plan_sync(rd_plan *plan);
+
+/* info[0..8] = lmax, nlat, nphi, nlm, nz, C, d, nr, 0. */
+int rd_plan_info(const rd_plan *plan, int *info);
+
+/* Per-stage HIP-event timing of rd_nonlinear_uxw (stages: radial, u
+ * synthesis, omega synthesis, phi/cross step, analysis, tail), recorded on
+ * the plan's stream without host synchronisation and read back by
+ * rd_plan_stage_times (which waits).  No counterpart in the reference, whose
+ * gtimer handles (src/utils/gtimer_mod.f90) time host wall clock only.
+ * rd_plan_stage_times returns the number of stages on success. */
+int rd_plan_profile(rd_plan *plan, int enable);
+int rd_plan_stage_times(rd_plan *plan, double *ms, long *count, int nmax);
+
+/* Host copies of the grid: cos/sin(theta), Gauss weights (nlat), z and kz
+ * (nz), l and m per lm (nlm).  Any pointer may be NULL.  Replaces the grid
+ * and wavenumber construction of box_init
+ * (src/pseudo/pseudospec_mod.f90:168-227). */
+int rd_plan_grid(const rd_plan *plan, double *ct, double *st, double *wt, double *z,
+                 double *kz, int *ell, int *emm);
+
+/* ---- 2D transforms, plane by plane (nplanes <= nz) ------------------- */
+
+/* in(nlat,nphi,nplanes) -> out(nlm,nplanes).  The input is left intact.
+ * Replaces fftp2d_real_to_complex_xy, src/sphp/sphp_sca.F90:273-329. */
+int rd_sht_analysis(rd_plan *plan, int nplanes, const double *in, double *out);
+
+/* in(nlm,nplanes) -> out(nlat,nphi,nplanes); Im of m = 0 is ignored.
+ * Replaces fftp2d_complex_to_real_xy, src/sphp/sphp_sca.F90:436-492. */
+int rd_sht_synthesis(rd_plan *plan, int nplanes, const double *in, double *out);
+
+/* (v_r, v_theta, v_phi) -> (Q, S, T); S, T of l = 0 are 0.
+ * Replaces fftp2d_vec_to_qst, src/sphp/sphp_vec.F90:49-106. */
+int rd_vsht_analysis(rd_plan *plan, int nplanes, const double *vr, const double *vt,
+                     const double *vp, double *q, double *s, double *t);
+
+/* (Q, S, T) -> (v_r, v_theta, v_phi).
+ * Replaces fftp2d_qst_to_vec, src/sphp/sphp_vec.F90:109-165. */
+int rd_vsht_synthesis(rd_plan *plan, int nplanes, const double *q, const double *s,
+                      const double *t, double *vr, double *vt, double *vp);
+
+/* ---- radial (z) transforms, in place on (nlm, nz) ------------------- */
+
+/* FC-Gram continuation of rows nz-C+1..nz, forward c2c FFT (sign -1) along
+ * k, division by nz.  Replaces fftp1d_real_to_complex_z,
+ * src/sphp/sphp_sca.F90:332-397. */
+int rd_z_forward(rd_plan *plan, double *inout);
+
+/* Backward c2c FFT (sign +1), unnormalized.  Replaces
+ * fftp1d_complex_to_real_z, src/sphp/sphp_sca.F90:495-530. */
+int rd_z_inverse(rd_plan *plan, double *inout);
+
+/* 2D then z: in(nlat,nphi,nz) -> out(nlm,nz) in (lm,kz).
+ * Replaces fftp3d_real_to_complex, src/sphp/sphp_sca.F90:237-270. */
+int rd_3d_forward(rd_plan *plan, const double *in, double *out);
+
+/* z (in place on in, which is destroyed) then 2D: in(nlm,nz) -> out.
+ * Replaces fftp3d_complex_to_real, src/sphp/sphp_sca.F90:400-433. */
+int rd_3d_inverse(rd_plan *plan, double *in, double *out);
+
+/* The planes <-> lm transposes.  With the whole domain on one GPU the
+ * reference's exchange reduces to copying the nplanes columns (its nprocs =
+ * 1 path, src/sphp/sphp_sca.F90:634-635).  Replaces fftp3d_redist_k_to_lm /
+ * fftp3d_redist_lm_to_k, src/sphp/sphp_sca.F90:629-760. */
+int rd_redist_k_to_lm(rd_plan *plan, int nplanes, const double *in, double *out);
+int rd_redist_lm_to_k(rd_plan *plan, int nplanes, const double *in, double *out);
+
+/* ---- pseudospectral operators, src/pseudo/pseudospec_hd.f90 ---------- */
+
+/* d^p a / dr^p; esp = 'r' (lm,r) or 'k' (lm,kz).  Replaces derivr /
+ * derivr_r / derivr_k, pseudospec_hd.f90:18-135. */
+int rd_derivr(rd_plan *plan, int p, char esp, const double *a, double *b);
+
+/* b = -l(l+1) a.  Replaces laplang, pseudospec_hd.f90:138-172. */
+int rd_laplang(rd_plan *plan, const double *a, double *b);
+
+/* b = r^p a.  Replaces multr, pseudospec_hd.f90:175-208. */
+int rd_multr(rd_plan *plan, int p, const double *a, double *b);
+
+/* b = d2a/dr2 - l(l+1) a / r^2.  Replaces dlr, pseudospec_hd.f90:254-288. */
+int rd_dlr(rd_plan *plan, const double *a, double *b);
+
+/* z = a x + b y on (nlm, nz) complex.  Replaces saxpby_c, :291-326. */
+int rd_saxpby_c(rd_plan *plan, double *z, const double *x, double a, const double *y, double b);
+
+/* z = a x + b y on (nlat, nphi, nplanes) real.  Replaces saxpby_r, :329-366. */
+int rd_saxpby_r(rd_plan *plan, int nplanes, double *z, const double *x, double a,
+                const double *y, double b);
+
+/* C = A x B pointwise on the physical planes (k < min(nplanes, nz - C)).
+ * Replaces prodvec, pseudospec_hd.f90:211-251. */
+int rd_prodvec(rd_plan *plan, int nplanes, const double *ar, const double *at, const double *ap,
+               const double *br, const double *bt, const double *bp, double *cr, double *ct,
+               double *cp);
+
+/* Exponential radial filter in place; esp 'r' or 'k'.  Replaces fc_filter,
+ * fc_filter_k, fc_filter_r, pseudospec_hd.f90:369-467. */
+int rd_fc_filter(rd_plan *plan, char esp, double *a);
+
+/* b = a on (nlm, nz).  Replaces copy, pseudospec_hd.f90:470-497. */
+int rd_copy(rd_plan *plan, const double *a, double *b);
+
+/* ---- the composed nonlinear term ------------------------------------ */
+
+/* N = u x omega with u = curl curl (W r) + curl (Z r):  W, Z (nlm, nz) in
+ * (lm, r) -> NQ, NS, NT (nlm, nz) on the physical rows (rows nr..nz-1 are
+ * set to zero).  W and Z are not modified.  prec selects RD_PREC_*.
+ * Composition of derivr, dlr, multr, laplang (pseudospec_hd.f90),
+ * fftp2d_qst_to_vec and fftp2d_vec_to_qst (src/sphp/sphp_vec.F90:49-165) and
+ * prodvec (pseudospec_hd.f90:211-251); the reference has no routine of its
+ * own for it. */
+int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ,
+                     double *NS, double *NT);
+
+/* ---- boundary conditions, src/boundary/ ----------------------------- */
+
+/* a(lm,1) = a(lm,nz-C) = 0.  Replaces dirichlet, boundary_mod.f90:62-93. */
+int rd_dirichlet(rd_plan *plan, double *a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REDONDITO_H */

@@ -1,0 +1,309 @@
+"""ctypes front end of the CPU oracle (oracle/sphp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the CPU-baseline leg of bench.py, always as the checker.  The product
+path (redondito_amd) never imports it.
+
+Arrays follow the reference layouts: spectral (lm, k) complex128 with lm
+fastest, i.e. numpy shape (nplanes, nlm); physical (theta, phi, k) float64
+with theta fastest, i.e. numpy shape (nplanes, nphi, nlat).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int
+_L = ctypes.c_long
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_sht_create.restype = _P
+        L.orc_sht_create.argtypes = [_I, _I, _I]
+        L.orc_sht_destroy.argtypes = [_P]
+        L.orc_sht_grid.argtypes = [_P, _D, _D, _D]
+        L.orc_sht_tables.argtypes = [_P, _D, _D, _D]
+        for nm in ("orc_sh_to_spat", "orc_spat_to_sh"):
+            getattr(L, nm).argtypes = [_P, _I, _D, _D]
+        for nm in ("orc_shqst_to_spat", "orc_spat_to_shqst"):
+            getattr(L, nm).argtypes = [_P, _I, _D, _D, _D, _D, _D, _D]
+        L.orc_fc_dir.argtypes = [_I, _I, _D, _D, _D]
+        L.orc_fc_create.restype = _P
+        L.orc_fc_create.argtypes = [_I, _I, _I, _D]
+        L.orc_fc_destroy.argtypes = [_P]
+        L.orc_z_forward.argtypes = [_P, _I, _L, _D]
+        L.orc_z_inverse.argtypes = [_P, _I, _L, _D]
+        L.orc_kz.argtypes = [_I, ctypes.c_double, _D]
+        L.orc_derivr_k.argtypes = [_I, _I, _L, _D, _I, _D, _D]
+        L.orc_derivr_r.argtypes = [_P, _D, _I, _I, _L, _D, _D]
+        L.orc_laplang.argtypes = [_I, _I, _L, ctypes.POINTER(ctypes.c_int), _D, _D]
+        L.orc_multr.argtypes = [_I, _I, _L, _D, _I, _D, _D]
+        L.orc_saxpby_c.argtypes = [_L, _D, _D, ctypes.c_double, _D, ctypes.c_double]
+        L.orc_dlr.argtypes = [_P, _D, _D, ctypes.POINTER(ctypes.c_int), _I, _L, _D, _D]
+        L.orc_prodvec.argtypes = [_L] + [_D] * 9
+        L.orc_fc_filter_k.argtypes = [_I, _I, _L, _D, ctypes.c_double, _D]
+        L.orc_nonlinear_uxw.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double,
+                                        _D, _D, _D, _D, _D, _I, _I]
+        L.orc_derivr_mat.argtypes = [_P, _D, _I, _D]
+        L.orc_invdl_setup.restype = _I
+        L.orc_invdl_setup.argtypes = [_I, _I, _D, _D, _D, _D]
+        L.orc_invdl.argtypes = [_I, _I, _I, _D, _D, _D]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.flags["C_CONTIGUOUS"]
+    assert a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(_D)
+
+
+def nlm_of(lmax):
+    return (lmax + 1) * (lmax + 2) // 2
+
+
+def lm_index(lmax, l, m):
+    """0-based position of (l, m) in the m-major order."""
+    return m * (2 * lmax + 1 - m) // 2 + l
+
+
+def ell_emm(lmax):
+    ell = np.empty(nlm_of(lmax), np.int32)
+    emm = np.empty(nlm_of(lmax), np.int32)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            ell[lm_index(lmax, l, m)] = l
+            emm[lm_index(lmax, l, m)] = m
+    return ell, emm
+
+
+def grid_nlat(lmax):
+    """nlat from the 3/2 rule of grid_init (pseudospec_mod.f90:66-68)."""
+    return 4 * ((3 * (lmax + 1) + 7) // 8)
+
+
+class SHT:
+    """Oracle spherical-harmonic transform on one (lmax, nlat, nphi) grid."""
+
+    def __init__(self, lmax, nlat, nphi):
+        self.lmax, self.nlat, self.nphi = lmax, nlat, nphi
+        self.nlm = nlm_of(lmax)
+        self.h = lib().orc_sht_create(lmax, nlat, nphi)
+        if not self.h:
+            raise ValueError("bad SHT sizes")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sht_destroy(self.h)
+            self.h = None
+
+    def grid(self):
+        ct, st, wt = (np.empty(self.nlat) for _ in range(3))
+        lib().orc_sht_grid(self.h, _d(ct), _d(st), _d(wt))
+        return ct, st, wt
+
+    def tables(self):
+        P, dP, mPs = (np.empty((self.nlm, self.nlat)) for _ in range(3))
+        lib().orc_sht_tables(self.h, _d(P), _d(dP), _d(mPs))
+        return P, dP, mPs
+
+    def _spec(self, a):
+        a = np.ascontiguousarray(a, np.complex128)
+        if a.ndim == 1:
+            a = a[None]
+        assert a.shape[1] == self.nlm
+        return a
+
+    def _phys(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        if f.ndim == 2:
+            f = f[None]
+        assert f.shape[1:] == (self.nphi, self.nlat)
+        return f
+
+    def synth(self, a):
+        a = self._spec(a)
+        f = np.empty((a.shape[0], self.nphi, self.nlat))
+        lib().orc_sh_to_spat(self.h, a.shape[0], _d(a), _d(f))
+        return f
+
+    def anal(self, f):
+        f = self._phys(f)
+        a = np.empty((f.shape[0], self.nlm), np.complex128)
+        lib().orc_spat_to_sh(self.h, f.shape[0], _d(f), _d(a))
+        return a
+
+    def vsynth(self, q, s, t):
+        q, s, t = self._spec(q), self._spec(s), self._spec(t)
+        out = [np.empty((q.shape[0], self.nphi, self.nlat)) for _ in range(3)]
+        lib().orc_shqst_to_spat(self.h, q.shape[0], _d(q), _d(s), _d(t), *[_d(o) for o in out])
+        return out
+
+    def vanal(self, vr, vt, vp):
+        vr, vt, vp = self._phys(vr), self._phys(vt), self._phys(vp)
+        out = [np.empty((vr.shape[0], self.nlm), np.complex128) for _ in range(3)]
+        lib().orc_spat_to_shqst(self.h, vr.shape[0], _d(vr), _d(vt), _d(vp), *[_d(o) for o in out])
+        return out
+
+
+def load_tables(tdir, C, d):
+    """A (C x d) and Q (d x d) in the Fortran stream layout of
+    load_dirichlet_tables (fcgram_mod.f90:227-254)."""
+    A = np.fromfile(os.path.join(tdir, f"A{C}-{d}.dat"), "<f8")
+    Q = np.fromfile(os.path.join(tdir, f"Q{d}.dat"), "<f8")
+    if A.size != C * d or Q.size != d * d:
+        raise ValueError("FC-Gram table size mismatch")
+    return A.reshape(d, C).T.copy(), Q.reshape(d, d).T.copy()
+
+
+def fc_dir(A, Q):
+    C, d = A.shape
+    Af = np.asfortranarray(A).ravel(order="F").copy()
+    Qf = np.asfortranarray(Q).ravel(order="F").copy()
+    out = np.empty(C * d)
+    lib().orc_fc_dir(C, d, _d(Af), _d(Qf), _d(out))
+    return out.reshape(d, C).T.copy()
+
+
+class FC:
+    """Oracle radial plan: n total points, C continuation, d matching."""
+
+    def __init__(self, n, C=0, d=0, dir_=None):
+        self.n, self.C, self.d = n, C, d
+        if C > 0:
+            dv = np.asfortranarray(dir_).ravel(order="F").copy()
+            self.h = lib().orc_fc_create(n, C, d, _d(dv))
+        else:
+            self.h = lib().orc_fc_create(n, 0, 0, None)
+        if not self.h:
+            raise ValueError("bad FC-Gram sizes")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_fc_destroy(self.h)
+            self.h = None
+
+    def forward(self, a):
+        """a: (n, ncols) complex, modified in place (continuation rows too)."""
+        assert a.shape[0] == self.n and a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
+        lib().orc_z_forward(self.h, a.shape[1], a.shape[1], _d(a))
+        return a
+
+    def inverse(self, a):
+        assert a.shape[0] == self.n and a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
+        lib().orc_z_inverse(self.h, a.shape[1], a.shape[1], _d(a))
+        return a
+
+    def derivr_r(self, a, kz, p):
+        b = np.empty_like(a)
+        lib().orc_derivr_r(self.h, _d(kz), p, a.shape[1], a.shape[1], _d(np.ascontiguousarray(a)), _d(b))
+        return b
+
+    def dlr(self, a, kz, z, ell):
+        b = np.empty_like(a)
+        ell = np.ascontiguousarray(ell, np.int32)
+        lib().orc_dlr(self.h, _d(kz), _d(z), ell.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                      a.shape[1], a.shape[1], _d(np.ascontiguousarray(a)), _d(b))
+        return b
+
+    def derivr_mat(self, kz, p):
+        N = self.n - self.C
+        dm = np.empty((N, N), order="F")
+        out = np.empty(N * N)
+        lib().orc_derivr_mat(self.h, _d(kz), p, _d(out))
+        dm[:] = out.reshape(N, N).T
+        return np.ascontiguousarray(dm)
+
+
+def kz(n, dz):
+    out = np.empty(n)
+    lib().orc_kz(n, dz, _d(out))
+    return out
+
+
+def derivr_k(a, kzv, p):
+    b = np.empty_like(a)
+    lib().orc_derivr_k(a.shape[0], a.shape[1], a.shape[1], _d(kzv), p, _d(np.ascontiguousarray(a)), _d(b))
+    return b
+
+
+def laplang(a, ell):
+    b = np.empty_like(a)
+    ell = np.ascontiguousarray(ell, np.int32)
+    lib().orc_laplang(a.shape[0], a.shape[1], a.shape[1],
+                      ell.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(np.ascontiguousarray(a)), _d(b))
+    return b
+
+
+def multr(a, z, p):
+    b = np.empty_like(a)
+    lib().orc_multr(a.shape[0], a.shape[1], a.shape[1], _d(z), p, _d(np.ascontiguousarray(a)), _d(b))
+    return b
+
+
+def saxpby_c(x, a, y, b):
+    z = np.empty_like(x)
+    lib().orc_saxpby_c(x.size, _d(z), _d(np.ascontiguousarray(x)), a, _d(np.ascontiguousarray(y)), b)
+    return z
+
+
+def prodvec(a, b):
+    """a, b: tuples of 3 float64 arrays of equal shape -> 3 arrays."""
+    a = [np.ascontiguousarray(x, np.float64) for x in a]
+    b = [np.ascontiguousarray(x, np.float64) for x in b]
+    c = [np.empty_like(a[0]) for _ in range(3)]
+    lib().orc_prodvec(a[0].size, *[_d(x) for x in a + b + c])
+    return c
+
+
+def fc_filter_k(a, kzv, dkz):
+    a = np.ascontiguousarray(a)
+    lib().orc_fc_filter_k(a.shape[0], a.shape[1], a.shape[1], _d(kzv), dkz, _d(a))
+    return a
+
+
+def nonlinear_uxw(sht, fc, rint, dz, W, Z, plane0=0, nplanes=None):
+    """N = u x omega as (NQ, NS, NT), each (n, nlm) complex; W, Z (n, nlm)."""
+    n = fc.n
+    if nplanes is None:
+        nplanes = n - fc.C
+    W = np.ascontiguousarray(W, np.complex128)
+    Z = np.ascontiguousarray(Z, np.complex128)
+    assert W.shape == (n, sht.nlm) and Z.shape == W.shape
+    out = [np.empty_like(W) for _ in range(3)]
+    lib().orc_nonlinear_uxw(sht.h, fc.h, rint, dz, _d(W), _d(Z), *[_d(o) for o in out], plane0, nplanes)
+    return out
+
+
+def invdl_setup(N, lmax, d1, d2, z):
+    binv = np.zeros((lmax, N - 4, N))
+    d1f = np.asfortranarray(d1).ravel(order="F").copy()
+    d2f = np.asfortranarray(d2).ravel(order="F").copy()
+    zz = np.ascontiguousarray(z, np.float64)
+    rc = lib().orc_invdl_setup(N, lmax, _d(d1f), _d(d2f), _d(zz), _d(binv))
+    if rc:
+        raise RuntimeError(f"LU of B_l failed for l = {rc}")
+    return binv
+
+
+def invdl(N, n, lmax, binv, phi):
+    phi = np.ascontiguousarray(phi, np.complex128)
+    out = np.empty_like(phi)
+    lib().orc_invdl(N, n, lmax, _d(binv), _d(phi), _d(out))
+    return out

@@ -1,0 +1,585 @@
+// C-ABI entry points (include/redondito.h): plan life cycle and the
+// reference's sphp / pseudo / boundary routines on device pointers.
+#include "rd_internal.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace rd;
+
+namespace {
+
+inline bool bad_plan(const rd_plan *p) { return p == nullptr; }
+
+Plan *P(rd_plan *p) { return reinterpret_cast<Plan *>(p); }
+const Plan *P(const rd_plan *p) { return reinterpret_cast<const Plan *>(p); }
+
+int upload_twiddles(Plan *p, int n, double2 **dst)
+{
+    std::vector<double2> h(n);
+    for (int t = 0; t < n; t++) {
+        const double a = 2.0 * M_PI * (double)t / (double)n;
+        h[t] = make_double2(cos(a), -sin(a));
+    }
+    RD_HIP(hipMalloc(dst, sizeof(double2) * n));
+    RD_HIP(hipMemcpyAsync(*dst, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice, p->stream));
+    return RD_OK;
+}
+
+void free_plan(Plan *p)
+{
+    if (!p) return;
+    void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
+                   p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
+                   p->d_ana_off, p->d_ana_ncol, p->d_ana_tiles, p->tw_phi, p->tw_z};
+    for (void *d : dev)
+        if (d) (void)hipFree(d);
+    for (int i = 0; i < 6; i++) {
+        if (p->F[i]) (void)hipFree(p->F[i]);
+        if (p->S[i]) (void)hipFree(p->S[i]);
+    }
+    if (p->ev) {
+        for (int i = 0; i < kProfSlots * (ST_COUNT + 1); i++) (void)hipEventDestroy(p->ev[i]);
+        free(p->ev);
+        free(p->slot_used);
+    }
+    if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
+    free(p->h_ct);
+    free(p->h_st);
+    free(p->h_wt);
+    free(p->h_z);
+    free(p->h_kz);
+    free(p->h_ell);
+    free(p->h_emm);
+    delete p;
+}
+
+int synth_scalar(Plan *p, int nplanes, const double *in, double *out)
+{
+    int rc = launch_synthesis(p, false, nplanes, (const double2 *)in, nullptr, nullptr, p->nlm, p->F[0],
+                              nullptr, nullptr);
+    if (rc) return rc;
+    const double2 *F[1] = {p->F[0]};
+    double *f[1] = {out};
+    return launch_phi_inverse(p, 1, nplanes, F, f);
+}
+
+int anal_scalar(Plan *p, int nplanes, const double *in, double *out)
+{
+    const double *f[1] = {in};
+    double2 *F[1] = {p->F[0]};
+    int rc = launch_phi_forward(p, 1, nplanes, f, F);
+    if (rc) return rc;
+    return launch_analysis(p, false, nplanes, p->F[0], nullptr, nullptr, (double2 *)out, nullptr,
+                           nullptr, p->nlm);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rd_error_string(int code)
+{
+    switch (code) {
+    case RD_OK: return "ok";
+    case RD_ERR_ARG: return "bad argument or size";
+    case RD_ERR_HIP: return "HIP runtime error";
+    case RD_ERR_TABLE: return "FC-Gram table missing or of the wrong size";
+    case RD_ERR_NOMEM: return "device allocation failed";
+    case RD_ERR_LU: return "singular boundary matrix";
+    case RD_ERR_STATE: return "required setup not done";
+    default: return "unknown error";
+    }
+}
+
+int rd_load_dirichlet_tables(const char *tdir, int C, int d, double *dir)
+{
+    if (!tdir || C <= 0 || d <= 0 || !dir) return RD_ERR_ARG;
+    char pa[1024], pq[1024];
+    snprintf(pa, sizeof pa, "%s/A%d-%d.dat", tdir, C, d);
+    snprintf(pq, sizeof pq, "%s/Q%d.dat", tdir, d);
+    std::vector<double> A((size_t)C * d), Q((size_t)d * d);
+    FILE *fa = fopen(pa, "rb");
+    if (!fa) {
+        fprintf(stderr, "redondito: could not find table %s\n", pa);
+        return RD_ERR_TABLE;
+    }
+    size_t na = fread(A.data(), sizeof(double), A.size(), fa);
+    int extra = fgetc(fa);
+    fclose(fa);
+    FILE *fq = fopen(pq, "rb");
+    if (!fq) {
+        fprintf(stderr, "redondito: could not find table %s\n", pq);
+        return RD_ERR_TABLE;
+    }
+    size_t nq = fread(Q.data(), sizeof(double), Q.size(), fq);
+    int extraq = fgetc(fq);
+    fclose(fq);
+    if (na != A.size() || nq != Q.size() || extra != EOF || extraq != EOF) {
+        fprintf(stderr, "redondito: FC-Gram table of the wrong size (%s, %s)\n", pa, pq);
+        return RD_ERR_TABLE;
+    }
+    // dir = A . Q^T, all column major
+    for (int i = 0; i < C; i++)
+        for (int j = 0; j < d; j++) {
+            double s = 0.0;
+            for (int k = 0; k < d; k++) s += A[i + (size_t)C * k] * Q[j + (size_t)d * k];
+            dir[i + (size_t)C * j] = s;
+        }
+    return RD_OK;
+}
+
+int rd_plan_create(rd_plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz,
+                   double rint, const double *dir)
+{
+    if (!out) return RD_ERR_ARG;
+    *out = nullptr;
+    if (lmax < 0 || nlat < 2 || (nlat & 1) || nlat <= lmax || nphi <= 2 * lmax || nphi < 2) return RD_ERR_ARG;
+    if (nz < 1 || C < 0) return RD_ERR_ARG;
+    if (C > 0 && (d < 1 || nz - C < 2 * d || !dir)) return RD_ERR_ARG;
+    if (C == 0 && d != 0) return RD_ERR_ARG;
+    if (!(dz > 0.0)) return RD_ERR_ARG;
+
+    Plan *p = new Plan();
+    memset(p, 0, sizeof(Plan));
+    p->lmax = lmax;
+    p->L = lmax + 1;
+    p->nlat = nlat;
+    p->nphi = nphi;
+    p->nlm = (lmax + 1) * (lmax + 2) / 2;
+    p->nth = nlat / 2;
+    p->nthp = (p->nth + 63) / 64 * 64;
+    p->nthp16 = (p->nth + 15) / 16 * 16;
+    p->nz = nz;
+    p->C = C;
+    p->d = d;
+    p->nr = nz - C;
+    p->dz = dz;
+    p->rint = rint;
+    p->seq_phi = make_fft_seq(nphi);
+    p->seq_z = make_fft_seq(nz);
+
+    int rc = RD_OK;
+#define TRY(x)                  \
+    do {                        \
+        rc = (x);               \
+        if (rc) {               \
+            free_plan(p);       \
+            return rc;          \
+        }                       \
+    } while (0)
+#define TRYH(x)                      \
+    do {                             \
+        if ((x) != hipSuccess) {     \
+            free_plan(p);            \
+            return RD_ERR_HIP;       \
+        }                            \
+    } while (0)
+
+    TRYH(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
+    p->stream = p->own_stream;
+
+    // host grid: z, kz (pseudospec_mod.f90:196-200,223-227), l and m per lm
+    p->h_ct = (double *)malloc(sizeof(double) * nlat);
+    p->h_st = (double *)malloc(sizeof(double) * nlat);
+    p->h_wt = (double *)malloc(sizeof(double) * nlat);
+    p->h_z = (double *)malloc(sizeof(double) * nz);
+    p->h_kz = (double *)malloc(sizeof(double) * nz);
+    p->h_ell = (int *)malloc(sizeof(int) * p->nlm);
+    p->h_emm = (int *)malloc(sizeof(int) * p->nlm);
+    for (int k = 0; k < nz; k++) p->h_z[k] = rint + dz * k;
+    const double dk = 2.0 * M_PI / (dz * nz);
+    if (nz % 2 == 0) {
+        for (int k = 0; k < nz / 2; k++) {
+            p->h_kz[k] = k * dk;
+            p->h_kz[k + nz / 2] = (k - nz / 2) * dk;
+        }
+    } else {
+        for (int k = 0; k < nz; k++) p->h_kz[k] = (k <= (nz - 1) / 2 ? k : k - nz) * dk;
+    }
+    for (int m = 0; m <= lmax; m++)
+        for (int l = m; l <= lmax; l++) {
+            const long i = (long)m * (2 * lmax + 1 - m) / 2 + l;
+            p->h_ell[i] = l;
+            p->h_emm[i] = m;
+        }
+
+    TRYH(hipMalloc(&p->d_ct, sizeof(double) * nlat));
+    TRYH(hipMalloc(&p->d_st, sizeof(double) * nlat));
+    TRYH(hipMalloc(&p->d_wt, sizeof(double) * nlat));
+    TRYH(hipMalloc(&p->d_z, sizeof(double) * nz));
+    TRYH(hipMalloc(&p->d_kz, sizeof(double) * nz));
+    TRYH(hipMalloc(&p->d_ell, sizeof(int) * p->nlm));
+    TRYH(hipMemcpyAsync(p->d_z, p->h_z, sizeof(double) * nz, hipMemcpyHostToDevice, p->stream));
+    TRYH(hipMemcpyAsync(p->d_kz, p->h_kz, sizeof(double) * nz, hipMemcpyHostToDevice, p->stream));
+    TRYH(hipMemcpyAsync(p->d_ell, p->h_ell, sizeof(int) * p->nlm, hipMemcpyHostToDevice, p->stream));
+    if (C > 0) {
+        TRYH(hipMalloc(&p->d_dir, sizeof(double) * C * d));
+        TRYH(hipMemcpyAsync(p->d_dir, dir, sizeof(double) * C * d, hipMemcpyHostToDevice, p->stream));
+    }
+    TRY(upload_twiddles(p, nphi, &p->tw_phi));
+    TRY(upload_twiddles(p, nz, &p->tw_z));
+    TRY(build_legendre_tables(p));
+
+    p->fsize = (long)nz * p->L * nlat;
+    p->ssize = (long)p->nlm * nz;
+    for (int i = 0; i < 6; i++) {
+        if (hipMalloc(&p->F[i], sizeof(double2) * p->fsize) != hipSuccess) {
+            free_plan(p);
+            return RD_ERR_NOMEM;
+        }
+        if (hipMalloc(&p->S[i], sizeof(double2) * p->ssize) != hipSuccess) {
+            free_plan(p);
+            return RD_ERR_NOMEM;
+        }
+    }
+    TRYH(hipStreamSynchronize(p->stream));
+#undef TRY
+#undef TRYH
+    *out = reinterpret_cast<rd_plan *>(p);
+    return RD_OK;
+}
+
+int rd_plan_destroy(rd_plan *plan)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    (void)hipStreamSynchronize(p->stream);
+    free_plan(p);
+    return RD_OK;
+}
+
+int rd_plan_set_stream(rd_plan *plan, void *s)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    p->stream = s ? (hipStream_t)s : p->own_stream;
+    return RD_OK;
+}
+
+int rd_plan_sync(rd_plan *plan)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    RD_HIP(hipStreamSynchronize(P(plan)->stream));
+    return RD_OK;
+}
+
+int rd_plan_info(const rd_plan *plan, int *info)
+{
+    if (bad_plan(plan) || !info) return RD_ERR_ARG;
+    const Plan *p = P(plan);
+    int v[9] = {p->lmax, p->nlat, p->nphi, p->nlm, p->nz, p->C, p->d, p->nr, 0};
+    memcpy(info, v, sizeof v);
+    return RD_OK;
+}
+
+int rd_plan_grid(const rd_plan *plan, double *ct, double *st, double *wt, double *z, double *kz,
+                 int *ell, int *emm)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    const Plan *p = P(plan);
+    if (ct) memcpy(ct, p->h_ct, sizeof(double) * p->nlat);
+    if (st) memcpy(st, p->h_st, sizeof(double) * p->nlat);
+    if (wt) memcpy(wt, p->h_wt, sizeof(double) * p->nlat);
+    if (z) memcpy(z, p->h_z, sizeof(double) * p->nz);
+    if (kz) memcpy(kz, p->h_kz, sizeof(double) * p->nz);
+    if (ell) memcpy(ell, p->h_ell, sizeof(int) * p->nlm);
+    if (emm) memcpy(emm, p->h_emm, sizeof(int) * p->nlm);
+    return RD_OK;
+}
+
+int rd_sht_analysis(rd_plan *plan, int nplanes, const double *in, double *out)
+{
+    if (bad_plan(plan) || !in || !out) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    if (nplanes == 0) return RD_OK;
+    return anal_scalar(p, nplanes, in, out);
+}
+
+int rd_sht_synthesis(rd_plan *plan, int nplanes, const double *in, double *out)
+{
+    if (bad_plan(plan) || !in || !out) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    if (nplanes == 0) return RD_OK;
+    return synth_scalar(p, nplanes, in, out);
+}
+
+int rd_vsht_analysis(rd_plan *plan, int nplanes, const double *vr, const double *vt, const double *vp,
+                     double *q, double *s, double *t)
+{
+    if (bad_plan(plan) || !vr || !vt || !vp || !q || !s || !t) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    if (nplanes == 0) return RD_OK;
+    const double *f[3] = {vr, vt, vp};
+    double2 *F[3] = {p->F[0], p->F[1], p->F[2]};
+    int rc = launch_phi_forward(p, 3, nplanes, f, F);
+    if (rc) return rc;
+    return launch_analysis(p, true, nplanes, p->F[0], p->F[1], p->F[2], (double2 *)q, (double2 *)s,
+                           (double2 *)t, p->nlm);
+}
+
+int rd_vsht_synthesis(rd_plan *plan, int nplanes, const double *q, const double *s, const double *t,
+                      double *vr, double *vt, double *vp)
+{
+    if (bad_plan(plan) || !vr || !vt || !vp || !q || !s || !t) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    if (nplanes == 0) return RD_OK;
+    int rc = launch_synthesis(p, true, nplanes, (const double2 *)q, (const double2 *)s,
+                              (const double2 *)t, p->nlm, p->F[0], p->F[1], p->F[2]);
+    if (rc) return rc;
+    const double2 *F[3] = {p->F[0], p->F[1], p->F[2]};
+    double *f[3] = {vr, vt, vp};
+    return launch_phi_inverse(p, 3, nplanes, F, f);
+}
+
+int rd_z_forward(rd_plan *plan, double *inout)
+{
+    if (bad_plan(plan) || !inout) return RD_ERR_ARG;
+    return launch_z_forward(P(plan), (double2 *)inout);
+}
+
+int rd_z_inverse(rd_plan *plan, double *inout)
+{
+    if (bad_plan(plan) || !inout) return RD_ERR_ARG;
+    return launch_z_inverse(P(plan), (double2 *)inout);
+}
+
+int rd_3d_forward(rd_plan *plan, const double *in, double *out)
+{
+    if (bad_plan(plan) || !in || !out) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    int rc = anal_scalar(p, p->nz, in, out);
+    if (rc) return rc;
+    return launch_z_forward(p, (double2 *)out);
+}
+
+int rd_3d_inverse(rd_plan *plan, double *in, double *out)
+{
+    if (bad_plan(plan) || !in || !out) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    int rc = launch_z_inverse(p, (double2 *)in);
+    if (rc) return rc;
+    return synth_scalar(p, p->nz, in, out);
+}
+
+int rd_redist_k_to_lm(rd_plan *plan, int nplanes, const double *in, double *out)
+{
+    if (bad_plan(plan) || !in || !out) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    if (in == out || nplanes == 0) return RD_OK;
+    RD_HIP(hipMemcpyAsync(out, in, sizeof(double2) * (size_t)p->nlm * nplanes, hipMemcpyDeviceToDevice,
+                          p->stream));
+    return RD_OK;
+}
+
+int rd_redist_lm_to_k(rd_plan *plan, int nplanes, const double *in, double *out)
+{
+    return rd_redist_k_to_lm(plan, nplanes, in, out);
+}
+
+int rd_derivr(rd_plan *plan, int order, char esp, const double *a, double *b)
+{
+    if (bad_plan(plan) || !a || !b || order < 0) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (esp == 'r') {
+        if (a == b) return RD_ERR_ARG;
+        return launch_derivr_r(p, order, (const double2 *)a, (double2 *)b);
+    }
+    if (esp == 'k') return launch_derivr_k(p, order, (const double2 *)a, (double2 *)b);
+    fprintf(stderr, "redondito: unknown space %c in derivr\n", esp);
+    return RD_ERR_ARG;
+}
+
+int rd_laplang(rd_plan *plan, const double *a, double *b)
+{
+    if (bad_plan(plan) || !a || !b) return RD_ERR_ARG;
+    return launch_laplang(P(plan), (const double2 *)a, (double2 *)b);
+}
+
+int rd_multr(rd_plan *plan, int pw, const double *a, double *b)
+{
+    if (bad_plan(plan) || !a || !b) return RD_ERR_ARG;
+    return launch_multr(P(plan), pw, (const double2 *)a, (double2 *)b);
+}
+
+int rd_dlr(rd_plan *plan, const double *a, double *b)
+{
+    if (bad_plan(plan) || !a || !b || a == b) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    int rc = launch_derivr_r(p, 2, (const double2 *)a, (double2 *)b);
+    if (!rc) rc = launch_laplang(p, (const double2 *)a, p->S[0]);
+    if (!rc) rc = launch_multr(p, -2, p->S[0], p->S[1]);
+    if (!rc) rc = launch_saxpby(p, 2L * p->nlm * p->nz, b, b, 1.0, (const double *)p->S[1], 1.0);
+    return rc;
+}
+
+int rd_saxpby_c(rd_plan *plan, double *z, const double *x, double a, const double *y, double b)
+{
+    if (bad_plan(plan) || !z || !x || !y) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    return launch_saxpby(p, 2L * p->nlm * p->nz, z, x, a, y, b);
+}
+
+int rd_saxpby_r(rd_plan *plan, int nplanes, double *z, const double *x, double a, const double *y,
+                double b)
+{
+    if (bad_plan(plan) || !z || !x || !y) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    return launch_saxpby(p, (long)p->nlat * p->nphi * nplanes, z, x, a, y, b);
+}
+
+int rd_prodvec(rd_plan *plan, int nplanes, const double *ar, const double *at, const double *ap,
+               const double *br, const double *bt, const double *bp, double *cr, double *ct, double *cp)
+{
+    if (bad_plan(plan) || !ar || !at || !ap || !br || !bt || !bp || !cr || !ct || !cp) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
+    const int np = nplanes < p->nr ? nplanes : p->nr;
+    return launch_prodvec(p, (long)p->nlat * p->nphi * np, ar, at, ap, br, bt, bp, cr, ct, cp);
+}
+
+int rd_fc_filter(rd_plan *plan, char esp, double *a)
+{
+    if (bad_plan(plan) || !a) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (esp == 'k') return launch_fc_filter_k(p, (double2 *)a);
+    if (esp == 'r') {
+        int rc = launch_z_forward(p, (double2 *)a);
+        if (!rc) rc = launch_fc_filter_k(p, (double2 *)a);
+        if (!rc) rc = launch_z_inverse(p, (double2 *)a);
+        return rc;
+    }
+    fprintf(stderr, "redondito: unknown space %c in fc_filter\n", esp);
+    return RD_ERR_ARG;
+}
+
+int rd_copy(rd_plan *plan, const double *a, double *b)
+{
+    if (bad_plan(plan) || !a || !b) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (a == b) return RD_OK;
+    RD_HIP(hipMemcpyAsync(b, a, sizeof(double2) * (size_t)p->nlm * p->nz, hipMemcpyDeviceToDevice,
+                          p->stream));
+    return RD_OK;
+}
+
+static int harvest_slot(Plan *p, int s)
+{
+    if (!p->slot_used[s]) return RD_OK;
+    hipEvent_t *e = p->ev + (long)s * (ST_COUNT + 1);
+    RD_HIP(hipEventSynchronize(e[ST_COUNT]));
+    for (int i = 0; i < ST_COUNT; i++) {
+        float ms = 0.f;
+        RD_HIP(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+        p->stage_ms[i] += ms;
+        p->stage_n[i] += 1;
+    }
+    p->slot_used[s] = 0;
+    return RD_OK;
+}
+
+int rd_plan_profile(rd_plan *plan, int enable)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    const int nev = kProfSlots * (ST_COUNT + 1);
+    if (enable && !p->profiling) {
+        p->ev = (hipEvent_t *)calloc(nev, sizeof(hipEvent_t));
+        p->slot_used = (int *)calloc(kProfSlots, sizeof(int));
+        if (!p->ev || !p->slot_used) return RD_ERR_NOMEM;
+        for (int i = 0; i < nev; i++) RD_HIP(hipEventCreate(&p->ev[i]));
+        p->next_slot = 0;
+    }
+    if (!enable && p->profiling) {
+        RD_HIP(hipStreamSynchronize(p->stream));
+        for (int i = 0; i < nev; i++) (void)hipEventDestroy(p->ev[i]);
+        free(p->ev);
+        free(p->slot_used);
+        p->ev = nullptr;
+        p->slot_used = nullptr;
+    }
+    p->profiling = enable ? 1 : 0;
+    for (int i = 0; i < kMaxStages; i++) {
+        p->stage_ms[i] = 0.0;
+        p->stage_n[i] = 0;
+    }
+    return RD_OK;
+}
+
+int rd_plan_stage_times(rd_plan *plan, double *ms, long *count, int nmax)
+{
+    if (bad_plan(plan) || !ms || !count) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->profiling)
+        for (int s = 0; s < kProfSlots; s++) {
+            int rc = harvest_slot(p, s);
+            if (rc) return rc;
+        }
+    for (int i = 0; i < nmax && i < ST_COUNT; i++) {
+        ms[i] = p->stage_ms[i];
+        count[i] = p->stage_n[i];
+    }
+    return ST_COUNT;
+}
+
+int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ, double *NS,
+                     double *NT)
+{
+    if (bad_plan(plan) || !W || !Z || !NQ || !NS || !NT) return RD_ERR_ARG;
+    if (prec != RD_PREC_FP64) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    const int nr = p->nr;
+    const bool prof = p->profiling != 0;
+    hipEvent_t *ev = nullptr;
+    if (prof) {
+        const int s = p->next_slot;
+        p->next_slot = (s + 1) % kProfSlots;
+        int hrc = harvest_slot(p, s);
+        if (hrc) return hrc;
+        ev = p->ev + (long)s * (ST_COUNT + 1);
+        p->slot_used[s] = 1;
+    }
+#define MARK(i) \
+    if (prof) RD_HIP(hipEventRecord(ev[i], p->stream))
+    MARK(0);
+    int rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, p->S);
+    MARK(1);
+    if (!rc) rc = launch_synthesis(p, true, nr, p->S[0], p->S[1], p->S[2], p->nlm, p->F[0], p->F[1], p->F[2]);
+    MARK(2);
+    if (!rc) rc = launch_synthesis(p, true, nr, p->S[3], p->S[4], p->S[5], p->nlm, p->F[3], p->F[4], p->F[5]);
+    MARK(3);
+    if (!rc) {
+        const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
+        double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
+        rc = launch_phi_cross(p, nr, Fin, Fout);
+    }
+    MARK(4);
+    if (!rc)
+        rc = launch_analysis(p, true, nr, p->F[0], p->F[1], p->F[2], (double2 *)NQ, (double2 *)NS,
+                             (double2 *)NT, p->nlm);
+    MARK(5);
+    if (!rc) rc = launch_zero_rows(p, (double2 *)NQ, nr, p->nz - nr);
+    if (!rc) rc = launch_zero_rows(p, (double2 *)NS, nr, p->nz - nr);
+    if (!rc) rc = launch_zero_rows(p, (double2 *)NT, nr, p->nz - nr);
+    MARK(6);
+#undef MARK
+    return rc;
+}
+
+int rd_dirichlet(rd_plan *plan, double *a)
+{
+    if (bad_plan(plan) || !a) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->nr < 1) return RD_ERR_STATE;
+    return launch_dirichlet(p, (double2 *)a);
+}
+
+}  // extern "C"

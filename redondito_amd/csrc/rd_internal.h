@@ -1,0 +1,121 @@
+// Internal declarations of the MI355X pseudospectral library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/redondito.h"
+
+namespace rd {
+
+constexpr int kMaxStages = 24;
+
+// Radix sequence of one FFT length; the Stockham stages run in this order.
+struct FftSeq {
+    int n;
+    int nst;
+    int rad[kMaxStages];
+};
+
+FftSeq make_fft_seq(int n);
+
+// Sizes and device tables of one plan.
+struct Plan {
+    int lmax, L, nlat, nphi, nlm, nth, nthp, nthp16;
+    int nz, C, d, nr;
+    double dz, rint;
+    hipStream_t stream;
+    hipStream_t own_stream;
+    // grid (host copies)
+    double *h_ct, *h_st, *h_wt, *h_z, *h_kz;
+    int *h_ell, *h_emm;
+    // device grid
+    double *d_ct, *d_st, *d_wt, *d_z, *d_kz, *d_dir;
+    int *d_ell;
+    // associated-Legendre tables: synthesis [m,p][row l_p][theta nthp],
+    // analysis [m,p][theta nthp16][col l_p] with weights (and 1/l(l+1))
+    double *syn_P, *syn_D, *syn_M;
+    double *ana_P, *ana_D, *ana_M;
+    long *d_syn_off, *d_ana_off;   // per (m,p)
+    int *d_ana_ncol;               // per m
+    long syn_rows, ana_elems;
+    int *d_ana_tiles;              // (m, l-tile) pairs of the analysis grid
+    int n_ana_tiles;
+    // FFTs
+    FftSeq seq_phi, seq_z;
+    double2 *tw_phi, *tw_z;        // exp(-2 pi i t / n)
+    // workspace
+    double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
+    double2 *S[6];                 // spectral scratch (nlm, nz)
+    long fsize, ssize;             // in elements of the type
+    // per-stage event timing of rd_nonlinear_uxw: a ring of event slots,
+    // harvested lazily so that timing adds no host synchronisation
+    int profiling;
+    hipEvent_t *ev;                // kProfSlots x (ST_COUNT + 1)
+    int *slot_used;
+    int next_slot;
+    double stage_ms[kMaxStages];
+    long stage_n[kMaxStages];
+};
+
+constexpr int kProfSlots = 512;
+
+// Stages of rd_nonlinear_uxw, in order.
+enum UxwStage { ST_RADIAL = 0, ST_SYN_U, ST_SYN_W, ST_PHI_CROSS, ST_ANALYSIS, ST_TAIL, ST_COUNT };
+
+// ---------------------------------------------------------------- kernels
+// Legendre contractions over all planes (Fourier layout F[k][m][theta]).
+int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s,
+                     const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2);
+int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
+                    const double2 *g2, double2 *q, double2 *s, double2 *t, long ld);
+int build_legendre_tables(Plan *p);
+
+// phi transforms between F[k][m][theta] and f(theta, phi, k).
+int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f);
+int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f, double2 *const *F);
+// Fused: six Fourier fields (u_r,u_t,u_p,w_r,w_t,w_p) -> u x w -> three Fourier fields.
+int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout);
+
+// Radial transforms on (nlm, nz) complex, in place.
+int launch_z_forward(Plan *p, double2 *a);
+int launch_z_inverse(Plan *p, double2 *a);
+// Radial derivative of order p in (lm,r): b = derivr(a, p).
+int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b);
+// Fused radial step of the nonlinear term: W, Z -> the six (Q,S,T) arrays.
+int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst);
+
+// Pointwise operators.
+int launch_derivr_k(Plan *p, int order, const double2 *a, double2 *b);
+int launch_laplang(Plan *p, const double2 *a, double2 *b);
+int launch_multr(Plan *p, int pw, const double2 *a, double2 *b);
+int launch_saxpby(Plan *p, long n, double *z, const double *x, double a, const double *y, double b);
+int launch_prodvec(Plan *p, long npts, const double *ar, const double *at, const double *ap,
+                   const double *br, const double *bt, const double *bp, double *cr, double *ct,
+                   double *cp);
+int launch_fc_filter_k(Plan *p, double2 *a);
+int launch_dirichlet(Plan *p, double2 *a);
+int launch_zero_rows(Plan *p, double2 *a, int row0, int nrows);
+
+}  // namespace rd
+
+#define RD_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "redondito: %s failed: %s (%s:%d)\n", #call,                 \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                           \
+            return RD_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+
+#define RD_LAUNCH_CHECK()                                                                \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "redondito: launch failed: %s (%s:%d)\n",                    \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                           \
+            return RD_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)

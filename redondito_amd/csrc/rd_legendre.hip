@@ -1,0 +1,538 @@
+// Associated-Legendre tables and the per-m Legendre contractions (FP64 MFMA).
+//
+// Synthesis, for every m and every plane k (SHTns SH_to_spat / SHqst_to_spat
+// semantics, called per plane by src/sphp/sphp_sca.F90:484-486 and
+// src/sphp/sphp_vec.F90:152-160):
+//   F_m(theta, k) = sum_l P~_lm(theta) a_lm(k)
+// run as a GEMM  [theta x l] . [l x (k, re/im)]  with the equatorial symmetry
+// P~_lm(pi - theta) = (-1)^(l-m) P~_lm(theta): the even and odd l - m halves are
+// accumulated separately over the northern nodes and combined into north and
+// south rows.  Analysis is the transposed contraction with the Gauss weights
+// (spat_to_SH / spat_to_SHqst, src/sphp/sphp_sca.F90:311-313,
+// src/sphp/sphp_vec.F90:88-92).
+//
+// Each workgroup is 4 wave64s; each wave owns a 16-row slab of the output
+// and issues v_mfma_f64_16x16x4f64.  The spectral operand is staged through
+// LDS with coalesced loads along l; the table operand is read straight from
+// HBM/L2 in the layout the MFMA A-operand wants (16 consecutive doubles per
+// lane group).
+#include "rd_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace rd {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ long lm_index(int lmax, int l, int m)
+{
+    return (long)m * (2 * lmax + 1 - m) / 2 + l;
+}
+
+// ------------------------------------------------------------------ grid
+// Gauss-Legendre nodes, north to south, Newton on theta (one thread per
+// northern node) so sin(theta) keeps its relative accuracy at the poles.
+__global__ void gauss_kernel(int nlat, double *ct, double *st, double *wt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nlat + 1) / 2) return;
+    double th = M_PI * (i + 0.75) / (nlat + 0.5);
+    double dpdt = 1.0;
+    for (int it = 0; it < 100; it++) {
+        double x = cos(th), s = sin(th);
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= nlat; k++) {
+            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dpdt = nlat * (x * p1 - p0) / s;
+        double dth = p1 / dpdt;
+        th -= dth;
+        if (fabs(dth) < 1e-17) break;
+    }
+    double x = cos(th), s = sin(th);
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= nlat; k++) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    dpdt = nlat * (x * p1 - p0) / s;
+    double w = 2.0 / (dpdt * dpdt);
+    ct[i] = x;
+    st[i] = s;
+    wt[i] = w;
+    ct[nlat - 1 - i] = -x;
+    st[nlat - 1 - i] = s;
+    wt[nlat - 1 - i] = w;
+}
+
+// One thread per (m, northern node): the FP64 recurrence
+//   P~_mm = c_m s^m,  P~_{m+1,m} = sqrt(2m+3) x P~_mm,
+//   P~_lm = a_lm (x P~_{l-1,m} - b_lm P~_{l-2,m})
+// run on P~ and on D = s^m dQ/dx (P~ = s^m Q(x)) with a power-of-two scale
+// for the sectoral start, then
+//   dP~/dtheta = (m x P~ - s^2 D) / s,   m P~ / s.
+__global__ void legendre_tables_kernel(int lmax, int nth, int nthp, int nthp16, const double *ct,
+                                       const double *st, const double *wt, const long *syn_off,
+                                       const long *ana_off, const int *ana_ncol, double *sP,
+                                       double *sD, double *sM, double *aP, double *aD, double *aM)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = blockIdx.y;
+    if (i >= nth) return;
+    double cm = sqrt((2.0 * m + 1.0) / (4.0 * M_PI));
+    for (int k = 1; k <= m; k++) cm *= sqrt((2.0 * k - 1.0) / (2.0 * k));
+    double x = ct[i], s = st[i], w = wt[i];
+    double v = cm;
+    int e2 = 0;
+    for (int k = 0; k < m; k++) {
+        v *= s;
+        if (v < 0x1p-400) {
+            v *= 0x1p400;
+            e2 -= 400;
+        }
+    }
+    double p1 = v, p2 = 0.0, d1 = 0.0, d2 = 0.0;
+    int ncol = ana_ncol[m];
+    for (int l = m; l <= lmax; l++) {
+        double pc, dc;
+        if (l == m) {
+            pc = p1;
+            dc = 0.0;
+        } else if (l == m + 1) {
+            double a = sqrt(2.0 * m + 3.0);
+            pc = a * x * p1;
+            dc = a * p1;
+        } else {
+            double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+            double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+            pc = a * (x * p1 - b * p2);
+            dc = a * (p1 + x * d1 - b * d2);
+        }
+        if (l > m) {
+            p2 = p1;
+            p1 = pc;
+            d2 = d1;
+            d1 = dc;
+        }
+        double P = ldexp(pc, e2), Dq = ldexp(dc, e2);
+        double dP = (m * x * P - s * s * Dq) / s;
+        double mP = m * P / s;
+        int par = (l - m) & 1, lp = (l - m) >> 1;
+        long so = (syn_off[2 * m + par] + lp) * nthp + i;
+        sP[so] = P;
+        sD[so] = dP;
+        sM[so] = mP;
+        double ll = (double)l * (l + 1);
+        double il = l > 0 ? 1.0 / ll : 0.0;
+        long ao = ana_off[2 * m + par] + (long)i * ncol + lp;
+        aP[ao] = w * P;
+        aD[ao] = w * dP * il;
+        aM[ao] = w * mP * il;
+    }
+}
+
+static inline int rup(int a, int b) { return (a + b - 1) / b * b; }
+
+int build_legendre_tables(Plan *p)
+{
+    const int L = p->L;
+    std::vector<long> syn_off(2 * L), ana_off(2 * L);
+    std::vector<int> ana_ncol(L);
+    long rows = 0, elems = 0;
+    std::vector<int> tiles;
+    for (int m = 0; m < L; m++) {
+        int ne = (L - m + 1) / 2;             // l - m even count (>= odd count)
+        int rowsm = rup(ne, 16);
+        syn_off[2 * m] = rows;
+        syn_off[2 * m + 1] = rows + rowsm;
+        rows += 2 * rowsm;
+        int ncol = rup(ne, 32);
+        ana_ncol[m] = ncol;
+        ana_off[2 * m] = elems;
+        ana_off[2 * m + 1] = elems + (long)p->nthp16 * ncol;
+        elems += 2L * p->nthp16 * ncol;
+        for (int lt = 0; lt < ncol / 32; lt++) tiles.push_back(m | (lt << 16));
+    }
+    p->syn_rows = rows;
+    p->ana_elems = elems;
+    p->n_ana_tiles = (int)tiles.size();
+    size_t sb = sizeof(double) * rows * p->nthp, ab = sizeof(double) * elems;
+    RD_HIP(hipMalloc(&p->syn_P, sb));
+    RD_HIP(hipMalloc(&p->syn_D, sb));
+    RD_HIP(hipMalloc(&p->syn_M, sb));
+    RD_HIP(hipMalloc(&p->ana_P, ab));
+    RD_HIP(hipMalloc(&p->ana_D, ab));
+    RD_HIP(hipMalloc(&p->ana_M, ab));
+    RD_HIP(hipMalloc(&p->d_syn_off, sizeof(long) * 2 * L));
+    RD_HIP(hipMalloc(&p->d_ana_off, sizeof(long) * 2 * L));
+    RD_HIP(hipMalloc(&p->d_ana_ncol, sizeof(int) * L));
+    RD_HIP(hipMalloc(&p->d_ana_tiles, sizeof(int) * tiles.size()));
+    hipStream_t st = p->stream;
+    RD_HIP(hipMemsetAsync(p->syn_P, 0, sb, st));
+    RD_HIP(hipMemsetAsync(p->syn_D, 0, sb, st));
+    RD_HIP(hipMemsetAsync(p->syn_M, 0, sb, st));
+    RD_HIP(hipMemsetAsync(p->ana_P, 0, ab, st));
+    RD_HIP(hipMemsetAsync(p->ana_D, 0, ab, st));
+    RD_HIP(hipMemsetAsync(p->ana_M, 0, ab, st));
+    RD_HIP(hipMemcpyAsync(p->d_syn_off, syn_off.data(), sizeof(long) * 2 * L, hipMemcpyHostToDevice, st));
+    RD_HIP(hipMemcpyAsync(p->d_ana_off, ana_off.data(), sizeof(long) * 2 * L, hipMemcpyHostToDevice, st));
+    RD_HIP(hipMemcpyAsync(p->d_ana_ncol, ana_ncol.data(), sizeof(int) * L, hipMemcpyHostToDevice, st));
+    RD_HIP(hipMemcpyAsync(p->d_ana_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
+
+    hipLaunchKernelGGL(gauss_kernel, dim3((p->nlat / 2 + 63) / 64), dim3(64), 0, st, p->nlat, p->d_ct,
+                       p->d_st, p->d_wt);
+    RD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(legendre_tables_kernel, dim3((p->nth + 63) / 64, L), dim3(64), 0, st, p->lmax,
+                       p->nth, p->nthp, p->nthp16, p->d_ct, p->d_st, p->d_wt, p->d_syn_off,
+                       p->d_ana_off, p->d_ana_ncol, p->syn_P, p->syn_D, p->syn_M, p->ana_P,
+                       p->ana_D, p->ana_M);
+    RD_LAUNCH_CHECK();
+    RD_HIP(hipMemcpyAsync(p->h_ct, p->d_ct, sizeof(double) * p->nlat, hipMemcpyDeviceToHost, st));
+    RD_HIP(hipMemcpyAsync(p->h_st, p->d_st, sizeof(double) * p->nlat, hipMemcpyDeviceToHost, st));
+    RD_HIP(hipMemcpyAsync(p->h_wt, p->d_wt, sizeof(double) * p->nlat, hipMemcpyDeviceToHost, st));
+    RD_HIP(hipStreamSynchronize(st));
+    return RD_OK;
+}
+
+// ------------------------------------------------------------- synthesis
+// Workgroup tile: one m, 64 northern nodes (16 per wave), SJT real columns
+// (SJT/2 planes).  The l loop runs in chunks of 32 (16 per parity).
+constexpr int SKC = 32;
+
+template <bool VEC>
+struct SynCfg {
+    static constexpr int NF = VEC ? 3 : 1;
+    static constexpr int JB = VEC ? 2 : 4;       // 16-column blocks per wave
+    static constexpr int JT = 16 * JB;           // columns per workgroup
+    static constexpr int JTP = JT + 2;           // LDS row pitch (doubles)
+};
+
+struct SynArgs {
+    const double *tP, *tD, *tM;
+    const long *off;
+    const double2 *in[3];
+    double2 *out[3];
+    long ld;
+    int lmax, nlat, nth, nthp, nplanes, n_tt, n_jt;
+};
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
+{
+    using Cfg = SynCfg<VEC>;
+    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
+    __shared__ double Bs[NF][SKC][JTP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int L = A.lmax + 1;
+    int bid = blockIdx.x;
+    const int jt = bid % A.n_jt;
+    bid /= A.n_jt;
+    const int tt = bid % A.n_tt;
+    const int m = bid / A.n_tt;
+    const int k0 = jt * (JT / 2);
+    const int th0 = tt * 64 + wave * 16;
+    const int col = lane & 15, kk = lane >> 4;
+    const double sgn_i = (col & 1) ? 1.0 : -1.0;   // (i X)_j = sgn_i * X_{j^1}
+
+    v4d acc[NF][2][JB];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int b = 0; b < JB; b++) acc[f][s][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
+    const long lm0 = lm_index(A.lmax, m, m);
+
+    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
+        // stage the spectral chunk: NF x (JT/2 planes) x SKC l's
+        constexpr int PER = (JT / 2) * SKC;
+        for (int e = tid; e < NF * PER; e += 256) {
+            int f = e / PER, r = e % PER;
+            int pl = r / SKC, ll = r % SKC;
+            int l = lbase + ll, k = k0 + pl;
+            double2 v = make_double2(0.0, 0.0);
+            if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
+            Bs[f][ll][2 * pl] = v.x;
+            Bs[f][ll][2 * pl + 1] = v.y;
+        }
+        __syncthreads();
+        const int rowbase = (lbase - m) >> 1;
+#pragma unroll
+        for (int st = 0; st < SKC / 8; st++) {
+            const int row = rowbase + 4 * st + kk;
+            const long ie = offe + (long)row * A.nthp + th0 + col;
+            const long io = offo + (long)row * A.nthp + th0 + col;
+            const double Pe = A.tP[ie], Po = A.tP[io];
+            double De = 0, Do = 0, Me = 0, Mo = 0;
+            if (VEC) {
+                De = A.tD[ie];
+                Do = A.tD[io];
+                Me = A.tM[ie];
+                Mo = A.tM[io];
+            }
+            const int lle = 8 * st + 2 * kk, llo = lle + 1;
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const int j = b * 16 + col;
+                acc[0][0][b] = mfma(Pe, Bs[0][lle][j], acc[0][0][b]);
+                acc[0][1][b] = mfma(Po, Bs[0][llo][j], acc[0][1][b]);
+                if (VEC) {
+                    const double Se = Bs[1][lle][j], So = Bs[1][llo][j];
+                    const double Te = Bs[2][lle][j], To = Bs[2][llo][j];
+                    const double iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
+                    const double iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
+                    // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
+                    acc[1][0][b] = mfma(Do, So, acc[1][0][b]);
+                    acc[1][0][b] = mfma(Me, iTe, acc[1][0][b]);
+                    acc[1][1][b] = mfma(De, Se, acc[1][1][b]);
+                    acc[1][1][b] = mfma(Mo, iTo, acc[1][1][b]);
+                    // v_phi: symmetric = mP_e (iS)_e - dP_o T_o, antisymmetric = mP_o (iS)_o - dP_e T_e
+                    acc[2][0][b] = mfma(Me, iSe, acc[2][0][b]);
+                    acc[2][0][b] = mfma(Do, -To, acc[2][0][b]);
+                    acc[2][1][b] = mfma(Mo, iSo, acc[2][1][b]);
+                    acc[2][1][b] = mfma(De, -Te, acc[2][1][b]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // north = sym + anti, south = sym - anti; lane holds D[lane/16 + 4v][lane%16]
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        double *out = (double *)A.out[f];
+#pragma unroll
+        for (int b = 0; b < JB; b++) {
+            const int j = b * 16 + col;
+            const int k = k0 + (j >> 1), c = j & 1;
+            if (k >= A.nplanes) continue;
+            const long base = ((long)k * L + m) * A.nlat;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int th = th0 + kk + 4 * v;
+                if (th >= A.nth) continue;
+                const double sy = acc[f][0][b][v], an = acc[f][1][b][v];
+                out[2 * (base + th) + c] = sy + an;
+                out[2 * (base + A.nlat - 1 - th) + c] = sy - an;
+            }
+        }
+    }
+}
+
+int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s,
+                     const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2)
+{
+    SynArgs A;
+    A.tP = p->syn_P;
+    A.tD = p->syn_D;
+    A.tM = p->syn_M;
+    A.off = p->d_syn_off;
+    A.in[0] = q;
+    A.in[1] = vec ? s : q;
+    A.in[2] = vec ? t : q;
+    A.out[0] = f0;
+    A.out[1] = vec ? f1 : f0;
+    A.out[2] = vec ? f2 : f0;
+    A.ld = ld;
+    A.lmax = p->lmax;
+    A.nlat = p->nlat;
+    A.nth = p->nth;
+    A.nthp = p->nthp;
+    A.nplanes = nplanes;
+    A.n_tt = p->nthp / 64;
+    const int JT = vec ? SynCfg<true>::JT : SynCfg<false>::JT;
+    A.n_jt = (2 * nplanes + JT - 1) / JT;
+    dim3 grid((unsigned)p->L * A.n_tt * A.n_jt);
+    if (vec)
+        hipLaunchKernelGGL(synthesis_kernel<true>, grid, dim3(256), 0, p->stream, A);
+    else
+        hipLaunchKernelGGL(synthesis_kernel<false>, grid, dim3(256), 0, p->stream, A);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+// -------------------------------------------------------------- analysis
+// Workgroup tile: one m, 32 rows l_p per parity (2 waves x 16), AJT real
+// columns (2 waves x JB x 16).  The theta loop runs over the northern nodes
+// in chunks of 16.
+constexpr int ATC = 16;
+
+template <bool VEC>
+struct AnaCfg {
+    static constexpr int NF = VEC ? 3 : 1;
+    static constexpr int JB = VEC ? 2 : 4;
+    static constexpr int JT = 2 * 16 * JB;
+    static constexpr int JTP = JT + 2;
+};
+
+struct AnaArgs {
+    const double *tP, *tD, *tM;
+    const long *off;
+    const int *ncol;
+    const int *tiles;
+    const double2 *in[3];
+    double2 *out[3];
+    long ld;
+    int lmax, nlat, nth, nthp16, nplanes, n_jt;
+};
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
+{
+    using Cfg = AnaCfg<VEC>;
+    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
+    __shared__ double Gn[NF][ATC][JTP];
+    __shared__ double Gs[NF][ATC][JTP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int L = A.lmax + 1;
+    int bid = blockIdx.x;
+    const int jt = bid % A.n_jt;
+    const int tile = A.tiles[bid / A.n_jt];
+    const int m = tile & 0xffff, lt = tile >> 16;
+    const int wl = wave & 1, wj = wave >> 1;
+    const int k0 = jt * (JT / 2);
+    const int col = lane & 15, kk = lane >> 4;
+    const double sgn_mi = (col & 1) ? -1.0 : 1.0;   // (-i X)_j = sgn_mi * X_{j^1}
+    const int ncol = A.ncol[m];
+    const int lp0 = lt * 32 + wl * 16;
+
+    v4d acc[NF][2][JB];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int b = 0; b < JB; b++) acc[f][s][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
+
+    for (int tc = 0; tc < A.nth; tc += ATC) {
+        constexpr int PER = (JT / 2) * ATC;
+        for (int e = tid; e < NF * PER; e += 256) {
+            int f = e / PER, r = e % PER;
+            int pl = r / ATC, tl = r % ATC;
+            int th = tc + tl, k = k0 + pl;
+            double2 vn = make_double2(0.0, 0.0), vs = vn;
+            if (th < A.nth && k < A.nplanes) {
+                const long base = ((long)k * L + m) * A.nlat;
+                vn = A.in[f][base + th];
+                vs = A.in[f][base + A.nlat - 1 - th];
+            }
+            Gn[f][tl][2 * pl] = vn.x;
+            Gn[f][tl][2 * pl + 1] = vn.y;
+            Gs[f][tl][2 * pl] = vs.x;
+            Gs[f][tl][2 * pl + 1] = vs.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < ATC / 4; st++) {
+            const int tl = 4 * st + kk;
+            const long rowoff = (long)(tc + tl) * ncol + lp0 + col;
+            const double Pe = A.tP[offe + rowoff], Po = A.tP[offo + rowoff];
+            double De = 0, Do = 0, Me = 0, Mo = 0;
+            if (VEC) {
+                De = A.tD[offe + rowoff];
+                Do = A.tD[offo + rowoff];
+                Me = A.tM[offe + rowoff];
+                Mo = A.tM[offo + rowoff];
+            }
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const int j = (wj * JB + b) * 16 + col;
+                const double rn = Gn[0][tl][j], rs = Gs[0][tl][j];
+                acc[0][0][b] = mfma(Pe, rn + rs, acc[0][0][b]);
+                acc[0][1][b] = mfma(Po, rn - rs, acc[0][1][b]);
+                if (VEC) {
+                    const double tn = Gn[1][tl][j], ts = Gs[1][tl][j];
+                    const double pn = Gn[2][tl][j], ps = Gs[2][tl][j];
+                    const double mtn = sgn_mi * Gn[1][tl][j ^ 1], mts = sgn_mi * Gs[1][tl][j ^ 1];
+                    const double mpn = sgn_mi * Gn[2][tl][j ^ 1], mps = sgn_mi * Gs[2][tl][j ^ 1];
+                    // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
+                    acc[1][0][b] = mfma(De, tn - ts, acc[1][0][b]);
+                    acc[1][0][b] = mfma(Me, mpn + mps, acc[1][0][b]);
+                    // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
+                    acc[1][1][b] = mfma(Do, tn + ts, acc[1][1][b]);
+                    acc[1][1][b] = mfma(Mo, mpn - mps, acc[1][1][b]);
+                    // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
+                    acc[2][0][b] = mfma(Me, mtn + mts, acc[2][0][b]);
+                    acc[2][0][b] = mfma(De, ps - pn, acc[2][0][b]);
+                    // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
+                    acc[2][1][b] = mfma(Mo, mtn - mts, acc[2][1][b]);
+                    acc[2][1][b] = mfma(Do, -(pn + ps), acc[2][1][b]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    const long lm0 = lm_index(A.lmax, m, m);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        double *out = (double *)A.out[f];
+#pragma unroll
+        for (int par = 0; par < 2; par++)
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const int j = (wj * JB + b) * 16 + col;
+                const int k = k0 + (j >> 1), c = j & 1;
+                if (k >= A.nplanes) continue;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int lp = lp0 + kk + 4 * v;
+                    const int l = m + par + 2 * lp;
+                    if (l > A.lmax) continue;
+                    double val = acc[f][par][b][v];
+                    if (m == 0 && c == 1) val = 0.0;
+                    out[2 * (lm0 + (l - m) + k * A.ld) + c] = val;
+                }
+            }
+    }
+}
+
+int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
+                    const double2 *g2, double2 *q, double2 *s, double2 *t, long ld)
+{
+    AnaArgs A;
+    A.tP = p->ana_P;
+    A.tD = p->ana_D;
+    A.tM = p->ana_M;
+    A.off = p->d_ana_off;
+    A.ncol = p->d_ana_ncol;
+    A.tiles = p->d_ana_tiles;
+    A.in[0] = g0;
+    A.in[1] = vec ? g1 : g0;
+    A.in[2] = vec ? g2 : g0;
+    A.out[0] = q;
+    A.out[1] = vec ? s : q;
+    A.out[2] = vec ? t : q;
+    A.ld = ld;
+    A.lmax = p->lmax;
+    A.nlat = p->nlat;
+    A.nth = p->nth;
+    A.nthp16 = p->nthp16;
+    A.nplanes = nplanes;
+    const int JT = vec ? AnaCfg<true>::JT : AnaCfg<false>::JT;
+    A.n_jt = (2 * nplanes + JT - 1) / JT;
+    dim3 grid((unsigned)p->n_ana_tiles * A.n_jt);
+    if (vec)
+        hipLaunchKernelGGL(analysis_kernel<true>, grid, dim3(256), 0, p->stream, A);
+    else
+        hipLaunchKernelGGL(analysis_kernel<false>, grid, dim3(256), 0, p->stream, A);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+}  // namespace rd

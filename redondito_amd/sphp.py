@@ -1,0 +1,292 @@
+"""Host-side mirror of REDONDITO's sphp / pseudo interface over the C-ABI.
+
+Routine names, argument order and meaning follow the reference's Fortran
+(src/sphp/sphp_sca.F90, src/sphp/sphp_vec.F90, src/sphp/fcgram_mod.f90,
+src/pseudo/pseudospec_hd.f90, src/boundary/boundary_mod.f90); the arrays are
+torch tensors resident on the GPU:
+
+  spectral a(lm, k)          -> complex128 tensor of shape (nz, nlm)
+  physical f(theta, phi, k)  -> float64 tensor of shape (nplanes, nphi, nlat)
+
+The work is queued on torch's current stream (the plan is bound to it at
+every call), so ordering with surrounding torch code is the stream's.  There
+is no CPU fallback: loading fails loudly when the HIP library is absent.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "librdsphp.so")
+TABLE_DIR = os.path.join(_HERE, "tables")
+
+RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3 = 0, 1, 2
+
+_lib = None
+
+
+class RdError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RdError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.rd_error_string.restype = ctypes.c_char_p
+        L.rd_error_string.argtypes = [i]
+        L.rd_load_dirichlet_tables.argtypes = [ctypes.c_char_p, i, i, ctypes.POINTER(d)]
+        L.rd_plan_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, i, i, d, d, ctypes.POINTER(d)]
+        L.rd_plan_destroy.argtypes = [vp]
+        L.rd_plan_set_stream.argtypes = [vp, vp]
+        L.rd_plan_sync.argtypes = [vp]
+        L.rd_plan_info.argtypes = [vp, ctypes.POINTER(i)]
+        L.rd_plan_grid.argtypes = [vp] + [ctypes.POINTER(d)] * 5 + [ctypes.POINTER(i)] * 2
+        for nm in ("rd_sht_analysis", "rd_sht_synthesis", "rd_redist_k_to_lm", "rd_redist_lm_to_k"):
+            getattr(L, nm).argtypes = [vp, i, vp, vp]
+        for nm in ("rd_vsht_analysis", "rd_vsht_synthesis"):
+            getattr(L, nm).argtypes = [vp, i] + [vp] * 6
+        for nm in ("rd_z_forward", "rd_z_inverse", "rd_dirichlet"):
+            getattr(L, nm).argtypes = [vp, vp]
+        for nm in ("rd_3d_forward", "rd_3d_inverse", "rd_laplang", "rd_dlr", "rd_copy"):
+            getattr(L, nm).argtypes = [vp, vp, vp]
+        L.rd_derivr.argtypes = [vp, i, ctypes.c_char, vp, vp]
+        L.rd_multr.argtypes = [vp, i, vp, vp]
+        L.rd_saxpby_c.argtypes = [vp, vp, vp, d, vp, d]
+        L.rd_saxpby_r.argtypes = [vp, i, vp, vp, d, vp, d]
+        L.rd_prodvec.argtypes = [vp, i] + [vp] * 9
+        L.rd_fc_filter.argtypes = [vp, ctypes.c_char, vp]
+        L.rd_nonlinear_uxw.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.rd_plan_profile.argtypes = [vp, i]
+        L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RdError(f"{what}: {lib().rd_error_string(rc).decode()} ({rc})")
+
+
+def load_dirichlet_tables(tdir, C, d):
+    """dir = A.Q^T (C x d) from A{C}-{d}.dat and Q{d}.dat
+    (load_dirichlet_tables, src/sphp/fcgram_mod.f90:186-263)."""
+    import numpy as np
+    out = np.zeros(C * d)
+    _check(lib().rd_load_dirichlet_tables(str(tdir).encode(), C, d,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+           "load_dirichlet_tables")
+    return out.reshape(d, C).T.copy()
+
+
+def grid_nlat(lmax):
+    """nlat from the 3/2 rule of grid_init (src/pseudo/pseudospec_mod.f90:66-68)."""
+    return 4 * ((3 * (lmax + 1) + 7) // 8)
+
+
+class FCPlan:
+    """The FCPLAN of the reference: SHT grid + radial FC-Gram plan.
+
+    fcgram_create_plan(this, N=(nlat,nphi,nz), C=(0,0,Cz), o=(0,0,oz),
+    dx=(1,1,dz), tdir, ...) -> FCPlan(lmax, nlat, nphi, nz, Cz, oz, dz,
+    rint, tdir).  Cz = oz = 0 makes z periodic.
+    """
+
+    def __init__(self, lmax, nlat, nphi, nz, C=0, d=0, dz=1.0, rint=0.0, tdir=TABLE_DIR, device=None):
+        import numpy as np
+        self.lmax, self.nlat, self.nphi, self.nz, self.C, self.d = lmax, nlat, nphi, nz, C, d
+        self.nlm = (lmax + 1) * (lmax + 2) // 2
+        self.nr = nz - C
+        self.dz, self.rint = dz, rint
+        self.device = torch.device(device if device is not None else "cuda")
+        if C > 0 and d > 0:
+            self.dir = load_dirichlet_tables(tdir, C, d)
+            dv = np.asfortranarray(self.dir).ravel(order="F").copy()
+            dptr = dv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        elif C == 0 and d == 0:
+            self.dir, dptr = None, None
+        else:
+            raise RdError("Mismatch in continuation or matching points in z direction")
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(lib().rd_plan_create(ctypes.byref(h), lmax, nlat, nphi, nz, C, d, dz, rint, dptr),
+                   "fcgram_create_plan")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().rd_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def h(self):
+        """Plan handle bound to torch's current stream."""
+        lib().rd_plan_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        return self._h
+
+    STAGES = ("radial", "synthesis_u", "synthesis_w", "phi_cross", "analysis", "tail")
+
+    def profile(self, enable=True):
+        """Switch per-stage event timing of nonlinear_uxw on or off (resets it)."""
+        _check(lib().rd_plan_profile(self._h, 1 if enable else 0), "profile")
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} accumulated since profile(True)."""
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_long * 8)()
+        n = lib().rd_plan_stage_times(self._h, ms, cnt, 8)
+        if n < 0:
+            _check(n, "stage_times")
+        return {self.STAGES[i]: (ms[i], cnt[i]) for i in range(n)}
+
+    def grid(self):
+        """(ct, st, wt, z, kz, ell, emm) as numpy arrays (box_init's grids)."""
+        import numpy as np
+        ct, st, wt = (np.zeros(self.nlat) for _ in range(3))
+        z, kz = np.zeros(self.nz), np.zeros(self.nz)
+        ell, emm = np.zeros(self.nlm, np.int32), np.zeros(self.nlm, np.int32)
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        Pi = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        _check(lib().rd_plan_grid(self._h, P(ct), P(st), P(wt), P(z), P(kz), Pi(ell), Pi(emm)), "grid")
+        return ct, st, wt, z, kz, ell, emm
+
+    # -- shape checks -------------------------------------------------
+    def _spec(self, a, rows=None):
+        rows = self.nz if rows is None else rows
+        if a.dtype != torch.complex128 or not a.is_cuda or not a.is_contiguous():
+            raise RdError("spectral arrays must be contiguous complex128 CUDA tensors")
+        if a.shape != (rows, self.nlm):
+            raise RdError(f"spectral shape {tuple(a.shape)} != ({rows}, {self.nlm})")
+        return ctypes.c_void_p(a.data_ptr())
+
+    def _phys(self, f, nplanes):
+        if f.dtype != torch.float64 or not f.is_cuda or not f.is_contiguous():
+            raise RdError("physical arrays must be contiguous float64 CUDA tensors")
+        if f.shape != (nplanes, self.nphi, self.nlat):
+            raise RdError(f"physical shape {tuple(f.shape)} != ({nplanes}, {self.nphi}, {self.nlat})")
+        return ctypes.c_void_p(f.data_ptr())
+
+
+# ---------------------------------------------------------------- sphp
+def fftp2d_real_to_complex_xy(plan, inp, out):
+    """in(nlat,nphi,planes) -> out(lm,planes) (src/sphp/sphp_sca.F90:273-329)."""
+    n = inp.shape[0]
+    _check(lib().rd_sht_analysis(plan.h, n, plan._phys(inp, n), plan._spec(out, n)), "fftp2d_real_to_complex_xy")
+
+
+def fftp2d_complex_to_real_xy(plan, inp, out):
+    """in(lm,planes) -> out(nlat,nphi,planes) (src/sphp/sphp_sca.F90:436-492)."""
+    n = inp.shape[0]
+    _check(lib().rd_sht_synthesis(plan.h, n, plan._spec(inp, n), plan._phys(out, n)), "fftp2d_complex_to_real_xy")
+
+
+def fftp2d_vec_to_qst(plan, vr, vt, vp, q, s, t):
+    """(v_r,v_theta,v_phi) -> (Q,S,T) (src/sphp/sphp_vec.F90:49-106)."""
+    n = vr.shape[0]
+    _check(lib().rd_vsht_analysis(plan.h, n, plan._phys(vr, n), plan._phys(vt, n), plan._phys(vp, n),
+                                  plan._spec(q, n), plan._spec(s, n), plan._spec(t, n)), "fftp2d_vec_to_qst")
+
+
+def fftp2d_qst_to_vec(plan, q, s, t, vr, vt, vp):
+    """(Q,S,T) -> (v_r,v_theta,v_phi) (src/sphp/sphp_vec.F90:109-165)."""
+    n = q.shape[0]
+    _check(lib().rd_vsht_synthesis(plan.h, n, plan._spec(q, n), plan._spec(s, n), plan._spec(t, n),
+                                   plan._phys(vr, n), plan._phys(vt, n), plan._phys(vp, n)), "fftp2d_qst_to_vec")
+
+
+def fftp1d_real_to_complex_z(plan, inout):
+    """In place, continuation + forward FFT + 1/nz (src/sphp/sphp_sca.F90:332-397)."""
+    _check(lib().rd_z_forward(plan.h, plan._spec(inout)), "fftp1d_real_to_complex_z")
+
+
+def fftp1d_complex_to_real_z(plan, inout):
+    """In place, backward FFT (src/sphp/sphp_sca.F90:495-530)."""
+    _check(lib().rd_z_inverse(plan.h, plan._spec(inout)), "fftp1d_complex_to_real_z")
+
+
+def fftp3d_real_to_complex(plan, inp, out):
+    """(src/sphp/sphp_sca.F90:237-270)."""
+    _check(lib().rd_3d_forward(plan.h, plan._phys(inp, plan.nz), plan._spec(out)), "fftp3d_real_to_complex")
+
+
+def fftp3d_complex_to_real(plan, inp, out):
+    """in is destroyed (src/sphp/sphp_sca.F90:400-433)."""
+    _check(lib().rd_3d_inverse(plan.h, plan._spec(inp), plan._phys(out, plan.nz)), "fftp3d_complex_to_real")
+
+
+def fftp3d_redist_k_to_lm(plan, inp, out):
+    """(src/sphp/sphp_sca.F90:629-693); a copy on one GPU."""
+    n = inp.shape[0]
+    _check(lib().rd_redist_k_to_lm(plan.h, n, plan._spec(inp, n), plan._spec(out, n)), "fftp3d_redist_k_to_lm")
+
+
+def fftp3d_redist_lm_to_k(plan, inp, out):
+    """(src/sphp/sphp_sca.F90:696-760); a copy on one GPU."""
+    n = inp.shape[0]
+    _check(lib().rd_redist_lm_to_k(plan.h, n, plan._spec(inp, n), plan._spec(out, n)), "fftp3d_redist_lm_to_k")
+
+
+# -------------------------------------------------------------- pseudo
+def derivr(plan, a, b, p, esp):
+    """b = d^p a / dr^p in (lm,r) ('r') or (lm,kz) ('k') (pseudospec_hd.f90:18-135)."""
+    if esp not in ("r", "k"):
+        raise RdError(f"Unknown space {esp} in derivr")
+    _check(lib().rd_derivr(plan.h, p, esp.encode(), plan._spec(a), plan._spec(b)), "derivr")
+
+
+def laplang(plan, a, b):
+    _check(lib().rd_laplang(plan.h, plan._spec(a), plan._spec(b)), "laplang")
+
+
+def multr(plan, a, b, p):
+    _check(lib().rd_multr(plan.h, p, plan._spec(a), plan._spec(b)), "multr")
+
+
+def dlr(plan, a, b):
+    _check(lib().rd_dlr(plan.h, plan._spec(a), plan._spec(b)), "dlr")
+
+
+def saxpby_c(plan, z, x, a, y, b):
+    _check(lib().rd_saxpby_c(plan.h, plan._spec(z), plan._spec(x), a, plan._spec(y), b), "saxpby_c")
+
+
+def saxpby_r(plan, z, x, a, y, b):
+    n = z.shape[0]
+    _check(lib().rd_saxpby_r(plan.h, n, plan._phys(z, n), plan._phys(x, n), a, plan._phys(y, n), b), "saxpby_r")
+
+
+def prodvec(plan, ar, at, ap, br, bt, bp, cr, ct, cp):
+    n = ar.shape[0]
+    args = [plan._phys(x, n) for x in (ar, at, ap, br, bt, bp, cr, ct, cp)]
+    _check(lib().rd_prodvec(plan.h, n, *args), "prodvec")
+
+
+def fc_filter(plan, a, esp):
+    if esp not in ("r", "k"):
+        raise RdError(f"Unknown space {esp} in fc_filter")
+    _check(lib().rd_fc_filter(plan.h, esp.encode(), plan._spec(a)), "fc_filter")
+
+
+def copy(plan, a, b):
+    _check(lib().rd_copy(plan.h, plan._spec(a), plan._spec(b)), "copy")
+
+
+def dirichlet(plan, a):
+    _check(lib().rd_dirichlet(plan.h, plan._spec(a)), "dirichlet")
+
+
+def nonlinear_uxw(plan, W, Z, NQ, NS, NT, prec=RD_PREC_FP64):
+    """N = u x omega, u = curl curl (W r) + curl (Z r), as (Q,S,T) in (lm, r)."""
+    _check(lib().rd_nonlinear_uxw(plan.h, prec, plan._spec(W), plan._spec(Z), plan._spec(NQ),
+                                  plan._spec(NS), plan._spec(NT)), "nonlinear_uxw")

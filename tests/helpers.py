@@ -1,0 +1,152 @@
+"""Shared test inputs: the reference's known-answer fields and seeded spectra."""
+import numpy as np
+
+PI = np.pi
+
+
+def lm_index(lmax, l, m):
+    return m * (2 * lmax + 1 - m) // 2 + l
+
+
+def nlm_of(lmax):
+    return (lmax + 1) * (lmax + 2) // 2
+
+
+def ell_of(lmax):
+    ell = np.empty(nlm_of(lmax), np.int32)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            ell[lm_index(lmax, l, m)] = l
+    return ell
+
+
+def emm_of(lmax):
+    emm = np.empty(nlm_of(lmax), np.int32)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            emm[lm_index(lmax, l, m)] = m
+    return emm
+
+
+def rell2(x, y):
+    x, y = np.asarray(x), np.asarray(y)
+    return float(np.linalg.norm((x - y).ravel()) / max(np.linalg.norm(y.ravel()), 1e-300))
+
+
+def sectoral_c(l):
+    """c = sqrt((2l+1)/(4 pi)) (2l-1)!!/sqrt((2l)!) (src/prueba_sphp.f90:165-174)."""
+    c = np.sqrt((2 * l + 1) / (4 * PI))
+    for k in range(1, l + 1):
+        c *= np.sqrt((2 * k - 1.0) / (2 * k))
+    return c
+
+
+def legendre_p(l, x):
+    p0, p1 = np.ones_like(x), x
+    if l == 0:
+        return p0
+    for k in range(1, l):
+        p0, p1 = p1, ((2 * k + 1) * x * p1 - k * p0) / (k + 1)
+    return p1
+
+
+def scalar_case(ic, lmax, cth, sth, ph):
+    """Field f(phi, theta) and exact coefficients a of prueba_sphp case ic
+    (src/prueba_sphp.f90:121-184); arrays are (nphi, nlat)."""
+    a = np.zeros(nlm_of(lmax), complex)
+    L = lambda l, m: lm_index(lmax, l, m)
+    if ic == 1:
+        f = np.ones_like(cth); a[L(0, 0)] = np.sqrt(4 * PI)
+    elif ic == 2:
+        f = cth.copy(); a[L(1, 0)] = np.sqrt(4 * PI / 3)
+    elif ic == 3:
+        f = sth * np.cos(ph); a[L(1, 1)] = np.sqrt(2 * PI / 3)
+    elif ic == 4:
+        f = sth * np.sin(ph); a[L(1, 1)] = -1j * np.sqrt(2 * PI / 3)
+    elif ic == 5:
+        f = 3 * cth ** 2 - 1; a[L(2, 0)] = 4 * np.sqrt(PI / 5)
+    elif ic == 6:
+        f = sth ** 2 * np.cos(2 * ph); a[L(2, 2)] = 2 * np.sqrt(2 * PI / 15)
+    elif ic == 7:
+        f = legendre_p(lmax, cth); a[L(lmax, 0)] = np.sqrt(4 * PI / (2 * lmax + 1))
+    elif ic == 8:
+        f = sth ** lmax * np.cos(lmax * ph); a[L(lmax, lmax)] = 1 / (2 * sectoral_c(lmax))
+    elif ic == 9:
+        f = 2 - 3 * cth + sth * np.sin(ph) + 0.5 * sth ** 2 * np.cos(2 * ph)
+        a[L(0, 0)] = 2 * np.sqrt(4 * PI); a[L(1, 0)] = -3 * np.sqrt(4 * PI / 3)
+        a[L(1, 1)] = -1j * np.sqrt(2 * PI / 3); a[L(2, 2)] = np.sqrt(2 * PI / 15)
+    else:
+        raise ValueError(ic)
+    return f, a
+
+
+def vector_case(ic, lmax, cth, sth, ph):
+    """(fr, ft, fp) and exact (aq, as, at) of prueba_vec case ic
+    (src/prueba_vec.f90:139-209)."""
+    n = nlm_of(lmax)
+    aq, as_, at = np.zeros(n, complex), np.zeros(n, complex), np.zeros(n, complex)
+    z = np.zeros_like(cth)
+    fr, ft, fp = z.copy(), z.copy(), z.copy()
+    L = lambda l, m: lm_index(lmax, l, m)
+    if ic == 1:
+        fr = cth.copy(); aq[L(1, 0)] = np.sqrt(4 * PI / 3)
+    elif ic == 2:
+        ft = -sth; as_[L(1, 0)] = np.sqrt(4 * PI / 3)
+    elif ic == 3:
+        fp = sth.copy(); at[L(1, 0)] = np.sqrt(4 * PI / 3)
+    elif ic == 4:
+        ft = cth * np.cos(ph); fp = -np.sin(ph); as_[L(1, 1)] = np.sqrt(2 * PI / 3)
+    elif ic == 5:
+        ft = -np.sin(ph); fp = -cth * np.cos(ph); at[L(1, 1)] = np.sqrt(2 * PI / 3)
+    elif ic == 6:
+        fr = legendre_p(lmax, cth); aq[L(lmax, 0)] = np.sqrt(4 * PI / (2 * lmax + 1))
+    elif ic == 7:
+        ft = -lmax * sth ** (lmax - 1) * np.sin(lmax * ph)
+        fp = -lmax * sth ** (lmax - 1) * cth * np.cos(lmax * ph)
+        at[L(lmax, lmax)] = 1 / (2 * sectoral_c(lmax))
+    elif ic == 8:
+        fr = cth.copy(); ft = cth * np.cos(ph) - np.sin(ph); fp = -np.sin(ph) - cth * np.cos(ph)
+        aq[L(1, 0)] = np.sqrt(4 * PI / 3); as_[L(1, 1)] = np.sqrt(2 * PI / 3); at[L(1, 1)] = np.sqrt(2 * PI / 3)
+    else:
+        raise ValueError(ic)
+    return (fr, ft, fp), (aq, as_, at)
+
+
+def grid_arrays(ct, st, nphi):
+    """(cth, sth, ph) of shape (nphi, nlat) for the theta-contiguous planes."""
+    ph = 2 * PI * np.arange(nphi) / nphi
+    cth = np.broadcast_to(ct[None, :], (nphi, ct.size)).copy()
+    sth = np.broadcast_to(st[None, :], (nphi, st.size)).copy()
+    phg = np.broadcast_to(ph[:, None], (nphi, ct.size)).copy()
+    return cth, sth, phg
+
+
+def random_spectrum(lmax, nplanes, seed, slope=0.0, zero_l0=False):
+    """Seeded complex normal coefficients, scaled by (l+1)^-slope, m = 0 real."""
+    rng = np.random.default_rng(seed)
+    n = nlm_of(lmax)
+    a = rng.standard_normal((nplanes, n)) + 1j * rng.standard_normal((nplanes, n))
+    a *= (ell_of(lmax) + 1.0) ** (-slope)
+    a[:, : lmax + 1] = a[:, : lmax + 1].real
+    if zero_l0:
+        a[:, 0] = 0
+    return a
+
+
+def shell_fields(lmax, nr, n, rint, rext, seed, slope=2.0):
+    """W, Z (n, nlm) in (lm, r): c_lm (l+1)^-slope sin(pi (r - ri)/(ro - ri)) on the
+    nr physical shells, l = 0 zero, m = 0 real; continuation rows zero."""
+    rng = np.random.default_rng(seed)
+    nl = nlm_of(lmax)
+    r = rint + (rext - rint) * np.arange(nr) / (nr - 1)
+    prof = np.sin(PI * (r - rint) / (rext - rint))
+    out = []
+    for f in range(2):
+        c = rng.standard_normal(nl) + 1j * rng.standard_normal(nl)
+        c *= (ell_of(lmax) + 1.0) ** (-slope)
+        c[: lmax + 1] = c[: lmax + 1].real
+        c[0] = 0
+        a = np.zeros((n, nl), complex)
+        a[:nr] = prof[:, None] * c[None, :]
+        out.append(a)
+    return out

@@ -1,0 +1,463 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Every oracle function is checked against something it does not restate:
+the reference's own known-answer tests (src/prueba_*.f90, levels from
+LEEME.md), library routines (numpy leggauss, scipy sph_harm_y, numpy.fft),
+closed forms (solid-body rotation), and brute force on tiny grids.
+"""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from oracle import oracle as O
+from tests import helpers as H
+
+TABLES = None
+
+
+def tables():
+    import os
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "redondito_amd", "tables")
+
+
+def fcdir():
+    A, Q = O.load_tables(tables(), 25, 5)
+    return O.fc_dir(A, Q)
+
+
+# ------------------------------------------------------------------ grid
+@pytest.mark.parametrize("nlat", [12, 24, 48, 192, 384])
+def test_gauss_nodes_vs_numpy(nlat):
+    S = O.SHT(7, nlat, 2 * nlat) if nlat > 7 else None
+    ct, st, wt = S.grid()
+    x, w = np.polynomial.legendre.leggauss(nlat)
+    assert np.all(np.diff(ct) < 0)            # north to south
+    assert np.max(np.abs(ct[::-1] - x)) < 2e-15
+    # numpy's polar weights carry up to ~1e-10 relative error (absolute ~1e-14);
+    # 40-digit mpmath pins ours in relative terms
+    assert np.max(np.abs(wt[::-1] - w)) < 5e-14
+    import mpmath as mp
+    mp.mp.dps = 40
+    for i in ((0, nlat // 3) if nlat <= 48 else ()):
+        x0 = mp.findroot(lambda t: mp.legendre(nlat, t), mp.mpf(ct[i]))
+        dp = mp.diff(lambda t: mp.legendre(nlat, t), x0)
+        w0 = 2 / ((1 - x0 ** 2) * dp ** 2)
+        assert abs(float((ct[i] - x0) / x0)) < 4e-16
+        assert abs(float((wt[i] - w0) / w0)) < 5e-14
+    assert np.max(np.abs(st - np.sqrt(1 - ct ** 2)) / st) < 1e-10
+    # exactness for polynomials of degree <= 2 nlat - 1
+    for deg in (0, 2, nlat, 2 * nlat - 2):
+        exact = 2.0 / (deg + 1) if deg % 2 == 0 else 0.0
+        assert abs(np.sum(wt * ct ** deg) - exact) < 1e-13
+
+
+def test_legendre_vs_scipy():
+    """P~_lm = orthonormal Y_lm(theta, 0) without the Condon-Shortley phase."""
+    lmax, nlat = 40, 48
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    ct, st, wt = S.grid()
+    P, dP, mPs = S.tables()
+    th = np.arccos(ct)
+    err, derr = 0.0, 0.0
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            y = sp.sph_harm_y(l, m, th, 0.0).real * (-1) ** m
+            i = H.lm_index(lmax, l, m)
+            err = max(err, np.max(np.abs(y - P[i])))
+            # d/dtheta by a centred difference of scipy's function
+            h = 1e-6
+            yd = (sp.sph_harm_y(l, m, th + h, 0.0).real - sp.sph_harm_y(l, m, th - h, 0.0).real) * (-1) ** m / (2 * h)
+            derr = max(derr, np.max(np.abs(yd - dP[i])) / max(1.0, np.max(np.abs(yd))))
+            assert np.allclose(mPs[i], m * P[i] / st, atol=1e-13)
+    assert err < 1e-13
+    assert derr < 1e-7
+
+
+def test_legendre_discrete_orthonormality():
+    lmax, nlat = 63, 96
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    wt = S.grid()[2]
+    P = S.tables()[0]
+    for m in (0, 1, 17, 63):
+        rows = [H.lm_index(lmax, l, m) for l in range(m, lmax + 1)]
+        G = (P[rows] * wt[None]) @ P[rows].T * 2 * np.pi
+        assert np.max(np.abs(G - np.eye(len(rows)))) < 1e-13
+
+
+def test_legendre_no_underflow_garbage_lmax511():
+    """High m near the poles: values are finite and the sectoral closed form holds."""
+    lmax, nlat = 511, 768
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    ct, st, wt = S.grid()
+    P, dP, mPs = S.tables()
+    assert np.all(np.isfinite(P)) and np.all(np.isfinite(dP)) and np.all(np.isfinite(mPs))
+    i = nlat // 2 - 1                        # near the equator
+    l = 300
+    ref = H.sectoral_c(l) * st[i] ** l
+    assert abs(P[H.lm_index(lmax, l, l), i] - ref) < 1e-13 * abs(ref)
+
+
+# ---------------------------------------------------------- scalar SHT
+def test_prueba_sphp_cases():
+    """src/prueba_sphp.f90:121-184; LEEME.md:34 reports 1e-30..1e-13."""
+    lmax, nz = 15, 6
+    nlat, nphi = 24, 48
+    S = O.SHT(lmax, nlat, nphi)
+    ct, st, wt = S.grid()
+    cth, sth, ph = H.grid_arrays(ct, st, nphi)
+    k = np.arange(1, nz + 1)[:, None, None]
+    for ic in range(1, 10):
+        f, a = H.scalar_case(ic, lmax, cth, sth, ph)
+        b = S.anal(k * f[None])
+        g = S.synth(k[:, :, 0] * a[None])
+        assert np.max(np.abs(b - k[:, :, 0] * a[None])) < 1e-13, ic
+        assert np.max(np.abs(g - k * f[None])) < 1e-12, ic
+
+
+def test_random_round_trip():
+    lmax = 63
+    S = O.SHT(lmax, 96, 192)
+    a = H.random_spectrum(lmax, 3, seed=7)
+    assert H.rell2(S.anal(S.synth(a)), a) < 1e-13
+
+
+def test_synthesis_brute_force_vs_scipy():
+    """Direct sum of scipy's harmonics on the grid (tiny lmax)."""
+    lmax, nlat, nphi = 6, 10, 20
+    S = O.SHT(lmax, nlat, nphi)
+    ct, st, wt = S.grid()
+    a = H.random_spectrum(lmax, 1, seed=3)[0]
+    th = np.arccos(ct)
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    TH, PH = np.meshgrid(th, ph)
+    f = np.zeros_like(TH)
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            y = sp.sph_harm_y(l, m, TH, PH) * (-1) ** m
+            c = a[H.lm_index(lmax, l, m)]
+            f += (c * y).real if m == 0 else 2 * (c * y).real
+    assert np.max(np.abs(S.synth(a)[0] - f)) < 1e-13
+
+
+# ---------------------------------------------------------- vector SHT
+def test_prueba_vec_cases():
+    """src/prueba_vec.f90:139-209; LEEME.md:42 reports <= 6e-14 / 8e-13."""
+    lmax, nz = 15, 6
+    nlat, nphi = 24, 48
+    S = O.SHT(lmax, nlat, nphi)
+    ct, st, wt = S.grid()
+    cth, sth, ph = H.grid_arrays(ct, st, nphi)
+    k = np.arange(1, nz + 1)[:, None, None]
+    for ic in range(1, 9):
+        fs, cs = H.vector_case(ic, lmax, cth, sth, ph)
+        out = S.vanal(*[k * f[None] for f in fs])
+        ea = max(np.max(np.abs(o - k[:, :, 0] * c[None])) for o, c in zip(out, cs))
+        vs = S.vsynth(*[k[:, :, 0] * c[None] for c in cs])
+        es = max(np.max(np.abs(v - k * f[None])) for v, f in zip(vs, fs))
+        assert ea < 1e-13 and es < 2e-12, (ic, ea, es)
+
+
+def test_vector_brute_force():
+    """v_theta, v_phi from finite differences of the scalar syntheses of S and T."""
+    lmax, nlat, nphi = 8, 12, 24
+    Sh = O.SHT(lmax, nlat, nphi)
+    ct, st, wt = Sh.grid()
+    s = H.random_spectrum(lmax, 1, seed=1, zero_l0=True)[0]
+    t = H.random_spectrum(lmax, 1, seed=2, zero_l0=True)[0]
+    vr, vt, vp = Sh.vsynth(np.zeros_like(s), s, t)
+    th = np.arccos(ct)
+
+    def field(c, TH, PH):
+        f = np.zeros_like(TH)
+        for l in range(lmax + 1):
+            for m in range(l + 1):
+                y = sp.sph_harm_y(l, m, TH, PH) * (-1) ** m
+                v = c[H.lm_index(lmax, l, m)]
+                f += (v * y).real if m == 0 else 2 * (v * y).real
+        return f
+
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    TH, PH = np.meshgrid(th, ph)
+    h = 1e-5
+    dth = lambda c: (field(c, TH + h, PH) - field(c, TH - h, PH)) / (2 * h)
+    dph = lambda c: (field(c, TH, PH + h) - field(c, TH, PH - h)) / (2 * h)
+    sth = np.sin(TH)
+    et = np.max(np.abs(vt[0] - (dth(s) + dph(t) / sth)))
+    ep = np.max(np.abs(vp[0] - (dph(s) / sth - dth(t))))
+    assert et < 1e-7 and ep < 1e-7
+
+
+# ----------------------------------------------------------- radial / FFT
+@pytest.mark.parametrize("n", [16, 49, 57, 64, 89, 121, 153, 281, 768])
+def test_fft_vs_numpy(n):
+    fc = O.FC(n)
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, 5)) + 1j * rng.standard_normal((n, 5))
+    f = fc.forward(a.copy())
+    assert H.rell2(f, np.fft.fft(a, axis=0) / n) < 1e-14
+    b = fc.inverse(f.copy())
+    assert H.rell2(b, a) < 1e-14
+
+
+def test_prueba_z_periodic():
+    """src/prueba_z.f90:45-75; LEEME.md:34 reports 3e-16."""
+    nz, q, p, nlm = 16, 3, 5, 36
+    lm = np.arange(1, nlm + 1)
+    cl = lm / nlm - 0.5j
+    dl = 0.3 + 1j * (1.0 - lm / nlm)
+    j = np.arange(nz)[:, None]
+    f0 = cl[None] * np.cos(2 * np.pi * q * j / nz) + dl[None] * np.sin(2 * np.pi * p * j / nz)
+    e = np.zeros((nz, nlm), complex)
+    e[q] = cl / 2; e[nz - q] = cl / 2; e[p] = -1j * dl / 2; e[nz - p] = 1j * dl / 2
+    fc = O.FC(nz)
+    f = fc.forward(f0.copy())
+    assert np.max(np.abs(f - e)) < 1e-15
+    assert np.max(np.abs(fc.inverse(f) - f0)) < 1e-14
+
+
+def test_fcgram_tables_format_and_gram_basis():
+    """Tables in the reference's stream layout; Q orthogonal and spanning the
+    polynomials of degree < d on the matching points."""
+    A, Q = O.load_tables(tables(), 25, 5)
+    assert A.shape == (25, 5) and Q.shape == (5, 5)
+    assert np.max(np.abs(Q.T @ Q - np.eye(5))) < 1e-14
+    x = np.arange(5.0)
+    V = np.vander(x, 5, increasing=True)
+    assert np.max(np.abs(Q @ (Q.T @ V) - V)) < 1e-11
+    dr = O.fc_dir(A, Q)
+    assert np.allclose(dr, A @ Q.T, atol=1e-15)
+
+
+@pytest.mark.parametrize("nz,published", [(64, 2.5e-6), (128, 4.9e-8), (256, 2.2e-9)])
+def test_prueba_z_fcgram_convergence(nz, published):
+    """src/prueba_z.f90:79-125, LEEME.md:34: the FC-Gram tables built here
+    reproduce the published error levels of the reference's tables."""
+    C, d = 25, 5
+    dz = 1.0 / (nz - C - 1)
+    fc = O.FC(nz, C, d, fcdir())
+    zz = np.arange(nz - C) * dz
+    f = np.zeros((nz, 3), complex)
+    cl = np.array([0.3 - 0.5j, 1.0, -0.2j])
+    f[: nz - C] = cl[None] * (np.sin(2.3 * zz + 0.4) + zz ** 2)[:, None]
+    e = cl[None] * (2.3 * np.cos(2.3 * zz + 0.4) + 2 * zz)[:, None]
+    fc.forward(f)
+    kn = np.arange(nz, dtype=float)
+    kn[kn > nz / 2] -= nz
+    kn[nz // 2] = 0
+    f *= (1j * 2 * np.pi * kn / (nz * dz))[:, None]
+    fc.inverse(f)
+    err = np.max(np.abs(f[: nz - C] - e)) / np.max(np.abs(e))
+    assert 0.5 * published < err < 1.5 * published
+
+
+def test_fcgram_no_continuation_control():
+    """LEEME.md:34: without the continuation the error is O(10)-O(100)."""
+    nz, C = 128, 25
+    dz = 1.0 / (nz - C - 1)
+    fc = O.FC(nz - C)
+    zz = np.arange(nz - C) * dz
+    f = (np.sin(2.3 * zz + 0.4) + zz ** 2)[:, None].astype(complex)
+    e = (2.3 * np.cos(2.3 * zz + 0.4) + 2 * zz)[:, None]
+    fc.forward(f)
+    n = nz - C
+    kn = np.fft.fftfreq(n, 1.0 / n)
+    f *= (1j * 2 * np.pi * kn / (n * dz))[:, None]
+    fc.inverse(f)
+    assert np.max(np.abs(f - e)) / np.max(np.abs(e)) > 1.0
+
+
+def test_kz_conventions():
+    """pseudospec_mod.f90:223-227: even n keeps Nyquist at -n/2; odd n uses the
+    symmetric ordering (the reference loop is undefined there)."""
+    k8 = O.kz(8, 1.0) / (2 * np.pi / 8)
+    assert np.allclose(k8, [0, 1, 2, 3, -4, -3, -2, -1])
+    k7 = O.kz(7, 1.0) / (2 * np.pi / 7)
+    assert np.allclose(k7, [0, 1, 2, 3, -3, -2, -1])
+
+
+# ------------------------------------------------------------ operators
+def _pseudo(lmax=7, nz=128):
+    C, d, rint, rext = 25, 5, 0.5, 1.5
+    dz = (rext - rint) / (nz - C - 1)
+    fc = O.FC(nz, C, d, fcdir())
+    z = rint + dz * np.arange(nz)
+    return fc, O.kz(nz, dz), z, H.ell_of(lmax), dz
+
+
+def test_prueba_pseudo_levels():
+    """src/prueba_pseudo.f90:42-115; LEEME.md:45: d1 2.3e-7, d2 3.6e-5, D_l 2.8e-6,
+    laplang / multr / saxpby / copy at roundoff, filter change 1.7e-11."""
+    fc, kz, z, ell, dz = _pseudo()
+    nz, C = 128, 25
+    nlm = ell.size
+    n = nz - C
+    lm = np.arange(1, nlm + 1)
+    cl = (1.0 + 0.1 * lm) + 1j * (0.5 - 0.05 * lm)
+    g0 = np.exp(z) * np.sin(3 * z)
+    g1 = np.exp(z) * (np.sin(3 * z) + 3 * np.cos(3 * z))
+    g2 = np.exp(z) * (-8 * np.sin(3 * z) + 6 * np.cos(3 * z))
+    a = cl[None] * g0[:, None]
+    rel = lambda x, y: np.max(np.abs(x[:n] - y[:n])) / np.max(np.abs(y[:n]))
+    d1 = fc.derivr_r(a, kz, 1)
+    d2 = fc.derivr_r(a, kz, 2)
+    e1 = rel(d1, cl[None] * g1[:, None])
+    e2 = rel(d2, cl[None] * g2[:, None])
+    c = fc.forward(a.copy())
+    e = O.derivr_k(c, kz, 1)
+    e = fc.inverse(e)
+    e3 = rel(e, d1)
+    lap = O.laplang(a, ell)
+    e4 = rel(lap, -(ell * (ell + 1.0))[None] * a)
+    mu = O.multr(a, z, -2)
+    e5 = rel(mu, a / (z ** 2)[:, None])
+    dl = fc.dlr(a, kz, z, ell)
+    e6 = rel(dl, cl[None] * g2[:, None] - (ell * (ell + 1.0))[None] * a / (z ** 2)[:, None])
+    sx = O.saxpby_c(a, 2.0, dl, -3.0)
+    e7 = rel(sx, 2 * a - 3 * dl)
+    filt = O.fc_filter_k(fc.forward(a.copy()), kz, 2 * np.pi / (dz * nz))
+    e8 = rel(fc.inverse(filt), a)
+    assert 0.3 * 2.3e-7 < e1 < 3 * 2.3e-7
+    assert 0.3 * 3.6e-5 < e2 < 3 * 3.6e-5
+    assert e3 < 1e-13 and e4 < 1e-15 and e5 < 1e-15 and e7 < 1e-15
+    assert 0.3 * 2.8e-6 < e6 < 3 * 2.8e-6
+    assert e8 < 1e-9
+
+
+def test_prodvec_hand_cross():
+    rng = np.random.default_rng(0)
+    a = [rng.standard_normal(50) for _ in range(3)]
+    b = [rng.standard_normal(50) for _ in range(3)]
+    c = O.prodvec(a, b)
+    ref = np.cross(np.stack(a, 1), np.stack(b, 1))
+    assert np.max(np.abs(np.stack(c, 1) - ref)) < 1e-15
+
+
+# -------------------------------------------------------- nonlinear term
+def test_uxw_solid_body_rotation():
+    """Z_10 = Omega r^2 sqrt(4 pi/3): u = Omega z x r, omega = 2 Omega z,
+    u x omega = grad(Omega^2 r^2 sin^2 theta)."""
+    lmax, nr, C, d, ri, ro = 15, 64, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat = O.grid_nlat(lmax)
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    fc = O.FC(n, C, d, fcdir())
+    r = ri + dz * np.arange(nr)
+    Om = 0.7
+    W = np.zeros((n, S.nlm), complex)
+    Z = np.zeros_like(W)
+    Z[:nr, H.lm_index(lmax, 1, 0)] = Om * r ** 2 * np.sqrt(4 * np.pi / 3)
+    NQ, NS, NT = O.nonlinear_uxw(S, fc, ri, dz, W, Z)
+    eQ = np.zeros((nr, S.nlm), complex)
+    eS = np.zeros_like(eQ)
+    eQ[:, H.lm_index(lmax, 0, 0)] = 2 * Om ** 2 * r * (2 / 3) * np.sqrt(4 * np.pi)
+    eQ[:, H.lm_index(lmax, 2, 0)] = -2 * Om ** 2 * r * (2 / 3) * np.sqrt(4 * np.pi / 5)
+    eS[:, H.lm_index(lmax, 2, 0)] = -Om ** 2 * r * (2 / 3) * np.sqrt(4 * np.pi / 5)
+    assert H.rell2(NQ[:nr], eQ) < 1e-9
+    assert H.rell2(NS[:nr], eS) < 1e-12
+    assert np.max(np.abs(NT[:nr])) < 1e-12
+    assert np.all(NQ[nr:] == 0)
+
+
+def test_uxw_brute_force_small():
+    """Independent numpy restatement on a tiny shell: u and omega evaluated on
+    the grid from scipy harmonics and the oracle's radial derivatives, the cross
+    product, and the analysis by direct Gauss-Legendre / trapezoid quadrature."""
+    lmax, nr, C, d, ri, ro = 5, 24, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat, nphi = 8, 16
+    S = O.SHT(lmax, nlat, nphi)
+    fc = O.FC(n, C, d, fcdir())
+    W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=5, slope=1.0)
+    NQ, NS, NT = O.nonlinear_uxw(S, fc, ri, dz, W, Z)
+    kz = O.kz(n, dz)
+    z = ri + dz * np.arange(n)
+    ell = H.ell_of(lmax)
+    dW = fc.derivr_r(W, kz, 1)
+    d2W = fc.derivr_r(W, kz, 2)
+    dZ = fc.derivr_r(Z, kz, 1)
+    x, wg = np.polynomial.legendre.leggauss(nlat)
+    th = np.arccos(x[::-1])
+    wg = wg[::-1]
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    TH, PH = np.meshgrid(th, ph)
+    k = 7                                   # one shell
+    r = z[k]
+    ll = ell * (ell + 1.0)
+    uq, us, ut = ll * W[k] / r ** 2, dW[k] / r, Z[k] / r
+    wq, ws, wt = ll * Z[k] / r ** 2, dZ[k] / r, -(d2W[k] - ll * W[k] / r ** 2) / r
+    Y, dY, mY = {}, {}, {}
+    h = 1e-6
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            Y[l, m] = sp.sph_harm_y(l, m, TH, PH) * (-1) ** m
+            dY[l, m] = (sp.sph_harm_y(l, m, TH + h, PH) - sp.sph_harm_y(l, m, TH - h, PH)) * (-1) ** m / (2 * h)
+            mY[l, m] = 1j * m * Y[l, m] / np.sin(TH)
+
+    def re_sum(terms):
+        out = np.zeros_like(TH)
+        for (l, m), v in terms:
+            out += v.real if m == 0 else 2 * v.real
+        return out
+
+    def vec(q, s, t):
+        keys = [(l, m) for l in range(lmax + 1) for m in range(l + 1)]
+        i = lambda l, m: H.lm_index(lmax, l, m)
+        vr = re_sum([((l, m), q[i(l, m)] * Y[l, m]) for l, m in keys])
+        vt = re_sum([((l, m), s[i(l, m)] * dY[l, m] + t[i(l, m)] * mY[l, m]) for l, m in keys])
+        vp = re_sum([((l, m), s[i(l, m)] * mY[l, m] - t[i(l, m)] * dY[l, m]) for l, m in keys])
+        return vr, vt, vp
+
+    u = vec(uq, us, ut)
+    w = vec(wq, ws, wt)
+    N = np.cross(np.stack(u, -1), np.stack(w, -1))
+    Nr, Nt, Np = N[..., 0], N[..., 1], N[..., 2]
+    dphi = 2 * np.pi / nphi
+    for l in range(1, lmax + 1):
+        for m in range(l + 1):
+            i = H.lm_index(lmax, l, m)
+            yc = np.conj(Y[l, m])
+            q = np.sum(wg[None] * Nr * yc) * dphi
+            s = np.sum(wg[None] * (Nt * np.conj(dY[l, m]) + Np * np.conj(mY[l, m]))) * dphi / (l * (l + 1))
+            t = np.sum(wg[None] * (Nt * np.conj(mY[l, m]) - Np * np.conj(dY[l, m]))) * dphi / (l * (l + 1))
+            assert abs(q - NQ[k, i]) < 1e-7 * (1 + abs(q)), (l, m)
+            assert abs(s - NS[k, i]) < 1e-7 * (1 + abs(s)), (l, m)
+            assert abs(t - NT[k, i]) < 1e-7 * (1 + abs(t)), (l, m)
+
+
+# -------------------------------------------------------------- boundary
+def test_derivr_mat_matches_derivr():
+    fc, kz, z, ell, dz = _pseudo(lmax=3, nz=60)
+    N = 60 - 25
+    D1 = fc.derivr_mat(kz, 1)
+    rng = np.random.default_rng(2)
+    a = np.zeros((60, 4), complex)
+    a[:N] = rng.standard_normal((N, 4))
+    d = fc.derivr_r(a, kz, 1)
+    assert np.max(np.abs(D1 @ a[:N].real - d[:N].real)) < 1e-10
+
+
+def test_invdl_recovers_admissible_w():
+    """src/prueba_boundary.f90:58-112: W0 = c (r - r1)^2 (r - r2)^2 e^r satisfies the
+    walls; phi = D_l W0 gives back W0 to truncation (LEEME.md:45: 6e-8)."""
+    lmax, nz, C = 7, 128, 25
+    fc, kz, z, ell, dz = _pseudo(lmax, nz)
+    N = nz - C
+    D1 = fc.derivr_mat(kz, 1)
+    D2 = fc.derivr_mat(kz, 2)
+    binv = O.invdl_setup(N, lmax, D1, D2, z[:N])
+    r1, r2 = z[0], z[N - 1]
+    w0 = (z - r1) ** 2 * (z - r2) ** 2 * np.exp(z)
+    nlm = ell.size
+    lm = np.arange(1, nlm + 1)
+    c = (1.0 + 0.1 * lm) + 1j * (0.5 - 0.05 * lm)
+    c[ell == 0] = 0
+    W0 = np.zeros((nz, nlm), complex)
+    W0[:N] = c[None] * w0[:N, None]
+    phi = fc.dlr(W0, kz, z, ell)
+    W = O.invdl(N, nz, lmax, binv, phi)
+    err = np.max(np.abs(W[:N] - W0[:N])) / np.max(np.abs(W0[:N]))
+    assert err < 1e-6
+    assert np.max(np.abs(W[0])) == 0 or np.max(np.abs(W[0])) < 1e-13
+    assert np.max(np.abs((D1 @ W[:N])[0])) < 1e-10 * np.max(np.abs(W))
