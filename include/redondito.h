@@ -17,6 +17,8 @@
 #ifndef REDONDITO_H
 #define REDONDITO_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -182,6 +184,30 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
 
 /* a(lm,1) = a(lm,nz-C) = 0.  Replaces dirichlet, boundary_mod.f90:62-93. */
 int rd_dirichlet(rd_plan *plan, double *a);
+
+/* ---- device selection and the host -> device registry ---------------
+ * Replaces src/memmgt/gdevice_mod.f90 for the Fortran drop-in layer: a host
+ * array gets a device copy registered by its host address; an interior
+ * address (one plane of the array) resolves to the same offset. */
+
+/* device = rank mod number of devices.  device_init, gdevice_mod.f90:112-141. */
+int rd_device_init(int rank);
+/* device_sync, gdevice_mod.f90:144-155. */
+int rd_device_sync(void);
+/* gdev_alloc, gdevice_mod.f90:158-193. */
+int rd_dev_alloc(const void *host, size_t nbytes);
+/* gdev_free, gdevice_mod.f90:196-217. */
+int rd_dev_free(const void *host);
+/* Device address of a registered host address, NULL if none
+ * (gdev_lookup, gdevice_mod.f90:220-235). */
+void *rd_dev_lookup(const void *host);
+/* gdev_present, gdevice_mod.f90:305-317. */
+int rd_dev_present(const void *host);
+/* gdev_update_to / gdev_update_from, gdevice_mod.f90:238-279. */
+int rd_dev_update_to(const void *host, size_t nbytes);
+int rd_dev_update_from(void *host, size_t nbytes);
+/* gdev_copy, gdevice_mod.f90:282-302. */
+int rd_dev_copy(const void *hdst, const void *hsrc, size_t nbytes);
 
 #ifdef __cplusplus
 }

@@ -103,38 +103,55 @@ static void fft_exec(const orc_fft *p, int sign, const cplx *in, long is, cplx *
 /* Grid and associated-Legendre tables                                  */
 /* ------------------------------------------------------------------ */
 
-/* Gauss-Legendre nodes from north to south, Newton on theta so that
- * sin(theta) keeps full relative accuracy near the poles. */
+/* Coefficients of P_n(cos t) = sum_{k=0..n} h_k cos((n - 2k) t),
+ * h_k = g_k g_{n-k}, g_k = prod_{j<=k} (2j-1)/(2j). */
+void orc_gauss_coefs(int n, double *h)
+{
+    double *g = malloc(sizeof(double) * (n + 1));
+    g[0] = 1.0;
+    for (int k = 1; k <= n; k++) g[k] = g[k - 1] * (2.0 * k - 1.0) / (2.0 * k);
+    for (int k = 0; k <= n; k++) h[k] = g[k] * g[n - k];
+    free(g);
+}
+
+/* P_n(cos t) and dP_n/dt from the cosine series, the derivative summed with
+ * Neumaier compensation. */
+static void legendre_cos_series(int n, const double *h, double t, double *P, double *dP)
+{
+    double p = (n % 2 == 0) ? h[n / 2] : 0.0;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < (n + 1) / 2; k++) {
+        int m = n - 2 * k;
+        p += 2.0 * h[k] * cos(m * t);
+        double term = -2.0 * h[k] * m * sin(m * t);
+        double u = s + term;
+        c += (fabs(s) >= fabs(term)) ? (s - u) + term : (term - u) + s;
+        s = u;
+    }
+    *P = p;
+    *dP = s + c;
+}
+
+/* Gauss-Legendre nodes from north to south.  Newton runs on theta with
+ * P_n(cos theta) evaluated from its cosine series, which keeps theta (and so
+ * sin theta) accurate to an ulp at the poles, where cos theta alone cannot;
+ * w = 2 / (dP_n/dtheta)^2. */
 void orc_gauss(int nlat, double *ct, double *st, double *wt)
 {
+    double *h = malloc(sizeof(double) * (nlat + 1));
+    orc_gauss_coefs(nlat, h);
     for (int i = 0; i < (nlat + 1) / 2; i++) {
         double th = M_PI * (i + 0.75) / (nlat + 0.5);
-        double dpdt = 1.0;
-        for (int it = 0; it < 100; it++) {
-            double x = cos(th), s = sin(th);
-            double p0 = 1.0, p1 = x;
-            for (int k = 2; k <= nlat; k++) {
-                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-                p0 = p1;
-                p1 = p2;
-            }
-            /* dP_n/dtheta = -sin(th) P_n'(x) = n (x P_n - P_{n-1}) / sin(th) */
-            dpdt = nlat * (x * p1 - p0) / s;
-            double dth = p1 / dpdt;
+        double P, dP;
+        int extra = 0;
+        for (int it = 0; it < 60; it++) {
+            legendre_cos_series(nlat, h, th, &P, &dP);
+            double dth = P / dP;
             th -= dth;
-            if (fabs(dth) < 1e-17) break;
+            if (fabs(dth) < 1e-15 * th && ++extra > 2) break;
         }
-        {
-            double x = cos(th), s = sin(th);
-            double p0 = 1.0, p1 = x;
-            for (int k = 2; k <= nlat; k++) {
-                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-                p0 = p1;
-                p1 = p2;
-            }
-            dpdt = nlat * (x * p1 - p0) / s;
-        }
-        double w = 2.0 / (dpdt * dpdt);
+        legendre_cos_series(nlat, h, th, &P, &dP);
+        double w = 2.0 / (dP * dP);
         ct[i] = cos(th);
         st[i] = sin(th);
         wt[i] = w;
@@ -143,6 +160,7 @@ void orc_gauss(int nlat, double *ct, double *st, double *wt)
         wt[nlat - 1 - i] = w;
     }
     if (nlat % 2 == 1) { ct[nlat / 2] = 0.0; st[nlat / 2] = 1.0; }
+    free(h);
 }
 
 static inline long lm_index(int lmax, int l, int m)

@@ -36,6 +36,43 @@ def _compile(src):
     return obj
 
 
+FDIR = os.path.join(HERE, "fortran")
+FLIB = os.path.join(OUT_DIR, "librdsphp_f.so")
+FTEST = os.path.join(OUT_DIR, "prueba_rd")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{r.stderr}")
+
+
+def build_fortran(verbose=True):
+    """The Fortran drop-in modules (librdsphp_f.so, .mod files in _lib/mod)
+    and the known-answer program prueba_rd, linked to librdsphp.so."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("amdflang not found: Fortran drop-in layer not built")
+        return None
+    mod = os.path.join(OUT_DIR, "mod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for src in ("rd_fortran_mods.f90", "rd_sphp.f90"):
+        obj = os.path.join(OUT_DIR, src[:-4] + ".o")
+        _run([FC, "-O2", "-fPIC", "-module-dir", mod, "-c", os.path.join(FDIR, src), "-o", obj], f"amdflang {src}")
+        objs.append(obj)
+    _run([FC, "-shared", "-fPIC", "-o", FLIB] + objs + ["-L" + OUT_DIR, "-lrdsphp", "-Wl,-rpath,$ORIGIN"],
+         "link librdsphp_f.so")
+    _run([FC, "-O2", "-I", mod, os.path.join(FDIR, "prueba_rd.f90"), "-o", FTEST,
+          "-L" + OUT_DIR, "-lrdsphp_f", "-lrdsphp", "-Wl,-rpath,$ORIGIN"], "build prueba_rd")
+    for o in objs:
+        os.remove(o)
+    if verbose:
+        print(f"built {FLIB} and {FTEST}")
+    return FLIB
+
+
 def build(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = sources()
@@ -50,6 +87,7 @@ def build(verbose=True):
         os.remove(o)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
 
 

@@ -49,16 +49,44 @@ __device__ __forceinline__ double2 cmuli(double2 a, int sign)
 {
     return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
-__device__ __forceinline__ double2 twid(const double2 *tw, int t, int sign)
+// exp(sign 2 pi i t / n) from a table of the first nh entries of
+// exp(-2 pi i t / n); nh = n/2 (even n) uses exp(-i pi) = -1 for the rest.
+__device__ __forceinline__ double2 twid(const double2 *tw, int t, int nh, int sign)
 {
-    double2 w = tw[t];
+    double2 w;
+    if (t < nh) {
+        w = tw[t];
+    } else {
+        w = tw[t - nh];
+        w.x = -w.x;
+        w.y = -w.y;
+    }
     return sign > 0 ? make_double2(w.x, -w.y) : w;
+}
+
+// Copies the twiddle table into LDS (n/2 entries for even n, else n) and
+// returns the number of entries held.
+__device__ int load_twiddles(double2 *dst, const double2 *tw, int n)
+{
+    const int nh = (n % 2 == 0) ? n / 2 : n;
+    for (int t = threadIdx.x; t < nh; t += blockDim.x) dst[t] = tw[t];
+    return nh;
+}
+
+// Workgroup index remapped so that consecutive logical tiles run on the same
+// XCD (the hardware deals workgroups round-robin over the 8 XCDs, each with
+// its own L2): neighbouring tiles then share cache lines in one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
+{
+    const int q = nblocks / 8, r = nblocks % 8;
+    const int x = bid % 8, i = bid / 8;
+    return x * q + (x < r ? x : r) + i;
 }
 
 // One radix-R Stockham stage over nb sequences of length n (a -> b).
 template <int R>
 __device__ void fft_stage(const double2 *a, double2 *b, int nb, int n, int Ns, int sign,
-                          const double2 *tw)
+                          const double2 *tw, int nh)
 {
     const int nbf = n / R, tot = nb * nbf, step = n / (Ns * R);
     for (int bf = threadIdx.x; bf < tot; bf += blockDim.x) {
@@ -71,7 +99,7 @@ __device__ void fft_stage(const double2 *a, double2 *b, int nb, int n, int Ns, i
         for (int r = 0; r < R; r++) v[r] = in[j + r * nbf];
         if (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, r * k * step, sign));
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, r * k * step, nh, sign));
         }
         if (R == 2) {
             double2 t0 = cadd(v[0], v[1]), t1 = csub(v[0], v[1]);
@@ -94,15 +122,33 @@ __device__ void fft_stage(const double2 *a, double2 *b, int nb, int n, int Ns, i
             v[3] = csub(d02, d13);
         }
         const int idxD = (j / Ns) * Ns * R + k;
+        if (R <= 4) {
 #pragma unroll
-        for (int q = 0; q < R; q++) out[idxD + q * Ns] = v[q];
+            for (int q = 0; q < R; q++) out[idxD + q * Ns] = v[q];
+        } else {
+            // direct R-point DFT: the inputs stay in registers, the roots come
+            // from the LDS table, one output at a time
+            const int e = n / R;
+            for (int q = 0; q < R; q++) {
+                double2 acc = v[0];
+                int idx = 0;
+                const int eq = q * e;
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    idx += eq;
+                    if (idx >= n) idx -= n;
+                    acc = cadd(acc, cmul(v[r], twid(tw, idx, nh, sign)));
+                }
+                out[idxD + q * Ns] = acc;
+            }
+        }
     }
 }
 
 // Generic radix: every thread computes whole output elements,
 // out[idxD + q Ns] = sum_r in[j + r nbf] W_{Ns R}^{r (k + q Ns)}.
 __device__ void fft_stage_gen(const double2 *a, double2 *b, int nb, int n, int Ns, int R, int sign,
-                              const double2 *tw)
+                              const double2 *tw, int nh)
 {
     const int nbf = n / R, step = n / (Ns * R);
     for (int o = threadIdx.x; o < nb * n; o += blockDim.x) {
@@ -115,7 +161,7 @@ __device__ void fft_stage_gen(const double2 *a, double2 *b, int nb, int n, int N
         double2 s = make_double2(0.0, 0.0);
         int idx = 0;
         for (int r = 0; r < R; r++) {
-            s = cadd(s, cmul(in[j + r * nbf], twid(tw, idx, sign)));
+            s = cadd(s, cmul(in[j + r * nbf], twid(tw, idx, nh, sign)));
             idx += e;
             if (idx >= n) idx -= n;
         }
@@ -124,21 +170,30 @@ __device__ void fft_stage_gen(const double2 *a, double2 *b, int nb, int n, int N
 }
 
 // Batched in-LDS FFT; returns the buffer that holds the result (a or b).
+// Radices 2, 3 and 4 have hand-written butterflies; RB (> 4, chosen per
+// length) gets a register DFT; any other factor uses the generic stage.
+template <int RB>
 __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftSeq &seq, int sign,
-                            const double2 *tw)
+                            const double2 *tw, int nh)
 {
     const int n = seq.n;
     int Ns = 1;
     for (int s = 0; s < seq.nst; s++) {
         const int R = seq.rad[s];
         if (R == 4)
-            fft_stage<4>(a, b, nb, n, Ns, sign, tw);
+            fft_stage<4>(a, b, nb, n, Ns, sign, tw, nh);
         else if (R == 2)
-            fft_stage<2>(a, b, nb, n, Ns, sign, tw);
+            fft_stage<2>(a, b, nb, n, Ns, sign, tw, nh);
         else if (R == 3)
-            fft_stage<3>(a, b, nb, n, Ns, sign, tw);
-        else
-            fft_stage_gen(a, b, nb, n, Ns, R, sign, tw);
+            fft_stage<3>(a, b, nb, n, Ns, sign, tw, nh);
+        else {
+            if constexpr (RB > 4) {
+                if (R == RB) fft_stage<RB>(a, b, nb, n, Ns, sign, tw, nh);
+                else fft_stage_gen(a, b, nb, n, Ns, R, sign, tw, nh);
+            } else {
+                fft_stage_gen(a, b, nb, n, Ns, R, sign, tw, nh);
+            }
+        }
         __syncthreads();
         double2 *t = a;
         a = b;
@@ -146,6 +201,18 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftSeq &seq, i
         Ns *= R;
     }
     return a;
+}
+
+// The register-DFT radix of a length: its largest prime factor if that is
+// one of 5, 7, 11, 13, 17, 19, else 0.
+int big_radix(const FftSeq &seq)
+{
+    int rb = 0;
+    for (int s = 0; s < seq.nst; s++) {
+        const int R = seq.rad[s];
+        if (R == 5 || R == 7 || R == 11 || R == 13 || R == 17 || R == 19) rb = R > rb ? R : rb;
+    }
+    return rb;
 }
 
 // Dynamic LDS above 64 KiB has to be allowed per kernel; the request is
@@ -183,12 +250,14 @@ __global__ __launch_bounds__(256) void phi_inverse_kernel(PhiPtrs P, int lmax, i
 {
     extern __shared__ double2 sm[];
     const int ntt = (nlat + 3) / 4;
-    const int k = blockIdx.x / ntt, th0 = (blockIdx.x % ntt) * 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / ntt, th0 = (t % ntt) * 4;
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const double2 *F = P.F[blockIdx.y];
     double *f = P.fo[blockIdx.y];
-    double2 *A = sm, *B = sm + 2 * nphi;
+    double2 *A = sm, *B = sm + 2 * nphi, *twl = sm + 4 * nphi;
+    const int nh = load_twiddles(twl, tw, nphi);
     for (int e = threadIdx.x; e < 2 * nphi; e += blockDim.x) {
         const int b = e / nphi, m = e - b * nphi;
         const int r1 = th0 + 2 * b, r2 = r1 + 1;
@@ -208,7 +277,7 @@ __global__ __launch_bounds__(256) void phi_inverse_kernel(PhiPtrs P, int lmax, i
         A[e] = z;
     }
     __syncthreads();
-    double2 *R = lds_fft(A, B, 2, seq, +1, tw);
+    double2 *R = lds_fft<0>(A, B, 2, seq, +1, twl, nh);
     for (int e = threadIdx.x; e < 2 * nphi; e += blockDim.x) {
         const int b = e / nphi, j = e - b * nphi;
         const int r1 = th0 + 2 * b, r2 = r1 + 1;
@@ -225,12 +294,14 @@ __global__ __launch_bounds__(256) void phi_forward_kernel(PhiPtrs P, int lmax, i
 {
     extern __shared__ double2 sm[];
     const int ntt = (nlat + 3) / 4;
-    const int k = blockIdx.x / ntt, th0 = (blockIdx.x % ntt) * 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / ntt, th0 = (t % ntt) * 4;
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const double *f = P.f[blockIdx.y];
     double2 *F = P.Fo[blockIdx.y];
-    double2 *A = sm, *B = sm + 2 * nphi;
+    double2 *A = sm, *B = sm + 2 * nphi, *twl = sm + 4 * nphi;
+    const int nh = load_twiddles(twl, tw, nphi);
     for (int e = threadIdx.x; e < 2 * nphi; e += blockDim.x) {
         const int b = e / nphi, j = e - b * nphi;
         const int r1 = th0 + 2 * b, r2 = r1 + 1;
@@ -238,7 +309,7 @@ __global__ __launch_bounds__(256) void phi_forward_kernel(PhiPtrs P, int lmax, i
         A[e] = make_double2(r1 < nlat ? f[o + r1] : 0.0, r2 < nlat ? f[o + r2] : 0.0);
     }
     __syncthreads();
-    double2 *R = lds_fft(A, B, 2, seq, -1, tw);
+    double2 *R = lds_fft<0>(A, B, 2, seq, -1, twl, nh);
     const double sc = 0.5 * (2.0 * M_PI / nphi);
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int b = e / L, m = e - b * L;
@@ -259,14 +330,16 @@ __global__ __launch_bounds__(256) void phi_forward_kernel(PhiPtrs P, int lmax, i
 // product N = u x w (prodvec, src/pseudo/pseudospec_hd.f90:235-248) is packed
 // as (N_r + i N_theta, N_phi) and transformed forward.  Rows of Fout may alias
 // rows of Fin: each workgroup reads its own row completely before writing it.
-__global__ __launch_bounds__(256) void phi_cross_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
+__global__ __launch_bounds__(512) void phi_cross_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
                                                         int nplanes, FftSeq seq, const double2 *tw)
 {
     extern __shared__ double2 sm[];
-    const int k = blockIdx.x / nlat, th = blockIdx.x % nlat;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
     if (k >= nplanes) return;
     const int L = lmax + 1;
-    double2 *A = sm, *B = sm + 3 * nphi;
+    double2 *A = sm, *B = sm + 3 * nphi, *twl = sm + 6 * nphi;
+    const int nh = load_twiddles(twl, tw, nphi);
     for (int e = threadIdx.x; e < 3 * nphi; e += blockDim.x) {
         const int x = e / nphi, m = e - x * nphi;
         double2 z = make_double2(0.0, 0.0);
@@ -284,7 +357,7 @@ __global__ __launch_bounds__(256) void phi_cross_kernel(PhiPtrs P, int lmax, int
         A[e] = z;
     }
     __syncthreads();
-    double2 *R = lds_fft(A, B, 3, seq, +1, tw);
+    double2 *R = lds_fft<0>(A, B, 3, seq, +1, twl, nh);
     double2 *O = (R == A) ? B : A;
     for (int j = threadIdx.x; j < nphi; j += blockDim.x) {
         const double2 zr = R[j], zt = R[nphi + j], zp = R[2 * nphi + j];
@@ -296,7 +369,7 @@ __global__ __launch_bounds__(256) void phi_cross_kernel(PhiPtrs P, int lmax, int
         O[nphi + j] = make_double2(np, 0.0);
     }
     __syncthreads();
-    double2 *Y = lds_fft(O, R, 2, seq, -1, tw);
+    double2 *Y = lds_fft<0>(O, R, 2, seq, -1, twl, nh);
     const double sc = 0.5 * (2.0 * M_PI / nphi);
     for (int m = threadIdx.x; m < L; m += blockDim.x) {
         const double2 yp = Y[m], ym = Y[m == 0 ? 0 : nphi - m];
@@ -319,7 +392,7 @@ int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *
         P.fo[i] = f[i];
     }
     const int ntt = (p->nlat + 3) / 4;
-    size_t shm = sizeof(double2) * 4 * p->nphi;
+    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
     {
         int lrc = allow_lds(phi_inverse_kernel, shm);
         if (lrc) return lrc;
@@ -338,7 +411,7 @@ int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f
         P.Fo[i] = F[i];
     }
     const int ntt = (p->nlat + 3) / 4;
-    size_t shm = sizeof(double2) * 4 * p->nphi;
+    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
     {
         int lrc = allow_lds(phi_forward_kernel, shm);
         if (lrc) return lrc;
@@ -354,12 +427,12 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
     PhiPtrs P = {};
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
     for (int i = 0; i < 3; i++) P.Fo[i] = Fout[i];
-    size_t shm = sizeof(double2) * 6 * p->nphi;
+    size_t shm = sizeof(double2) * (6 * p->nphi + p->nphi / 2);
     {
         int lrc = allow_lds(phi_cross_kernel, shm);
         if (lrc) return lrc;
     }
-    hipLaunchKernelGGL(phi_cross_kernel, dim3(nplanes * p->nlat), dim3(256), shm, p->stream, P,
+    hipLaunchKernelGGL(phi_cross_kernel, dim3(nplanes * p->nlat), dim3(512), shm, p->stream, P,
                        p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
     RD_LAUNCH_CHECK();
     return RD_OK;
@@ -415,17 +488,19 @@ __device__ void z_store(const double2 *X, double2 *a, long ld, int c0, int ncols
 
 // mode 0: forward (continuation, FFT -1, /n); 1: inverse (FFT +1);
 // 2: derivative of order `order` in (lm,r), output to a from ain.
-__global__ __launch_bounds__(256) void z_kernel(ZArgs A, int mode, FftSeq seq, const double2 *tw)
+template <int RB>
+__global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, FftSeq seq, const double2 *tw)
 {
     extern __shared__ double2 sm[];
     const int n = A.n, c0 = blockIdx.x * ZTC;
-    double2 *X = sm, *Y = sm + ZTC * n;
+    double2 *X = sm, *Y = sm + ZTC * n, *twl = sm + 2 * ZTC * n;
+    const int nh = load_twiddles(twl, tw, n);
     const double2 *src = (mode == 2) ? A.ain : A.a;
     const int nr = n - A.C;
     if (mode == 1) {
         z_load(X, src, A.ld, c0, A.ncols, n, n);
         __syncthreads();
-        double2 *R = lds_fft(X, Y, ZTC, seq, +1, tw);
+        double2 *R = lds_fft<RB>(X, Y, ZTC, seq, +1, twl, nh);
         z_store(R, A.a, A.ld, c0, A.ncols, n, n, 1.0);
         return;
     }
@@ -435,7 +510,7 @@ __global__ __launch_bounds__(256) void z_kernel(ZArgs A, int mode, FftSeq seq, c
         fc_continue(X, ZTC, n, A.C, A.d, A.dir);
         __syncthreads();
     }
-    double2 *R = lds_fft(X, Y, ZTC, seq, -1, tw);
+    double2 *R = lds_fft<RB>(X, Y, ZTC, seq, -1, twl, nh);
     if (mode == 0) {
         z_store(R, A.a, A.ld, c0, A.ncols, n, n, 1.0 / n);
         return;
@@ -455,7 +530,7 @@ __global__ __launch_bounds__(256) void z_kernel(ZArgs A, int mode, FftSeq seq, c
     }
     __syncthreads();
     double2 *O = (R == X) ? Y : X;
-    double2 *R2 = lds_fft(R, O, ZTC, seq, +1, tw);
+    double2 *R2 = lds_fft<RB>(R, O, ZTC, seq, +1, twl, nh);
     z_store(R2, A.a, A.ld, c0, A.ncols, n, n, 1.0);
 }
 
@@ -472,13 +547,26 @@ static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order
     A.C = p->C;
     A.d = p->d;
     A.order = order;
-    size_t shm = sizeof(double2) * 2 * ZTC * p->nz;
-    {
-        int lrc = allow_lds(z_kernel, shm);
-        if (lrc) return lrc;
+    size_t shm = sizeof(double2) * (2 * ZTC + 1) * p->nz;
+    const dim3 grid((p->nlm + ZTC - 1) / ZTC), block(512);
+#define RD_Z_CASE(RBV)                                                                         \
+    case RBV: {                                                                                \
+        int lrc = allow_lds(z_kernel<RBV>, shm);                                               \
+        if (lrc) return lrc;                                                                   \
+        hipLaunchKernelGGL(z_kernel<RBV>, grid, block, shm, p->stream, A, mode, p->seq_z, p->tw_z); \
+        break;                                                                                 \
     }
-    hipLaunchKernelGGL(z_kernel, dim3((p->nlm + ZTC - 1) / ZTC), dim3(256), shm, p->stream, A, mode,
-                       p->seq_z, p->tw_z);
+    switch (big_radix(p->seq_z)) {
+        RD_Z_CASE(0)
+        RD_Z_CASE(5)
+        RD_Z_CASE(7)
+        RD_Z_CASE(11)
+        RD_Z_CASE(13)
+        RD_Z_CASE(17)
+        RD_Z_CASE(19)
+    default: return RD_ERR_ARG;
+    }
+#undef RD_Z_CASE
     RD_LAUNCH_CHECK();
     return RD_OK;
 }
@@ -515,11 +603,13 @@ __device__ void spec_mul_ikz(const double2 *S, double2 *D, const double *kz, int
     }
 }
 
-__global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq seq, const double2 *tw)
+template <int RB>
+__global__ __launch_bounds__(512) void uxw_radial_kernel(UxwRadArgs A, FftSeq seq, const double2 *tw)
 {
     extern __shared__ double2 sm[];
     const int n = A.n, c0 = blockIdx.x * ZTC, nr = n - A.C;
-    double2 *X = sm, *Y = sm + ZTC * n, *S = sm + 2 * ZTC * n;
+    double2 *X = sm, *Y = sm + ZTC * n, *S = sm + 2 * ZTC * n, *twl = sm + 3 * ZTC * n;
+    const int nh = load_twiddles(twl, tw, n);
     const double inv_n = 1.0 / n;
     for (int fld = 0; fld < 2; fld++) {
         const double2 *src = fld == 0 ? A.W : A.Z;
@@ -529,14 +619,14 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq se
             fc_continue(X, ZTC, n, A.C, A.d, A.dir);
             __syncthreads();
         }
-        double2 *R = lds_fft(X, Y, ZTC, seq, -1, tw);
+        double2 *R = lds_fft<RB>(X, Y, ZTC, seq, -1, twl, nh);
         for (int e = threadIdx.x; e < ZTC * n; e += blockDim.x) S[e] = cscale(R[e], inv_n);
         __syncthreads();
         const int nder = fld == 0 ? 2 : 1;
         for (int p = 1; p <= nder; p++) {
             spec_mul_ikz(S, X, A.kz, n, p);
             __syncthreads();
-            double2 *D = lds_fft(X, Y, ZTC, seq, +1, tw);
+            double2 *D = lds_fft<RB>(X, Y, ZTC, seq, +1, twl, nh);
             for (int e = threadIdx.x; e < nr * ZTC; e += blockDim.x) {
                 const int r = e / ZTC, c = e - r * ZTC;
                 const int lm = c0 + c;
@@ -580,13 +670,26 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     A.n = p->nz;
     A.C = p->C;
     A.d = p->d;
-    size_t shm = sizeof(double2) * 3 * ZTC * p->nz;
-    {
-        int lrc = allow_lds(uxw_radial_kernel, shm);
-        if (lrc) return lrc;
+    size_t shm = sizeof(double2) * (3 * ZTC + 1) * p->nz;
+    const dim3 grid((p->nlm + ZTC - 1) / ZTC), block(512);
+#define RD_U_CASE(RBV)                                                                          \
+    case RBV: {                                                                                 \
+        int lrc = allow_lds(uxw_radial_kernel<RBV>, shm);                                       \
+        if (lrc) return lrc;                                                                    \
+        hipLaunchKernelGGL(uxw_radial_kernel<RBV>, grid, block, shm, p->stream, A, p->seq_z, p->tw_z); \
+        break;                                                                                  \
     }
-    hipLaunchKernelGGL(uxw_radial_kernel, dim3((p->nlm + ZTC - 1) / ZTC), dim3(256), shm, p->stream, A,
-                       p->seq_z, p->tw_z);
+    switch (big_radix(p->seq_z)) {
+        RD_U_CASE(0)
+        RD_U_CASE(5)
+        RD_U_CASE(7)
+        RD_U_CASE(11)
+        RD_U_CASE(13)
+        RD_U_CASE(17)
+        RD_U_CASE(19)
+    default: return RD_ERR_ARG;
+    }
+#undef RD_U_CASE
     RD_LAUNCH_CHECK();
     return RD_OK;
 }

@@ -36,36 +36,46 @@ __device__ __forceinline__ long lm_index(int lmax, int l, int m)
 }
 
 // ------------------------------------------------------------------ grid
-// Gauss-Legendre nodes, north to south, Newton on theta (one thread per
-// northern node) so sin(theta) keeps its relative accuracy at the poles.
-__global__ void gauss_kernel(int nlat, double *ct, double *st, double *wt)
+// P_n(cos t) = sum_k h_k cos((n - 2k) t) and its t-derivative (Neumaier
+// compensated), h_k = g_k g_{n-k}, g_k = prod_{j<=k} (2j-1)/(2j).
+__device__ void legendre_cos_series(int n, const double *h, double t, double *P, double *dP)
+{
+    double p = (n % 2 == 0) ? h[n / 2] : 0.0;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < (n + 1) / 2; k++) {
+        const int m = n - 2 * k;
+        double sn, cs;
+        sincos(m * t, &sn, &cs);
+        p += 2.0 * h[k] * cs;
+        const double term = -2.0 * h[k] * m * sn;
+        const double u = s + term;
+        c += (fabs(s) >= fabs(term)) ? (s - u) + term : (term - u) + s;
+        s = u;
+    }
+    *P = p;
+    *dP = s + c;
+}
+
+// Gauss-Legendre nodes, north to south (one thread per northern node):
+// Newton on theta with P_n(cos theta) from its cosine series, so theta and
+// sin(theta) stay accurate to an ulp at the poles; w = 2 / (dP_n/dtheta)^2.
+__global__ void gauss_kernel(int nlat, const double *h, double *ct, double *st, double *wt)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nlat + 1) / 2) return;
     double th = M_PI * (i + 0.75) / (nlat + 0.5);
-    double dpdt = 1.0;
-    for (int it = 0; it < 100; it++) {
-        double x = cos(th), s = sin(th);
-        double p0 = 1.0, p1 = x;
-        for (int k = 2; k <= nlat; k++) {
-            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-            p0 = p1;
-            p1 = p2;
-        }
-        dpdt = nlat * (x * p1 - p0) / s;
-        double dth = p1 / dpdt;
+    double P, dP;
+    int extra = 0;
+    for (int it = 0; it < 60; it++) {
+        legendre_cos_series(nlat, h, th, &P, &dP);
+        const double dth = P / dP;
         th -= dth;
-        if (fabs(dth) < 1e-17) break;
+        if (fabs(dth) < 1e-15 * th && ++extra > 2) break;
     }
-    double x = cos(th), s = sin(th);
-    double p0 = 1.0, p1 = x;
-    for (int k = 2; k <= nlat; k++) {
-        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-        p0 = p1;
-        p1 = p2;
-    }
-    dpdt = nlat * (x * p1 - p0) / s;
-    double w = 2.0 / (dpdt * dpdt);
+    legendre_cos_series(nlat, h, th, &P, &dP);
+    const double w = 2.0 / (dP * dP);
+    double s, x;
+    sincos(th, &s, &x);
     ct[i] = x;
     st[i] = s;
     wt[i] = w;
@@ -188,7 +198,15 @@ int build_legendre_tables(Plan *p)
     RD_HIP(hipMemcpyAsync(p->d_ana_ncol, ana_ncol.data(), sizeof(int) * L, hipMemcpyHostToDevice, st));
     RD_HIP(hipMemcpyAsync(p->d_ana_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
 
-    hipLaunchKernelGGL(gauss_kernel, dim3((p->nlat / 2 + 63) / 64), dim3(64), 0, st, p->nlat, p->d_ct,
+    // cosine-series coefficients of P_nlat (host FP64, shared with the nodes kernel)
+    std::vector<double> g(p->nlat + 1), h(p->nlat + 1);
+    g[0] = 1.0;
+    for (int k = 1; k <= p->nlat; k++) g[k] = g[k - 1] * (2.0 * k - 1.0) / (2.0 * k);
+    for (int k = 0; k <= p->nlat; k++) h[k] = g[k] * g[p->nlat - k];
+    double *d_h = nullptr;
+    RD_HIP(hipMalloc(&d_h, sizeof(double) * (p->nlat + 1)));
+    RD_HIP(hipMemcpyAsync(d_h, h.data(), sizeof(double) * (p->nlat + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(gauss_kernel, dim3((p->nlat / 2 + 63) / 64), dim3(64), 0, st, p->nlat, d_h, p->d_ct,
                        p->d_st, p->d_wt);
     RD_LAUNCH_CHECK();
     hipLaunchKernelGGL(legendre_tables_kernel, dim3((p->nth + 63) / 64, L), dim3(64), 0, st, p->lmax,
@@ -200,6 +218,7 @@ int build_legendre_tables(Plan *p)
     RD_HIP(hipMemcpyAsync(p->h_st, p->d_st, sizeof(double) * p->nlat, hipMemcpyDeviceToHost, st));
     RD_HIP(hipMemcpyAsync(p->h_wt, p->d_wt, sizeof(double) * p->nlat, hipMemcpyDeviceToHost, st));
     RD_HIP(hipStreamSynchronize(st));
+    RD_HIP(hipFree(d_h));
     return RD_OK;
 }
 

@@ -1,0 +1,273 @@
+!=================================================================
+! Known-answer checks of the Fortran drop-in layer, in the style of the
+! reference's prueba_sphp / prueba_vec / prueba_z / prueba_3d programs:
+! host arrays with registered device copies, the reference's routine
+! names and argument lists, exact answers from closed forms.  Prints one
+! CHECK line per case and stops with code 1 if any case fails.
+!=================================================================
+      PROGRAM prueba_rd
+      USE iso_c_binding
+      USE gdevice
+      USE fcgram
+      USE fft
+      IMPLICIT NONE
+      DOUBLE PRECISION, PARAMETER :: pi = 3.14159265358979323846D0
+      INTEGER :: nfail = 0
+
+      CALL device_init(0)
+      CALL casos_sht()
+      CALL casos_vec()
+      CALL caso_z()
+      CALL caso_uxw()
+      IF (nfail .gt. 0) THEN
+         PRINT '(A,I0,A)', 'prueba_rd: ', nfail, ' failed'
+         STOP 1
+      ENDIF
+      PRINT '(A)', 'prueba_rd: all passed'
+
+      CONTAINS
+
+      SUBROUTINE report(name,err,tol)
+      CHARACTER(len=*), INTENT(IN) :: name
+      DOUBLE PRECISION, INTENT(IN) :: err,tol
+      IF (err .le. tol) THEN
+         PRINT '(A,A40,ES10.2,A,ES8.1,A)', 'CHECK ', name, err, ' <= ', tol, ' PASS'
+      ELSE
+         PRINT '(A,A40,ES10.2,A,ES8.1,A)', 'CHECK ', name, err, ' >  ', tol, ' FAIL'
+         nfail = nfail + 1
+      ENDIF
+      END SUBROUTINE report
+
+      INTEGER FUNCTION lm(lmax,l,m)
+      INTEGER, INTENT(IN) :: lmax,l,m
+      lm = m*(2*lmax+1-m)/2 + l + 1
+      END FUNCTION lm
+
+! The grid of a plan: cos and sin of the Gauss nodes.
+      SUBROUTINE grilla(plan,ct,st)
+      TYPE(FCPLAN), INTENT(IN) :: plan
+      DOUBLE PRECISION, TARGET, INTENT(OUT) :: ct(:),st(:)
+      CALL rd_check(rd_plan_grid(plan%h,C_LOC(ct),C_LOC(st),C_NULL_PTR,C_NULL_PTR, &
+           C_NULL_PTR,C_NULL_PTR,C_NULL_PTR),'grid')
+      END SUBROUTINE grilla
+
+! prueba_sphp cases 3, 4, 8 and 9 (src/prueba_sphp.f90:121-184), plane k x k
+      SUBROUTINE casos_sht()
+      INTEGER, PARAMETER :: lmax = 15, nz = 6, nlat = 24, nphi = 48
+      INTEGER, PARAMETER :: nlm = (lmax+1)*(lmax+2)/2
+      TYPE(FCPLAN) :: plan
+      DOUBLE PRECISION, TARGET :: f3(nlat,nphi,nz), ct(nlat), st(nlat)
+      DOUBLE COMPLEX, TARGET :: b3(nlm,nz)
+      DOUBLE COMPLEX :: a(nlm)
+      DOUBLE PRECISION :: f(nlat,nphi), c, ph, err1, err2
+      INTEGER :: ic, i, j, k
+      CHARACTER(len=40) :: nombre
+      CALL fcgram_create_plan(plan,(/nlat,nphi,nz/),(/0,0,0/),(/0,0,0/), &
+           (/1.0D0,1.0D0,1.0D0/),'.',0,lmax)
+      CALL grilla(plan,ct,st)
+      CALL gdev_alloc(C_LOC(f3),INT(8*SIZE(f3),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(b3),INT(16*SIZE(b3),C_SIZE_T))
+      DO ic = 1,4
+         a = (0.0D0,0.0D0)
+         DO j = 1,nphi
+            ph = 2*pi*(j-1)/nphi
+            DO i = 1,nlat
+               SELECT CASE (ic)
+               CASE (1)
+                  f(i,j) = st(i)*COS(ph)
+               CASE (2)
+                  f(i,j) = st(i)*SIN(ph)
+               CASE (3)
+                  f(i,j) = st(i)**lmax*COS(lmax*ph)
+               CASE (4)
+                  f(i,j) = 2-3*ct(i)+st(i)*SIN(ph)+0.5D0*st(i)**2*COS(2*ph)
+               END SELECT
+            END DO
+         END DO
+         SELECT CASE (ic)
+         CASE (1)
+            nombre = 'sht sin(th) cos(ph)'
+            a(lm(lmax,1,1)) = SQRT(2*pi/3)
+         CASE (2)
+            nombre = 'sht sin(th) sin(ph)'
+            a(lm(lmax,1,1)) = DCMPLX(0.0D0,-SQRT(2*pi/3))
+         CASE (3)
+            nombre = 'sht sin(th)^lmax cos(lmax ph)'
+            c = SQRT((2*lmax+1)/(4*pi))
+            DO k = 1,lmax
+               c = c*SQRT((2*k-1.0D0)/(2*k))
+            END DO
+            a(lm(lmax,lmax,lmax)) = 1/(2*c)
+         CASE (4)
+            nombre = 'sht sum 2(1) - 3(2) + (4) + (6)/2'
+            a(lm(lmax,0,0)) = 2*SQRT(4*pi)
+            a(lm(lmax,1,0)) = -3*SQRT(4*pi/3)
+            a(lm(lmax,1,1)) = DCMPLX(0.0D0,-SQRT(2*pi/3))
+            a(lm(lmax,2,2)) = SQRT(2*pi/15)
+         END SELECT
+         DO k = 1,nz
+            f3(:,:,k) = k*f
+         END DO
+         CALL gdev_update_to(C_LOC(f3),INT(8*SIZE(f3),C_SIZE_T))
+         CALL fftp2d_real_to_complex_xy(plan,f3,b3,0)
+         CALL gdev_update_from(C_LOC(b3),INT(16*SIZE(b3),C_SIZE_T))
+         err1 = 0
+         DO k = 1,nz
+            err1 = MAX(err1,MAXVAL(ABS(b3(:,k)-k*a)))
+         END DO
+         DO k = 1,nz
+            b3(:,k) = k*a
+         END DO
+         CALL gdev_update_to(C_LOC(b3),INT(16*SIZE(b3),C_SIZE_T))
+         CALL fftp2d_complex_to_real_xy(plan,b3,f3,0)
+         CALL gdev_update_from(C_LOC(f3),INT(8*SIZE(f3),C_SIZE_T))
+         err2 = 0
+         DO k = 1,nz
+            err2 = MAX(err2,MAXVAL(ABS(f3(:,:,k)-k*f)))
+         END DO
+         CALL report(TRIM(nombre)//' analysis',err1,1.0D-12)
+         CALL report(TRIM(nombre)//' synthesis',err2,1.0D-12)
+      END DO
+      CALL gdev_free(C_LOC(f3))
+      CALL gdev_free(C_LOC(b3))
+      CALL fcgram_destroy_plan(plan)
+      END SUBROUTINE casos_sht
+
+! prueba_vec case 8 (src/prueba_vec.f90:198-207): Q = cos th, S = T = sin th cos ph
+      SUBROUTINE casos_vec()
+      INTEGER, PARAMETER :: lmax = 15, nz = 6, nlat = 24, nphi = 48
+      INTEGER, PARAMETER :: nlm = (lmax+1)*(lmax+2)/2
+      TYPE(FCPLAN) :: plan
+      DOUBLE PRECISION, TARGET :: vr(nlat,nphi,nz), vt(nlat,nphi,nz), vp(nlat,nphi,nz)
+      DOUBLE PRECISION, TARGET :: ct(nlat), st(nlat)
+      DOUBLE COMPLEX, TARGET :: q(nlm,nz), s(nlm,nz), t(nlm,nz)
+      DOUBLE COMPLEX :: aq(nlm), as(nlm), at(nlm)
+      DOUBLE PRECISION :: ph, err1, err2
+      INTEGER :: i, j, k
+      CALL fcgram_create_plan(plan,(/nlat,nphi,nz/),(/0,0,0/),(/0,0,0/), &
+           (/1.0D0,1.0D0,1.0D0/),'.',0,lmax)
+      CALL grilla(plan,ct,st)
+      CALL gdev_alloc(C_LOC(vr),INT(8*SIZE(vr),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(vt),INT(8*SIZE(vt),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(vp),INT(8*SIZE(vp),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(q),INT(16*SIZE(q),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(s),INT(16*SIZE(s),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(t),INT(16*SIZE(t),C_SIZE_T))
+      aq = 0; as = 0; at = 0
+      aq(lm(lmax,1,0)) = SQRT(4*pi/3)
+      as(lm(lmax,1,1)) = SQRT(2*pi/3)
+      at(lm(lmax,1,1)) = SQRT(2*pi/3)
+      DO k = 1,nz
+         DO j = 1,nphi
+            ph = 2*pi*(j-1)/nphi
+            DO i = 1,nlat
+               vr(i,j,k) = k*ct(i)
+               vt(i,j,k) = k*(ct(i)*COS(ph)-SIN(ph))
+               vp(i,j,k) = k*(-SIN(ph)-ct(i)*COS(ph))
+            END DO
+         END DO
+      END DO
+      CALL gdev_update_to(C_LOC(vr),INT(8*SIZE(vr),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(vt),INT(8*SIZE(vt),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(vp),INT(8*SIZE(vp),C_SIZE_T))
+      CALL fftp2d_vec_to_qst(plan,vr,vt,vp,q,s,t,0)
+      CALL gdev_update_from(C_LOC(q),INT(16*SIZE(q),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(s),INT(16*SIZE(s),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(t),INT(16*SIZE(t),C_SIZE_T))
+      err1 = 0
+      DO k = 1,nz
+         err1 = MAX(err1,MAXVAL(ABS(q(:,k)-k*aq)),MAXVAL(ABS(s(:,k)-k*as)),MAXVAL(ABS(t(:,k)-k*at)))
+      END DO
+      CALL fftp2d_qst_to_vec(plan,q,s,t,vr,vt,vp,0)
+      CALL gdev_update_from(C_LOC(vt),INT(8*SIZE(vt),C_SIZE_T))
+      err2 = 0
+      DO k = 1,nz
+         DO j = 1,nphi
+            ph = 2*pi*(j-1)/nphi
+            err2 = MAX(err2,MAXVAL(ABS(vt(:,j,k)-k*(ct*COS(ph)-SIN(ph)))))
+         END DO
+      END DO
+      CALL report('vec sum of cases 1, 4, 5 analysis',err1,1.0D-12)
+      CALL report('vec round trip v_theta',err2,1.0D-12)
+      CALL gdev_free(C_LOC(vr)); CALL gdev_free(C_LOC(vt)); CALL gdev_free(C_LOC(vp))
+      CALL gdev_free(C_LOC(q)); CALL gdev_free(C_LOC(s)); CALL gdev_free(C_LOC(t))
+      CALL fcgram_destroy_plan(plan)
+      END SUBROUTINE casos_vec
+
+! prueba_z (b) at nz = 128 (src/prueba_z.f90:79-125) through derivr of
+! the global plan: published error 4.9e-8 (LEEME.md:34).
+      SUBROUTINE caso_z()
+      INTEGER, PARAMETER :: lmax = 7, nz = 128, C = 25, od = 5
+      INTEGER, PARAMETER :: nlm = (lmax+1)*(lmax+2)/2, nlat = 12, nphi = 24
+      DOUBLE COMPLEX, TARGET :: f(nlm,nz), e(nlm,nz), d(nlm,nz)
+      DOUBLE PRECISION :: dz, zj, err
+      INTEGER :: j
+      CHARACTER(len=256) :: tdir
+      CALL GET_ENVIRONMENT_VARIABLE('RD_TABLE_DIR',tdir)
+      dz = 1.0D0/(nz-C-1)
+      CALL fcgram_create_plan(planfc,(/nlat,nphi,nz/),(/0,0,C/),(/0,0,od/), &
+           (/1.0D0,1.0D0,dz/),TRIM(tdir),0,lmax)
+      f = 0; e = 0
+      DO j = 1,nz-C
+         zj = (j-1)*dz
+         f(:,j) = DCMPLX(0.3D0,-0.5D0)*(SIN(2.3D0*zj+0.4D0)+zj**2)
+         e(:,j) = DCMPLX(0.3D0,-0.5D0)*(2.3D0*COS(2.3D0*zj+0.4D0)+2*zj)
+      END DO
+      CALL gdev_alloc(C_LOC(f),INT(16*SIZE(f),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(d),INT(16*SIZE(d),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(f),INT(16*SIZE(f),C_SIZE_T))
+      CALL derivr(f,d,1,'r')
+      CALL gdev_update_from(C_LOC(d),INT(16*SIZE(d),C_SIZE_T))
+      err = MAXVAL(ABS(d(:,1:nz-C)-e(:,1:nz-C)))/MAXVAL(ABS(e(:,1:nz-C)))
+      CALL report('derivr FC-Gram nz=128 (published 4.9e-8)',err,1.0D-7)
+      CALL gdev_free(C_LOC(f)); CALL gdev_free(C_LOC(d))
+      CALL fcgram_destroy_plan(planfc)
+      END SUBROUTINE caso_z
+
+! u x omega of a solid-body rotation: NQ_00, NQ_20, NS_20 in closed form.
+      SUBROUTINE caso_uxw()
+      INTEGER, PARAMETER :: lmax = 15, nr = 64, C = 25, od = 5, nz = nr + C
+      INTEGER, PARAMETER :: nlat = 24, nphi = 48, nlm = (lmax+1)*(lmax+2)/2
+      DOUBLE PRECISION, PARAMETER :: ri = 0.35D0, ro = 1.0D0, om = 0.7D0
+      DOUBLE COMPLEX, TARGET :: w(nlm,nz), z(nlm,nz), nq(nlm,nz), ns(nlm,nz), nt(nlm,nz)
+      DOUBLE PRECISION :: dz, r, eq, es, et, sc
+      INTEGER :: k
+      CHARACTER(len=256) :: tdir
+      CALL GET_ENVIRONMENT_VARIABLE('RD_TABLE_DIR',tdir)
+      dz = (ro-ri)/(nr-1)
+      CALL fcgram_create_plan(planfc,(/nlat,nphi,nz/),(/0,0,C/),(/0,0,od/), &
+           (/1.0D0,1.0D0,dz/),TRIM(tdir),0,lmax,ri)
+      w = 0; z = 0
+      DO k = 1,nr
+         r = ri+(k-1)*dz
+         z(lm(lmax,1,0),k) = om*r**2*SQRT(4*pi/3)
+      END DO
+      CALL gdev_alloc(C_LOC(w),INT(16*SIZE(w),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(z),INT(16*SIZE(z),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(nq),INT(16*SIZE(nq),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(ns),INT(16*SIZE(ns),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(nt),INT(16*SIZE(nt),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(w),INT(16*SIZE(w),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(z),INT(16*SIZE(z),C_SIZE_T))
+      CALL nonlinear_uxw(w,z,nq,ns,nt,0)
+      CALL gdev_update_from(C_LOC(nq),INT(16*SIZE(nq),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(ns),INT(16*SIZE(ns),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(nt),INT(16*SIZE(nt),C_SIZE_T))
+      eq = 0; es = 0; et = 0; sc = 0
+      DO k = 1,nr
+         r = ri+(k-1)*dz
+         sc = MAX(sc,2*om**2*r*(2.0D0/3)*SQRT(4*pi))
+         eq = MAX(eq,ABS(nq(lm(lmax,0,0),k)-2*om**2*r*(2.0D0/3)*SQRT(4*pi)), &
+                  ABS(nq(lm(lmax,2,0),k)+2*om**2*r*(2.0D0/3)*SQRT(4*pi/5)))
+         es = MAX(es,ABS(ns(lm(lmax,2,0),k)+om**2*r*(2.0D0/3)*SQRT(4*pi/5)))
+         et = MAX(et,MAXVAL(ABS(nt(:,k))))
+      END DO
+      CALL report('u x omega solid body NQ (rel)',eq/sc,1.0D-9)
+      CALL report('u x omega solid body NS (rel)',es/sc,1.0D-12)
+      CALL report('u x omega solid body NT (abs)',et,1.0D-12)
+      CALL gdev_free(C_LOC(w)); CALL gdev_free(C_LOC(z))
+      CALL gdev_free(C_LOC(nq)); CALL gdev_free(C_LOC(ns)); CALL gdev_free(C_LOC(nt))
+      CALL fcgram_destroy_plan(planfc)
+      END SUBROUTINE caso_uxw
+
+      END PROGRAM prueba_rd

@@ -1,0 +1,391 @@
+!=================================================================
+! Fortran drop-in modules over the C-ABI of librdsphp
+! (include/redondito.h).  They keep the names the reference's callers
+! USE:
+!   gdevice  -- device_init, device_sync, gdev_alloc, gdev_free,
+!               gdev_update_to/from, gdev_copy, gdev_present,
+!               gdev_active (src/memmgt/gdevice_mod.f90)
+!   fcgram   -- TYPE(FCCOORD), TYPE(FFTPLAN), TYPE(FCPLAN),
+!               fcgram_create_plan, fcgram_destroy_plan,
+!               load_dirichlet_tables (src/sphp/fcgram_mod.f90,
+!               src/sphp-gpu/sphp_mod.F90:43-54)
+!   fft      -- the global plan planfc (src/pseudo/pseudospec_mod.f90)
+! Field arrays stay the caller's host arrays; their device copies are
+! registered with gdev_alloc and found by host address, as in the
+! reference's GPU build.  Errors print and STOP, as the reference does.
+!=================================================================
+
+      MODULE rd_capi
+      USE iso_c_binding
+      IMPLICIT NONE
+      INTERFACE
+        FUNCTION rd_error_string(code) BIND(C,NAME='rd_error_string')
+          IMPORT :: C_INT, C_PTR
+          INTEGER(C_INT), VALUE :: code
+          TYPE(C_PTR) :: rd_error_string
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_load_dirichlet_tables(tdir,C,d,dir) &
+                BIND(C,NAME='rd_load_dirichlet_tables')
+          IMPORT :: C_INT, C_CHAR, C_DOUBLE
+          CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: tdir
+          INTEGER(C_INT), VALUE :: C, d
+          REAL(C_DOUBLE), DIMENSION(*), INTENT(OUT) :: dir
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_plan_create(plan,lmax,nlat,nphi,nz,C,d,dz,rint,dir) &
+                BIND(C,NAME='rd_plan_create')
+          IMPORT :: C_INT, C_PTR, C_DOUBLE
+          TYPE(C_PTR), INTENT(OUT) :: plan
+          INTEGER(C_INT), VALUE :: lmax, nlat, nphi, nz, C, d
+          REAL(C_DOUBLE), VALUE :: dz, rint
+          TYPE(C_PTR), VALUE :: dir
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_plan_destroy(plan) BIND(C,NAME='rd_plan_destroy')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_plan_sync(plan) BIND(C,NAME='rd_plan_sync')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_plan_grid(plan,ct,st,wt,z,kz,ell,emm) &
+                BIND(C,NAME='rd_plan_grid')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, ct, st, wt, z, kz, ell, emm
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_sht_analysis(plan,nplanes,a,b) BIND(C,NAME='rd_sht_analysis')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+          INTEGER(C_INT), VALUE :: nplanes
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_sht_synthesis(plan,nplanes,a,b) BIND(C,NAME='rd_sht_synthesis')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+          INTEGER(C_INT), VALUE :: nplanes
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_vsht_analysis(plan,nplanes,vr,vt,vp,q,s,t) &
+                BIND(C,NAME='rd_vsht_analysis')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, vr, vt, vp, q, s, t
+          INTEGER(C_INT), VALUE :: nplanes
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_vsht_synthesis(plan,nplanes,q,s,t,vr,vt,vp) &
+                BIND(C,NAME='rd_vsht_synthesis')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, q, s, t, vr, vt, vp
+          INTEGER(C_INT), VALUE :: nplanes
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_z_forward(plan,a) BIND(C,NAME='rd_z_forward')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_z_inverse(plan,a) BIND(C,NAME='rd_z_inverse')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_3d_forward(plan,a,b) BIND(C,NAME='rd_3d_forward')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_3d_inverse(plan,a,b) BIND(C,NAME='rd_3d_inverse')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_derivr(plan,p,esp,a,b) BIND(C,NAME='rd_derivr')
+          IMPORT :: C_INT, C_PTR, C_CHAR
+          TYPE(C_PTR), VALUE :: plan, a, b
+          INTEGER(C_INT), VALUE :: p
+          CHARACTER(KIND=C_CHAR), VALUE :: esp
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_laplang(plan,a,b) BIND(C,NAME='rd_laplang')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_multr(plan,p,a,b) BIND(C,NAME='rd_multr')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+          INTEGER(C_INT), VALUE :: p
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dlr(plan,a,b) BIND(C,NAME='rd_dlr')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_saxpby_c(plan,z,x,a,y,b) BIND(C,NAME='rd_saxpby_c')
+          IMPORT :: C_INT, C_PTR, C_DOUBLE
+          TYPE(C_PTR), VALUE :: plan, z, x, y
+          REAL(C_DOUBLE), VALUE :: a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_saxpby_r(plan,nplanes,z,x,a,y,b) BIND(C,NAME='rd_saxpby_r')
+          IMPORT :: C_INT, C_PTR, C_DOUBLE
+          TYPE(C_PTR), VALUE :: plan, z, x, y
+          INTEGER(C_INT), VALUE :: nplanes
+          REAL(C_DOUBLE), VALUE :: a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_prodvec(plan,nplanes,ar,at,ap,br,bt,bp,cr,ct,cp) &
+                BIND(C,NAME='rd_prodvec')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, ar, at, ap, br, bt, bp, cr, ct, cp
+          INTEGER(C_INT), VALUE :: nplanes
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_fc_filter(plan,esp,a) BIND(C,NAME='rd_fc_filter')
+          IMPORT :: C_INT, C_PTR, C_CHAR
+          TYPE(C_PTR), VALUE :: plan, a
+          CHARACTER(KIND=C_CHAR), VALUE :: esp
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_copy(plan,a,b) BIND(C,NAME='rd_copy')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dirichlet(plan,a) BIND(C,NAME='rd_dirichlet')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_nonlinear_uxw(plan,prec,w,z,nq,ns,nt) &
+                BIND(C,NAME='rd_nonlinear_uxw')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, w, z, nq, ns, nt
+          INTEGER(C_INT), VALUE :: prec
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_device_init(rank) BIND(C,NAME='rd_device_init')
+          IMPORT :: C_INT
+          INTEGER(C_INT), VALUE :: rank
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_device_sync() BIND(C,NAME='rd_device_sync')
+          IMPORT :: C_INT
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_alloc(h,nb) BIND(C,NAME='rd_dev_alloc')
+          IMPORT :: C_INT, C_PTR, C_SIZE_T
+          TYPE(C_PTR), VALUE :: h
+          INTEGER(C_SIZE_T), VALUE :: nb
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_free(h) BIND(C,NAME='rd_dev_free')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: h
+        END FUNCTION
+        TYPE(C_PTR) FUNCTION rd_dev_lookup(h) BIND(C,NAME='rd_dev_lookup')
+          IMPORT :: C_PTR
+          TYPE(C_PTR), VALUE :: h
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_present(h) BIND(C,NAME='rd_dev_present')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: h
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_update_to(h,nb) BIND(C,NAME='rd_dev_update_to')
+          IMPORT :: C_INT, C_PTR, C_SIZE_T
+          TYPE(C_PTR), VALUE :: h
+          INTEGER(C_SIZE_T), VALUE :: nb
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_update_from(h,nb) BIND(C,NAME='rd_dev_update_from')
+          IMPORT :: C_INT, C_PTR, C_SIZE_T
+          TYPE(C_PTR), VALUE :: h
+          INTEGER(C_SIZE_T), VALUE :: nb
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_dev_copy(hd,hs,nb) BIND(C,NAME='rd_dev_copy')
+          IMPORT :: C_INT, C_PTR, C_SIZE_T
+          TYPE(C_PTR), VALUE :: hd, hs
+          INTEGER(C_SIZE_T), VALUE :: nb
+        END FUNCTION
+      END INTERFACE
+
+      CONTAINS
+
+! Prints the library's message for a failed call and stops, like the
+! reference's PRINT + STOP error handling.
+      SUBROUTINE rd_check(rc,what)
+      INTEGER(C_INT), INTENT(IN) :: rc
+      CHARACTER(len=*), INTENT(IN) :: what
+      CHARACTER(len=128), POINTER :: msg
+      CHARACTER(len=128) :: txt
+      INTEGER :: i
+      IF (rc .eq. 0) RETURN
+      CALL C_F_POINTER(rd_error_string(rc),msg)
+      txt = msg
+      i = INDEX(txt,C_NULL_CHAR)
+      IF (i .gt. 0) txt(i:) = ' '
+      PRINT *, '[ERROR] ', what, ': ', TRIM(txt)
+      STOP
+      END SUBROUTINE rd_check
+
+! Device address of a registered host array (stops when it has none,
+! as the reference does for an unregistered array).
+      FUNCTION rd_dptr(h,what) RESULT(d)
+      TYPE(C_PTR), INTENT(IN) :: h
+      CHARACTER(len=*), INTENT(IN) :: what
+      TYPE(C_PTR) :: d
+      d = rd_dev_lookup(h)
+      IF (.NOT. C_ASSOCIATED(d)) THEN
+         PRINT *, '[ERROR] ', what, ': array has no device copy (gdev_alloc)'
+         STOP
+      ENDIF
+      END FUNCTION rd_dptr
+
+      END MODULE rd_capi
+
+!=================================================================
+      MODULE gdevice
+      USE iso_c_binding
+      USE rd_capi
+      IMPLICIT NONE
+! Kept for source compatibility: the device path is the only path.
+      LOGICAL, SAVE :: gdev_active = .TRUE.
+      CONTAINS
+
+      SUBROUTINE device_init(myrank)
+      INTEGER, INTENT(IN) :: myrank
+      CALL rd_check(rd_device_init(INT(myrank,C_INT)),'device_init')
+      END SUBROUTINE device_init
+
+      SUBROUTINE device_sync()
+      CALL rd_check(rd_device_sync(),'device_sync')
+      END SUBROUTINE device_sync
+
+      SUBROUTINE gdev_alloc(hp,nbytes)
+      TYPE(C_PTR), INTENT(IN) :: hp
+      INTEGER(C_SIZE_T), INTENT(IN) :: nbytes
+      CALL rd_check(rd_dev_alloc(hp,nbytes),'gdev_alloc')
+      END SUBROUTINE gdev_alloc
+
+      SUBROUTINE gdev_free(hp)
+      TYPE(C_PTR), INTENT(IN) :: hp
+      CALL rd_check(rd_dev_free(hp),'gdev_free')
+      END SUBROUTINE gdev_free
+
+      SUBROUTINE gdev_update_to(hp,nbytes)
+      TYPE(C_PTR), INTENT(IN) :: hp
+      INTEGER(C_SIZE_T), INTENT(IN) :: nbytes
+      CALL rd_check(rd_dev_update_to(hp,nbytes),'gdev_update_to')
+      END SUBROUTINE gdev_update_to
+
+      SUBROUTINE gdev_update_from(hp,nbytes)
+      TYPE(C_PTR), INTENT(IN) :: hp
+      INTEGER(C_SIZE_T), INTENT(IN) :: nbytes
+      CALL rd_check(rd_dev_update_from(hp,nbytes),'gdev_update_from')
+      END SUBROUTINE gdev_update_from
+
+      SUBROUTINE gdev_copy(hdst,hsrc,nbytes)
+      TYPE(C_PTR), INTENT(IN) :: hdst, hsrc
+      INTEGER(C_SIZE_T), INTENT(IN) :: nbytes
+      CALL rd_check(rd_dev_copy(hdst,hsrc,nbytes),'gdev_copy')
+      END SUBROUTINE gdev_copy
+
+      LOGICAL FUNCTION gdev_present(hp)
+      TYPE(C_PTR), INTENT(IN) :: hp
+      gdev_present = rd_dev_present(hp) .ne. 0
+      END FUNCTION gdev_present
+
+      END MODULE gdevice
+
+!=================================================================
+      MODULE fcgram
+      USE iso_c_binding
+      USE rd_capi
+      IMPLICIT NONE
+
+      TYPE FCCOORD
+         DOUBLE PRECISION, ALLOCATABLE, DIMENSION(:,:) :: dir
+         DOUBLE PRECISION :: dx
+         INTEGER          :: N,C,d
+      END TYPE FCCOORD
+
+      TYPE FFTPLAN
+         INTEGER :: nlat,nphi,nz,nlm,lmax
+      END TYPE FFTPLAN
+
+      TYPE FCPLAN
+         TYPE(FCCOORD)      :: x,y,z
+         TYPE(FFTPLAN)      :: planrc, plancr
+         CHARACTER(len=128) :: tdir
+         TYPE(C_PTR)        :: h = C_NULL_PTR
+      END TYPE FCPLAN
+
+      CONTAINS
+
+!-----------------------------------------------------------------
+! fcgram_create_plan(this,N,C,o,dx,tdir,fftflags,lmax), the
+! reference's constructor (src/sphp/fcgram_mod.f90:49-150): only z may
+! be non-periodic; the z tables are read from tdir.  rint is the first
+! radius (z(1)); it only enters the nonlinear term (1/r factors) and
+! defaults to 0.
+      SUBROUTINE fcgram_create_plan(this,N,C,o,dx,tdir,fftflags,lmax,rint)
+      TYPE(FCPLAN), INTENT(OUT)    :: this
+      INTEGER, INTENT(IN)          :: N(3), C(3), o(3), fftflags
+      DOUBLE PRECISION, INTENT(IN) :: dx(3)
+      CHARACTER(len=*), INTENT(IN) :: tdir
+      INTEGER, INTENT(IN)          :: lmax
+      DOUBLE PRECISION, INTENT(IN), OPTIONAL :: rint
+      DOUBLE PRECISION, TARGET, ALLOCATABLE :: dv(:)
+      DOUBLE PRECISION :: r0
+      INTEGER :: nlm
+
+      IF (C(1).ne.0 .OR. o(1).ne.0 .OR. C(2).ne.0 .OR. o(2).ne.0) THEN
+         PRINT*, "[ERROR] FC-Gram continuation is only ", &
+            "supported in z (Cx = Cy = 0 and ox = oy = 0). Aborting..."
+         STOP
+      ENDIF
+      IF ((C(3).eq.0) .neqv. (o(3).eq.0)) THEN
+         PRINT*, "[ERROR] Mismatch in continuation", &
+            " or matching points in z direction. Aborting..."
+         STOP
+      ENDIF
+      this%x%N = N(1); this%y%N = N(2); this%z%N = N(3)
+      this%x%C = 0;    this%y%C = 0;    this%z%C = C(3)
+      this%x%d = 0;    this%y%d = 0;    this%z%d = o(3)
+      this%x%dx = dx(1); this%y%dx = dx(2); this%z%dx = dx(3)
+      this%tdir = tdir
+      nlm = (lmax+1)*(lmax+2)/2
+      this%planrc = FFTPLAN(N(1),N(2),N(3),nlm,lmax)
+      this%plancr = this%planrc
+      r0 = 0.0D0
+      IF (PRESENT(rint)) r0 = rint
+      IF (C(3) .gt. 0) THEN
+         CALL load_dirichlet_tables(this%z,tdir)
+         ALLOCATE(dv(C(3)*o(3)))
+         dv = RESHAPE(this%z%dir,(/C(3)*o(3)/))
+         CALL rd_check(rd_plan_create(this%h,INT(lmax,C_INT),INT(N(1),C_INT), &
+              INT(N(2),C_INT),INT(N(3),C_INT),INT(C(3),C_INT),INT(o(3),C_INT), &
+              REAL(dx(3),C_DOUBLE),REAL(r0,C_DOUBLE),C_LOC(dv)),'fcgram_create_plan')
+         DEALLOCATE(dv)
+      ELSE
+         CALL rd_check(rd_plan_create(this%h,INT(lmax,C_INT),INT(N(1),C_INT), &
+              INT(N(2),C_INT),INT(N(3),C_INT),0_C_INT,0_C_INT, &
+              REAL(dx(3),C_DOUBLE),REAL(r0,C_DOUBLE),C_NULL_PTR),'fcgram_create_plan')
+      ENDIF
+      END SUBROUTINE fcgram_create_plan
+
+      SUBROUTINE fcgram_destroy_plan(this)
+      TYPE(FCPLAN), INTENT(INOUT) :: this
+      IF (ALLOCATED(this%z%dir)) DEALLOCATE(this%z%dir)
+      IF (C_ASSOCIATED(this%h)) CALL rd_check(rd_plan_destroy(this%h),'fcgram_destroy_plan')
+      this%h = C_NULL_PTR
+      END SUBROUTINE fcgram_destroy_plan
+
+! dir = A.Q^T from A{C}-{d}.dat and Q{d}.dat (fcgram_mod.f90:186-263).
+      SUBROUTINE load_dirichlet_tables(coord,tdir)
+      TYPE(FCCOORD), INTENT(INOUT) :: coord
+      CHARACTER(len=*), INTENT(IN) :: tdir
+      DOUBLE PRECISION, ALLOCATABLE :: dv(:)
+      IF (ALLOCATED(coord%dir)) THEN
+         PRINT*, "[WARNING] Multiple allocations ", &
+             "attempted at load_dir_tables. Skipping..."
+         RETURN
+      ENDIF
+      ALLOCATE(coord%dir(coord%C,coord%d), dv(coord%C*coord%d))
+      CALL rd_check(rd_load_dirichlet_tables(TRIM(tdir)//C_NULL_CHAR, &
+           INT(coord%C,C_INT),INT(coord%d,C_INT),dv),'load_dirichlet_tables')
+      coord%dir = RESHAPE(dv,(/coord%C,coord%d/))
+      DEALLOCATE(dv)
+      END SUBROUTINE load_dirichlet_tables
+
+! Waits for the plan's queued device work (each reference routine is
+! synchronous).
+      SUBROUTINE fc_sync(this)
+      TYPE(FCPLAN), INTENT(IN) :: this
+      CALL rd_check(rd_plan_sync(this%h),'device_sync')
+      END SUBROUTINE fc_sync
+
+      END MODULE fcgram
+
+!=================================================================
+      MODULE fft
+      USE fcgram
+      TYPE(FCPLAN), SAVE, TARGET :: planfc
+      END MODULE fft

@@ -33,6 +33,28 @@ def rell2(x, y):
     return float(np.linalg.norm((x - y).ravel()) / max(np.linalg.norm(y.ravel()), 1e-300))
 
 
+def rell2_joint(xs, ys):
+    """Relative L2 of several outputs taken as one vector."""
+    num = sum(float(np.linalg.norm((np.asarray(x) - np.asarray(y)).ravel())) ** 2 for x, y in zip(xs, ys))
+    den = sum(float(np.linalg.norm(np.asarray(y).ravel())) ** 2 for y in ys)
+    return (num / max(den, 1e-300)) ** 0.5
+
+
+def noise_floor(fn, inputs, seed=0, rel=2.0 ** -52):
+    """FP64 conditioning of fn at these inputs: the relative L2 change of each
+    output when every input gets independent relative noise of one ulp.  Two
+    correct FP64 implementations with different rounding differ by about this."""
+    rng = np.random.default_rng(seed)
+    base = fn(*inputs)
+    pert = [x * (1 + rel * rng.standard_normal(x.shape)) if np.iscomplexobj(x) is False
+            else x.real * (1 + rel * rng.standard_normal(x.shape)) + 1j * x.imag * (1 + rel * rng.standard_normal(x.shape))
+            for x in inputs]
+    out = fn(*pert)
+    if isinstance(base, (list, tuple)):
+        return [rell2(o, b) for o, b in zip(out, base)]
+    return rell2(out, base)
+
+
 def sectoral_c(l):
     """c = sqrt((2l+1)/(4 pi)) (2l-1)!!/sqrt((2l)!) (src/prueba_sphp.f90:165-174)."""
     c = np.sqrt((2 * l + 1) / (4 * PI))

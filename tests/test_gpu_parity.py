@@ -54,7 +54,7 @@ def test_grid_matches_oracle(lmax, nlat, nphi):
     oc = O.SHT(lmax, nlat, nphi)
     c2, s2, w2 = oc.grid()
     assert np.max(np.abs(ct - c2)) < 1e-15
-    assert np.max(np.abs(st - s2) / s2) < 1e-14
+    assert np.max(np.abs(st - s2) / s2) < 1e-15
     assert np.max(np.abs(wt - w2) / w2) < 1e-13
     assert np.array_equal(ell, H.ell_of(lmax)) and np.array_equal(emm, H.emm_of(lmax))
 
@@ -279,9 +279,14 @@ def test_prueba_pseudo_operators_vs_oracle():
     Cc = A.clone()
     S.fc_filter(plan, Cc, "r")
     out["fc_filter_change"] = rel(to_host(Cc), a)
+    # FP64 conditioning of the FC-Gram operators at this input (the continuation
+    # matrix has entries ~4e3): GPU and oracle may differ by a few times this
+    floor = H.noise_floor(lambda x: [fc.derivr_r(x, kz, 1), fc.derivr_r(x, kz, 2), fc.dlr(x, kz, z, ell)], [a])
     for k, v in out.items():
         print(f"{k:22s} {v:.2e}")
-    assert out["derivr1_vs_oracle"] < TOL and out["derivr2_vs_oracle"] < TOL and out["dlr_vs_oracle"] < TOL
+    print("noise floor (d1, d2, dlr):", ", ".join(f"{f:.2e}" for f in floor))
+    tol = [max(TOL, 30 * f) for f in floor]
+    assert out["derivr1_vs_oracle"] < tol[0] and out["derivr2_vs_oracle"] < tol[1] and out["dlr_vs_oracle"] < tol[2]
     assert out["derivr1_exact"] < 2 * 2.3e-7 and out["derivr2_exact"] < 2 * 3.6e-5 and out["dlr_exact"] < 2 * 2.8e-6
     assert out["derivr_k_vs_r"] < 1e-13 and out["laplang"] < 1e-15 and out["multr"] < 1e-15
     assert out["saxpby"] < 1e-15 and out["copy"] == 0.0 and out["fc_filter_change"] < 1e-9
@@ -342,7 +347,7 @@ def test_prueba_3d_periodic():
 
 
 # -------------------------------------------------------- nonlinear term
-@pytest.mark.parametrize("lmax,nr", [(15, 24), (31, 24), (63, 40)])
+@pytest.mark.parametrize("lmax,nr", [(15, 24), (31, 24), (63, 40), (127, 96)])
 def test_nonlinear_uxw_vs_oracle(lmax, nr):
     S = sphp()
     C, d, rint, rext = 25, 5, 0.35, 1.0
@@ -358,11 +363,16 @@ def test_nonlinear_uxw_vs_oracle(lmax, nr):
     Wd, Zd = to_dev(W), to_dev(Z)
     out = [torch.full_like(Wd, 3.0) for _ in range(3)]
     S.nonlinear_uxw(plan, Wd, Zd, *out)
-    errs = [H.rell2(to_host(o)[:nr], r[:nr]) for o, r in zip(out, ref)]
-    tail = max(float(np.max(np.abs(to_host(o)[nr:]))) for o in out)
+    got = [to_host(o) for o in out]
+    errs = [H.rell2(g[:nr], r[:nr]) for g, r in zip(got, ref)]
+    joint = H.rell2_joint([g[:nr] for g in got], [r[:nr] for r in ref])
+    floor = H.noise_floor(lambda w, zz: [x[:nr] for x in O.nonlinear_uxw(oc, fc, rint, dz, w, zz)], [W, Z])
+    tail = max(float(np.max(np.abs(g[nr:]))) for g in got)
     assert np.array_equal(to_host(Wd), W) and np.array_equal(to_host(Zd), Z)
-    print(f"lmax={lmax} nr={nr}: NQ {errs[0]:.2e} NS {errs[1]:.2e} NT {errs[2]:.2e}")
-    assert max(errs) < TOL and tail == 0.0
+    print(f"lmax={lmax} nr={nr}: joint {joint:.2e}; NQ {errs[0]:.2e} NS {errs[1]:.2e} NT {errs[2]:.2e}; "
+          f"noise floor NQ {floor[0]:.2e} NS {floor[1]:.2e} NT {floor[2]:.2e}")
+    assert joint < TOL and tail == 0.0
+    assert all(e < max(TOL, 30 * f) for e, f in zip(errs, floor))
 
 
 def test_nonlinear_solid_body_rotation():
