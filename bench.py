@@ -75,16 +75,45 @@ def cpu_baseline(W, Z, nlat, nphi, n, dz, gpu_out, budget_s=15.0):
     }, {"joint": errs[0], "NQ": errs[1], "NS": errs[2], "NT": errs[3], "shells": nsample}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters of a profiler run (each copy reads and writes nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/hbm_traffic_uxw_lmax255_nr128.json), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_uxw_lmax255_nr128.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this path runs on one GPU only; --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        copy_calibration()
+        return
 
     import torch
     import redondito_amd.sphp as S
@@ -161,7 +190,7 @@ def main():
                    "nphi": nphi, "nr": NR, "fcgram_C": C, "fcgram_d": D, "radial_fft_len": n,
                    "form": "fp64"},
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                     "frac": round(achieved / peak, 4), "traffic": None},
+                     "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "eval_roofline_frac": round((t_roof + t_roof_leg) / (ms_step * 1e-3), 4),
         "rel_err_vs_fp64": 0.0,

@@ -129,6 +129,7 @@ __device__ void fft_stage(const double2 *a, double2 *b, int nb, int n, int Ns, i
             // direct R-point DFT: the inputs stay in registers, the roots come
             // from the LDS table, one output at a time
             const int e = n / R;
+#pragma unroll 1
             for (int q = 0; q < R; q++) {
                 double2 acc = v[0];
                 int idx = 0;
@@ -204,13 +205,14 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftSeq &seq, i
 }
 
 // The register-DFT radix of a length: its largest prime factor if that is
-// one of 5, 7, 11, 13, 17, 19, else 0.
+// one of 5, 7, 11, 13, else 0 (the generic stage is faster for 17 and 19,
+// whose register DFTs need too many VGPRs).
 int big_radix(const FftSeq &seq)
 {
     int rb = 0;
     for (int s = 0; s < seq.nst; s++) {
         const int R = seq.rad[s];
-        if (R == 5 || R == 7 || R == 11 || R == 13 || R == 17 || R == 19) rb = R > rb ? R : rb;
+        if (R == 5 || R == 7 || R == 11 || R == 13) rb = R > rb ? R : rb;
     }
     return rb;
 }
@@ -422,11 +424,189 @@ int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f
     return RD_OK;
 }
 
+// ------------------------------------------------ phi / cross, nphi = 768
+// The same step for nphi = 3 x 16 x 16 with at most 256 stored m:
+//   inverse: m = 256 a + b, phi index j = 3 q + r.  The middle block a = 1 of
+//   the padded spectrum is zero, so the radix-3 step is one complex FMA,
+//   fused with the loads:  Y_r[b] = (Z[b] + w3^{2r} Z[512 + b]) w768^{b r};
+//   then z[3 q + r] = IDFT_256(Y_r)[q], run as 16 x 16 with both 16-point
+//   DFTs in registers and one LDS transpose between them.
+//   forward: decimation in time j = 3 n1 + n2, the two 256-point DFTs again
+//   as 16 x 16, and only the outputs m <= lmax (and their mirrors) are
+//   formed.
+// Seven barriers per row instead of one per Stockham pass.
+
+// Length-16 DFT in registers, out[k] = sum_j v[j] exp(sign 2 pi i j k / 16),
+// as 4 x 4 with the w16 twiddles as constants.
+__device__ __forceinline__ void dft4_inplace(double2 &a, double2 &b, double2 &c, double2 &d, int sign)
+{
+    const double2 s02 = cadd(a, c), d02 = csub(a, c);
+    const double2 s13 = cadd(b, d), d13 = cmuli(csub(b, d), sign);
+    a = cadd(s02, s13);
+    c = csub(s02, s13);
+    b = cadd(d02, d13);
+    d = csub(d02, d13);
+}
+
+__device__ __forceinline__ void dft16(double2 (&v)[16], int sign)
+{
+    // step 1: 4-point DFTs over j1 for each j0 (elements v[4 j1 + j0])
+#pragma unroll
+    for (int j0 = 0; j0 < 4; j0++) dft4_inplace(v[j0], v[4 + j0], v[8 + j0], v[12 + j0], sign);
+    // after step 1, v[4 k0 + j0] holds A[j0][k0]; twiddle by w16^(j0 k0)
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, c2 = 0.70710678118654752440;
+    const double sg = sign > 0 ? 1.0 : -1.0;
+    const double2 w1 = make_double2(c1, sg * s1), w2 = make_double2(c2, sg * c2), w3 = make_double2(s1, sg * c1);
+    const double2 w6 = make_double2(-c2, sg * c2), w9 = make_double2(-c1, -sg * s1);
+    v[4 + 1] = cmul(v[4 + 1], w1);   // j0 = 1, k0 = 1
+    v[8 + 1] = cmul(v[8 + 1], w2);   // j0 = 1, k0 = 2
+    v[12 + 1] = cmul(v[12 + 1], w3); // j0 = 1, k0 = 3
+    v[4 + 2] = cmul(v[4 + 2], w2);   // j0 = 2, k0 = 1
+    v[8 + 2] = cmuli(v[8 + 2], sign); // j0 = 2, k0 = 2: w16^4 = sign i
+    v[12 + 2] = cmul(v[12 + 2], w6); // j0 = 2, k0 = 3
+    v[4 + 3] = cmul(v[4 + 3], w3);   // j0 = 3, k0 = 1
+    v[8 + 3] = cmul(v[8 + 3], w6);   // j0 = 3, k0 = 2
+    v[12 + 3] = cmul(v[12 + 3], w9); // j0 = 3, k0 = 3
+    // step 3: 4-point DFTs over j0 for each k0 -> out[k0 + 4 k1]
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0++) dft4_inplace(v[4 * k0], v[4 * k0 + 1], v[4 * k0 + 2], v[4 * k0 + 3], sign);
+    // now v[4 k0 + k1] = out[k0 + 4 k1]; reorder to natural order
+    double2 o[16];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0++)
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) o[k0 + 4 * k1] = v[4 * k0 + k1];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = o[i];
+}
+
+__global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                           const double2 *tw)
+{
+    constexpr int N = 768, NH = 384;
+    __shared__ double2 twl[NH];
+    __shared__ double2 bufA[9 * 256];
+    __shared__ double2 bufB[3 * 768];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    for (int i = threadIdx.x; i < NH; i += blockDim.x) twl[i] = tw[i];
+    // the row's six coefficient vectors, m = 0..255 (zero above lmax), into bufB
+    for (int e = threadIdx.x; e < 6 * 256; e += blockDim.x) {
+        const int f = e >> 8, m = e & 255;
+        double2 v = make_double2(0.0, 0.0);
+        if (m <= lmax) v = P.F[f][((long)k * L + m) * nlat + th];
+        if (m == 0) v.y = 0.0;
+        bufB[e] = v;
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+
+    // ---- radix-3 step: Y_r[b] = (Z[b] + w3^{2r} Z[512 + b]) w768^{b r}, into bufA
+    for (int e = tid; e < 9 * 256; e += blockDim.x) {
+        const int xr = e >> 8, b = e & 255;
+        const int x = xr / 3, r = xr - 3 * x;
+        const double2 u = bufB[x * 256 + b], w = bufB[(3 + x) * 256 + b];
+        const double2 zb = make_double2(u.x - w.y, u.y + w.x);
+        const int mm = (256 - b) & 255;          // mirror of m = 512 + b (m = 512 is zero)
+        const double2 uc = bufB[x * 256 + mm], wc = bufB[(3 + x) * 256 + mm];
+        const double2 zc = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
+        const double2 w3 = twid(twl, (512 * r) % N, NH, +1);
+        bufA[e] = cmul(cadd(zb, cmul(zc, w3)), twid(twl, b * r, NH, +1));
+    }
+    __syncthreads();
+    // ---- inverse, first 16-point pass over b2 of b = 16 b2 + b1: bufA -> bufB
+    if (tid < 144) {
+        const int xr = tid / 16, b1 = tid % 16;
+        double2 v[16];
+#pragma unroll
+        for (int b2 = 0; b2 < 16; b2++) v[b2] = bufA[xr * 256 + 16 * b2 + b1];
+        dft16(v, +1);
+#pragma unroll
+        for (int q1 = 0; q1 < 16; q1++)
+            bufB[(xr * 16 + b1) * 16 + q1] = cmul(v[q1], twid(twl, 3 * b1 * q1, NH, +1));
+    }
+    __syncthreads();
+    // ---- inverse, second pass over b1 -> grid values u_x + i w_x at j = 3 q + r, into bufA
+    if (tid < 144) {
+        const int xr = tid / 16, q1 = tid % 16;
+        const int x = xr / 3, r = xr - 3 * x;
+        double2 v[16];
+#pragma unroll
+        for (int b1 = 0; b1 < 16; b1++) v[b1] = bufB[(xr * 16 + b1) * 16 + q1];
+        dft16(v, +1);
+#pragma unroll
+        for (int q2 = 0; q2 < 16; q2++) bufA[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
+    }
+    __syncthreads();
+    // ---- N = u x w (prodvec), packed as (N_r + i N_theta, N_phi)
+    for (int j = tid; j < N; j += blockDim.x) {
+        const double2 zr = bufA[j], zt = bufA[N + j], zp = bufA[2 * N + j];
+        const double nr = zt.x * zp.y - zp.x * zt.y;
+        const double nt = zp.x * zr.y - zr.x * zp.y;
+        const double np = zr.x * zt.y - zt.x * zr.y;
+        bufA[j] = make_double2(nr, nt);
+        bufA[N + j] = make_double2(np, 0.0);
+    }
+    __syncthreads();
+    // ---- forward, first pass over a2 of n1 = 16 a2 + a1, j = 3 n1 + n2
+    if (tid < 96) {
+        const int s = tid / 48, rem = tid % 48, n2 = rem / 16, a1 = rem % 16;
+        double2 v[16];
+#pragma unroll
+        for (int a2 = 0; a2 < 16; a2++) v[a2] = bufA[s * N + 3 * (16 * a2 + a1) + n2];
+        dft16(v, -1);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++)
+            bufB[((s * 3 + n2) * 16 + a1) * 16 + k1] = cmul(v[k1], twid(twl, 3 * a1 * k1, NH, -1));
+    }
+    __syncthreads();
+    // ---- forward, second pass over a1 -> Yf[s][n2][k1 + 16 k2]
+    if (tid < 96) {
+        const int s = tid / 48, rem = tid % 48, n2 = rem / 16, k1 = rem % 16;
+        double2 v[16];
+#pragma unroll
+        for (int a1 = 0; a1 < 16; a1++) v[a1] = bufB[((s * 3 + n2) * 16 + a1) * 16 + k1];
+        dft16(v, -1);
+#pragma unroll
+        for (int k2 = 0; k2 < 16; k2++) bufA[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
+    }
+    __syncthreads();
+    // ---- X_s[m] = sum_n2 w768^{-n2 m} Yf[s][n2][m mod 256], unpack, scale, store
+    const double sc = 0.5 * (2.0 * M_PI / N);
+    for (int m = tid; m < L; m += blockDim.x) {
+        const int mneg = (256 - m) & 255;
+        double2 xp = make_double2(0.0, 0.0), xm = xp, x1 = xp;
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++) {
+            xp = cadd(xp, cmul(bufA[n2 * 256 + m], twid(twl, n2 * m, NH, -1)));
+            xm = cadd(xm, cmul(bufA[n2 * 256 + mneg], twid(twl, n2 * m, NH, +1)));
+            x1 = cadd(x1, cmul(bufA[(3 + n2) * 256 + m], twid(twl, n2 * m, NH, -1)));
+        }
+        if (m == 0) xm = xp;
+        double2 gr = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
+        double2 gt = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
+        double2 gp = cscale(x1, 2.0 * sc);
+        if (m == 0) { gr.y = 0.0; gt.y = 0.0; gp.y = 0.0; }
+        const long o = ((long)k * L + m) * nlat + th;
+        P.Fo[0][o] = gr;
+        P.Fo[1][o] = gt;
+        P.Fo[2][o] = gp;
+    }
+}
+
 int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout)
 {
     PhiPtrs P = {};
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
     for (int i = 0; i < 3; i++) P.Fo[i] = Fout[i];
+    if (p->nphi == 768 && p->lmax <= 255) {
+        hipLaunchKernelGGL(phi_cross768_kernel, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P, p->lmax,
+                           p->nlat, nplanes, p->tw_phi);
+        RD_LAUNCH_CHECK();
+        return RD_OK;
+    }
     size_t shm = sizeof(double2) * (6 * p->nphi + p->nphi / 2);
     {
         int lrc = allow_lds(phi_cross_kernel, shm);
@@ -534,21 +714,21 @@ __global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, FftSeq seq, c
     z_store(R2, A.a, A.ld, c0, A.ncols, n, n, 1.0);
 }
 
-static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order)
+static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order, int ncols, long ld)
 {
     ZArgs A;
     A.a = a;
     A.ain = ain;
     A.dir = p->d_dir;
     A.kz = p->d_kz;
-    A.ld = p->nlm;
-    A.ncols = p->nlm;
+    A.ld = ld;
+    A.ncols = ncols;
     A.n = p->nz;
     A.C = p->C;
     A.d = p->d;
     A.order = order;
     size_t shm = sizeof(double2) * (2 * ZTC + 1) * p->nz;
-    const dim3 grid((p->nlm + ZTC - 1) / ZTC), block(512);
+    const dim3 grid((ncols + ZTC - 1) / ZTC), block(512);
 #define RD_Z_CASE(RBV)                                                                         \
     case RBV: {                                                                                \
         int lrc = allow_lds(z_kernel<RBV>, shm);                                               \
@@ -571,9 +751,16 @@ static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order
     return RD_OK;
 }
 
-int launch_z_forward(Plan *p, double2 *a) { return z_launch(p, 0, a, nullptr, 0); }
-int launch_z_inverse(Plan *p, double2 *a) { return z_launch(p, 1, a, nullptr, 0); }
-int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b) { return z_launch(p, 2, b, a, order); }
+int launch_z_forward(Plan *p, double2 *a) { return z_launch(p, 0, a, nullptr, 0, p->nlm, p->nlm); }
+int launch_z_inverse(Plan *p, double2 *a) { return z_launch(p, 1, a, nullptr, 0, p->nlm, p->nlm); }
+int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b)
+{
+    return z_launch(p, 2, b, a, order, p->nlm, p->nlm);
+}
+int launch_derivr_cols(Plan *p, int order, const double2 *a, double2 *b, int ncols, long ld)
+{
+    return z_launch(p, 2, b, a, order, ncols, ld);
+}
 
 // Radial step of the nonlinear term for a tile of lm columns:
 //   W^ = FFT(continue(W))/n, Z^ likewise;
@@ -590,68 +777,73 @@ struct UxwRadArgs {
     int ncols, n, C, d;
 };
 
-__device__ void spec_mul_ikz(const double2 *S, double2 *D, const double *kz, int n, int p)
+constexpr int UTC = 4;   // lm columns per workgroup in the fused radial step
+
+// Columns c of a batch held in LDS at X + c*n: rows nr..n-1 from the FC-Gram
+// continuation of rows 0..d-1 and nr-d..nr-1.
+__device__ void z_load_cols(double2 *X, const double2 *a, long ld, int c0, int ncols, int n, int rows,
+                            int nc)
 {
-    for (int e = threadIdx.x; e < ZTC * n; e += blockDim.x) {
-        const int r = e % n;
-        double kp = 1.0;
-        for (int q = 0; q < p; q++) kp *= kz[r];
-        double2 v = cscale(S[e], kp);
-        if (p == 1) v = make_double2(-v.y, v.x);
-        else if (p == 2) v = make_double2(-v.x, -v.y);
-        D[e] = v;
+    for (int e = threadIdx.x; e < rows * nc; e += blockDim.x) {
+        const int r = e / nc, c = e - r * nc;
+        X[c * n + r] = (c0 + c < ncols) ? a[(long)r * ld + c0 + c] : make_double2(0.0, 0.0);
     }
 }
 
+// Both forward transforms (W and Z columns) run as one batch, the three
+// derivative spectra (i kz W^, -kz^2 W^, i kz Z^) as one inverse batch.
 template <int RB>
-__global__ __launch_bounds__(512) void uxw_radial_kernel(UxwRadArgs A, FftSeq seq, const double2 *tw)
+__global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq seq, const double2 *tw)
 {
     extern __shared__ double2 sm[];
-    const int n = A.n, c0 = blockIdx.x * ZTC, nr = n - A.C;
-    double2 *X = sm, *Y = sm + ZTC * n, *S = sm + 2 * ZTC * n, *twl = sm + 3 * ZTC * n;
+    const int n = A.n, nr = n - A.C;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * UTC;
+    double2 *P = sm, *Q = sm + 3 * UTC * n, *twl = sm + 6 * UTC * n;
     const int nh = load_twiddles(twl, tw, n);
+    const int rows = A.C > 0 ? nr : n;
+    z_load_cols(P, A.W, A.ld, c0, A.ncols, n, rows, UTC);
+    z_load_cols(P + UTC * n, A.Z, A.ld, c0, A.ncols, n, rows, UTC);
+    __syncthreads();
+    if (A.C > 0) {
+        fc_continue(P, 2 * UTC, n, A.C, A.d, A.dir);
+        __syncthreads();
+    }
+    double2 *R = lds_fft<RB>(P, Q, 2 * UTC, seq, -1, twl, nh);
+    double2 *I = (R == P) ? Q : P;
+    // spectra / n times (i kz)^p: sequences [0,UTC) dW, [UTC,2UTC) d2W, [2UTC,3UTC) dZ
     const double inv_n = 1.0 / n;
-    for (int fld = 0; fld < 2; fld++) {
-        const double2 *src = fld == 0 ? A.W : A.Z;
-        z_load(X, src, A.ld, c0, A.ncols, n, A.C > 0 ? nr : n);
-        __syncthreads();
-        if (A.C > 0) {
-            fc_continue(X, ZTC, n, A.C, A.d, A.dir);
-            __syncthreads();
+    for (int e = threadIdx.x; e < 3 * UTC * n; e += blockDim.x) {
+        const int sq = e / n, r = e - sq * n;
+        const int kind = sq / UTC, c = sq - kind * UTC;
+        const double k = A.kz[r];
+        double2 v = R[(kind == 2 ? UTC + c : c) * n + r];
+        if (kind == 1) {
+            v = cscale(v, -k * k * inv_n);
+        } else {
+            v = cscale(v, k * inv_n);
+            v = make_double2(-v.y, v.x);
         }
-        double2 *R = lds_fft<RB>(X, Y, ZTC, seq, -1, twl, nh);
-        for (int e = threadIdx.x; e < ZTC * n; e += blockDim.x) S[e] = cscale(R[e], inv_n);
-        __syncthreads();
-        const int nder = fld == 0 ? 2 : 1;
-        for (int p = 1; p <= nder; p++) {
-            spec_mul_ikz(S, X, A.kz, n, p);
-            __syncthreads();
-            double2 *D = lds_fft<RB>(X, Y, ZTC, seq, +1, twl, nh);
-            for (int e = threadIdx.x; e < nr * ZTC; e += blockDim.x) {
-                const int r = e / ZTC, c = e - r * ZTC;
-                const int lm = c0 + c;
-                if (lm >= A.ncols) continue;
-                const double rr = A.z[r], r1 = 1.0 / rr, r2 = 1.0 / (rr * rr);
-                const long o = (long)r * A.ld + lm;
-                const double2 dv = D[c * n + r];
-                if (fld == 0 && p == 1) {
-                    A.out[1][o] = cscale(dv, r1);               // u: S
-                } else if (fld == 0 && p == 2) {
-                    const double ll = (double)A.ell[lm] * (A.ell[lm] + 1);
-                    const double2 w = A.W[o];
-                    const double2 dl = make_double2(dv.x - ll * w.x * r2, dv.y - ll * w.y * r2);
-                    A.out[5][o] = cscale(dl, -r1);              // omega: T
-                    A.out[0][o] = cscale(w, ll * r2);           // u: Q
-                } else {
-                    const double ll = (double)A.ell[lm] * (A.ell[lm] + 1);
-                    const double2 zz = A.Z[o];
-                    A.out[4][o] = cscale(dv, r1);               // omega: S
-                    A.out[2][o] = cscale(zz, r1);               // u: T
-                    A.out[3][o] = cscale(zz, ll * r2);          // omega: Q
-                }
-            }
-            __syncthreads();
-        }
+        I[e] = v;
+    }
+    __syncthreads();
+    double2 *D = lds_fft<RB>(I, R, 3 * UTC, seq, +1, twl, nh);
+    // u:     Q = l(l+1) W / r^2, S = dW / r, T = Z / r
+    // omega: Q = l(l+1) Z / r^2, S = dZ / r, T = -(d2W - l(l+1) W / r^2) / r
+    for (int e = threadIdx.x; e < nr * UTC; e += blockDim.x) {
+        const int r = e / UTC, c = e - r * UTC;
+        const int lm = c0 + c;
+        if (lm >= A.ncols) continue;
+        const double rr = A.z[r], r1 = 1.0 / rr, r2 = 1.0 / (rr * rr);
+        const double ll = (double)A.ell[lm] * (A.ell[lm] + 1);
+        const long o = (long)r * A.ld + lm;
+        const double2 w = A.W[o], zz = A.Z[o];
+        const double2 dw = D[c * n + r], d2w = D[(UTC + c) * n + r], dz = D[(2 * UTC + c) * n + r];
+        A.out[0][o] = cscale(w, ll * r2);
+        A.out[1][o] = cscale(dw, r1);
+        A.out[2][o] = cscale(zz, r1);
+        A.out[3][o] = cscale(zz, ll * r2);
+        A.out[4][o] = cscale(dz, r1);
+        A.out[5][o] = cscale(make_double2(d2w.x - ll * w.x * r2, d2w.y - ll * w.y * r2), -r1);
     }
 }
 
@@ -670,8 +862,8 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     A.n = p->nz;
     A.C = p->C;
     A.d = p->d;
-    size_t shm = sizeof(double2) * (3 * ZTC + 1) * p->nz;
-    const dim3 grid((p->nlm + ZTC - 1) / ZTC), block(512);
+    size_t shm = sizeof(double2) * (6 * UTC + 1) * p->nz;
+    const dim3 grid((p->nlm + UTC - 1) / UTC), block(256);
 #define RD_U_CASE(RBV)                                                                          \
     case RBV: {                                                                                 \
         int lrc = allow_lds(uxw_radial_kernel<RBV>, shm);                                       \

@@ -45,6 +45,10 @@ struct Plan {
     // FFTs
     FftSeq seq_phi, seq_z;
     double2 *tw_phi, *tw_z;        // exp(-2 pi i t / n)
+    // FC-Gram derivative matrices on the physical shells, transposed and
+    // zero-padded to nrp = nr rounded up to 16: DT[k][r] = D_p[r][k]
+    double *d_DT1, *d_DT2;
+    int nrp;
     // workspace
     double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
     double2 *S[6];                 // spectral scratch (nlm, nz)
@@ -85,6 +89,8 @@ int launch_z_inverse(Plan *p, double2 *a);
 int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b);
 // Fused radial step of the nonlinear term: W, Z -> the six (Q,S,T) arrays.
 int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst);
+// Radial derivative of order p of ncols columns with plane stride ld.
+int launch_derivr_cols(Plan *p, int order, const double2 *a, double2 *b, int ncols, long ld);
 
 // Pointwise operators.
 int launch_derivr_k(Plan *p, int order, const double2 *a, double2 *b);

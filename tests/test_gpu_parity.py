@@ -347,7 +347,7 @@ def test_prueba_3d_periodic():
 
 
 # -------------------------------------------------------- nonlinear term
-@pytest.mark.parametrize("lmax,nr", [(15, 24), (31, 24), (63, 40), (127, 96)])
+@pytest.mark.parametrize("lmax,nr", [(15, 24), (31, 24), (63, 40), (127, 96), (255, 12)])
 def test_nonlinear_uxw_vs_oracle(lmax, nr):
     S = sphp()
     C, d, rint, rext = 25, 5, 0.35, 1.0
