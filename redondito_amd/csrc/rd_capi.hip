@@ -33,7 +33,7 @@ void free_plan(Plan *p)
     if (!p) return;
     void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
                    p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
-                   p->d_ana_off, p->d_ana_ncol, p->d_ana_tiles, p->tw_phi, p->tw_z, p->d_DT1, p->d_DT2};
+                   p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z};
     for (void *d : dev)
         if (d) (void)hipFree(d);
     for (int i = 0; i < 6; i++) {

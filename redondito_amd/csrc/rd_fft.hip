@@ -73,16 +73,6 @@ __device__ int load_twiddles(double2 *dst, const double2 *tw, int n)
     return nh;
 }
 
-// Workgroup index remapped so that consecutive logical tiles run on the same
-// XCD (the hardware deals workgroups round-robin over the 8 XCDs, each with
-// its own L2): neighbouring tiles then share cache lines in one L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
-{
-    const int q = nblocks / 8, r = nblocks % 8;
-    const int x = bid % 8, i = bid / 8;
-    return x * q + (x < r ? x : r) + i;
-}
-
 // One radix-R Stockham stage over nb sequences of length n (a -> b).
 template <int R>
 __device__ void fft_stage(const double2 *a, double2 *b, int nb, int n, int Ns, int sign,
@@ -480,18 +470,32 @@ __device__ __forceinline__ void dft16(double2 (&v)[16], int sign)
     for (int i = 0; i < 16; i++) v[i] = o[i];
 }
 
+// exp(sign 2 pi i t / 768) from a quarter table of exp(-2 pi i t / 768),
+// t < 192, using w(t + 192 q) = (-i)^q w(t).
+__device__ __forceinline__ double2 twq(const double2 *tw, int t, int sign)
+{
+    const int q = t / 192;
+    const double2 w = tw[t - 192 * q];
+    double2 r;
+    if (q == 0) r = w;
+    else if (q == 1) r = make_double2(w.y, -w.x);
+    else if (q == 2) r = make_double2(-w.x, -w.y);
+    else r = make_double2(-w.y, w.x);
+    return sign > 0 ? make_double2(r.x, -r.y) : r;
+}
+
 __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                            const double2 *tw)
 {
-    constexpr int N = 768, NH = 384;
-    __shared__ double2 twl[NH];
+    constexpr int N = 768, NQ = 192;
+    __shared__ double2 twl[NQ];
     __shared__ double2 bufA[9 * 256];
-    __shared__ double2 bufB[3 * 768];
+    __shared__ double2 bufB[144 * 17];   // 16 x 16 transposes padded to 17
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int k = t / nlat, th = t % nlat;
     if (k >= nplanes) return;
     const int L = lmax + 1;
-    for (int i = threadIdx.x; i < NH; i += blockDim.x) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < NQ; i += blockDim.x) twl[i] = tw[i];
     // the row's six coefficient vectors, m = 0..255 (zero above lmax), into bufB
     for (int e = threadIdx.x; e < 6 * 256; e += blockDim.x) {
         const int f = e >> 8, m = e & 255;
@@ -512,8 +516,8 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         const int mm = (256 - b) & 255;          // mirror of m = 512 + b (m = 512 is zero)
         const double2 uc = bufB[x * 256 + mm], wc = bufB[(3 + x) * 256 + mm];
         const double2 zc = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
-        const double2 w3 = twid(twl, (512 * r) % N, NH, +1);
-        bufA[e] = cmul(cadd(zb, cmul(zc, w3)), twid(twl, b * r, NH, +1));
+        const double2 w3 = twq(twl, (512 * r) % N, +1);
+        bufA[e] = cmul(cadd(zb, cmul(zc, w3)), twq(twl, b * r, +1));
     }
     __syncthreads();
     // ---- inverse, first 16-point pass over b2 of b = 16 b2 + b1: bufA -> bufB
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         dft16(v, +1);
 #pragma unroll
         for (int q1 = 0; q1 < 16; q1++)
-            bufB[(xr * 16 + b1) * 16 + q1] = cmul(v[q1], twid(twl, 3 * b1 * q1, NH, +1));
+            bufB[(xr * 16 + b1) * 17 + q1] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
     }
     __syncthreads();
     // ---- inverse, second pass over b1 -> grid values u_x + i w_x at j = 3 q + r, into bufA
@@ -534,7 +538,7 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         const int x = xr / 3, r = xr - 3 * x;
         double2 v[16];
 #pragma unroll
-        for (int b1 = 0; b1 < 16; b1++) v[b1] = bufB[(xr * 16 + b1) * 16 + q1];
+        for (int b1 = 0; b1 < 16; b1++) v[b1] = bufB[(xr * 16 + b1) * 17 + q1];
         dft16(v, +1);
 #pragma unroll
         for (int q2 = 0; q2 < 16; q2++) bufA[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         dft16(v, -1);
 #pragma unroll
         for (int k1 = 0; k1 < 16; k1++)
-            bufB[((s * 3 + n2) * 16 + a1) * 16 + k1] = cmul(v[k1], twid(twl, 3 * a1 * k1, NH, -1));
+            bufB[((s * 3 + n2) * 16 + a1) * 17 + k1] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
     }
     __syncthreads();
     // ---- forward, second pass over a1 -> Yf[s][n2][k1 + 16 k2]
@@ -567,7 +571,7 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         const int s = tid / 48, rem = tid % 48, n2 = rem / 16, k1 = rem % 16;
         double2 v[16];
 #pragma unroll
-        for (int a1 = 0; a1 < 16; a1++) v[a1] = bufB[((s * 3 + n2) * 16 + a1) * 16 + k1];
+        for (int a1 = 0; a1 < 16; a1++) v[a1] = bufB[((s * 3 + n2) * 16 + a1) * 17 + k1];
         dft16(v, -1);
 #pragma unroll
         for (int k2 = 0; k2 < 16; k2++) bufA[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
@@ -580,9 +584,9 @@ __global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, 
         double2 xp = make_double2(0.0, 0.0), xm = xp, x1 = xp;
 #pragma unroll
         for (int n2 = 0; n2 < 3; n2++) {
-            xp = cadd(xp, cmul(bufA[n2 * 256 + m], twid(twl, n2 * m, NH, -1)));
-            xm = cadd(xm, cmul(bufA[n2 * 256 + mneg], twid(twl, n2 * m, NH, +1)));
-            x1 = cadd(x1, cmul(bufA[(3 + n2) * 256 + m], twid(twl, n2 * m, NH, -1)));
+            xp = cadd(xp, cmul(bufA[n2 * 256 + m], twq(twl, n2 * m, -1)));
+            xm = cadd(xm, cmul(bufA[n2 * 256 + mneg], twq(twl, n2 * m, +1)));
+            x1 = cadd(x1, cmul(bufA[(3 + n2) * 256 + m], twq(twl, n2 * m, -1)));
         }
         if (m == 0) xm = xp;
         double2 gr = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));

@@ -20,6 +20,18 @@ struct FftSeq {
 
 FftSeq make_fft_seq(int n);
 
+// Workgroup index remapped so that consecutive logical tiles run on the same
+// XCD (the hardware deals workgroups round-robin over the 8 XCDs, each with
+// its own L2): tiles that share operands then share one L2.  Only for grids
+// whose tiles cost the same; uneven work goes through per-XCD queues.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
+{
+    const int q = nblocks / 8, r = nblocks % 8;
+    const int x = bid % 8, i = bid / 8;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // Sizes and device tables of one plan.
 struct Plan {
     int lmax, L, nlat, nphi, nlm, nth, nthp, nthp16;
@@ -40,15 +52,13 @@ struct Plan {
     long *d_syn_off, *d_ana_off;   // per (m,p)
     int *d_ana_ncol;               // per m
     long syn_rows, ana_elems;
-    int *d_ana_tiles;              // (m, l-tile) pairs of the analysis grid
-    int n_ana_tiles;
+    // Per-XCD work queues, stored interleaved (entry e of XCD x at e*8+x, -1 =
+    // empty): synthesis entries m | tt<<16, analysis entries m | lt<<16.
+    int *d_syn_q, *d_ana_q;
+    int syn_qlen, ana_qlen;
     // FFTs
     FftSeq seq_phi, seq_z;
     double2 *tw_phi, *tw_z;        // exp(-2 pi i t / n)
-    // FC-Gram derivative matrices on the physical shells, transposed and
-    // zero-padded to nrp = nr rounded up to 16: DT[k][r] = D_p[r][k]
-    double *d_DT1, *d_DT2;
-    int nrp;
     // workspace
     double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
     double2 *S[6];                 // spectral scratch (nlm, nz)

@@ -18,6 +18,7 @@
 // lane group).
 #include "rd_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -152,13 +153,55 @@ __global__ void legendre_tables_kernel(int lmax, int nth, int nthp, int nthp16, 
 
 static inline int rup(int a, int b) { return (a + b - 1) / b * b; }
 
+// Interleave per-XCD queues: entry e of queue x goes to e*kXcds + x, shorter
+// queues padded with -1.  Workgroup b then runs on XCD b % kXcds.
+static std::vector<int> interleave_queues(const std::vector<std::vector<int>> &q, int *qlen)
+{
+    size_t len = 0;
+    for (const auto &v : q) len = std::max(len, v.size());
+    std::vector<int> out(len * kXcds, -1);
+    for (int x = 0; x < kXcds; x++)
+        for (size_t e = 0; e < q[x].size(); e++) out[e * kXcds + x] = q[x][e];
+    *qlen = (int)len;
+    return out;
+}
+
+// Synthesis: the work of a tile of order m is proportional to L - m, so m and
+// L-1-m are paired into chunks of equal cost, dealt round-robin to the XCDs.
+static std::vector<int> synthesis_queues(int L, int n_tt, int *qlen)
+{
+    std::vector<std::vector<int>> q(kXcds);
+    int c = 0;
+    for (int a = 0, b = L - 1; a <= b; a++, b--, c++) {
+        auto &dst = q[c % kXcds];
+        for (int tt = 0; tt < n_tt; tt++) dst.push_back(a | (tt << 16));
+        if (b != a)
+            for (int tt = 0; tt < n_tt; tt++) dst.push_back(b | (tt << 16));
+    }
+    return interleave_queues(q, qlen);
+}
+
+// Analysis: tiles cost the same; all l-tiles of one m share its Fourier
+// coefficients, so whole m groups go to the least-loaded XCD (largest first).
+static std::vector<int> analysis_queues(const std::vector<int> &ncol, int *qlen)
+{
+    const int L = (int)ncol.size();
+    std::vector<std::vector<int>> q(kXcds);
+    for (int m = 0; m < L; m++) {   // ncol is non-increasing in m
+        int x = 0;
+        for (int y = 1; y < kXcds; y++)
+            if (q[y].size() < q[x].size()) x = y;
+        for (int lt = 0; lt < ncol[m] / 32; lt++) q[x].push_back(m | (lt << 16));
+    }
+    return interleave_queues(q, qlen);
+}
+
 int build_legendre_tables(Plan *p)
 {
     const int L = p->L;
     std::vector<long> syn_off(2 * L), ana_off(2 * L);
     std::vector<int> ana_ncol(L);
     long rows = 0, elems = 0;
-    std::vector<int> tiles;
     for (int m = 0; m < L; m++) {
         int ne = (L - m + 1) / 2;             // l - m even count (>= odd count)
         int rowsm = rup(ne, 16);
@@ -170,11 +213,11 @@ int build_legendre_tables(Plan *p)
         ana_off[2 * m] = elems;
         ana_off[2 * m + 1] = elems + (long)p->nthp16 * ncol;
         elems += 2L * p->nthp16 * ncol;
-        for (int lt = 0; lt < ncol / 32; lt++) tiles.push_back(m | (lt << 16));
     }
     p->syn_rows = rows;
     p->ana_elems = elems;
-    p->n_ana_tiles = (int)tiles.size();
+    std::vector<int> synq = synthesis_queues(L, p->nthp / 64, &p->syn_qlen);
+    std::vector<int> anaq = analysis_queues(ana_ncol, &p->ana_qlen);
     size_t sb = sizeof(double) * rows * p->nthp, ab = sizeof(double) * elems;
     RD_HIP(hipMalloc(&p->syn_P, sb));
     RD_HIP(hipMalloc(&p->syn_D, sb));
@@ -185,7 +228,8 @@ int build_legendre_tables(Plan *p)
     RD_HIP(hipMalloc(&p->d_syn_off, sizeof(long) * 2 * L));
     RD_HIP(hipMalloc(&p->d_ana_off, sizeof(long) * 2 * L));
     RD_HIP(hipMalloc(&p->d_ana_ncol, sizeof(int) * L));
-    RD_HIP(hipMalloc(&p->d_ana_tiles, sizeof(int) * tiles.size()));
+    RD_HIP(hipMalloc(&p->d_syn_q, sizeof(int) * synq.size()));
+    RD_HIP(hipMalloc(&p->d_ana_q, sizeof(int) * anaq.size()));
     hipStream_t st = p->stream;
     RD_HIP(hipMemsetAsync(p->syn_P, 0, sb, st));
     RD_HIP(hipMemsetAsync(p->syn_D, 0, sb, st));
@@ -196,7 +240,8 @@ int build_legendre_tables(Plan *p)
     RD_HIP(hipMemcpyAsync(p->d_syn_off, syn_off.data(), sizeof(long) * 2 * L, hipMemcpyHostToDevice, st));
     RD_HIP(hipMemcpyAsync(p->d_ana_off, ana_off.data(), sizeof(long) * 2 * L, hipMemcpyHostToDevice, st));
     RD_HIP(hipMemcpyAsync(p->d_ana_ncol, ana_ncol.data(), sizeof(int) * L, hipMemcpyHostToDevice, st));
-    RD_HIP(hipMemcpyAsync(p->d_ana_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
+    RD_HIP(hipMemcpyAsync(p->d_syn_q, synq.data(), sizeof(int) * synq.size(), hipMemcpyHostToDevice, st));
+    RD_HIP(hipMemcpyAsync(p->d_ana_q, anaq.data(), sizeof(int) * anaq.size(), hipMemcpyHostToDevice, st));
 
     // cosine-series coefficients of P_nlat (host FP64, shared with the nodes kernel)
     std::vector<double> g(p->nlat + 1), h(p->nlat + 1);
@@ -238,10 +283,11 @@ struct SynCfg {
 struct SynArgs {
     const double *tP, *tD, *tM;
     const long *off;
+    const int *queue;
     const double2 *in[3];
     double2 *out[3];
     long ld;
-    int lmax, nlat, nth, nthp, nplanes, n_tt, n_jt;
+    int lmax, nlat, nth, nthp, nplanes, n_jt;
 };
 
 template <bool VEC>
@@ -253,11 +299,13 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
-    int bid = blockIdx.x;
-    const int jt = bid % A.n_jt;
-    bid /= A.n_jt;
-    const int tt = bid % A.n_tt;
-    const int m = bid / A.n_tt;
+    // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
+    // back to back on one XCD, so its table rows are read from HBM once
+    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int jt = i % A.n_jt;
+    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    if (ent < 0) return;
+    const int m = ent & 0xffff, tt = ent >> 16;
     const int k0 = jt * (JT / 2);
     const int th0 = tt * 64 + wave * 16;
     const int col = lane & 15, kk = lane >> 4;
@@ -370,10 +418,10 @@ int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const dou
     A.nth = p->nth;
     A.nthp = p->nthp;
     A.nplanes = nplanes;
-    A.n_tt = p->nthp / 64;
+    A.queue = p->d_syn_q;
     const int JT = vec ? SynCfg<true>::JT : SynCfg<false>::JT;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
-    dim3 grid((unsigned)p->L * A.n_tt * A.n_jt);
+    dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
     if (vec)
         hipLaunchKernelGGL(synthesis_kernel<true>, grid, dim3(256), 0, p->stream, A);
     else
@@ -400,7 +448,7 @@ struct AnaArgs {
     const double *tP, *tD, *tM;
     const long *off;
     const int *ncol;
-    const int *tiles;
+    const int *queue;
     const double2 *in[3];
     double2 *out[3];
     long ld;
@@ -417,10 +465,13 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
-    int bid = blockIdx.x;
-    const int jt = bid % A.n_jt;
-    const int tile = A.tiles[bid / A.n_jt];
-    const int m = tile & 0xffff, lt = tile >> 16;
+    // per-XCD queue entry (m, l tile); the plane tiles of one entry run back
+    // to back on one XCD
+    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int jt = i % A.n_jt;
+    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    if (ent < 0) return;
+    const int m = ent & 0xffff, lt = ent >> 16;
     const int wl = wave & 1, wj = wave >> 1;
     const int k0 = jt * (JT / 2);
     const int col = lane & 15, kk = lane >> 4;
@@ -530,7 +581,7 @@ int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const dou
     A.tM = p->ana_M;
     A.off = p->d_ana_off;
     A.ncol = p->d_ana_ncol;
-    A.tiles = p->d_ana_tiles;
+    A.queue = p->d_ana_q;
     A.in[0] = g0;
     A.in[1] = vec ? g1 : g0;
     A.in[2] = vec ? g2 : g0;
@@ -545,7 +596,7 @@ int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const dou
     A.nplanes = nplanes;
     const int JT = vec ? AnaCfg<true>::JT : AnaCfg<false>::JT;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
-    dim3 grid((unsigned)p->n_ana_tiles * A.n_jt);
+    dim3 grid((unsigned)(kXcds * p->ana_qlen * A.n_jt));
     if (vec)
         hipLaunchKernelGGL(analysis_kernel<true>, grid, dim3(256), 0, p->stream, A);
     else
