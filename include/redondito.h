@@ -185,6 +185,39 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
 /* a(lm,1) = a(lm,nz-C) = 0.  Replaces dirichlet, boundary_mod.f90:62-93. */
 int rd_dirichlet(rd_plan *plan, double *a);
 
+#define RD_BC_NOSLIP 0      /* 'noslip': W = dW/dr = Z = 0 (vboundary.f90:10-31) */
+
+/* FC-Gram radial derivative matrix of order p on the N = nr physical points,
+ * d[i + N*j] = d^p e_j/dr^p at r_i (device, N x N column major).  Replaces
+ * derivr_mat, boundary_mod.f90:96-137. */
+int rd_derivr_mat(rd_plan *plan, int p, double *d);
+
+/* Wall types (RD_BC_*) and the per-l recovery matrices M_l = columns 3..N-2 of
+ * the row-scaled B_l^-1, built and LU-factorised on the device in FP64.
+ * Returns RD_ERR_LU (and prints the l) when a B_l is singular.  Replaces
+ * setup_bc / v_setup / invDl_setup, boundary_mod.f90:35-58,140-241 and
+ * vboundary.f90:10-56. */
+int rd_bc_setup(rd_plan *plan, int bcint, int bcext);
+
+/* Copies (stream-ordered, device to device) of the tables built by
+ * rd_bc_setup: binv (N x (N-4) per l = 1..lmax, the module array binv of
+ * boundary_mod.f90:19-22) and d2bor (4 x N, :27-28).  Either destination may
+ * be NULL. */
+int rd_bc_tables(rd_plan *plan, double *binv, double *d2bor);
+
+/* W = M_l phi(3:N-2) for l >= 1; l = 0 and the continuation rows are zero.
+ * Replaces invDl, boundary_mod.f90:244-292. */
+int rd_invdl(rd_plan *plan, const double *phi, double *W);
+
+/* Z = 0 at the walls, W = M_l phi, phi = D_l W.  Replaces v_imposebc,
+ * vboundary.f90:59-88. */
+int rd_v_imposebc(rd_plan *plan, double *tor, double *phi, double *pol);
+
+/* As rd_v_imposebc, with phi recomputed only on rows 1, 2, N-1, N and the
+ * l = 0 mode of phi set to zero.  Replaces v_imposebc_fast,
+ * vboundary.f90:91-130. */
+int rd_v_imposebc_fast(rd_plan *plan, double *tor, double *phi, double *pol);
+
 /* ---- device selection and the host -> device registry ---------------
  * Replaces src/memmgt/gdevice_mod.f90 for the Fortran drop-in layer: a host
  * array gets a device copy registered by its host address; an interior

@@ -307,3 +307,40 @@ def invdl(N, n, lmax, binv, phi):
     out = np.empty_like(phi)
     lib().orc_invdl(N, n, lmax, _d(binv), _d(phi), _d(out))
     return out
+
+
+def dirichlet(a, N):
+    """dirichlet (boundary_mod.f90:62-93): rows 1 and N = nz - C set to zero."""
+    a = np.array(a, np.complex128, copy=True)
+    a[0] = 0
+    a[N - 1] = 0
+    return a
+
+
+def d2bor(d2):
+    """Rows 1, 2, N-1, N of d2/dr2 (boundary_mod.f90:230-235), shape (4, N)."""
+    N = d2.shape[0]
+    return np.ascontiguousarray(d2[[0, 1, N - 2, N - 1]])
+
+
+def v_imposebc(fc, kz, z, ell, binv, lmax, tor, phi):
+    """vboundary.f90:59-88: Z = 0 at the walls, W = M_l phi, phi = D_l W.
+    Returns (tor, phi, pol)."""
+    N = fc.n - fc.C
+    tor = dirichlet(tor, N)
+    pol = invdl(N, fc.n, lmax, binv, phi)
+    return tor, fc.dlr(pol, kz, z, ell), pol
+
+
+def v_imposebc_fast(fc, kz, z, ell, binv, d2b, lmax, tor, phi):
+    """vboundary.f90:91-130: phi recomputed on rows 1, 2, N-1, N only, l = 0
+    mode of phi zero.  Returns (tor, phi, pol)."""
+    N = fc.n - fc.C
+    tor = dirichlet(tor, N)
+    pol = invdl(N, fc.n, lmax, binv, phi)
+    phi = np.array(phi, np.complex128, copy=True)
+    ll = ell * (ell + 1.0)
+    for k, i in enumerate((0, 1, N - 2, N - 1)):
+        phi[i] = d2b[k] @ pol[:N] - ll * pol[i] / z[i] ** 2
+    phi[:, ell == 0] = 0
+    return tor, phi, pol

@@ -31,6 +31,7 @@ int upload_twiddles(Plan *p, int n, double2 **dst)
 void free_plan(Plan *p)
 {
     if (!p) return;
+    free_bc(p);
     void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
                    p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
                    p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z};
@@ -580,6 +581,75 @@ int rd_dirichlet(rd_plan *plan, double *a)
     Plan *p = P(plan);
     if (p->nr < 1) return RD_ERR_STATE;
     return launch_dirichlet(p, (double2 *)a);
+}
+
+int rd_derivr_mat(rd_plan *plan, int order, double *d)
+{
+    if (bad_plan(plan) || !d || order < 0) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->nr < 1) return RD_ERR_STATE;
+    return build_derivr_mat(p, order, d);
+}
+
+int rd_bc_setup(rd_plan *plan, int bcint, int bcext)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    if (bcint != RD_BC_NOSLIP || bcext != RD_BC_NOSLIP) {
+        fprintf(stderr, "redondito: unknown boundary condition type %d/%d\n", bcint, bcext);
+        return RD_ERR_ARG;
+    }
+    Plan *p = P(plan);
+    if (p->nr < 6 || p->lmax < 1) return RD_ERR_STATE;
+    int failed_l = 0;
+    int rc = build_bc(p, &failed_l);
+    if (rc == RD_ERR_LU)
+        fprintf(stderr, "redondito: LU of B_l failed for l = %d in invDl_setup\n", failed_l);
+    if (rc) free_bc(p);
+    return rc;
+}
+
+int rd_bc_tables(rd_plan *plan, double *binv, double *d2bor)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (!p->bc_ready) return RD_ERR_STATE;
+    const size_t N = (size_t)p->nr;
+    if (binv)
+        RD_HIP(hipMemcpyAsync(binv, p->bc_binv, sizeof(double) * N * (N - 4) * p->lmax,
+                              hipMemcpyDeviceToDevice, p->stream));
+    if (d2bor)
+        RD_HIP(hipMemcpyAsync(d2bor, p->bc_d2bor, sizeof(double) * 4 * N, hipMemcpyDeviceToDevice, p->stream));
+    return RD_OK;
+}
+
+int rd_invdl(rd_plan *plan, const double *phi, double *W)
+{
+    if (bad_plan(plan) || !phi || !W || phi == W) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (!p->bc_ready) return RD_ERR_STATE;
+    return launch_invdl(p, (const double2 *)phi, (double2 *)W);
+}
+
+int rd_v_imposebc(rd_plan *plan, double *tor, double *phi, double *pol)
+{
+    if (bad_plan(plan) || !tor || !phi || !pol || phi == pol || tor == pol || tor == phi) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (!p->bc_ready) return RD_ERR_STATE;
+    int rc = launch_dirichlet(p, (double2 *)tor);
+    if (!rc) rc = launch_invdl(p, (const double2 *)phi, (double2 *)pol);
+    if (!rc) rc = rd_dlr(plan, pol, phi);
+    return rc;
+}
+
+int rd_v_imposebc_fast(rd_plan *plan, double *tor, double *phi, double *pol)
+{
+    if (bad_plan(plan) || !tor || !phi || !pol || phi == pol || tor == pol || tor == phi) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (!p->bc_ready) return RD_ERR_STATE;
+    int rc = launch_dirichlet(p, (double2 *)tor);
+    if (!rc) rc = launch_invdl(p, (const double2 *)phi, (double2 *)pol);
+    if (!rc) rc = launch_imposebc_rows(p, (const double2 *)pol, (double2 *)phi);
+    return rc;
 }
 
 }  // extern "C"

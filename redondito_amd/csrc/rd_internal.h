@@ -59,6 +59,12 @@ struct Plan {
     // FFTs
     FftSeq seq_phi, seq_z;
     double2 *tw_phi, *tw_z;        // exp(-2 pi i t / n)
+    // no-slip wall solves, built by rd_bc_setup (column major, N = nr):
+    // d1, d2 N x N; binv N x (N-4) per l = 1..lmax; d2bor 4 x N
+    int bc_ready;
+    double *bc_d1, *bc_d2, *bc_binv, *bc_d2bor;
+    int *bc_queue;                 // (l | m0 << 16) tiles of invDl
+    int bc_qlen;
     // workspace
     double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
     double2 *S[6];                 // spectral scratch (nlm, nz)
@@ -113,6 +119,13 @@ int launch_prodvec(Plan *p, long npts, const double *ar, const double *at, const
 int launch_fc_filter_k(Plan *p, double2 *a);
 int launch_dirichlet(Plan *p, double2 *a);
 int launch_zero_rows(Plan *p, double2 *a, int row0, int nrows);
+
+// Wall solves.
+int build_derivr_mat(Plan *p, int order, double *d);
+int build_bc(Plan *p, int *failed_l);
+void free_bc(Plan *p);
+int launch_invdl(Plan *p, const double2 *phi, double2 *W);
+int launch_imposebc_rows(Plan *p, const double2 *pol, double2 *phi);
 
 }  // namespace rd
 

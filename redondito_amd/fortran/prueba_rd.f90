@@ -19,6 +19,7 @@
       CALL casos_vec()
       CALL caso_z()
       CALL caso_uxw()
+      CALL caso_boundary()
       IF (nfail .gt. 0) THEN
          PRINT '(A,I0,A)', 'prueba_rd: ', nfail, ' failed'
          STOP 1
@@ -269,5 +270,110 @@
       CALL gdev_free(C_LOC(nq)); CALL gdev_free(C_LOC(ns)); CALL gdev_free(C_LOC(nt))
       CALL fcgram_destroy_plan(planfc)
       END SUBROUTINE caso_uxw
+
+! prueba_boundary (src/prueba_boundary.f90:58-128): lmax = 7, nz = 128,
+! rint = 0.5, rext = 1.5, through the boundary module, at the levels
+! LEEME.md:45 records (walls exact, dW/dr 1e-14, interior phi 3e-12,
+! twice 2e-13, W vs W0 6e-8, fast == full, fast phi 3e-12).
+      SUBROUTINE caso_boundary()
+      USE boundary
+      INTEGER, PARAMETER :: lmax = 7, nz = 128, C = 25, od = 5, n = nz - C
+      INTEGER, PARAMETER :: nlm = (lmax+1)*(lmax+2)/2, nlat = 12, nphi = 24
+      INTEGER, PARAMETER :: nf = 9
+      TYPE(BCPLAN) :: bcp
+      CHARACTER(len=15) :: bck(2)
+      DOUBLE COMPLEX, TARGET :: a(nlm,nz,nf), z0(nlm,nz), w0(nlm,nz)
+      DOUBLE PRECISION, TARGET :: zz(nz), d1(n,n)
+      INTEGER(C_INT), TARGET :: ell(nlm)
+      DOUBLE PRECISION :: dz, r1, r2, err(10), wm, dmax(2)
+      DOUBLE COMPLEX :: cl
+      INTEGER :: k, i, j
+      INTEGER(C_SIZE_T) :: nb
+      CHARACTER(len=256) :: tdir
+! a(:,:,j): 1 f0, 2 tor, 3 fi, 4 pol, 5 pol2, 6 fs, 7 tor3, 8 fi3, 9 pol3
+      CALL GET_ENVIRONMENT_VARIABLE('RD_TABLE_DIR',tdir)
+      dz = 1.0D0/(n-1)
+      CALL fcgram_create_plan(planfc,(/nlat,nphi,nz/),(/0,0,C/),(/0,0,od/), &
+           (/1.0D0,1.0D0,dz/),TRIM(tdir),0,lmax,0.5D0)
+      CALL rd_check(rd_plan_grid(planfc%h,C_NULL_PTR,C_NULL_PTR,C_NULL_PTR,C_LOC(zz), &
+           C_NULL_PTR,C_LOC(ell),C_NULL_PTR),'grid')
+      r1 = zz(1)
+      r2 = zz(n)
+      bck(1) = 'noslip'
+      bck(2) = 'NoSlip '
+      CALL setup_bc(bcp,planfc,bck,'v')
+      DO k = 1,nz
+         DO i = 1,nlm
+            cl = DCMPLX(1.0D0+0.1D0*i,0.5D0-0.05D0*i)
+            z0(i,k) = cl*(1.0D0+EXP(zz(k))*SIN(3*zz(k)))
+            w0(i,k) = cl*(zz(k)-r1)**2*(zz(k)-r2)**2*EXP(zz(k))
+         END DO
+      END DO
+      nb = INT(16*nlm*nz,C_SIZE_T)
+      CALL gdev_alloc(C_LOC(w0),nb)
+      DO j = 1,nf
+         CALL gdev_alloc(C_LOC(a(1,1,j)),nb)
+      END DO
+      CALL gdev_update_to(C_LOC(w0),nb)
+      CALL dlr(w0,a(1,1,1))
+      CALL gdev_update_from(C_LOC(a(1,1,1)),nb)
+! what a stage leaves on the wall rows and in l = 0
+      a(:,:,2) = z0
+      a(:,:,3) = a(:,:,1)
+      a(:,1,3) = a(:,1,3)+1.0D0
+      a(:,2,3) = a(:,2,3)-2.0D0
+      a(:,n-1,3) = a(:,n-1,3)+3.0D0
+      a(:,n,3) = a(:,n,3)-4.0D0
+      DO i = 1,nlm
+         IF (ell(i) .eq. 0) a(i,:,3) = (5.0D0,1.0D0)
+      END DO
+      a(:,:,6) = a(:,:,3)
+      a(:,:,7) = z0
+      a(:,:,8) = a(:,:,3)
+      DO j = 2,8
+         CALL gdev_update_to(C_LOC(a(1,1,j)),nb)
+      END DO
+      CALL v_imposebc(a(1,1,2),a(1,1,3),a(1,1,4))
+      CALL v_imposebc(a(1,1,2),a(1,1,3),a(1,1,5))
+      CALL v_imposebc_fast(a(1,1,7),a(1,1,8),a(1,1,9))
+      DO j = 2,nf
+         CALL gdev_update_from(C_LOC(a(1,1,j)),nb)
+      END DO
+      CALL derivr_mat(1,d1)
+      err = 0
+      err(1) = MAX(MAXVAL(ABS(a(:,1,2))),MAXVAL(ABS(a(:,n,2))))
+      err(2) = MAX(MAXVAL(ABS(a(:,2:n-1,2)-z0(:,2:n-1))),MAXVAL(ABS(a(:,n+1:nz,2)-z0(:,n+1:nz))))
+      DO i = 1,nlm
+         IF (ell(i) .eq. 0) THEN
+            err(5) = MAX(err(5),MAXVAL(ABS(a(i,:,3))),MAXVAL(ABS(a(i,:,4))))
+            CYCLE
+         ENDIF
+         wm = MAXVAL(ABS(a(i,1:n,4)))
+         err(3) = MAX(err(3),MAX(ABS(a(i,1,4)),ABS(a(i,n,4)))/wm)
+         err(4) = MAX(err(4),MAX(ABS(SUM(d1(1,:)*a(i,1:n,4))),ABS(SUM(d1(n,:)*a(i,1:n,4))))*(r2-r1)/wm)
+         err(6) = MAX(err(6),MAXVAL(ABS(a(i,3:n-2,3)-a(i,3:n-2,1)))/MAXVAL(ABS(a(i,3:n-2,1))))
+         err(8) = MAX(err(8),MAXVAL(ABS(a(i,1:n,4)-w0(i,1:n)))/MAXVAL(ABS(w0(i,1:n))))
+      END DO
+      err(7) = MAXVAL(ABS(a(:,:,5)-a(:,:,4)))/MAXVAL(ABS(a(:,:,4)))
+      err(9) = MAX(MAXVAL(ABS(a(:,:,9)-a(:,:,4))),MAXVAL(ABS(a(:,:,7)-a(:,:,2))))
+      dmax(1) = MAXVAL(ABS(a(:,1:n,8)-a(:,1:n,3)))
+      dmax(2) = MAXVAL(ABS(a(:,1:n,3)))
+      err(10) = dmax(1)/dmax(2)
+      CALL report('boundary: Z at the walls',err(1),0.0D0)
+      CALL report('boundary: Z off the walls, change',err(2),0.0D0)
+      CALL report('boundary: W at the walls (rel)',err(3),1.0D-13)
+      CALL report('boundary: dW/dr at the walls (rel)',err(4),1.0D-12)
+      CALL report('boundary: phi and W at l = 0',err(5),0.0D0)
+      CALL report('boundary: interior phi change',err(6),3.0D-11)
+      CALL report('boundary: applied twice, W change',err(7),2.0D-12)
+      CALL report('boundary: W vs W0 (truncation 6e-8)',err(8),6.0D-7)
+      CALL report('boundary: fast, W and Z vs full',err(9),0.0D0)
+      CALL report('boundary: fast, phi vs full',err(10),3.0D-11)
+      CALL gdev_free(C_LOC(w0))
+      DO j = 1,nf
+         CALL gdev_free(C_LOC(a(1,1,j)))
+      END DO
+      CALL fcgram_destroy_plan(planfc)
+      END SUBROUTINE caso_boundary
 
       END PROGRAM prueba_rd

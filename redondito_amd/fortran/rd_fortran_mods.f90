@@ -145,6 +145,28 @@
           TYPE(C_PTR), VALUE :: plan, w, z, nq, ns, nt
           INTEGER(C_INT), VALUE :: prec
         END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_derivr_mat(plan,p,d) BIND(C,NAME='rd_derivr_mat')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, d
+          INTEGER(C_INT), VALUE :: p
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_bc_setup(plan,bcint,bcext) BIND(C,NAME='rd_bc_setup')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan
+          INTEGER(C_INT), VALUE :: bcint, bcext
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_invdl(plan,a,b) BIND(C,NAME='rd_invdl')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, a, b
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_v_imposebc(plan,tor,phi,pol) BIND(C,NAME='rd_v_imposebc')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, tor, phi, pol
+        END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_v_imposebc_fast(plan,tor,phi,pol) BIND(C,NAME='rd_v_imposebc_fast')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, tor, phi, pol
+        END FUNCTION
         INTEGER(C_INT) FUNCTION rd_device_init(rank) BIND(C,NAME='rd_device_init')
           IMPORT :: C_INT
           INTEGER(C_INT), VALUE :: rank
@@ -389,3 +411,117 @@
       USE fcgram
       TYPE(FCPLAN), SAVE, TARGET :: planfc
       END MODULE fft
+
+!=================================================================
+! boundary -- TYPE(BCPLAN), setup_bc, v_parsebc, v_setup, dirichlet,
+! derivr_mat, invDl_setup, invDl, v_imposebc, v_imposebc_fast
+! (src/boundary/boundary_mod.f90, src/boundary/vboundary.f90).  The
+! tables (binv, d2bor) live in the plan on the device; the routines use
+! the global plan planfc, as the reference's do.
+      MODULE boundary
+      USE iso_c_binding
+      USE rd_capi
+      USE fcgram
+      IMPLICIT NONE
+
+      TYPE BCPLAN
+         INTEGER :: bcint, bcext
+      END TYPE BCPLAN
+
+      CONTAINS
+
+      SUBROUTINE setup_bc(this,planfc,bckind,field)
+      TYPE(BCPLAN), INTENT(INOUT)   :: this
+      TYPE(FCPLAN), INTENT(INOUT)   :: planfc
+      CHARACTER(len=15), INTENT(IN) :: bckind(2)
+      CHARACTER(len=*), INTENT(IN)  :: field
+      IF (field .eq. 'v') CALL v_setup(this,planfc,bckind)
+      END SUBROUTINE setup_bc
+
+      FUNCTION v_parsebc(str) RESULT(bc)
+      CHARACTER(len=*), INTENT(IN) :: str
+      INTEGER :: bc
+      IF (str .eq. 'noslip') THEN
+         bc = 0
+      ELSE
+         PRINT*, "[ERROR] Unknown boundary ", "condition type", str, " Aborting..."
+         STOP
+      ENDIF
+      END FUNCTION v_parsebc
+
+      SUBROUTINE v_setup(this,planfc,bckind)
+      TYPE(BCPLAN), INTENT(INOUT)   :: this
+      TYPE(FCPLAN), INTENT(INOUT)   :: planfc
+      CHARACTER(len=15), INTENT(IN) :: bckind(2)
+      this%bcint = v_parsebc(lowtrim(bckind(1)))
+      this%bcext = v_parsebc(lowtrim(bckind(2)))
+      CALL rd_check(rd_bc_setup(planfc%h,INT(this%bcint,C_INT),INT(this%bcext,C_INT)), &
+           'invDl_setup')
+      END SUBROUTINE v_setup
+
+! The M_l of both walls without slip, on the global plan.
+      SUBROUTINE invDl_setup()
+      USE fft
+      CALL rd_check(rd_bc_setup(planfc%h,0_C_INT,0_C_INT),'invDl_setup')
+      END SUBROUTINE invDl_setup
+
+      SUBROUTINE dirichlet(a)
+      USE fft
+      DOUBLE COMPLEX, INTENT(INOUT), TARGET :: a(*)
+      CALL rd_check(rd_dirichlet(planfc%h,rd_dptr(C_LOC(a),'a')),'dirichlet')
+      CALL fc_sync(planfc)
+      END SUBROUTINE dirichlet
+
+! d(i,j) = d^p e_j / dr^p at z(i) on the N = nz - Cz physical points;
+! d is a host array, filled through a temporary device copy.
+      SUBROUTINE derivr_mat(p,d)
+      USE fft
+      INTEGER, INTENT(IN) :: p
+      DOUBLE PRECISION, INTENT(OUT), TARGET :: d(:,:)
+      INTEGER(C_SIZE_T) :: nb
+      nb = 8_C_SIZE_T*SIZE(d,KIND=C_SIZE_T)
+      CALL rd_check(rd_dev_alloc(C_LOC(d),nb),'derivr_mat')
+      CALL rd_check(rd_derivr_mat(planfc%h,INT(p,C_INT),rd_dev_lookup(C_LOC(d))),'derivr_mat')
+      CALL rd_check(rd_dev_update_from(C_LOC(d),nb),'derivr_mat')
+      CALL rd_check(rd_dev_free(C_LOC(d)),'derivr_mat')
+      END SUBROUTINE derivr_mat
+
+      SUBROUTINE invDl(a,b)
+      USE fft
+      DOUBLE COMPLEX, INTENT(IN), TARGET  :: a(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET :: b(*)
+      CALL rd_check(rd_invdl(planfc%h,rd_dptr(C_LOC(a),'a'),rd_dptr(C_LOC(b),'b')),'invDl')
+      CALL fc_sync(planfc)
+      END SUBROUTINE invDl
+
+      SUBROUTINE v_imposebc(tor,phi,pol)
+      USE fft
+      DOUBLE COMPLEX, INTENT(INOUT), TARGET :: tor(*),phi(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET   :: pol(*)
+      CALL rd_check(rd_v_imposebc(planfc%h,rd_dptr(C_LOC(tor),'tor'), &
+           rd_dptr(C_LOC(phi),'phi'),rd_dptr(C_LOC(pol),'pol')),'v_imposebc')
+      CALL fc_sync(planfc)
+      END SUBROUTINE v_imposebc
+
+      SUBROUTINE v_imposebc_fast(tor,phi,pol)
+      USE fft
+      DOUBLE COMPLEX, INTENT(INOUT), TARGET :: tor(*),phi(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET   :: pol(*)
+      CALL rd_check(rd_v_imposebc_fast(planfc%h,rd_dptr(C_LOC(tor),'tor'), &
+           rd_dptr(C_LOC(phi),'phi'),rd_dptr(C_LOC(pol),'pol')),'v_imposebc_fast')
+      CALL fc_sync(planfc)
+      END SUBROUTINE v_imposebc_fast
+
+! Lower case, without leading or trailing blanks (the reference's
+! preprocess/lowcase).
+      FUNCTION lowtrim(s) RESULT(o)
+      CHARACTER(len=*), INTENT(IN) :: s
+      CHARACTER(len=LEN(s)) :: o
+      INTEGER :: j
+      o = TRIM(ADJUSTL(s))
+      DO j = 1,LEN(o)
+         IF (o(j:j) >= 'A' .AND. o(j:j) <= 'Z') o(j:j) = ACHAR(IACHAR(o(j:j))+32)
+      END DO
+      END FUNCTION lowtrim
+
+      END MODULE boundary

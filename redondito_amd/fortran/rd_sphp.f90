@@ -213,15 +213,6 @@
       CALL fc_sync(planfc)
       END SUBROUTINE copy
 
-! src/boundary/boundary_mod.f90:62-93
-      SUBROUTINE dirichlet(a)
-      USE fft
-      IMPLICIT NONE
-      DOUBLE COMPLEX, INTENT(INOUT), TARGET :: a(*)
-      CALL rd_check(rd_dirichlet(planfc%h,rd_dptr(C_LOC(a),'a')),'dirichlet')
-      CALL fc_sync(planfc)
-      END SUBROUTINE dirichlet
-
 ! The composed nonlinear term N = u x omega (no reference routine):
 ! W, Z in (lm,r) -> NQ, NS, NT in (lm,r); prec 0 = FP64.
       SUBROUTINE nonlinear_uxw(w,z,nq,ns,nt,prec)

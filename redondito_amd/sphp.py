@@ -64,6 +64,12 @@ def lib():
         L.rd_nonlinear_uxw.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
+        L.rd_derivr_mat.argtypes = [vp, i, vp]
+        L.rd_bc_setup.argtypes = [vp, i, i]
+        L.rd_bc_tables.argtypes = [vp, vp, vp]
+        L.rd_invdl.argtypes = [vp, vp, vp]
+        L.rd_v_imposebc.argtypes = [vp, vp, vp, vp]
+        L.rd_v_imposebc_fast.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -290,3 +296,70 @@ def nonlinear_uxw(plan, W, Z, NQ, NS, NT, prec=RD_PREC_FP64):
     """N = u x omega, u = curl curl (W r) + curl (Z r), as (Q,S,T) in (lm, r)."""
     _check(lib().rd_nonlinear_uxw(plan.h, prec, plan._spec(W), plan._spec(Z), plan._spec(NQ),
                                   plan._spec(NS), plan._spec(NT)), "nonlinear_uxw")
+
+
+# ------------------------------------------------------------ boundary
+RD_BC_NOSLIP = 0
+_BC_KINDS = {"noslip": RD_BC_NOSLIP}
+
+
+def v_parsebc(s):
+    """Wall type from its name (v_parsebc, src/boundary/vboundary.f90:10-31)."""
+    key = s.strip().lower()
+    if key not in _BC_KINDS:
+        raise RdError(f"Unknown boundary condition type {s}")
+    return _BC_KINDS[key]
+
+
+class BCPlan:
+    """BCPLAN (src/boundary/boundary_mod.f90:14-16): the wall types."""
+
+    def __init__(self):
+        self.bcint = self.bcext = None
+
+
+def setup_bc(this, planfc, bckind, field):
+    """Wall types and the per-l recovery matrices M_l on the device
+    (setup_bc / v_setup / invDl_setup, boundary_mod.f90:35-58,140-241)."""
+    if field != "v":
+        return
+    this.bcint, this.bcext = v_parsebc(bckind[0]), v_parsebc(bckind[1])
+    _check(lib().rd_bc_setup(planfc.h, this.bcint, this.bcext), "invDl_setup")
+
+
+def derivr_mat(plan, p):
+    """d^p/dr^p on the nr physical points as an (nr, nr) float64 tensor,
+    d[i, j] = d^p e_j / dr^p at r_i (derivr_mat, boundary_mod.f90:96-137)."""
+    N = plan.nz - plan.C
+    # column-major N x N on the device == the transpose of a row-major tensor
+    dt = torch.empty((N, N), dtype=torch.float64, device=plan.device)
+    _check(lib().rd_derivr_mat(plan.h, p, ctypes.c_void_p(dt.data_ptr())), "derivr_mat")
+    return dt.t()
+
+
+def bc_tables(plan):
+    """Copies of the wall tables: binv as (lmax, N-4, N) with
+    binv[l-1, j, i] = M_l[i, j], and d2bor as (N, 4) with d2bor[j, k] = entry
+    j of wall row k (rows 1, 2, N-1, N of d2/dr2)."""
+    N = plan.nz - plan.C
+    binv = torch.empty((plan.lmax, N - 4, N), dtype=torch.float64, device=plan.device)
+    d2bor = torch.empty((N, 4), dtype=torch.float64, device=plan.device)
+    _check(lib().rd_bc_tables(plan.h, ctypes.c_void_p(binv.data_ptr()), ctypes.c_void_p(d2bor.data_ptr())),
+           "bc_tables")
+    return binv, d2bor
+
+
+def invDl(plan, a, b):
+    """W = M_l phi(3:N-2) (invDl, boundary_mod.f90:244-292)."""
+    _check(lib().rd_invdl(plan.h, plan._spec(a), plan._spec(b)), "invDl")
+
+
+def v_imposebc(tor, phi, pol, plan):
+    """Z = 0 at the walls, W = M_l phi, phi = D_l W (vboundary.f90:59-88)."""
+    _check(lib().rd_v_imposebc(plan.h, plan._spec(tor), plan._spec(phi), plan._spec(pol)), "v_imposebc")
+
+
+def v_imposebc_fast(tor, phi, pol, plan):
+    """As v_imposebc, phi recomputed on rows 1, 2, N-1, N only (vboundary.f90:91-130)."""
+    _check(lib().rd_v_imposebc_fast(plan.h, plan._spec(tor), plan._spec(phi), plan._spec(pol)),
+           "v_imposebc_fast")

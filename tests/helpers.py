@@ -172,3 +172,55 @@ def shell_fields(lmax, nr, n, rint, rext, seed, slope=2.0):
         a[:nr] = prof[:, None] * c[None, :]
         out.append(a)
     return out
+
+
+# ------------------------------------------------------------ boundary
+def boundary_inputs(z, ell, N):
+    """The fields of src/prueba_boundary.f90:58-78 on the rows of z: Z0, the
+    admissible W0 = c (r - r1)^2 (r - r2)^2 e^r and c_lm per mode."""
+    nlm = ell.size
+    lm = np.arange(1, nlm + 1)
+    cl = (1.0 + 0.1 * lm) + 1j * (0.5 - 0.05 * lm)
+    r1, r2 = z[0], z[N - 1]
+    z0 = cl[None] * (1.0 + np.exp(z) * np.sin(3 * z))[:, None]
+    w0 = cl[None] * ((z - r1) ** 2 * (z - r2) ** 2 * np.exp(z))[:, None]
+    return z0, w0
+
+
+def boundary_stage_phi(f0, ell, N):
+    """phi as a Runge-Kutta stage leaves it: wall rows and the l = 0 mode
+    disturbed (src/prueba_boundary.f90:80-89)."""
+    fi = np.array(f0, np.complex128, copy=True)
+    fi[0] += 1.0
+    fi[1] -= 2.0
+    fi[N - 2] += 3.0
+    fi[N - 1] -= 4.0
+    fi[:, ell == 0] = 5.0 + 1.0j
+    return fi
+
+
+def boundary_errs(z, ell, N, d1, z0, w0, f0, tor, fi, pol, pol2, tor3, fi3, pol3):
+    """The ten checks of src/prueba_boundary.f90:92-128, in order."""
+    err = np.zeros(10)
+    l0 = ell == 0
+    nl = ~l0
+    r1, r2 = z[0], z[N - 1]
+    err[0] = max(np.abs(tor[0]).max(), np.abs(tor[N - 1]).max())
+    err[1] = max(np.abs(tor[1:N - 1] - z0[1:N - 1]).max(), np.abs(tor[N:] - z0[N:]).max())
+    err[4] = max(np.abs(fi[:, l0]).max(), np.abs(pol[:, l0]).max())
+    wm = np.abs(pol[:N, nl]).max(axis=0)
+    err[2] = (np.maximum(np.abs(pol[0, nl]), np.abs(pol[N - 1, nl])) / wm).max()
+    dw = d1 @ pol[:N, nl]
+    err[3] = (np.maximum(np.abs(dw[0]), np.abs(dw[N - 1])) * (r2 - r1) / wm).max()
+    err[5] = (np.abs(fi[2:N - 2, nl] - f0[2:N - 2, nl]).max(axis=0) / np.abs(f0[2:N - 2, nl]).max(axis=0)).max()
+    err[7] = (np.abs(pol[:N, nl] - w0[:N, nl]).max(axis=0) / np.abs(w0[:N, nl]).max(axis=0)).max()
+    err[6] = np.abs(pol2 - pol).max() / np.abs(pol).max()
+    err[8] = max(np.abs(pol3 - pol).max(), np.abs(tor3 - tor).max())
+    err[9] = np.abs(fi3[:N] - fi[:N]).max() / np.abs(fi[:N]).max()
+    return err
+
+
+# published levels of prueba_boundary (LEEME.md:45) with margins for the
+# FC-Gram tables built here: walls exact, dW/dr 1e-14, interior phi 3e-12,
+# twice 2e-13, W vs W0 6e-8 (truncation), fast == full, fast phi 3e-12
+BOUNDARY_LIMITS = (0.0, 0.0, 1e-13, 1e-12, 0.0, 3e-11, 2e-12, 6e-7, 0.0, 3e-11)

@@ -461,3 +461,58 @@ def test_invdl_recovers_admissible_w():
     assert err < 1e-6
     assert np.max(np.abs(W[0])) == 0 or np.max(np.abs(W[0])) < 1e-13
     assert np.max(np.abs((D1 @ W[:N])[0])) < 1e-10 * np.max(np.abs(W))
+
+
+def test_prueba_boundary_levels():
+    """src/prueba_boundary.f90 with lmax = 7, nz = 128, rint = 0.5, rext = 1.5:
+    the ten published checks (LEEME.md:45)."""
+    lmax, nz, C = 7, 128, 25
+    fc, kz, z, ell, dz = _pseudo(lmax, nz)
+    N = nz - C
+    d1 = fc.derivr_mat(kz, 1)
+    d2 = fc.derivr_mat(kz, 2)
+    binv = O.invdl_setup(N, lmax, d1, d2, z[:N])
+    d2b = O.d2bor(d2)
+    z0, w0 = H.boundary_inputs(z, ell, N)
+    f0 = fc.dlr(w0, kz, z, ell)
+    fs = H.boundary_stage_phi(f0, ell, N)
+    tor, fi, pol = O.v_imposebc(fc, kz, z, ell, binv, lmax, z0, fs)
+    _, _, pol2 = O.v_imposebc(fc, kz, z, ell, binv, lmax, tor, fi)
+    tor3, fi3, pol3 = O.v_imposebc_fast(fc, kz, z, ell, binv, d2b, lmax, z0, fs)
+    err = H.boundary_errs(z, ell, N, d1, z0, w0, f0, tor, fi, pol, pol2, tor3, fi3, pol3)
+    print("prueba_boundary errs", " ".join(f"{e:.2e}" for e in err))
+    for k, (e, lim) in enumerate(zip(err, H.BOUNDARY_LIMITS)):
+        assert e <= lim, (k + 1, e, lim)
+
+
+def test_invdl_matches_scaled_lu_solve():
+    """W from M_l equals a direct solve of B_l W = P phi built independently
+    with numpy (LEEME.md:45: 1e-15), on a random phi."""
+    lmax, nz, C = 5, 90, 25
+    fc, kz, z, ell, dz = _pseudo(lmax, nz)
+    N = nz - C
+    d1 = fc.derivr_mat(kz, 1)
+    d2 = fc.derivr_mat(kz, 2)
+    binv = O.invdl_setup(N, lmax, d1, d2, z[:N])
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal((nz, ell.size)) + 1j * rng.standard_normal((nz, ell.size))
+    W = O.invdl(N, nz, lmax, binv, phi)
+    for lm in range(ell.size):
+        l = ell[lm]
+        if l == 0:
+            assert np.all(W[:, lm] == 0)
+            continue
+        B = d2.copy()
+        B[np.arange(N), np.arange(N)] -= l * (l + 1) / z[:N] ** 2
+        B[0] = 0
+        B[0, 0] = 1
+        B[1] = d1[0]
+        B[N - 2] = d1[N - 1]
+        B[N - 1] = 0
+        B[N - 1, N - 1] = 1
+        rhs = np.zeros(N, complex)
+        rhs[2:N - 2] = phi[2:N - 2, lm]
+        ref = np.linalg.solve(B, rhs)
+        # phi is O(1): absolute agreement (the reference quotes 1e-15)
+        assert np.max(np.abs(W[:N, lm] - ref)) <= 5e-14, lm
+        assert np.all(W[N:, lm] == 0)
