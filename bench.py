@@ -7,6 +7,12 @@ are seeded complex normal with spectrum (l+1)^-2 and radial profile
 sin(pi (r - ri)/(ro - ri)).  One step = one full evaluation
 (W, Z) -> (NQ, NS, NT) with all inputs resident in HBM.
 
+Three forms run on the same inputs: FP64, and the precision forms with the
+Legendre contractions on FP32 MFMA and as BF16x3 splits.  Each is timed over
+the same number of steps and its error against the FP64 output is reported
+per field and per degree band.  The headline value is the fastest form whose
+joint relative L2 error against FP64 is at most 1e-9.
+
 Prints one JSON line.  --gpus above 1 is refused: the path is single-GPU.
 """
 import argparse
@@ -22,8 +28,11 @@ sys.path.insert(0, ROOT)
 
 LMAX, NR, C, D = 255, 128, 25, 5
 RI, RO = 0.35, 1.0
+SEED = 20261203
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+FORMS = (("fp64", 0), ("fp32", 1), ("bf16x3", 2))
+ERR_BAR = 1e-9
 
 
 def workload():
@@ -34,14 +43,14 @@ def workload():
     return nlat, nphi, n, dz
 
 
-def seeded_fields(nlm, n):
+def seeded_fields(n):
     from tests import helpers as H
-    return H.shell_fields(LMAX, NR, n, RI, RO, seed=20261203, slope=2.0)
+    return H.shell_fields(LMAX, NR, n, RI, RO, seed=SEED, slope=2.0)
 
 
 def cpu_baseline(W, Z, nlat, nphi, n, dz, gpu_out, budget_s=15.0):
     """The oracle (CPU FP64 restatement, OpenMP) on a bounded sample of shells of
-    the same workload; also checks the GPU result on those shells."""
+    the same workload; also checks the GPU FP64 result on those shells."""
     from oracle import oracle as O
     from tests import helpers as H
     import redondito_amd.sphp as S
@@ -88,8 +97,8 @@ def copy_calibration(nbytes=1 << 30, reps=3):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (profiles/hbm_traffic_uxw_lmax255_nr128.json), or None."""
+    """HBM bytes per launch of a stage's kernel from the committed counter
+    summary (profiles/hbm_traffic_uxw_lmax255_nr128.json), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_uxw_lmax255_nr128.json")
     try:
         with open(path) as fh:
@@ -99,56 +108,70 @@ def measured_traffic(kernel):
         return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a 1 GiB device copy (counter calibration) and exit")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this path runs on one GPU only; --gpus above 1 is not supported", file=sys.stderr)
-        sys.exit(2)
-    if args.calibrate_copy:
-        copy_calibration()
-        return
-
+def time_form(S, plan, prec, Wd, Zd, out, steps, warmup):
+    """Warm up, then time exactly `steps` evaluations bracketed by device
+    synchronisations; returns (ms per step, per-stage ms)."""
     import torch
-    import redondito_amd.sphp as S
-
-    nlat, nphi, n, dz = workload()
-    dev = torch.device("cuda:0")
-    plan = S.FCPlan(LMAX, nlat, nphi, n, C, D, dz, RI, device=dev)
-    W, Z = seeded_fields(plan.nlm, n)
-    Wd = torch.from_numpy(W).to(dev)
-    Zd = torch.from_numpy(Z).to(dev)
-    out = [torch.zeros_like(Wd) for _ in range(3)]
-
-    for _ in range(args.warmup):
-        S.nonlinear_uxw(plan, Wd, Zd, *out)
+    for _ in range(warmup):
+        S.nonlinear_uxw(plan, Wd, Zd, *out, prec=prec)
     torch.cuda.synchronize()
-
-    # timed region, with per-stage HIP events on the plan's stream
     plan.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        S.nonlinear_uxw(plan, Wd, Zd, *out)
+    for _ in range(steps):
+        S.nonlinear_uxw(plan, Wd, Zd, *out, prec=prec)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stages = plan.stage_times()
     plan.profile(False)
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    evals = 1e3 / ms_step
+    return (t1 - t0) * 1e3 / steps, {k: v[0] / max(v[1], 1) for k, v in stages.items()}
 
-    L, nlm = LMAX + 1, plan.nlm
-    Sb = 16.0 * nlm * NR
-    Fm = 16.0 * nlat * L * NR
-    G = 8.0 * nlat * nphi * NR
-    leg_flops = 10.0 * nlat * nlm * NR       # one vector synthesis or analysis
-    per_stage = {k: v[0] / max(v[1], 1) for k, v in stages.items()}
+
+def time_calls(fn, reps=10):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def aux_timings(S, plan, Wd):
+    """Absolute scalar transforms (one SHT over all nr shells) and the FP64
+    wall solves at the same size."""
+    import torch
+    nlat, nphi, nlm = plan.nlat, plan.nphi, plan.nlm
+    f = torch.zeros((NR, nphi, nlat), dtype=torch.float64, device=Wd.device)
+    a = Wd[:NR].contiguous()
+    b = torch.zeros_like(a)
+    syn = time_calls(lambda: S.fftp2d_complex_to_real_xy(plan, a, f))
+    ana = time_calls(lambda: S.fftp2d_real_to_complex_xy(plan, f, b))
+    bcp = S.BCPlan()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.setup_bc(bcp, plan, ("noslip", "noslip"), "v")
+    torch.cuda.synchronize()
+    setup = (time.perf_counter() - t0) * 1e3
+    tor, phi, pol = Wd.clone(), Wd.clone(), torch.zeros_like(Wd)
+    inv = time_calls(lambda: S.invDl(plan, phi, pol))
+    full = time_calls(lambda: S.v_imposebc(tor, phi, pol, plan))
+    fast = time_calls(lambda: S.v_imposebc_fast(tor, phi, pol, plan))
+    return {
+        "scalar_synthesis_ms": round(syn, 4), "scalar_analysis_ms": round(ana, 4),
+        "transforms_per_s": round(2e3 / (syn + ana), 2),
+        "bc_setup_ms": round(setup, 2), "invDl_ms": round(inv, 4),
+        "v_imposebc_ms": round(full, 4), "v_imposebc_fast_ms": round(fast, 4),
+    }
+
+
+def roofline(form, per_stage, nlat, nlm):
+    """Dominant kernel of a form against its bound, algorithmic work per launch."""
+    L = LMAX + 1
+    Sb = 16.0 * nlm * NR                    # one spectral field on nr shells
+    Fm = 16.0 * nlat * L * NR               # one Fourier field (m <= lmax)
+    leg_flops = 10.0 * nlat * nlm * NR      # one vector synthesis or analysis
     alg = {
         "radial": ("hbm", 8 * Sb),
         "synthesis_u": ("mfma", leg_flops),
@@ -162,38 +185,95 @@ def main():
     if bound == "hbm":
         achieved, peak, unit = work / t_dom / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
-        achieved, peak, unit = work / t_dom / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
-    # stage-wise roofline time of the whole evaluation (FP64)
-    t_roof = (5 * Sb + 11 * Sb + 6 * Sb + 6 * Fm + 6 * Fm + 6 * G + 9 * G + 3 * G + 3 * Fm + 3 * Fm + 3 * Sb) / (HBM_PEAK_GBS * 1e9)
-    t_roof_leg = 3 * leg_flops / (FP64_PEAK_TFLOPS * 1e12)
+        achieved, peak, unit = work / t_dom / 1e12, MFMA_PEAK_TFLOPS[form], "TFLOP/s"
+    return {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom) if form == "fp64" else None}
 
-    host_out = [o.cpu().numpy() for o in out]
-    cpu = None
-    parity = None
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 form (for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this path runs on one GPU only; --gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        copy_calibration()
+        return
+
+    import torch
+    import redondito_amd.sphp as S
+    from tests import helpers as H
+
+    nlat, nphi, n, dz = workload()
+    dev = torch.device("cuda:0")
+    plan = S.FCPlan(LMAX, nlat, nphi, n, C, D, dz, RI, device=dev)
+    W, Z = seeded_fields(n)
+    Wd = torch.from_numpy(W).to(dev)
+    Zd = torch.from_numpy(Z).to(dev)
+
+    forms = {}
+    host = {}
+    for name, prec in FORMS:
+        if args.fp64_only and prec != 0:
+            continue
+        out = [torch.zeros_like(Wd) for _ in range(3)]
+        ms, stages = time_form(S, plan, prec, Wd, Zd, out, args.steps, args.warmup)
+        host[name] = [o.cpu().numpy() for o in out]
+        forms[name] = {"evals_per_s": round(1e3 / ms, 3), "ms_per_step": round(ms, 4),
+                       "stage_ms": {k: round(v, 4) for k, v in stages.items()}, "_stages": stages}
+    for name in forms:
+        rep = H.band_errors(host[name], host["fp64"], LMAX, NR)
+        forms[name]["rel_err_vs_fp64"] = {
+            "joint": float(f"{rep['joint']:.3e}"),
+            **{f: {b: float(f"{e:.3e}") for b, e in rep[f].items()} for f in ("NQ", "NS", "NT")}}
+        forms[name]["speedup_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / forms[name]["ms_per_step"], 3)
+
+    ok = [k for k in forms if forms[k]["rel_err_vs_fp64"]["joint"] <= ERR_BAR]
+    head = min(ok, key=lambda k: forms[k]["ms_per_step"])
+    nlm = plan.nlm
+    roof = roofline(head, forms[head]["_stages"], nlat, nlm)
+    # whole-evaluation bound, stage by stage: sum of max(flops / MFMA peak,
+    # algorithmic bytes / HBM peak) over the five kernels of one evaluation
+    L = LMAX + 1
+    Sb, Fm = 16.0 * nlm * NR, 16.0 * nlat * L * NR
+    bw, leg = HBM_PEAK_GBS * 1e9, 10.0 * nlat * nlm * NR / (MFMA_PEAK_TFLOPS[head] * 1e12)
+    t_roof = (8 * Sb / bw + 2 * max(leg, (3 * Sb + 3 * Fm) / bw) + 9 * Fm / bw
+              + max(leg, (3 * Fm + 3 * Sb) / bw))
+    aux = aux_timings(S, plan, Wd)
+
+    cpu = parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(W, Z, nlat, nphi, n, dz, host_out)
+        cpu, parity = cpu_baseline(W, Z, nlat, nphi, n, dz, host["fp64"])
 
+    for f in forms.values():
+        f.pop("_stages")
     line = {
         "metric": "nonlinear-term evals/s @lmax=255,nr=128: mixed-prec vs FP64, rel err <=1e-9",
-        "value": round(evals, 3),
+        "value": forms[head]["evals_per_s"],
         "unit": "nonlinear-term evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": forms[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 elsewhere",
+                  "bf16x3": "bf16x3 Legendre, f64 elsewhere"}[head],
         "data": "synthetic seeded W,Z: complex normal, (l+1)^-2, sin radial profile",
-        "config": {"workload": "u x omega nonlinear term, plain FP64 form", "lmax": LMAX, "nlat": nlat,
-                   "nphi": nphi, "nr": NR, "fcgram_C": C, "fcgram_d": D, "radial_fft_len": n,
-                   "form": "fp64"},
-        "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                     "frac": round(achieved / peak, 4), "traffic": measured_traffic(dom)},
-        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
-        "eval_roofline_frac": round((t_roof + t_roof_leg) / (ms_step * 1e-3), 4),
-        "rel_err_vs_fp64": 0.0,
+        "config": {"workload": "u x omega nonlinear term", "form": head, "lmax": LMAX, "nlat": nlat,
+                   "nphi": nphi, "nr": NR, "fcgram_C": C, "fcgram_d": D, "radial_fft_len": n},
+        "roofline": roof,
+        "eval_roofline_frac": round(t_roof / (forms[head]["ms_per_step"] * 1e-3), 4),
+        "forms": forms,
+        "aux": aux,
         "rel_l2_vs_oracle": parity,
         "cpu_baseline": cpu,
     }

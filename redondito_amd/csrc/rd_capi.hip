@@ -32,6 +32,7 @@ void free_plan(Plan *p)
 {
     if (!p) return;
     free_bc(p);
+    free_precision_tables(p);
     void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
                    p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
                    p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z};
@@ -59,7 +60,7 @@ void free_plan(Plan *p)
 
 int synth_scalar(Plan *p, int nplanes, const double *in, double *out)
 {
-    int rc = launch_synthesis(p, false, nplanes, (const double2 *)in, nullptr, nullptr, p->nlm, p->F[0],
+    int rc = launch_synthesis(p, RD_PREC_FP64, false, nplanes, (const double2 *)in, nullptr, nullptr, p->nlm, p->F[0],
                               nullptr, nullptr);
     if (rc) return rc;
     const double2 *F[1] = {p->F[0]};
@@ -73,7 +74,7 @@ int anal_scalar(Plan *p, int nplanes, const double *in, double *out)
     double2 *F[1] = {p->F[0]};
     int rc = launch_phi_forward(p, 1, nplanes, f, F);
     if (rc) return rc;
-    return launch_analysis(p, false, nplanes, p->F[0], nullptr, nullptr, (double2 *)out, nullptr,
+    return launch_analysis(p, RD_PREC_FP64, false, nplanes, p->F[0], nullptr, nullptr, (double2 *)out, nullptr,
                            nullptr, p->nlm);
 }
 
@@ -320,7 +321,7 @@ int rd_vsht_analysis(rd_plan *plan, int nplanes, const double *vr, const double 
     double2 *F[3] = {p->F[0], p->F[1], p->F[2]};
     int rc = launch_phi_forward(p, 3, nplanes, f, F);
     if (rc) return rc;
-    return launch_analysis(p, true, nplanes, p->F[0], p->F[1], p->F[2], (double2 *)q, (double2 *)s,
+    return launch_analysis(p, RD_PREC_FP64, true, nplanes, p->F[0], p->F[1], p->F[2], (double2 *)q, (double2 *)s,
                            (double2 *)t, p->nlm);
 }
 
@@ -331,7 +332,7 @@ int rd_vsht_synthesis(rd_plan *plan, int nplanes, const double *q, const double 
     Plan *p = P(plan);
     if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
     if (nplanes == 0) return RD_OK;
-    int rc = launch_synthesis(p, true, nplanes, (const double2 *)q, (const double2 *)s,
+    int rc = launch_synthesis(p, RD_PREC_FP64, true, nplanes, (const double2 *)q, (const double2 *)s,
                               (const double2 *)t, p->nlm, p->F[0], p->F[1], p->F[2]);
     if (rc) return rc;
     const double2 *F[3] = {p->F[0], p->F[1], p->F[2]};
@@ -535,7 +536,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
                      double *NT)
 {
     if (bad_plan(plan) || !W || !Z || !NQ || !NS || !NT) return RD_ERR_ARG;
-    if (prec != RD_PREC_FP64) return RD_ERR_ARG;
+    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32 && prec != RD_PREC_BF16X3) return RD_ERR_ARG;
     Plan *p = P(plan);
     const int nr = p->nr;
     const bool prof = p->profiling != 0;
@@ -553,9 +554,9 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     MARK(0);
     int rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, p->S);
     MARK(1);
-    if (!rc) rc = launch_synthesis(p, true, nr, p->S[0], p->S[1], p->S[2], p->nlm, p->F[0], p->F[1], p->F[2]);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, p->S[0], p->S[1], p->S[2], p->nlm, p->F[0], p->F[1], p->F[2]);
     MARK(2);
-    if (!rc) rc = launch_synthesis(p, true, nr, p->S[3], p->S[4], p->S[5], p->nlm, p->F[3], p->F[4], p->F[5]);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, p->S[3], p->S[4], p->S[5], p->nlm, p->F[3], p->F[4], p->F[5]);
     MARK(3);
     if (!rc) {
         const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
@@ -564,7 +565,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     }
     MARK(4);
     if (!rc)
-        rc = launch_analysis(p, true, nr, p->F[0], p->F[1], p->F[2], (double2 *)NQ, (double2 *)NS,
+        rc = launch_analysis(p, prec, true, nr, p->F[0], p->F[1], p->F[2], (double2 *)NQ, (double2 *)NS,
                              (double2 *)NT, p->nlm);
     MARK(5);
     if (!rc) rc = launch_zero_rows(p, (double2 *)NQ, nr, p->nz - nr);

@@ -49,6 +49,11 @@ struct Plan {
     // analysis [m,p][theta nthp16][col l_p] with weights (and 1/l(l+1))
     double *syn_P, *syn_D, *syn_M;
     double *ana_P, *ana_D, *ana_M;
+    // precision-form tables, built on first use: FP32 copies in the same
+    // layouts, and BF16 hi/lo splits with four consecutive K values packed
+    // (synthesis [m,p][row/4][theta][4], analysis [m,p][theta/4][col][4])
+    float *syn32_P, *syn32_D, *syn32_M, *ana32_P, *ana32_D, *ana32_M;
+    uint16_t *synb_hi[3], *synb_lo[3], *anab_hi[3], *anab_lo[3];
     long *d_syn_off, *d_ana_off;   // per (m,p)
     int *d_ana_ncol;               // per m
     long syn_rows, ana_elems;
@@ -85,12 +90,19 @@ constexpr int kProfSlots = 512;
 enum UxwStage { ST_RADIAL = 0, ST_SYN_U, ST_SYN_W, ST_PHI_CROSS, ST_ANALYSIS, ST_TAIL, ST_COUNT };
 
 // ---------------------------------------------------------------- kernels
-// Legendre contractions over all planes (Fourier layout F[k][m][theta]).
-int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s,
+// Legendre contractions over all planes (Fourier layout F[k][m][theta]);
+// prec is RD_PREC_FP64, RD_PREC_FP32 or RD_PREC_BF16X3.
+int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q, const double2 *s,
                      const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2);
-int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
+int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0, const double2 *g1,
                     const double2 *g2, double2 *q, double2 *s, double2 *t, long ld);
 int build_legendre_tables(Plan *p);
+int ensure_precision_tables(Plan *p, int prec);
+void free_precision_tables(Plan *p);
+int launch_synthesis_bf3(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
+                         long ld, double2 *f0, double2 *f1, double2 *f2);
+int launch_analysis_bf3(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
+                        const double2 *g2, double2 *q, double2 *s, double2 *t, long ld);
 
 // phi transforms between F[k][m][theta] and f(theta, phi, k).
 int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f);

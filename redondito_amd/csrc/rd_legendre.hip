@@ -25,11 +25,29 @@
 namespace rd {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ v4d mfma(double a, double b, v4d c)
-{
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+// 16x16x4 MFMA of one operand type.  A lane (i + 16k) holds A[i][k], B lane
+// (j + 16k) holds B[k][j] for both types; the accumulator rows differ:
+// lane l, register v holds D[row(l/16, v)][l%16] with row = l/16 + 4v for
+// F64 and 4(l/16) + v for F32 (measured on gfx950).
+template <typename T> struct Mf;
+template <> struct Mf<double> {
+    using V = v4d;
+    static __device__ __forceinline__ V mma(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kk, int v) { return kk + 4 * v; }
+};
+template <> struct Mf<float> {
+    using V = v4f;
+    static __device__ __forceinline__ V mma(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kk, int v) { return 4 * kk + v; }
+};
 
 __device__ __forceinline__ long lm_index(int lmax, int l, int m)
 {
@@ -280,8 +298,9 @@ struct SynCfg {
     static constexpr int JTP = JT + 2;           // LDS row pitch (doubles)
 };
 
+template <typename T>
 struct SynArgs {
-    const double *tP, *tD, *tM;
+    const T *tP, *tD, *tM;
     const long *off;
     const int *queue;
     const double2 *in[3];
@@ -290,12 +309,15 @@ struct SynArgs {
     int lmax, nlat, nth, nthp, nplanes, n_jt;
 };
 
-template <bool VEC>
-__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
+// T = double: the FP64 path; T = float: the FP32 precision form (tables and
+// operands rounded to FP32, FP32 accumulation, FP64 output).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs<T> A)
 {
     using Cfg = SynCfg<VEC>;
+    using V = typename Mf<T>::V;
     constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
-    __shared__ double Bs[NF][SKC][JTP];
+    __shared__ T Bs[NF][SKC][JTP];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
@@ -309,15 +331,15 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
     const int k0 = jt * (JT / 2);
     const int th0 = tt * 64 + wave * 16;
     const int col = lane & 15, kk = lane >> 4;
-    const double sgn_i = (col & 1) ? 1.0 : -1.0;   // (i X)_j = sgn_i * X_{j^1}
+    const T sgn_i = (col & 1) ? T(1) : T(-1);   // (i X)_j = sgn_i * X_{j^1}
 
-    v4d acc[NF][2][JB];
+    V acc[NF][2][JB];
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
         for (int s = 0; s < 2; s++)
 #pragma unroll
-            for (int b = 0; b < JB; b++) acc[f][s][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+            for (int b = 0; b < JB; b++) acc[f][s][b] = (V){0, 0, 0, 0};
 
     const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
     const long lm0 = lm_index(A.lmax, m, m);
@@ -331,8 +353,8 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
             int l = lbase + ll, k = k0 + pl;
             double2 v = make_double2(0.0, 0.0);
             if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-            Bs[f][ll][2 * pl] = v.x;
-            Bs[f][ll][2 * pl + 1] = v.y;
+            Bs[f][ll][2 * pl] = (T)v.x;
+            Bs[f][ll][2 * pl + 1] = (T)v.y;
         }
         __syncthreads();
         const int rowbase = (lbase - m) >> 1;
@@ -341,8 +363,8 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
             const int row = rowbase + 4 * st + kk;
             const long ie = offe + (long)row * A.nthp + th0 + col;
             const long io = offo + (long)row * A.nthp + th0 + col;
-            const double Pe = A.tP[ie], Po = A.tP[io];
-            double De = 0, Do = 0, Me = 0, Mo = 0;
+            const T Pe = A.tP[ie], Po = A.tP[io];
+            T De = 0, Do = 0, Me = 0, Mo = 0;
             if (VEC) {
                 De = A.tD[ie];
                 Do = A.tD[io];
@@ -353,30 +375,30 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = b * 16 + col;
-                acc[0][0][b] = mfma(Pe, Bs[0][lle][j], acc[0][0][b]);
-                acc[0][1][b] = mfma(Po, Bs[0][llo][j], acc[0][1][b]);
+                acc[0][0][b] = Mf<T>::mma(Pe, Bs[0][lle][j], acc[0][0][b]);
+                acc[0][1][b] = Mf<T>::mma(Po, Bs[0][llo][j], acc[0][1][b]);
                 if (VEC) {
-                    const double Se = Bs[1][lle][j], So = Bs[1][llo][j];
-                    const double Te = Bs[2][lle][j], To = Bs[2][llo][j];
-                    const double iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
-                    const double iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
+                    const T Se = Bs[1][lle][j], So = Bs[1][llo][j];
+                    const T Te = Bs[2][lle][j], To = Bs[2][llo][j];
+                    const T iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
+                    const T iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
                     // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
-                    acc[1][0][b] = mfma(Do, So, acc[1][0][b]);
-                    acc[1][0][b] = mfma(Me, iTe, acc[1][0][b]);
-                    acc[1][1][b] = mfma(De, Se, acc[1][1][b]);
-                    acc[1][1][b] = mfma(Mo, iTo, acc[1][1][b]);
+                    acc[1][0][b] = Mf<T>::mma(Do, So, acc[1][0][b]);
+                    acc[1][0][b] = Mf<T>::mma(Me, iTe, acc[1][0][b]);
+                    acc[1][1][b] = Mf<T>::mma(De, Se, acc[1][1][b]);
+                    acc[1][1][b] = Mf<T>::mma(Mo, iTo, acc[1][1][b]);
                     // v_phi: symmetric = mP_e (iS)_e - dP_o T_o, antisymmetric = mP_o (iS)_o - dP_e T_e
-                    acc[2][0][b] = mfma(Me, iSe, acc[2][0][b]);
-                    acc[2][0][b] = mfma(Do, -To, acc[2][0][b]);
-                    acc[2][1][b] = mfma(Mo, iSo, acc[2][1][b]);
-                    acc[2][1][b] = mfma(De, -Te, acc[2][1][b]);
+                    acc[2][0][b] = Mf<T>::mma(Me, iSe, acc[2][0][b]);
+                    acc[2][0][b] = Mf<T>::mma(Do, -To, acc[2][0][b]);
+                    acc[2][1][b] = Mf<T>::mma(Mo, iSo, acc[2][1][b]);
+                    acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
                 }
             }
         }
         __syncthreads();
     }
 
-    // north = sym + anti, south = sym - anti; lane holds D[lane/16 + 4v][lane%16]
+    // north = sym + anti, south = sym - anti
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         double *out = (double *)A.out[f];
@@ -388,7 +410,7 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
             const long base = ((long)k * L + m) * A.nlat;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int th = th0 + kk + 4 * v;
+                const int th = th0 + Mf<T>::row(kk, v);
                 if (th >= A.nth) continue;
                 const double sy = acc[f][0][b][v], an = acc[f][1][b][v];
                 out[2 * (base + th) + c] = sy + an;
@@ -398,13 +420,15 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs A)
     }
 }
 
-int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s,
-                     const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2)
+template <typename T>
+static int launch_synthesis_t(Plan *p, const T *tP, const T *tD, const T *tM, bool vec, int nplanes,
+                              const double2 *q, const double2 *s, const double2 *t, long ld, double2 *f0,
+                              double2 *f1, double2 *f2)
 {
-    SynArgs A;
-    A.tP = p->syn_P;
-    A.tD = p->syn_D;
-    A.tM = p->syn_M;
+    SynArgs<T> A;
+    A.tP = tP;
+    A.tD = tD;
+    A.tM = tM;
     A.off = p->d_syn_off;
     A.in[0] = q;
     A.in[1] = vec ? s : q;
@@ -423,11 +447,25 @@ int launch_synthesis(Plan *p, bool vec, int nplanes, const double2 *q, const dou
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
     if (vec)
-        hipLaunchKernelGGL(synthesis_kernel<true>, grid, dim3(256), 0, p->stream, A);
+        hipLaunchKernelGGL((synthesis_kernel<T, true>), grid, dim3(256), 0, p->stream, A);
     else
-        hipLaunchKernelGGL(synthesis_kernel<false>, grid, dim3(256), 0, p->stream, A);
+        hipLaunchKernelGGL((synthesis_kernel<T, false>), grid, dim3(256), 0, p->stream, A);
     RD_LAUNCH_CHECK();
     return RD_OK;
+}
+
+int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q, const double2 *s,
+                     const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2)
+{
+    if (prec == RD_PREC_FP64)
+        return launch_synthesis_t<double>(p, p->syn_P, p->syn_D, p->syn_M, vec, nplanes, q, s, t, ld, f0, f1,
+                                          f2);
+    int rc = ensure_precision_tables(p, prec);
+    if (rc) return rc;
+    if (prec == RD_PREC_FP32)
+        return launch_synthesis_t<float>(p, p->syn32_P, p->syn32_D, p->syn32_M, vec, nplanes, q, s, t, ld, f0,
+                                         f1, f2);
+    return launch_synthesis_bf3(p, vec, nplanes, q, s, t, ld, f0, f1, f2);
 }
 
 // -------------------------------------------------------------- analysis
@@ -444,8 +482,9 @@ struct AnaCfg {
     static constexpr int JTP = JT + 2;
 };
 
+template <typename T>
 struct AnaArgs {
-    const double *tP, *tD, *tM;
+    const T *tP, *tD, *tM;
     const long *off;
     const int *ncol;
     const int *queue;
@@ -455,13 +494,15 @@ struct AnaArgs {
     int lmax, nlat, nth, nthp16, nplanes, n_jt;
 };
 
-template <bool VEC>
-__global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
+// T as in synthesis_kernel.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void analysis_kernel(AnaArgs<T> A)
 {
     using Cfg = AnaCfg<VEC>;
+    using V = typename Mf<T>::V;
     constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
-    __shared__ double Gn[NF][ATC][JTP];
-    __shared__ double Gs[NF][ATC][JTP];
+    __shared__ T Gn[NF][ATC][JTP];
+    __shared__ T Gs[NF][ATC][JTP];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
@@ -475,17 +516,17 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
     const int wl = wave & 1, wj = wave >> 1;
     const int k0 = jt * (JT / 2);
     const int col = lane & 15, kk = lane >> 4;
-    const double sgn_mi = (col & 1) ? -1.0 : 1.0;   // (-i X)_j = sgn_mi * X_{j^1}
+    const T sgn_mi = (col & 1) ? T(-1) : T(1);   // (-i X)_j = sgn_mi * X_{j^1}
     const int ncol = A.ncol[m];
     const int lp0 = lt * 32 + wl * 16;
 
-    v4d acc[NF][2][JB];
+    V acc[NF][2][JB];
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
         for (int s = 0; s < 2; s++)
 #pragma unroll
-            for (int b = 0; b < JB; b++) acc[f][s][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+            for (int b = 0; b < JB; b++) acc[f][s][b] = (V){0, 0, 0, 0};
 
     const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
 
@@ -501,18 +542,18 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
                 vn = A.in[f][base + th];
                 vs = A.in[f][base + A.nlat - 1 - th];
             }
-            Gn[f][tl][2 * pl] = vn.x;
-            Gn[f][tl][2 * pl + 1] = vn.y;
-            Gs[f][tl][2 * pl] = vs.x;
-            Gs[f][tl][2 * pl + 1] = vs.y;
+            Gn[f][tl][2 * pl] = (T)vn.x;
+            Gn[f][tl][2 * pl + 1] = (T)vn.y;
+            Gs[f][tl][2 * pl] = (T)vs.x;
+            Gs[f][tl][2 * pl + 1] = (T)vs.y;
         }
         __syncthreads();
 #pragma unroll
         for (int st = 0; st < ATC / 4; st++) {
             const int tl = 4 * st + kk;
             const long rowoff = (long)(tc + tl) * ncol + lp0 + col;
-            const double Pe = A.tP[offe + rowoff], Po = A.tP[offo + rowoff];
-            double De = 0, Do = 0, Me = 0, Mo = 0;
+            const T Pe = A.tP[offe + rowoff], Po = A.tP[offo + rowoff];
+            T De = 0, Do = 0, Me = 0, Mo = 0;
             if (VEC) {
                 De = A.tD[offe + rowoff];
                 Do = A.tD[offo + rowoff];
@@ -522,26 +563,26 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = (wj * JB + b) * 16 + col;
-                const double rn = Gn[0][tl][j], rs = Gs[0][tl][j];
-                acc[0][0][b] = mfma(Pe, rn + rs, acc[0][0][b]);
-                acc[0][1][b] = mfma(Po, rn - rs, acc[0][1][b]);
+                const T rn = Gn[0][tl][j], rs = Gs[0][tl][j];
+                acc[0][0][b] = Mf<T>::mma(Pe, rn + rs, acc[0][0][b]);
+                acc[0][1][b] = Mf<T>::mma(Po, rn - rs, acc[0][1][b]);
                 if (VEC) {
-                    const double tn = Gn[1][tl][j], ts = Gs[1][tl][j];
-                    const double pn = Gn[2][tl][j], ps = Gs[2][tl][j];
-                    const double mtn = sgn_mi * Gn[1][tl][j ^ 1], mts = sgn_mi * Gs[1][tl][j ^ 1];
-                    const double mpn = sgn_mi * Gn[2][tl][j ^ 1], mps = sgn_mi * Gs[2][tl][j ^ 1];
+                    const T tn = Gn[1][tl][j], ts = Gs[1][tl][j];
+                    const T pn = Gn[2][tl][j], ps = Gs[2][tl][j];
+                    const T mtn = sgn_mi * Gn[1][tl][j ^ 1], mts = sgn_mi * Gs[1][tl][j ^ 1];
+                    const T mpn = sgn_mi * Gn[2][tl][j ^ 1], mps = sgn_mi * Gs[2][tl][j ^ 1];
                     // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
-                    acc[1][0][b] = mfma(De, tn - ts, acc[1][0][b]);
-                    acc[1][0][b] = mfma(Me, mpn + mps, acc[1][0][b]);
+                    acc[1][0][b] = Mf<T>::mma(De, tn - ts, acc[1][0][b]);
+                    acc[1][0][b] = Mf<T>::mma(Me, mpn + mps, acc[1][0][b]);
                     // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
-                    acc[1][1][b] = mfma(Do, tn + ts, acc[1][1][b]);
-                    acc[1][1][b] = mfma(Mo, mpn - mps, acc[1][1][b]);
+                    acc[1][1][b] = Mf<T>::mma(Do, tn + ts, acc[1][1][b]);
+                    acc[1][1][b] = Mf<T>::mma(Mo, mpn - mps, acc[1][1][b]);
                     // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
-                    acc[2][0][b] = mfma(Me, mtn + mts, acc[2][0][b]);
-                    acc[2][0][b] = mfma(De, ps - pn, acc[2][0][b]);
+                    acc[2][0][b] = Mf<T>::mma(Me, mtn + mts, acc[2][0][b]);
+                    acc[2][0][b] = Mf<T>::mma(De, ps - pn, acc[2][0][b]);
                     // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
-                    acc[2][1][b] = mfma(Mo, mtn - mts, acc[2][1][b]);
-                    acc[2][1][b] = mfma(Do, -(pn + ps), acc[2][1][b]);
+                    acc[2][1][b] = Mf<T>::mma(Mo, mtn - mts, acc[2][1][b]);
+                    acc[2][1][b] = Mf<T>::mma(Do, -(pn + ps), acc[2][1][b]);
                 }
             }
         }
@@ -561,7 +602,7 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
                 if (k >= A.nplanes) continue;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int lp = lp0 + kk + 4 * v;
+                    const int lp = lp0 + Mf<T>::row(kk, v);
                     const int l = m + par + 2 * lp;
                     if (l > A.lmax) continue;
                     double val = acc[f][par][b][v];
@@ -572,13 +613,15 @@ __global__ __launch_bounds__(256) void analysis_kernel(AnaArgs A)
     }
 }
 
-int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
-                    const double2 *g2, double2 *q, double2 *s, double2 *t, long ld)
+template <typename T>
+static int launch_analysis_t(Plan *p, const T *tP, const T *tD, const T *tM, bool vec, int nplanes,
+                             const double2 *g0, const double2 *g1, const double2 *g2, double2 *q, double2 *s,
+                             double2 *t, long ld)
 {
-    AnaArgs A;
-    A.tP = p->ana_P;
-    A.tD = p->ana_D;
-    A.tM = p->ana_M;
+    AnaArgs<T> A;
+    A.tP = tP;
+    A.tD = tD;
+    A.tM = tM;
     A.off = p->d_ana_off;
     A.ncol = p->d_ana_ncol;
     A.queue = p->d_ana_q;
@@ -598,11 +641,25 @@ int launch_analysis(Plan *p, bool vec, int nplanes, const double2 *g0, const dou
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->ana_qlen * A.n_jt));
     if (vec)
-        hipLaunchKernelGGL(analysis_kernel<true>, grid, dim3(256), 0, p->stream, A);
+        hipLaunchKernelGGL((analysis_kernel<T, true>), grid, dim3(256), 0, p->stream, A);
     else
-        hipLaunchKernelGGL(analysis_kernel<false>, grid, dim3(256), 0, p->stream, A);
+        hipLaunchKernelGGL((analysis_kernel<T, false>), grid, dim3(256), 0, p->stream, A);
     RD_LAUNCH_CHECK();
     return RD_OK;
+}
+
+int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0, const double2 *g1,
+                    const double2 *g2, double2 *q, double2 *s, double2 *t, long ld)
+{
+    if (prec == RD_PREC_FP64)
+        return launch_analysis_t<double>(p, p->ana_P, p->ana_D, p->ana_M, vec, nplanes, g0, g1, g2, q, s, t,
+                                         ld);
+    int rc = ensure_precision_tables(p, prec);
+    if (rc) return rc;
+    if (prec == RD_PREC_FP32)
+        return launch_analysis_t<float>(p, p->ana32_P, p->ana32_D, p->ana32_M, vec, nplanes, g0, g1, g2, q, s,
+                                        t, ld);
+    return launch_analysis_bf3(p, vec, nplanes, g0, g1, g2, q, s, t, ld);
 }
 
 }  // namespace rd

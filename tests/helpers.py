@@ -224,3 +224,28 @@ def boundary_errs(z, ell, N, d1, z0, w0, f0, tor, fi, pol, pol2, tor3, fi3, pol3
 # FC-Gram tables built here: walls exact, dW/dr 1e-14, interior phi 3e-12,
 # twice 2e-13, W vs W0 6e-8 (truncation), fast == full, fast phi 3e-12
 BOUNDARY_LIMITS = (0.0, 0.0, 1e-13, 1e-12, 0.0, 3e-11, 2e-12, 6e-7, 0.0, 3e-11)
+
+
+def l_bands(lmax):
+    """Degree bands [lo, hi) for error reports: 0-15, 16-31, then octaves."""
+    edges = [0, 16, 32]
+    while edges[-1] <= lmax:
+        edges.append(edges[-1] * 2)
+    edges[-1] = lmax + 1
+    return [(a, b) for a, b in zip(edges[:-1], edges[1:]) if a <= lmax]
+
+
+def band_errors(out, ref, lmax, nr):
+    """Relative L2 error of each output field (NQ, NS, NT) on the nr physical
+    shells, over all degrees and per degree band."""
+    ell = ell_of(lmax)
+    rep = {}
+    for name, o, r in zip(("NQ", "NS", "NT"), out, ref):
+        o, r = o[:nr], r[:nr]
+        d = {"all": rell2(o, r)}
+        for a, b in l_bands(lmax):
+            sel = (ell >= a) & (ell < b)
+            d[f"l{a}-{b - 1}"] = rell2(o[:, sel], r[:, sel])
+        rep[name] = d
+    rep["joint"] = rell2_joint([o[:nr] for o in out], [r[:nr] for r in ref])
+    return rep

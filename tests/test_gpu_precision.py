@@ -1,0 +1,83 @@
+"""Precision forms of the nonlinear term: Legendre contractions on FP32 MFMA
+and as BF16x3 splits with FP32 accumulation (tables, nodes, weights and the
+radial FC-Gram step stay FP64).  Their error is reported per output field and
+per degree band against the FP64 path and the oracle.
+
+Expected levels: FP32 rounding of tables and operands gives about 1e-7
+relative per contraction; BF16x3 keeps about 17 significant bits per operand
+(2^-17 = 8e-6).  The bounds below are those levels with headroom: the forms
+are reported, not held to the FP64 bar.
+"""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+BOUND = {1: 2e-6, 2: 5e-5}     # joint relative L2 vs FP64
+NAMES = {1: "fp32", 2: "bf16x3"}
+
+
+def _run(lmax, nr, seed):
+    from redondito_amd import sphp as S
+    C, d, ri, ro = 25, 5, 0.35, 1.0
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    plan = S.FCPlan(lmax, nlat, 2 * nlat, n, C, d, dz, ri)
+    W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=seed, slope=2.0)
+    Wd, Zd = (torch.from_numpy(x).to("cuda:0") for x in (W, Z))
+    outs = {}
+    for prec in (0, 1, 2):
+        o = [torch.zeros_like(Wd) for _ in range(3)]
+        S.nonlinear_uxw(plan, Wd, Zd, *o, prec=prec)
+        torch.cuda.synchronize()
+        outs[prec] = [x.cpu().numpy() for x in o]
+    return plan, W, Z, outs, dz, nlat
+
+
+@pytest.mark.parametrize("lmax,nr", [(31, 24), (127, 96), (255, 128)])
+def test_precision_forms_vs_fp64(lmax, nr):
+    plan, W, Z, outs, dz, nlat = _run(lmax, nr, seed=20261203)
+    for prec in (1, 2):
+        rep = H.band_errors(outs[prec], outs[0], lmax, nr)
+        print(f"[{NAMES[prec]} vs fp64 lmax={lmax} nr={nr}] " + json.dumps(
+            {k: (v if isinstance(v, float) else {b: float(f"{e:.2e}") for b, e in v.items()})
+             for k, v in rep.items()}))
+        assert rep["joint"] < BOUND[prec], (prec, rep["joint"])
+        # every band and field carries the same relative accuracy (no band
+        # loses digits): within 100x of the joint error
+        for f in ("NQ", "NS", "NT"):
+            assert rep[f]["all"] < 1e3 * BOUND[prec]
+    # the two reduced forms are independent roundings: they differ from each
+    # other by about the larger of their errors, not by 0
+    assert H.rell2_joint(outs[1], outs[2]) > 0
+
+
+def test_precision_forms_vs_oracle_small():
+    """Against the CPU oracle directly at lmax = 31, nr = 24."""
+    lmax, nr = 31, 24
+    plan, W, Z, outs, dz, nlat = _run(lmax, nr, seed=7)
+    A, Q = O.load_tables(__import__("redondito_amd.sphp", fromlist=["x"]).TABLE_DIR, 25, 5)
+    sht = O.SHT(lmax, nlat, 2 * nlat)
+    fc = O.FC(nr + 25, 25, 5, O.fc_dir(A, Q))
+    ref = O.nonlinear_uxw(sht, fc, 0.35, dz, W, Z)
+    assert H.band_errors(outs[0], ref, lmax, nr)["joint"] < 1e-12
+    for prec in (1, 2):
+        e = H.band_errors(outs[prec], ref, lmax, nr)["joint"]
+        print(f"{NAMES[prec]} vs oracle: {e:.2e}")
+        assert e < BOUND[prec]
+
+
+def test_bad_precision_rejected():
+    from redondito_amd import sphp as S
+    plan = S.FCPlan(7, 12, 24, 40, 25, 5, 0.1, 0.5)
+    W = torch.zeros((40, plan.nlm), dtype=torch.complex128, device="cuda:0")
+    o = [torch.zeros_like(W) for _ in range(3)]
+    with pytest.raises(S.RdError):
+        S.nonlinear_uxw(plan, W, W, *o, prec=3)
