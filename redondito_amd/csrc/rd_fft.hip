@@ -44,6 +44,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // sign * i * a
 __device__ __forceinline__ double2 cmuli(double2 a, int sign)
 {
@@ -484,115 +485,170 @@ __device__ __forceinline__ double2 twq(const double2 *tw, int t, int sign)
     return sign > 0 ? make_double2(r.x, -r.y) : r;
 }
 
-__global__ __launch_bounds__(256) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
-                                                           const double2 *tw)
+// One (plane, theta) row per workgroup, in place in a single LDS buffer so
+// that four rows run per CU.  Every phase reads its inputs into registers,
+// waits at a barrier, then writes its outputs over the same buffer:
+//   R3  radix-3 step of the inverse transform of the packed sequences
+//       z_x = u_x + i w_x (x = r, theta, phi): the spectrum block 256..511 is
+//       zero, so Y[3x+r][b] = (Z[b] + e^{2 pi i 512 r/768} Z[512+b]) e^{2 pi i b r/768}
+//   P1  first 16-point pass of the 256-point inverse FFTs over b = 16 b2 + b1
+//   P2  second inverse pass: grid values z_x at j = 3 (q1 + 16 q2) + r
+//   CR  N = u x w packed as (N_r + i N_theta, N_phi)
+//   F1  first forward pass over n1 = 16 a2 + a1 of j = 3 n1 + n2
+//   F2  second forward pass: Yf[3s+n2][k] = 256-point spectra
+//   FIN X[m] = sum_n2 e^{-2 pi i n2 m/768} Yf[n2][m mod 256], unpacking of the
+//       real spectra, scaling and store.
+// The 16 x 16 transposes use an XOR swizzle (column q ^ row) instead of
+// padding; twiddles come from a quarter table of the 768 roots.
+constexpr int PHI_BUF = 9 * 256;
+
+__global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                              const double2 *tw)
 {
     constexpr int N = 768, NQ = 192;
     __shared__ double2 twl[NQ];
-    __shared__ double2 bufA[9 * 256];
-    __shared__ double2 bufB[144 * 17];   // 16 x 16 transposes padded to 17
+    __shared__ double2 B[PHI_BUF];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int k = t / nlat, th = t % nlat;
     if (k >= nplanes) return;
     const int L = lmax + 1;
-    for (int i = threadIdx.x; i < NQ; i += blockDim.x) twl[i] = tw[i];
-    // the row's six coefficient vectors, m = 0..255 (zero above lmax), into bufB
-    for (int e = threadIdx.x; e < 6 * 256; e += blockDim.x) {
-        const int f = e >> 8, m = e & 255;
-        double2 v = make_double2(0.0, 0.0);
-        if (m <= lmax) v = P.F[f][((long)k * L + m) * nlat + th];
-        if (m == 0) v.y = 0.0;
-        bufB[e] = v;
-    }
-    __syncthreads();
     const int tid = threadIdx.x;
+    if (tid < NQ) twl[tid] = tw[tid];
+    // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax)
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
+        if (tid == 0) v.y = 0.0;
+        B[f * 256 + tid] = v;
+    }
+    __syncthreads();
 
-    // ---- radix-3 step: Y_r[b] = (Z[b] + w3^{2r} Z[512 + b]) w768^{b r}, into bufA
-    for (int e = tid; e < 9 * 256; e += blockDim.x) {
-        const int xr = e >> 8, b = e & 255;
-        const int x = xr / 3, r = xr - 3 * x;
-        const double2 u = bufB[x * 256 + b], w = bufB[(3 + x) * 256 + b];
-        const double2 zb = make_double2(u.x - w.y, u.y + w.x);
-        const int mm = (256 - b) & 255;          // mirror of m = 512 + b (m = 512 is zero)
-        const double2 uc = bufB[x * 256 + mm], wc = bufB[(3 + x) * 256 + mm];
-        const double2 zc = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
-        const double2 w3 = twq(twl, (512 * r) % N, +1);
-        bufA[e] = cmul(cadd(zb, cmul(zc, w3)), twq(twl, b * r, +1));
-    }
-    __syncthreads();
-    // ---- inverse, first 16-point pass over b2 of b = 16 b2 + b1: bufA -> bufB
-    if (tid < 144) {
-        const int xr = tid / 16, b1 = tid % 16;
-        double2 v[16];
+    // ---- R3 -> Y[3x+r][b]
+    {
+        const int b = tid, mm = (256 - b) & 255;
+        double2 zb[3], zc[3];
 #pragma unroll
-        for (int b2 = 0; b2 < 16; b2++) v[b2] = bufA[xr * 256 + 16 * b2 + b1];
-        dft16(v, +1);
-#pragma unroll
-        for (int q1 = 0; q1 < 16; q1++)
-            bufB[(xr * 16 + b1) * 17 + q1] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
-    }
-    __syncthreads();
-    // ---- inverse, second pass over b1 -> grid values u_x + i w_x at j = 3 q + r, into bufA
-    if (tid < 144) {
-        const int xr = tid / 16, q1 = tid % 16;
-        const int x = xr / 3, r = xr - 3 * x;
-        double2 v[16];
-#pragma unroll
-        for (int b1 = 0; b1 < 16; b1++) v[b1] = bufB[(xr * 16 + b1) * 17 + q1];
-        dft16(v, +1);
-#pragma unroll
-        for (int q2 = 0; q2 < 16; q2++) bufA[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
-    }
-    __syncthreads();
-    // ---- N = u x w (prodvec), packed as (N_r + i N_theta, N_phi)
-    for (int j = tid; j < N; j += blockDim.x) {
-        const double2 zr = bufA[j], zt = bufA[N + j], zp = bufA[2 * N + j];
-        const double nr = zt.x * zp.y - zp.x * zt.y;
-        const double nt = zp.x * zr.y - zr.x * zp.y;
-        const double np = zr.x * zt.y - zt.x * zr.y;
-        bufA[j] = make_double2(nr, nt);
-        bufA[N + j] = make_double2(np, 0.0);
-    }
-    __syncthreads();
-    // ---- forward, first pass over a2 of n1 = 16 a2 + a1, j = 3 n1 + n2
-    if (tid < 96) {
-        const int s = tid / 48, rem = tid % 48, n2 = rem / 16, a1 = rem % 16;
-        double2 v[16];
-#pragma unroll
-        for (int a2 = 0; a2 < 16; a2++) v[a2] = bufA[s * N + 3 * (16 * a2 + a1) + n2];
-        dft16(v, -1);
-#pragma unroll
-        for (int k1 = 0; k1 < 16; k1++)
-            bufB[((s * 3 + n2) * 16 + a1) * 17 + k1] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
-    }
-    __syncthreads();
-    // ---- forward, second pass over a1 -> Yf[s][n2][k1 + 16 k2]
-    if (tid < 96) {
-        const int s = tid / 48, rem = tid % 48, n2 = rem / 16, k1 = rem % 16;
-        double2 v[16];
-#pragma unroll
-        for (int a1 = 0; a1 < 16; a1++) v[a1] = bufB[((s * 3 + n2) * 16 + a1) * 17 + k1];
-        dft16(v, -1);
-#pragma unroll
-        for (int k2 = 0; k2 < 16; k2++) bufA[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
-    }
-    __syncthreads();
-    // ---- X_s[m] = sum_n2 w768^{-n2 m} Yf[s][n2][m mod 256], unpack, scale, store
-    const double sc = 0.5 * (2.0 * M_PI / N);
-    for (int m = tid; m < L; m += blockDim.x) {
-        const int mneg = (256 - m) & 255;
-        double2 xp = make_double2(0.0, 0.0), xm = xp, x1 = xp;
-#pragma unroll
-        for (int n2 = 0; n2 < 3; n2++) {
-            xp = cadd(xp, cmul(bufA[n2 * 256 + m], twq(twl, n2 * m, -1)));
-            xm = cadd(xm, cmul(bufA[n2 * 256 + mneg], twq(twl, n2 * m, +1)));
-            x1 = cadd(x1, cmul(bufA[(3 + n2) * 256 + m], twq(twl, n2 * m, -1)));
+        for (int x = 0; x < 3; x++) {
+            const double2 u = B[x * 256 + b], w = B[(3 + x) * 256 + b];
+            const double2 uc = B[x * 256 + mm], wc = B[(3 + x) * 256 + mm];
+            zb[x] = make_double2(u.x - w.y, u.y + w.x);
+            zc[x] = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
         }
+        __syncthreads();
+        const double2 w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
+        const double2 t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            B[(3 * x) * 256 + b] = cadd(zb[x], zc[x]);
+            B[(3 * x + 1) * 256 + b] = cmul(cadd(zb[x], cmul(zc[x], w3a)), t1);
+            B[(3 * x + 2) * 256 + b] = cmul(cadd(zb[x], cmul(zc[x], w3b)), t2);
+        }
+    }
+    __syncthreads();
+    // ---- P1: T[xr][b1][q1] = e^{2 pi i b1 q1/256} sum_b2 Y[xr][16 b2 + b1] e^{2 pi i b2 q1/16}
+    {
+        const int xr = tid >> 4, b1 = tid & 15;
+        double2 v[16];
+        if (tid < 144) {
+#pragma unroll
+            for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
+        }
+        __syncthreads();
+        if (tid < 144) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q1 = 0; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
+        }
+    }
+    __syncthreads();
+    // ---- P2: grid values z_x at j = 3 (q1 + 16 q2) + r, stored at x N + j
+    {
+        const int xr = tid >> 4, q1 = tid & 15;
+        const int x = xr / 3, r = xr - 3 * x;
+        double2 v[16];
+        if (tid < 144) {
+#pragma unroll
+            for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
+        }
+        __syncthreads();
+        if (tid < 144) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q2 = 0; q2 < 16; q2++) B[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
+        }
+    }
+    __syncthreads();
+    // ---- CR: (N_r + i N_theta) at j, N_phi at N + j
+    {
+        double2 o0[3], o1[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int j = tid + 256 * i;
+            const double2 zr = B[j], zt = B[N + j], zp = B[2 * N + j];
+            o0[i] = make_double2(zt.x * zp.y - zp.x * zt.y, zp.x * zr.y - zr.x * zp.y);
+            o1[i] = make_double2(zr.x * zt.y - zt.x * zr.y, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            B[tid + 256 * i] = o0[i];
+            B[N + tid + 256 * i] = o1[i];
+        }
+    }
+    __syncthreads();
+    // ---- F1: F[3s+n2][a1][k1] = e^{-2 pi i a1 k1/256} sum_a2 P_s[3 (16 a2 + a1) + n2] e^{-2 pi i a2 k1/16}
+    {
+        const int s = tid / 48, rem = tid % 48, n2 = rem >> 4, a1 = rem & 15;
+        double2 v[16];
+        if (tid < 96) {
+#pragma unroll
+            for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 3 * (16 * a2 + a1) + n2];
+        }
+        __syncthreads();
+        if (tid < 96) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++)
+                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
+        }
+    }
+    __syncthreads();
+    // ---- F2: Yf[3s+n2][k1 + 16 k2]
+    {
+        const int s = tid / 48, rem = tid % 48, n2 = rem >> 4, k1 = rem & 15;
+        double2 v[16];
+        if (tid < 96) {
+#pragma unroll
+            for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
+        }
+        __syncthreads();
+        if (tid < 96) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k2 = 0; k2 < 16; k2++) B[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
+        }
+    }
+    __syncthreads();
+    // ---- FIN
+    const double sc = 0.5 * (2.0 * M_PI / N);
+    const int m = tid;
+    if (m < L) {
+        const int mneg = (256 - m) & 255;
+        const double2 c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
+        const double2 xp = cadd(B[m], cadd(cmul(B[256 + m], c1), cmul(B[512 + m], c2)));
+        const double2 x1 = cadd(B[768 + m], cadd(cmul(B[1024 + m], c1), cmul(B[1280 + m], c2)));
+        double2 xm = cadd(B[mneg], cadd(cmul(B[256 + mneg], cconj(c1)), cmul(B[512 + mneg], cconj(c2))));
         if (m == 0) xm = xp;
         double2 gr = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
         double2 gt = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
         double2 gp = cscale(x1, 2.0 * sc);
-        if (m == 0) { gr.y = 0.0; gt.y = 0.0; gp.y = 0.0; }
+        if (m == 0) {
+            gr.y = 0.0;
+            gt.y = 0.0;
+            gp.y = 0.0;
+        }
         const long o = ((long)k * L + m) * nlat + th;
         P.Fo[0][o] = gr;
         P.Fo[1][o] = gt;

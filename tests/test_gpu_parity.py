@@ -347,14 +347,19 @@ def test_prueba_3d_periodic():
 
 
 # -------------------------------------------------------- nonlinear term
-@pytest.mark.parametrize("lmax,nr", [(15, 24), (31, 24), (63, 40), (127, 96), (255, 12)])
-def test_nonlinear_uxw_vs_oracle(lmax, nr):
+@pytest.mark.parametrize("lmax,nr,grid", [(15, 24, None), (31, 24, None), (63, 40, None), (127, 96, None),
+                                          (255, 12, None),
+                                          # nphi = 768 kernel with m > lmax zero, rows not a multiple of 8
+                                          (200, 11, (386, 768))])
+def test_nonlinear_uxw_vs_oracle(lmax, nr, grid):
     S = sphp()
     C, d, rint, rext = 25, 5, 0.35, 1.0
     n = nr + C
     dz = (rext - rint) / (nr - 1)
     nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
     nphi = 2 * nlat
+    if grid is not None:
+        nlat, nphi = grid
     plan = S.FCPlan(lmax, nlat, nphi, n, C, d, dz, rint)
     W, Z = H.shell_fields(lmax, nr, n, rint, rext, seed=20261203 + lmax, slope=2.0)
     oc = O.SHT(lmax, nlat, nphi)
