@@ -496,7 +496,7 @@ struct AnaArgs {
 
 // T as in synthesis_kernel.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void analysis_kernel(AnaArgs<T> A)
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void analysis_kernel(AnaArgs<T> A)
 {
     using Cfg = AnaCfg<VEC>;
     using V = typename Mf<T>::V;
