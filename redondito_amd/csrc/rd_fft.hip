@@ -907,6 +907,278 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq se
     }
 }
 
+// ------------------------------------------- factorised radial step, n = N1 N2
+// cos and sin of 2 pi m / R, correctly rounded.
+template <int R> struct DftConst;
+template <> struct DftConst<7> {
+    static constexpr double c[7] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191,
+                                    -0.9009688679024191, -0.2225209339563144, 0.6234898018587335};
+    static constexpr double s[7] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581,
+                                    -0.4338837391175581, -0.9749279121818236, -0.7818314824680298};
+};
+template <> struct DftConst<9> {
+    static constexpr double c[9] = {1.0, 0.766044443118978, 0.17364817766693036, -0.5, -0.9396926207859084,
+                                    -0.9396926207859084, -0.5, 0.17364817766693036, 0.766044443118978};
+    static constexpr double s[9] = {0.0, 0.6427876096865394, 0.984807753012208, 0.8660254037844386,
+                                    0.3420201433256687, -0.3420201433256687, -0.8660254037844386,
+                                    -0.984807753012208, -0.6427876096865394};
+};
+template <> struct DftConst<11> {
+    static constexpr double c[11] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
+                                     -0.6548607339452851, -0.9594929736144974, -0.9594929736144974,
+                                     -0.6548607339452851, -0.14231483827328514, 0.41541501300188644,
+                                     0.8412535328311812};
+    static constexpr double s[11] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
+                                     0.7557495743542583, 0.28173255684142967, -0.28173255684142967,
+                                     -0.7557495743542583, -0.9898214418809327, -0.9096319953545183,
+                                     -0.5406408174555976};
+};
+template <> struct DftConst<17> {
+    static constexpr double c[17] = {1.0, 0.9324722294043558, 0.7390089172206591, 0.44573835577653825,
+                                     0.092268359463302, -0.2736629900720829, -0.6026346363792564,
+                                     -0.8502171357296141, -0.9829730996839018, -0.9829730996839018,
+                                     -0.8502171357296141, -0.6026346363792564, -0.2736629900720829,
+                                     0.092268359463302, 0.44573835577653825, 0.7390089172206591,
+                                     0.9324722294043558};
+    static constexpr double s[17] = {0.0, 0.3612416661871529, 0.6736956436465572, 0.8951632913550623,
+                                     0.9957341762950345, 0.961825643172819, 0.7980172272802395,
+                                     0.5264321628773558, 0.18374951781657034, -0.18374951781657034,
+                                     -0.5264321628773558, -0.7980172272802395, -0.961825643172819,
+                                     -0.9957341762950345, -0.8951632913550623, -0.6736956436465572,
+                                     -0.3612416661871529};
+};
+
+// In-register DFT of odd length R, X[k] = sum_n x[n] e^{sign 2 pi i n k / R},
+// pairing n with R - n (a_j = x_j + x_{R-j}, b_j = x_j - x_{R-j}):
+//   X[k]   = x_0 + sum_j a_j cos(2 pi j k / R) + sign i sum_j b_j sin(2 pi j k / R)
+//   X[R-k] = the same with -sign.
+template <int R>
+__device__ __forceinline__ void dft_odd(double2 (&x)[R], int sign)
+{
+    constexpr int H = (R - 1) / 2;
+    double2 a[H], b[H];
+    double2 s0 = x[0];
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        a[j - 1] = cadd(x[j], x[R - j]);
+        b[j - 1] = csub(x[j], x[R - j]);
+        s0 = cadd(s0, a[j - 1]);
+    }
+    const double sg = sign > 0 ? 1.0 : -1.0;
+    double2 y[R];
+    y[0] = s0;
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        double2 C = x[0], S = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            const int m = (j * k) % R;
+            C.x += a[j - 1].x * DftConst<R>::c[m];
+            C.y += a[j - 1].y * DftConst<R>::c[m];
+            S.x += b[j - 1].x * DftConst<R>::s[m];
+            S.y += b[j - 1].y * DftConst<R>::s[m];
+        }
+        y[k] = make_double2(C.x - sg * S.y, C.y + sg * S.x);
+        y[R - k] = make_double2(C.x + sg * S.y, C.y - sg * S.x);
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) x[k] = y[k];
+}
+
+// The radial step of the nonlinear term for n = N1 N2 (n = 153 = 17 x 9 at
+// nr = 128, 121 = 11 x 11, 49 = 7 x 7), CB lm columns per workgroup.  With
+// r = n2 + N2 n1 and k = k1 + N1 k2:
+//   X[k] = sum_n2 e^{-2 pi i n2 k2/N2} e^{-2 pi i n2 k1/n} sum_n1 x[r] e^{-2 pi i n1 k1/N1}
+// Pass A (thread = column, n2) holds the N1 rows n2 + N2 n1 of a column: it
+// loads them straight from HBM (coalesced over the columns), applies the
+// FC-Gram continuation for the rows past nr in registers and does the N1-point
+// DFTs; pass B (thread = column, k1) does the N2-point DFTs and keeps the W and
+// Z spectra in registers, from which it starts the three inverse transforms
+// (i kz W^, -kz^2 W^, i kz Z^, each / n).  One LDS buffer of CB n values
+// carries every transpose.
+template <int N1, int N2>
+struct RadFact {
+    static constexpr int N = N1 * N2;
+    static constexpr int CB = 256 / (N1 > N2 ? N1 : N2);
+};
+
+template <int N1, int N2>
+__global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
+{
+    constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
+    __shared__ double2 buf[CB * N];
+    __shared__ double2 twl[N];   // e^{-2 pi i t / n}
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N; t += 256) twl[t] = tw[t];
+    const int nr = N - A.C;
+    const int c0 = blockIdx.x * CB;
+    const bool inA = tid < CB * N2, inB = tid < CB * N1;
+    const int ca = tid % CB, n2 = tid / CB;   // pass A
+    const int cb = tid % CB, k1 = tid / CB;   // pass B
+    const int lma = c0 + ca;
+    const bool okA = inA && lma < A.ncols;
+    const double inv_n = 1.0 / N;
+    double ll = 0.0;
+    if (okA) ll = (double)A.ell[lma] * (A.ell[lma] + 1);
+    __syncthreads();
+
+    // rows n2 + N2 n1 of column lma of src, the continuation rows from the table
+    auto load_rows = [&](const double2 *src, double2 (&x)[N1]) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const int r = n2 + N2 * n1;
+            double2 v = make_double2(0.0, 0.0);
+            if (okA) {
+                if (r < nr) {
+                    v = src[(long)r * A.ld + lma];
+                } else {
+                    const int ii = r - nr + 1;
+                    for (int jj = 1; jj <= A.d; jj++) {
+                        const double a1 = A.dir[(ii - 1) + A.C * (jj - 1)];
+                        const double a2 = A.dir[(A.C - ii) + A.C * (jj - 1)];
+                        const double2 xr = src[(long)(nr - A.d + jj - 1) * A.ld + lma];
+                        const double2 xl = src[(long)(A.d - jj) * A.ld + lma];
+                        v.x += a1 * xr.x + a2 * xl.x;
+                        v.y += a1 * xr.y + a2 * xl.y;
+                    }
+                }
+            }
+            x[n1] = v;
+        }
+    };
+    // pass A forward: N1-point DFTs, twiddles, to LDS as [column][k1][n2]
+    auto fwd_a = [&](double2 (&x)[N1]) {
+        dft_odd<N1>(x, -1);
+#pragma unroll
+        for (int q = 0; q < N1; q++) buf[ca * N + q * N2 + n2] = cmul(x[q], twl[(n2 * q) % N]);
+    };
+    // pass B forward: N2-point DFTs -> X[k1 + N1 k2]
+    auto fwd_b = [&](double2 (&y)[N2]) {
+#pragma unroll
+        for (int q = 0; q < N2; q++) y[q] = buf[cb * N + k1 * N2 + q];
+        dft_odd<N2>(y, -1);
+    };
+    // pass B inverse: N2-point DFTs of a spectrum, twiddles, to LDS as [column][n2][k1]
+    auto inv_b = [&](double2 (&y)[N2]) {
+        dft_odd<N2>(y, +1);
+#pragma unroll
+        for (int q = 0; q < N2; q++) buf[cb * N + q * N1 + k1] = cmul(y[q], cconj(twl[(q * k1) % N]));
+    };
+    // pass A inverse: N1-point DFTs -> values at rows n2 + N2 n1
+    auto inv_a = [&](double2 (&x)[N1]) {
+#pragma unroll
+        for (int q = 0; q < N1; q++) x[q] = buf[ca * N + n2 * N1 + q];
+        dft_odd<N1>(x, +1);
+    };
+
+    double2 XW[N2], XZ[N2], x[N1], y[N2];
+    // ---- W: u_Q = l(l+1) W / r^2, forward transform
+    if (inA) {
+        load_rows(A.W, x);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                if (r < nr) {
+                    const double rr = A.z[r];
+                    A.out[0][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
+                }
+            }
+        fwd_a(x);
+    }
+    __syncthreads();
+    if (inB) fwd_b(XW);
+    __syncthreads();
+    // ---- Z: u_T = Z / r, w_Q = l(l+1) Z / r^2, forward transform
+    if (inA) {
+        load_rows(A.Z, x);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                if (r < nr) {
+                    const double rr = A.z[r];
+                    A.out[2][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / rr);
+                    A.out[3][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
+                }
+            }
+        fwd_a(x);
+    }
+    __syncthreads();
+    if (inB) fwd_b(XZ);
+    __syncthreads();
+    // ---- u_S = dW / r
+    if (inB) {
+#pragma unroll
+        for (int q = 0; q < N2; q++) {
+            const double k = A.kz[k1 + N1 * q] * inv_n;
+            y[q] = make_double2(-k * XW[q].y, k * XW[q].x);
+        }
+        inv_b(y);
+    }
+    __syncthreads();
+    if (inA) {
+        inv_a(x);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                if (r < nr) A.out[1][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+            }
+    }
+    __syncthreads();
+    // ---- w_T = -(d2W - l(l+1) W / r^2) / r
+    if (inB) {
+#pragma unroll
+        for (int q = 0; q < N2; q++) {
+            const double k = A.kz[k1 + N1 * q];
+            y[q] = cscale(XW[q], -k * k * inv_n);
+        }
+        inv_b(y);
+    }
+    __syncthreads();
+    if (inA) {
+        inv_a(x);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                if (r < nr) {
+                    const long o = (long)r * A.ld + lma;
+                    const double rr = A.z[r], r2 = 1.0 / (rr * rr);
+                    const double2 w = A.W[o];
+                    A.out[5][o] = cscale(make_double2(x[n1].x - ll * w.x * r2, x[n1].y - ll * w.y * r2), -1.0 / rr);
+                }
+            }
+    }
+    __syncthreads();
+    // ---- w_S = dZ / r
+    if (inB) {
+#pragma unroll
+        for (int q = 0; q < N2; q++) {
+            const double k = A.kz[k1 + N1 * q] * inv_n;
+            y[q] = make_double2(-k * XZ[q].y, k * XZ[q].x);
+        }
+        inv_b(y);
+    }
+    __syncthreads();
+    if (inA) {
+        inv_a(x);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+            }
+    }
+}
+
+// Factor pair of n handled by uxw_radial_fact_kernel, or 0.
+static int radial_factorised(int n)
+{
+    return (n == 153 || n == 121 || n == 49) ? n : 0;
+}
+
 int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst)
 {
     UxwRadArgs A;
@@ -922,9 +1194,24 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     A.n = p->nz;
     A.C = p->C;
     A.d = p->d;
+    switch (radial_factorised(p->nz)) {
+#define RD_F_CASE(NV, F1, F2)                                                                   \
+    case NV: {                                                                                  \
+        constexpr int CB = RadFact<F1, F2>::CB;                                                 \
+        hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2>), dim3((p->nlm + CB - 1) / CB), dim3(256), 0, \
+                           p->stream, A, p->tw_z);                                              \
+        RD_LAUNCH_CHECK();                                                                      \
+        return RD_OK;                                                                           \
+    }
+        RD_F_CASE(153, 17, 9)
+        RD_F_CASE(121, 11, 11)
+        RD_F_CASE(49, 7, 7)
+#undef RD_F_CASE
+    default: break;
+    }
     size_t shm = sizeof(double2) * (6 * UTC + 1) * p->nz;
     const dim3 grid((p->nlm + UTC - 1) / UTC), block(256);
-#define RD_U_CASE(RBV)                                                                          \
+#define RD_U_CASE(RBV)                                                                        \
     case RBV: {                                                                                 \
         int lrc = allow_lds(uxw_radial_kernel<RBV>, shm);                                       \
         if (lrc) return lrc;                                                                    \

@@ -349,6 +349,8 @@ def test_prueba_3d_periodic():
 # -------------------------------------------------------- nonlinear term
 @pytest.mark.parametrize("lmax,nr,grid", [(15, 24, None), (31, 24, None), (63, 40, None), (127, 96, None),
                                           (255, 12, None),
+                                          # radial length 153 = 17 x 9 (the nr = 128 workload)
+                                          (47, 128, None),
                                           # nphi = 768 kernel with m > lmax zero, rows not a multiple of 8
                                           (200, 11, (386, 768))])
 def test_nonlinear_uxw_vs_oracle(lmax, nr, grid):
