@@ -61,7 +61,8 @@ int rd_plan_create(rd_plan **plan, int lmax, int nlat, int nphi, int nz, int C, 
 /* Replaces fcgram_destroy_plan (src/sphp/fcgram_mod.f90:153-182). */
 int rd_plan_destroy(rd_plan *plan);
 
-/* Orders the plan's work on an existing HIP stream (NULL: the plan's own). */
+/* Orders the plan's work on an existing HIP stream; NULL selects the default
+ * (null) stream.  A new plan runs on a non-blocking stream of its own. */
 int rd_plan_set_stream(rd_plan *plan, void *hip_stream);
 
 /* Waits for all work queued on the plan's stream (the reference's
@@ -241,6 +242,41 @@ int rd_dev_update_to(const void *host, size_t nbytes);
 int rd_dev_update_from(void *host, size_t nbytes);
 /* gdev_copy, gdevice_mod.f90:282-302. */
 int rd_dev_copy(const void *hdst, const void *hsrc, size_t nbytes);
+
+/* ---- batching form ------------------------------------------------------
+ * Many independent shell problems (each its own lmax, nlat, nphi, nz, dz,
+ * rint; one FC-Gram table C, d, dir for all) transformed together: the
+ * reference's fftp3d_complex_to_real / fftp3d_real_to_complex
+ * (src/sphp/sphp_sca.F90:237-270,400-433) applied to every problem, with the
+ * Legendre stage of all problems in one persistent launch per direction.
+ * prec is RD_PREC_FP64 or RD_PREC_FP32.  Arrays are device pointers, one per
+ * problem: spectral (nz, nlm) complex, physical (nz, nphi, nlat) real. */
+typedef struct rd_batch rd_batch;
+
+int rd_batch_create(rd_batch **b, int nprob, const int *lmax, const int *nlat, const int *nphi, const int *nz,
+                    int C, int d, const double *dz, const double *rint, const double *dir, int prec);
+int rd_batch_destroy(rd_batch *b);
+/* As rd_plan_set_stream: NULL selects the default stream. */
+int rd_batch_set_stream(rd_batch *b, void *stream);
+int rd_batch_sync(rd_batch *b);
+/* info: nprob, prec, plane-tile blocks, workgroups per CU, CUs, synthesis
+ * and analysis work items, captured (8 ints). */
+int rd_batch_info(const rd_batch *b, int *info);
+/* spec -> grid per problem; spec is not modified. */
+int rd_batch_3d_inverse(rd_batch *b, const double *const *spec, double *const *grid);
+/* grid -> spec per problem (FC-Gram continuation, forward radial FFT, 1/nz). */
+int rd_batch_3d_forward(rd_batch *b, const double *const *grid, double *const *spec);
+/* Captures the round trip spec -> grid -> spec_out of every problem into one
+ * hipGraph (replayed by rd_batch_replay on the batch stream). */
+int rd_batch_capture(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out);
+int rd_batch_replay(rd_batch *b);
+/* Tiles of the persistent launches: plane-tile blocks (2 or 4) and workgroups
+ * per CU (1..8).  Drops a captured graph. */
+int rd_batch_config(rd_batch *b, int jb, int wg_per_cu);
+/* Times the round trip (reps each) for every tile choice, keeps the fastest;
+ * reports the fixed default (4, 2) and the chosen times and the choice. */
+int rd_batch_tune(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out,
+                  int reps, double *ms_fixed, double *ms_tuned, int *choice);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,355 @@
+// Batching form: many independent shell problems of different sizes (lmax,
+// grid, radial length) transformed together.  Each problem has its own plan
+// (tables, twiddles, one Fourier and one spectral scratch array); the
+// Legendre stage of all problems runs as one persistent launch per direction
+// (work items ordered largest first, pulled from a counter), the longitude
+// and radial FFTs run per problem on the batch stream.  The whole round trip
+// can be captured once into a hipGraph and replayed, and a tuner times the
+// tile / grid choices of the persistent launches and keeps the fastest.
+//
+// Per problem p (the reference's fftp3d_complex_to_real and
+// fftp3d_real_to_complex, src/sphp/sphp_sca.F90:237-270,400-433, applied to
+// every plan of the batch):
+//   inverse: spec (nz, nlm) -> radial inverse FFT -> synthesis -> phi inverse
+//            -> grid (nz, nphi, nlat); spec is left unchanged (the radial
+//            transform runs on the plan's copy)
+//   forward: grid -> phi forward -> analysis -> FC-Gram continuation and
+//            radial forward FFT (1/nz) -> spec
+#include "rd_internal.h"
+
+#include <vector>
+
+using namespace rd;
+
+struct rd_batch {
+    int nprob, prec;
+    std::vector<Plan *> plans;
+    hipStream_t stream, own_stream;
+    BatchLeg leg;
+    int jb, wg_per_cu, n_cu;
+    std::vector<double2 *> ana_out;   // bound output arrays of the analysis
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    bool captured;
+    // the graph's own copies of the launch arguments and work counters, so
+    // that eager calls with other arrays do not change what a replay reads
+    void *g_syn_args, *g_ana_args;
+    int *g_counter;
+};
+
+namespace {
+
+int set_stream(rd_batch *b, hipStream_t s)
+{
+    b->stream = s;
+    for (Plan *p : b->plans) p->stream = s;
+    return RD_OK;
+}
+
+void drop_graph(rd_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void *d[] = {b->g_syn_args, b->g_ana_args, b->g_counter};
+    for (void *x : d)
+        if (x) (void)hipFree(x);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_syn_args = b->g_ana_args = nullptr;
+    b->g_counter = nullptr;
+    b->captured = false;
+}
+
+int nwg(const rd_batch *b) { return b->n_cu * b->wg_per_cu; }
+
+int bind(rd_batch *b, double *const *spec_out)
+{
+    const int np = b->nprob;
+    std::vector<const double2 *> sin(np), ain(np);
+    std::vector<double2 *> sout(np), aout(np);
+    for (int p = 0; p < np; p++) {
+        sin[p] = b->plans[p]->S[0];
+        sout[p] = b->plans[p]->F[0];
+        ain[p] = b->plans[p]->F[0];
+        aout[p] = spec_out ? (double2 *)spec_out[p] : b->ana_out[p];
+    }
+    b->ana_out = aout;
+    return batch_leg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
+}
+
+// the inverse chain, enqueued on the batch stream (no host synchronisation)
+int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
+{
+    for (int i = 0; i < b->nprob; i++) {
+        Plan *p = b->plans[i];
+        RD_HIP(hipMemcpyAsync(p->S[0], spec[i], sizeof(double2) * p->ssize, hipMemcpyDeviceToDevice, b->stream));
+        int rc = launch_z_inverse(p, p->S[0]);
+        if (rc) return rc;
+    }
+    int rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    for (int i = 0; i < b->nprob; i++) {
+        Plan *p = b->plans[i];
+        const double2 *F[1] = {p->F[0]};
+        double *f[1] = {grid[i]};
+        rc = launch_phi_inverse(p, 1, p->nz, F, f);
+        if (rc) return rc;
+    }
+    return RD_OK;
+}
+
+int enqueue_forward(rd_batch *b, const double *const *grid)
+{
+    for (int i = 0; i < b->nprob; i++) {
+        Plan *p = b->plans[i];
+        const double *f[1] = {grid[i]};
+        double2 *F[1] = {p->F[0]};
+        int rc = launch_phi_forward(p, 1, p->nz, f, F);
+        if (rc) return rc;
+    }
+    int rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    for (int i = 0; i < b->nprob; i++) {
+        rc = launch_z_forward(b->plans[i], b->ana_out[i]);
+        if (rc) return rc;
+    }
+    return RD_OK;
+}
+
+int configure(rd_batch *b, int jb, int wg_per_cu)
+{
+    if ((jb != 2 && jb != 4) || wg_per_cu < 1 || wg_per_cu > 8) return RD_ERR_ARG;
+    drop_graph(b);
+    RD_HIP(hipStreamSynchronize(b->stream));
+    batch_leg_free(&b->leg);
+    int rc = batch_leg_init(&b->leg, b->plans.data(), b->nprob, b->prec, jb);
+    if (rc) return rc;
+    b->jb = jb;
+    b->wg_per_cu = wg_per_cu;
+    return RD_OK;
+}
+
+void destroy(rd_batch *b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    batch_leg_free(&b->leg);
+    for (Plan *p : b->plans) {
+        if (!p) continue;
+        p->stream = p->own_stream;
+        plan_destroy(p);
+    }
+    if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat, const int *nphi, const int *nz,
+                    int C, int d, const double *dz, const double *rint, const double *dir, int prec)
+{
+    if (!out || nprob < 1 || !lmax || !nlat || !nphi || !nz || !dz || !rint) return RD_ERR_ARG;
+    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32) return RD_ERR_ARG;
+    *out = nullptr;
+    rd_batch *b = new rd_batch();
+    b->nprob = nprob;
+    b->prec = prec;
+    b->plans.assign(nprob, nullptr);
+    b->ana_out.assign(nprob, nullptr);
+    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        destroy(b);
+        return RD_ERR_HIP;
+    }
+    b->stream = b->own_stream;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        b->n_cu < 1)
+        b->n_cu = 256;
+    for (int i = 0; i < nprob; i++) {
+        int rc = plan_create(&b->plans[i], lmax[i], nlat[i], nphi[i], nz[i], C, d, dz[i], rint[i], dir, 1);
+        if (rc) {
+            destroy(b);
+            return rc;
+        }
+    }
+    set_stream(b, b->stream);
+    b->jb = 4;
+    b->wg_per_cu = 2;
+    int rc = batch_leg_init(&b->leg, b->plans.data(), nprob, prec, b->jb);
+    if (rc) {
+        destroy(b);
+        return rc;
+    }
+    *out = b;
+    return RD_OK;
+}
+
+int rd_batch_destroy(rd_batch *b)
+{
+    if (!b) return RD_ERR_ARG;
+    destroy(b);
+    return RD_OK;
+}
+
+int rd_batch_set_stream(rd_batch *b, void *s)
+{
+    if (!b) return RD_ERR_ARG;
+    return set_stream(b, (hipStream_t)s);   // NULL: the default stream
+}
+
+int rd_batch_sync(rd_batch *b)
+{
+    if (!b) return RD_ERR_ARG;
+    RD_HIP(hipStreamSynchronize(b->stream));
+    return RD_OK;
+}
+
+int rd_batch_info(const rd_batch *b, int *info)
+{
+    if (!b || !info) return RD_ERR_ARG;
+    info[0] = b->nprob;
+    info[1] = b->prec;
+    info[2] = b->jb;
+    info[3] = b->wg_per_cu;
+    info[4] = b->n_cu;
+    info[5] = b->leg.n_syn_items;
+    info[6] = b->leg.n_ana_items;
+    info[7] = b->captured ? 1 : 0;
+    return RD_OK;
+}
+
+int rd_batch_3d_inverse(rd_batch *b, const double *const *spec, double *const *grid)
+{
+    if (!b || !spec || !grid) return RD_ERR_ARG;
+    for (int i = 0; i < b->nprob; i++)
+        if (!spec[i] || !grid[i]) return RD_ERR_ARG;
+    int rc = bind(b, nullptr);
+    if (rc) return rc;
+    return enqueue_inverse(b, spec, grid);
+}
+
+int rd_batch_3d_forward(rd_batch *b, const double *const *grid, double *const *spec)
+{
+    if (!b || !grid || !spec) return RD_ERR_ARG;
+    for (int i = 0; i < b->nprob; i++)
+        if (!spec[i] || !grid[i]) return RD_ERR_ARG;
+    int rc = bind(b, spec);
+    if (rc) return rc;
+    return enqueue_forward(b, grid);
+}
+
+int rd_batch_capture(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out)
+{
+    if (!b || !spec || !grid || !spec_out) return RD_ERR_ARG;
+    for (int i = 0; i < b->nprob; i++)
+        if (!spec[i] || !grid[i] || !spec_out[i]) return RD_ERR_ARG;
+    drop_graph(b);
+    int rc = bind(b, spec_out);
+    if (rc) return rc;
+    RD_HIP(hipStreamSynchronize(b->stream));
+    const size_t ns = b->leg.arg_bytes_syn * b->nprob, na = b->leg.arg_bytes_ana * b->nprob;
+    RD_HIP(hipMalloc(&b->g_syn_args, ns));
+    RD_HIP(hipMalloc(&b->g_ana_args, na));
+    RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
+    RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
+    RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    // record on the batch's own (non-default) stream, with the graph's copies
+    // of the arguments and counters; the graph is launched later on whichever
+    // stream the batch is bound to
+    hipStream_t user = b->stream;
+    void *sa = b->leg.d_syn_args, *aa = b->leg.d_ana_args;
+    int *ct = b->leg.d_counter;
+    b->leg.d_syn_args = b->g_syn_args;
+    b->leg.d_ana_args = b->g_ana_args;
+    b->leg.d_counter = b->g_counter;
+    set_stream(b, b->own_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = enqueue_inverse(b, spec, grid);
+        if (!rc) rc = enqueue_forward(b, grid);
+        e = hipStreamEndCapture(b->own_stream, &g);
+    }
+    set_stream(b, user);
+    b->leg.d_syn_args = sa;
+    b->leg.d_ana_args = aa;
+    b->leg.d_counter = ct;
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : RD_ERR_HIP;
+    }
+    b->graph = g;
+    RD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->captured = true;
+    return RD_OK;
+}
+
+int rd_batch_replay(rd_batch *b)
+{
+    if (!b) return RD_ERR_ARG;
+    if (!b->captured) return RD_ERR_STATE;
+    RD_HIP(hipGraphLaunch(b->gexec, b->stream));
+    return RD_OK;
+}
+
+int rd_batch_config(rd_batch *b, int jb, int wg_per_cu)
+{
+    if (!b) return RD_ERR_ARG;
+    return configure(b, jb, wg_per_cu);
+}
+
+int rd_batch_tune(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out,
+                  int reps, double *ms_fixed, double *ms_tuned, int *choice)
+{
+    if (!b || !spec || !grid || !spec_out || reps < 1) return RD_ERR_ARG;
+    const int cand_jb[2] = {4, 2}, cand_wg[4] = {2, 1, 3, 4};
+    hipEvent_t e0, e1;
+    RD_HIP(hipEventCreate(&e0));
+    RD_HIP(hipEventCreate(&e1));
+    double best = 1e30, fixed = 0.0;
+    int bj = 4, bw = 2;
+    int rc = RD_OK;
+    for (int a = 0; a < 2 && !rc; a++)
+        for (int c = 0; c < 4 && !rc; c++) {
+            rc = configure(b, cand_jb[a], cand_wg[c]);
+            if (!rc) rc = bind(b, spec_out);
+            // one untimed round trip, then reps timed
+            if (!rc) rc = enqueue_inverse(b, spec, grid);
+            if (!rc) rc = enqueue_forward(b, grid);
+            if (rc) break;
+            (void)hipEventRecord(e0, b->stream);
+            for (int r = 0; r < reps && !rc; r++) {
+                rc = enqueue_inverse(b, spec, grid);
+                if (!rc) rc = enqueue_forward(b, grid);
+            }
+            (void)hipEventRecord(e1, b->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double t = ms / reps;
+            if (a == 0 && c == 0) fixed = t;
+            if (t < best) {
+                best = t;
+                bj = cand_jb[a];
+                bw = cand_wg[c];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = configure(b, bj, bw);
+    if (rc) return rc;
+    if (ms_fixed) *ms_fixed = fixed;
+    if (ms_tuned) *ms_tuned = best;
+    if (choice) {
+        choice[0] = bj;
+        choice[1] = bw;
+    }
+    return RD_OK;
+}
+
+}  // extern "C"

@@ -133,10 +133,12 @@ int rd_load_dirichlet_tables(const char *tdir, int C, int d, double *dir)
     return RD_OK;
 }
 
-int rd_plan_create(rd_plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz,
-                   double rint, const double *dir)
+// The plan constructor; nwork Fourier and spectral scratch arrays (6 for the
+// nonlinear term, 1 for scalar transforms).
+static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz,
+                            double rint, const double *dir, int nwork)
 {
-    if (!out) return RD_ERR_ARG;
+    if (!out || nwork < 1 || nwork > 6) return RD_ERR_ARG;
     *out = nullptr;
     if (lmax < 0 || nlat < 2 || (nlat & 1) || nlat <= lmax || nphi <= 2 * lmax || nphi < 2) return RD_ERR_ARG;
     if (nz < 1 || C < 0) return RD_ERR_ARG;
@@ -227,7 +229,8 @@ int rd_plan_create(rd_plan **out, int lmax, int nlat, int nphi, int nz, int C, i
 
     p->fsize = (long)nz * p->L * nlat;
     p->ssize = (long)p->nlm * nz;
-    for (int i = 0; i < 6; i++) {
+    p->nwork = nwork;
+    for (int i = 0; i < nwork; i++) {
         if (hipMalloc(&p->F[i], sizeof(double2) * p->fsize) != hipSuccess) {
             free_plan(p);
             return RD_ERR_NOMEM;
@@ -240,8 +243,18 @@ int rd_plan_create(rd_plan **out, int lmax, int nlat, int nphi, int nz, int C, i
     TRYH(hipStreamSynchronize(p->stream));
 #undef TRY
 #undef TRYH
-    *out = reinterpret_cast<rd_plan *>(p);
+    *out = p;
     return RD_OK;
+}
+
+int rd_plan_create(rd_plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz,
+                   double rint, const double *dir)
+{
+    if (!out) return RD_ERR_ARG;
+    Plan *p = nullptr;
+    const int rc = plan_create_impl(&p, lmax, nlat, nphi, nz, C, d, dz, rint, dir, 6);
+    *out = reinterpret_cast<rd_plan *>(p);
+    return rc;
 }
 
 int rd_plan_destroy(rd_plan *plan)
@@ -257,7 +270,7 @@ int rd_plan_set_stream(rd_plan *plan, void *s)
 {
     if (bad_plan(plan)) return RD_ERR_ARG;
     Plan *p = P(plan);
-    p->stream = s ? (hipStream_t)s : p->own_stream;
+    p->stream = (hipStream_t)s;   // NULL: the default stream
     return RD_OK;
 }
 
@@ -415,6 +428,7 @@ int rd_dlr(rd_plan *plan, const double *a, double *b)
 {
     if (bad_plan(plan) || !a || !b || a == b) return RD_ERR_ARG;
     Plan *p = P(plan);
+    if (p->nwork < 2) return RD_ERR_STATE;
     int rc = launch_derivr_r(p, 2, (const double2 *)a, (double2 *)b);
     if (!rc) rc = launch_laplang(p, (const double2 *)a, p->S[0]);
     if (!rc) rc = launch_multr(p, -2, p->S[0], p->S[1]);
@@ -538,6 +552,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     if (bad_plan(plan) || !W || !Z || !NQ || !NS || !NT) return RD_ERR_ARG;
     if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32 && prec != RD_PREC_BF16X3) return RD_ERR_ARG;
     Plan *p = P(plan);
+    if (p->nwork < 6) return RD_ERR_STATE;
     const int nr = p->nr;
     const bool prof = p->profiling != 0;
     hipEvent_t *ev = nullptr;
@@ -654,3 +669,20 @@ int rd_v_imposebc_fast(rd_plan *plan, double *tor, double *phi, double *pol)
 }
 
 }  // extern "C"
+
+namespace rd {
+
+int plan_create(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz, double rint,
+                const double *dir, int nwork)
+{
+    return plan_create_impl(out, lmax, nlat, nphi, nz, C, d, dz, rint, dir, nwork);
+}
+
+void plan_destroy(Plan *p)
+{
+    if (!p) return;
+    (void)hipStreamSynchronize(p->stream);
+    free_plan(p);
+}
+
+}  // namespace rd

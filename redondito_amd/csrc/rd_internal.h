@@ -70,7 +70,8 @@ struct Plan {
     double *bc_d1, *bc_d2, *bc_binv, *bc_d2bor;
     int *bc_queue;                 // (l | m0 << 16) tiles of invDl
     int bc_qlen;
-    // workspace
+    // workspace (nwork of each; 6 for the nonlinear term)
+    int nwork;
     double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
     double2 *S[6];                 // spectral scratch (nlm, nz)
     long fsize, ssize;             // in elements of the type
@@ -131,6 +132,31 @@ int launch_prodvec(Plan *p, long npts, const double *ar, const double *at, const
 int launch_fc_filter_k(Plan *p, double2 *a);
 int launch_dirichlet(Plan *p, double2 *a);
 int launch_zero_rows(Plan *p, double2 *a, int row0, int nrows);
+
+// Plan life cycle for internal users (the batch): nwork scratch arrays.
+int plan_create(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz, double rint,
+                const double *dir, int nwork);
+void plan_destroy(Plan *p);
+
+// Persistent Legendre launches over many problems: work items (problem, m,
+// tile, plane tile) pulled from a counter, largest first.
+struct BatchLeg {
+    int nprob, prec, jb;
+    void *d_syn_args, *d_ana_args;       // per-problem kernel arguments
+    int4 *d_syn_items, *d_ana_items;
+    int n_syn_items, n_ana_items;
+    int *d_counter;                      // [0] synthesis, [1] analysis
+    size_t arg_bytes_syn, arg_bytes_ana;
+    void *h_syn_args, *h_ana_args;       // host copies of the uploaded arguments
+};
+int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb);
+// Binds the per-problem operands (spectral and Fourier arrays); uploads the
+// arguments when they changed (synchronous on the stream).
+int batch_leg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in, double2 *const *syn_out,
+                   const double2 *const *ana_in, double2 *const *ana_out, hipStream_t s);
+int launch_batch_synthesis(BatchLeg *b, hipStream_t s, int nwg);
+int launch_batch_analysis(BatchLeg *b, hipStream_t s, int nwg);
+void batch_leg_free(BatchLeg *b);
 
 // Wall solves.
 int build_derivr_mat(Plan *p, int order, double *d);

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace rd {
@@ -290,10 +292,10 @@ int build_legendre_tables(Plan *p)
 // (SJT/2 planes).  The l loop runs in chunks of 32 (16 per parity).
 constexpr int SKC = 32;
 
-template <bool VEC>
+template <bool VEC, int JBV = (VEC ? 2 : 4)>
 struct SynCfg {
     static constexpr int NF = VEC ? 3 : 1;
-    static constexpr int JB = VEC ? 2 : 4;       // 16-column blocks per wave
+    static constexpr int JB = JBV;               // 16-column blocks per wave
     static constexpr int JT = 16 * JB;           // columns per workgroup
     static constexpr int JTP = JT + 2;           // LDS row pitch (doubles)
 };
@@ -309,25 +311,19 @@ struct SynArgs {
     int lmax, nlat, nth, nthp, nplanes, n_jt;
 };
 
+// One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
 // T = double: the FP64 path; T = float: the FP32 precision form (tables and
 // operands rounded to FP32, FP32 accumulation, FP64 output).
-template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs<T> A)
+template <typename T, bool VEC, int JBV>
+__device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int jt,
+                                         T (&Bs)[SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
 {
-    using Cfg = SynCfg<VEC>;
+    using Cfg = SynCfg<VEC, JBV>;
     using V = typename Mf<T>::V;
-    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
-    __shared__ T Bs[NF][SKC][JTP];
+    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
-    // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
-    // back to back on one XCD, so its table rows are read from HBM once
-    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
-    const int jt = i % A.n_jt;
-    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
-    if (ent < 0) return;
-    const int m = ent & 0xffff, tt = ent >> 16;
     const int k0 = jt * (JT / 2);
     const int th0 = tt * 64 + wave * 16;
     const int col = lane & 15, kk = lane >> 4;
@@ -420,6 +416,20 @@ __global__ __launch_bounds__(256) void synthesis_kernel(SynArgs<T> A)
     }
 }
 
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs<T> A)
+{
+    using Cfg = SynCfg<VEC>;
+    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
+    // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
+    // back to back on one XCD, so its table rows are read from HBM once
+    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int jt = i % A.n_jt;
+    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    if (ent < 0) return;
+    syn_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Bs);
+}
+
 template <typename T>
 static int launch_synthesis_t(Plan *p, const T *tP, const T *tD, const T *tM, bool vec, int nplanes,
                               const double2 *q, const double2 *s, const double2 *t, long ld, double2 *f0,
@@ -474,10 +484,10 @@ int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q,
 // in chunks of 16.
 constexpr int ATC = 16;
 
-template <bool VEC>
+template <bool VEC, int JBV = (VEC ? 2 : 4)>
 struct AnaCfg {
     static constexpr int NF = VEC ? 3 : 1;
-    static constexpr int JB = VEC ? 2 : 4;
+    static constexpr int JB = JBV;
     static constexpr int JT = 2 * 16 * JB;
     static constexpr int JTP = JT + 2;
 };
@@ -494,25 +504,19 @@ struct AnaArgs {
     int lmax, nlat, nth, nthp16, nplanes, n_jt;
 };
 
-// T as in synthesis_kernel.
-template <typename T, bool VEC>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void analysis_kernel(AnaArgs<T> A)
+// One analysis tile (m, l tile lt, plane tile jt) of one problem; T as in
+// syn_tile.
+template <typename T, bool VEC, int JBV>
+__device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int jt,
+                                         T (&Gn)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP],
+                                         T (&Gs)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP])
 {
-    using Cfg = AnaCfg<VEC>;
+    using Cfg = AnaCfg<VEC, JBV>;
     using V = typename Mf<T>::V;
-    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT, JTP = Cfg::JTP;
-    __shared__ T Gn[NF][ATC][JTP];
-    __shared__ T Gs[NF][ATC][JTP];
+    constexpr int NF = Cfg::NF, JB = Cfg::JB, JT = Cfg::JT;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
-    // per-XCD queue entry (m, l tile); the plane tiles of one entry run back
-    // to back on one XCD
-    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
-    const int jt = i % A.n_jt;
-    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
-    if (ent < 0) return;
-    const int m = ent & 0xffff, lt = ent >> 16;
     const int wl = wave & 1, wj = wave >> 1;
     const int k0 = jt * (JT / 2);
     const int col = lane & 15, kk = lane >> 4;
@@ -613,6 +617,21 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void analysis_kernel(A
     }
 }
 
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void analysis_kernel(AnaArgs<T> A)
+{
+    using Cfg = AnaCfg<VEC>;
+    __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
+    __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
+    // per-XCD queue entry (m, l tile); the plane tiles of one entry run back
+    // to back on one XCD
+    const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int jt = i % A.n_jt;
+    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    if (ent < 0) return;
+    ana_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Gn, Gs);
+}
+
 template <typename T>
 static int launch_analysis_t(Plan *p, const T *tP, const T *tD, const T *tM, bool vec, int nplanes,
                              const double2 *g0, const double2 *g1, const double2 *g2, double2 *q, double2 *s,
@@ -662,4 +681,235 @@ int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0,
     return launch_analysis_bf3(p, vec, nplanes, g0, g1, g2, q, s, t, ld);
 }
 
+// ------------------------------------------------ batched persistent launches
+// Many independent problems through one launch per direction: every
+// workgroup pulls work items (problem, m, tile, plane tile) from a counter
+// until the list is exhausted; the list is ordered largest first so that the
+// uneven m-blocks of problems of different lmax balance across the CUs.
+__global__ void zero_counters_kernel(int *c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+template <typename T, int JBV>
+__global__ __launch_bounds__(256) void batch_synthesis_kernel(const SynArgs<T> *probs, const int4 *items,
+                                                              int nitems, int *counter)
+{
+    using Cfg = SynCfg<false, JBV>;
+    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) return;
+        const int4 w = items[it];
+        const SynArgs<T> A = probs[w.x];
+        syn_tile<T, false, JBV>(A, w.y, w.z, w.w, Bs);
+    }
+}
+
+template <typename T, int JBV>
+__global__ __launch_bounds__(256) void batch_analysis_kernel(const AnaArgs<T> *probs, const int4 *items,
+                                                             int nitems, int *counter)
+{
+    using Cfg = AnaCfg<false, JBV>;
+    __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
+    __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) return;
+        const int4 w = items[it];
+        const AnaArgs<T> A = probs[w.x];
+        ana_tile<T, false, JBV>(A, w.y, w.z, w.w, Gn, Gs);
+    }
+}
+
+int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb)
+{
+    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32) return RD_ERR_ARG;
+    if (jb != 2 && jb != 4) return RD_ERR_ARG;
+    b->nprob = nprob;
+    b->prec = prec;
+    b->jb = jb;
+    b->arg_bytes_syn = prec == RD_PREC_FP64 ? sizeof(SynArgs<double>) : sizeof(SynArgs<float>);
+    b->arg_bytes_ana = prec == RD_PREC_FP64 ? sizeof(AnaArgs<double>) : sizeof(AnaArgs<float>);
+    std::vector<int4> si, ai;
+    std::vector<long> sc, ac;
+    for (int p = 0; p < nprob; p++) {
+        Plan *pl = plans[p];
+        if (prec == RD_PREC_FP32) {
+            int rc = ensure_precision_tables(pl, prec);
+            if (rc) return rc;
+        }
+        const int L = pl->L;
+        const int sjt = (2 * pl->nz + 16 * jb - 1) / (16 * jb), ntt = pl->nthp / 64;
+        const int ajt = (2 * pl->nz + 32 * jb - 1) / (32 * jb);
+        for (int m = 0; m < L; m++) {
+            for (int tt = 0; tt < ntt; tt++)
+                for (int jt = 0; jt < sjt; jt++) {
+                    si.push_back(make_int4(p, m, tt, jt));
+                    sc.push_back((L - m + SKC - 1) / SKC);
+                }
+            const int ncol = rup((L - m + 1) / 2, 32);
+            for (int lt = 0; lt < ncol / 32; lt++)
+                for (int jt = 0; jt < ajt; jt++) {
+                    ai.push_back(make_int4(p, m, lt, jt));
+                    ac.push_back((pl->nth + ATC - 1) / ATC);
+                }
+        }
+    }
+    auto order = [](std::vector<int4> &v, const std::vector<long> &cost) {
+        std::vector<size_t> idx(v.size());
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+        std::vector<int4> o(v.size());
+        for (size_t i = 0; i < idx.size(); i++) o[i] = v[idx[i]];
+        v.swap(o);
+    };
+    order(si, sc);
+    order(ai, ac);
+    b->n_syn_items = (int)si.size();
+    b->n_ana_items = (int)ai.size();
+    RD_HIP(hipMalloc(&b->d_syn_items, sizeof(int4) * si.size()));
+    RD_HIP(hipMalloc(&b->d_ana_items, sizeof(int4) * ai.size()));
+    RD_HIP(hipMalloc(&b->d_syn_args, b->arg_bytes_syn * nprob));
+    RD_HIP(hipMalloc(&b->d_ana_args, b->arg_bytes_ana * nprob));
+    RD_HIP(hipMalloc(&b->d_counter, 2 * sizeof(int)));
+    RD_HIP(hipMemcpy(b->d_syn_items, si.data(), sizeof(int4) * si.size(), hipMemcpyHostToDevice));
+    RD_HIP(hipMemcpy(b->d_ana_items, ai.data(), sizeof(int4) * ai.size(), hipMemcpyHostToDevice));
+    b->h_syn_args = calloc(nprob, b->arg_bytes_syn);
+    b->h_ana_args = calloc(nprob, b->arg_bytes_ana);
+    if (!b->h_syn_args || !b->h_ana_args) return RD_ERR_NOMEM;
+    return RD_OK;
+}
+
+template <typename T>
+static void fill_args(Plan *const *plans, int nprob, const T *const *ts, const T *const *ta,
+                      const double2 *const *syn_in, double2 *const *syn_out, const double2 *const *ana_in,
+                      double2 *const *ana_out, SynArgs<T> *sa, AnaArgs<T> *aa)
+{
+    for (int p = 0; p < nprob; p++) {
+        const Plan *pl = plans[p];
+        SynArgs<T> &S = sa[p];
+        S.tP = S.tD = S.tM = ts[p];
+        S.off = pl->d_syn_off;
+        S.queue = nullptr;
+        for (int f = 0; f < 3; f++) {
+            S.in[f] = syn_in ? syn_in[p] : nullptr;
+            S.out[f] = syn_out ? syn_out[p] : nullptr;
+        }
+        S.ld = pl->nlm;
+        S.lmax = pl->lmax;
+        S.nlat = pl->nlat;
+        S.nth = pl->nth;
+        S.nthp = pl->nthp;
+        S.nplanes = pl->nz;
+        S.n_jt = 0;
+        AnaArgs<T> &A = aa[p];
+        A.tP = A.tD = A.tM = ta[p];
+        A.off = pl->d_ana_off;
+        A.ncol = pl->d_ana_ncol;
+        A.queue = nullptr;
+        for (int f = 0; f < 3; f++) {
+            A.in[f] = ana_in ? ana_in[p] : nullptr;
+            A.out[f] = ana_out ? ana_out[p] : nullptr;
+        }
+        A.ld = pl->nlm;
+        A.lmax = pl->lmax;
+        A.nlat = pl->nlat;
+        A.nth = pl->nth;
+        A.nthp16 = pl->nthp16;
+        A.nplanes = pl->nz;
+        A.n_jt = 0;
+    }
+}
+
+int batch_leg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in, double2 *const *syn_out,
+                   const double2 *const *ana_in, double2 *const *ana_out, hipStream_t s)
+{
+    const int np = b->nprob;
+    std::vector<char> hs(b->arg_bytes_syn * np), ha(b->arg_bytes_ana * np);
+    if (b->prec == RD_PREC_FP64) {
+        std::vector<const double *> ts(np), ta(np);
+        for (int p = 0; p < np; p++) {
+            ts[p] = plans[p]->syn_P;
+            ta[p] = plans[p]->ana_P;
+        }
+        fill_args<double>(plans, np, ts.data(), ta.data(), syn_in, syn_out, ana_in, ana_out,
+                          (SynArgs<double> *)hs.data(), (AnaArgs<double> *)ha.data());
+    } else {
+        std::vector<const float *> ts(np), ta(np);
+        for (int p = 0; p < np; p++) {
+            ts[p] = plans[p]->syn32_P;
+            ta[p] = plans[p]->ana32_P;
+        }
+        fill_args<float>(plans, np, ts.data(), ta.data(), syn_in, syn_out, ana_in, ana_out,
+                         (SynArgs<float> *)hs.data(), (AnaArgs<float> *)ha.data());
+    }
+    const bool same = memcmp(hs.data(), b->h_syn_args, hs.size()) == 0 &&
+                      memcmp(ha.data(), b->h_ana_args, ha.size()) == 0;
+    if (same) return RD_OK;
+    RD_HIP(hipStreamSynchronize(s));
+    RD_HIP(hipMemcpy(b->d_syn_args, hs.data(), hs.size(), hipMemcpyHostToDevice));
+    RD_HIP(hipMemcpy(b->d_ana_args, ha.data(), ha.size(), hipMemcpyHostToDevice));
+    memcpy(b->h_syn_args, hs.data(), hs.size());
+    memcpy(b->h_ana_args, ha.data(), ha.size());
+    return RD_OK;
+}
+
+int launch_batch_synthesis(BatchLeg *b, hipStream_t s, int nwg)
+{
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counter, 1);
+    RD_LAUNCH_CHECK();
+#define RD_BS(T, J)                                                                                        \
+    hipLaunchKernelGGL((batch_synthesis_kernel<T, J>), dim3(nwg), dim3(256), 0, s,                         \
+                       (const SynArgs<T> *)b->d_syn_args, b->d_syn_items, b->n_syn_items, b->d_counter)
+    if (b->prec == RD_PREC_FP64) {
+        if (b->jb == 2) RD_BS(double, 2);
+        else RD_BS(double, 4);
+    } else {
+        if (b->jb == 2) RD_BS(float, 2);
+        else RD_BS(float, 4);
+    }
+#undef RD_BS
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+int launch_batch_analysis(BatchLeg *b, hipStream_t s, int nwg)
+{
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counter + 1, 1);
+    RD_LAUNCH_CHECK();
+#define RD_BA(T, J)                                                                                        \
+    hipLaunchKernelGGL((batch_analysis_kernel<T, J>), dim3(nwg), dim3(256), 0, s,                          \
+                       (const AnaArgs<T> *)b->d_ana_args, b->d_ana_items, b->n_ana_items, b->d_counter + 1)
+    if (b->prec == RD_PREC_FP64) {
+        if (b->jb == 2) RD_BA(double, 2);
+        else RD_BA(double, 4);
+    } else {
+        if (b->jb == 2) RD_BA(float, 2);
+        else RD_BA(float, 4);
+    }
+#undef RD_BA
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+void batch_leg_free(BatchLeg *b)
+{
+    void *d[] = {b->d_syn_args, b->d_ana_args, b->d_syn_items, b->d_ana_items, b->d_counter};
+    for (void *x : d)
+        if (x) (void)hipFree(x);
+    free(b->h_syn_args);
+    free(b->h_ana_args);
+    memset(b, 0, sizeof(*b));
+}
+
 }  // namespace rd
+

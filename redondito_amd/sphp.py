@@ -363,3 +363,110 @@ def v_imposebc_fast(tor, phi, pol, plan):
     """As v_imposebc, phi recomputed on rows 1, 2, N-1, N only (vboundary.f90:91-130)."""
     _check(lib().rd_v_imposebc_fast(plan.h, plan._spec(tor), plan._spec(phi), plan._spec(pol)),
            "v_imposebc_fast")
+
+
+# ------------------------------------------------------------ batching form
+class Batch:
+    """Many independent shell problems transformed together (the reference's
+    fftp3d_complex_to_real / fftp3d_real_to_complex per problem, with the
+    Legendre stage of all problems in one persistent launch per direction).
+
+    problems: sequence of (lmax, nlat, nphi, nz, dz, rint); one FC-Gram table
+    (C, d) for all.  Spectral arrays are complex128 (nz, nlm) tensors, grids
+    float64 (nz, nphi, nlat) tensors, one per problem.
+    """
+
+    def __init__(self, problems, C=25, d=5, prec=RD_PREC_FP32, tdir=TABLE_DIR, device=None):
+        import numpy as np
+        L = lib()
+        i, vp, dd = ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+        if not getattr(L, "_batch_bound", False):
+            L.rd_batch_create.argtypes = [ctypes.POINTER(vp), i] + [ctypes.POINTER(i)] * 4 + [
+                i, i, ctypes.POINTER(dd), ctypes.POINTER(dd), ctypes.POINTER(dd), i]
+            for nm in ("rd_batch_destroy", "rd_batch_sync", "rd_batch_replay"):
+                getattr(L, nm).argtypes = [vp]
+            L.rd_batch_set_stream.argtypes = [vp, vp]
+            L.rd_batch_info.argtypes = [vp, ctypes.POINTER(i)]
+            for nm in ("rd_batch_3d_inverse", "rd_batch_3d_forward"):
+                getattr(L, nm).argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+            L.rd_batch_capture.argtypes = [vp] + [ctypes.POINTER(vp)] * 3
+            L.rd_batch_config.argtypes = [vp, i, i]
+            L.rd_batch_tune.argtypes = [vp] + [ctypes.POINTER(vp)] * 3 + [i, ctypes.POINTER(dd),
+                                                                           ctypes.POINTER(dd), ctypes.POINTER(i)]
+            L._batch_bound = True
+        self.problems = [tuple(p) for p in problems]
+        self.device = torch.device(device if device is not None else "cuda")
+        n = len(self.problems)
+        arr = lambda k, t: (t * n)(*[p[k] for p in self.problems])
+        dirp = None
+        if C > 0:
+            dv = np.asfortranarray(load_dirichlet_tables(tdir, C, d)).ravel(order="F").copy()
+            self._dir = dv
+            dirp = dv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(L.rd_batch_create(ctypes.byref(h), n, arr(0, i), arr(1, i), arr(2, i), arr(3, i), C, d,
+                                     arr(4, dd), arr(5, dd), dirp, prec), "batch_create")
+        self._h = h
+        self.nlm = [(p[0] + 1) * (p[0] + 2) // 2 for p in self.problems]
+
+    @property
+    def h(self):
+        lib().rd_batch_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().rd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ptrs(self, ts, kind):
+        if len(ts) != len(self.problems):
+            raise RdError("one array per problem")
+        for t, p, nlm in zip(ts, self.problems, self.nlm):
+            if kind == "spec":
+                ok = t.dtype == torch.complex128 and tuple(t.shape) == (p[3], nlm)
+            else:
+                ok = t.dtype == torch.float64 and tuple(t.shape) == (p[3], p[2], p[1])
+            if not (ok and t.is_cuda and t.is_contiguous()):
+                raise RdError(f"bad {kind} array {tuple(t.shape)} {t.dtype}")
+        return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def info(self):
+        v = (ctypes.c_int * 8)()
+        _check(lib().rd_batch_info(self._h, v), "batch_info")
+        keys = ("nprob", "prec", "jb", "wg_per_cu", "n_cu", "syn_items", "ana_items", "captured")
+        return dict(zip(keys, list(v)))
+
+    def inverse(self, spec, grid):
+        _check(lib().rd_batch_3d_inverse(self.h, self._ptrs(spec, "spec"), self._ptrs(grid, "grid")),
+               "batch_3d_inverse")
+
+    def forward(self, grid, spec):
+        _check(lib().rd_batch_3d_forward(self.h, self._ptrs(grid, "grid"), self._ptrs(spec, "spec")),
+               "batch_3d_forward")
+
+    def capture(self, spec, grid, spec_out):
+        _check(lib().rd_batch_capture(self.h, self._ptrs(spec, "spec"), self._ptrs(grid, "grid"),
+                                      self._ptrs(spec_out, "spec")), "batch_capture")
+
+    def replay(self):
+        _check(lib().rd_batch_replay(self.h), "batch_replay")
+
+    def config(self, jb, wg_per_cu):
+        _check(lib().rd_batch_config(self.h, jb, wg_per_cu), "batch_config")
+
+    def tune(self, spec, grid, spec_out, reps=3):
+        """Returns (ms_fixed, ms_tuned, (jb, wg_per_cu)) per round trip."""
+        f, t = ctypes.c_double(), ctypes.c_double()
+        ch = (ctypes.c_int * 2)()
+        _check(lib().rd_batch_tune(self.h, self._ptrs(spec, "spec"), self._ptrs(grid, "grid"),
+                                   self._ptrs(spec_out, "spec"), reps, ctypes.byref(f), ctypes.byref(t), ch),
+               "batch_tune")
+        return f.value, t.value, (ch[0], ch[1])
