@@ -166,6 +166,72 @@ def aux_timings(S, plan, Wd):
     }
 
 
+def batch_problems(seed=SEED, n=48):
+    """48 shell problems: lmax from {31, 63, 95, 127, 191, 255}, nlat = 3(lmax+1)/2
+    (rounded to the grid rule), nphi = 2 nlat, nr from {32, 64, 128}, one seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        lmax = int(rng.choice([31, 63, 95, 127, 191, 255]))
+        nr = int(rng.choice([32, 64, 128]))
+        nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+        out.append((lmax, nlat, 2 * nlat, nr + C, (RO - RI) / (nr - 1), RI))
+    return out
+
+
+def batch_bench(reps):
+    """The batching form: 48 uneven scalar problems, FP32 MFMA Legendre, the
+    round trip spec -> grid -> spec of all of them, eager and hipGraph-replayed,
+    with tuned and fixed tiles."""
+    import torch
+    import redondito_amd.sphp as S
+    from tests import helpers as H
+    probs = batch_problems()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 1)
+    spec, grid, out = [], [], []
+    for lmax, nlat, nphi, nz, dz, ri in probs:
+        ell = H.ell_of(lmax)
+        c = (rng.standard_normal((nz, ell.size)) + 1j * rng.standard_normal((nz, ell.size))) * (ell + 1.0) ** -1.5
+        c[:, : lmax + 1] = c[:, : lmax + 1].real
+        spec.append(torch.from_numpy(c).to(dev))
+        grid.append(torch.empty((nz, nphi, nlat), dtype=torch.float64, device=dev))
+        out.append(torch.empty_like(spec[-1]))
+    b = S.Batch(probs, prec=S.RD_PREC_FP32)
+    ms_fixed, ms_tuned, choice = b.tune(spec, grid, out, reps=2)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    eager = timed(lambda: (b.inverse(spec, grid), b.forward(grid, out)))
+    b.capture(spec, grid, out)
+    graph = timed(b.replay)
+    # accuracy of the FP32 round trip on band-limited, FC-Gram-consistent data
+    b.forward(grid, spec)
+    b.inverse(spec, grid)
+    b.forward(grid, out)
+    torch.cuda.synchronize()
+    err = max(H.rell2(o.cpu().numpy(), s.cpu().numpy()) for o, s in zip(out, spec))
+    nsyn = sum(p[3] for p in probs)     # planes transformed per direction
+    info = b.info()
+    b.close()
+    return {
+        "problems": len(probs), "prec": "fp32 Legendre", "planes_per_direction": nsyn,
+        "round_trip_ms_eager": round(eager, 4), "round_trip_ms_graph": round(graph, 4),
+        "round_trip_ms_fixed_tiles": round(ms_fixed, 4), "round_trip_ms_tuned_tiles": round(ms_tuned, 4),
+        "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1]},
+        "transforms_per_s": round(2 * len(probs) * 1e3 / graph, 1),
+        "round_trip_rel_err": float(f"{err:.3e}"),
+        "work_items": {"synthesis": info["syn_items"], "analysis": info["ana_items"]},
+    }
+
+
 def roofline(form, per_stage, nlat, nlm):
     """Dominant kernel of a form against its bound, algorithmic work per launch."""
     L = LMAX + 1
@@ -197,6 +263,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 form (for profiler runs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -247,6 +314,7 @@ def main():
     t_roof = (8 * Sb / bw + 2 * max(leg, (3 * Sb + 3 * Fm) / bw) + 9 * Fm / bw
               + max(leg, (3 * Fm + 3 * Sb) / bw))
     aux = aux_timings(S, plan, Wd)
+    batch = None if (args.no_batch or args.fp64_only) else batch_bench(max(2, min(args.steps, 10)))
 
     cpu = parity = None
     if not args.no_cpu:
@@ -274,6 +342,7 @@ def main():
         "eval_roofline_frac": round(t_roof / (forms[head]["ms_per_step"] * 1e-3), 4),
         "forms": forms,
         "aux": aux,
+        "batch": batch,
         "rel_l2_vs_oracle": parity,
         "cpu_baseline": cpu,
     }
