@@ -181,6 +181,19 @@ int rd_copy(rd_plan *plan, const double *a, double *b);
 int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ,
                      double *NS, double *NT);
 
+/* One nonlinear step of the coupled state: velocity (poloidal W, toroidal Z),
+ * magnetic field B = curl curl (Pb r) + curl (Tb r) and a temperature scalar
+ * T, all (nlm, nz) in (lm, r).  Outputs (nlm, nz), rows nr..nz-1 zero:
+ *   (NQ, NS, NT) = u x omega, (MQ, MS, MT) = u x B  (Q, S, T vectors),
+ *   NH = u . grad T  (scalar; grad T has Q = dT/dr, S = T/r, T = 0).
+ * Inputs are not modified.  prec selects RD_PREC_* for every Legendre
+ * contraction.  Composition of the same reference operators as
+ * rd_nonlinear_uxw plus fftp2d_real_to_complex_xy
+ * (src/sphp/sphp_sca.F90:273-329) for the scalar; no reference routine. */
+int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double *Z, const double *Pb,
+                         const double *Tb, const double *T, double *NQ, double *NS, double *NT,
+                         double *MQ, double *MS, double *MT, double *NH);
+
 /* ---- boundary conditions, src/boundary/ ----------------------------- */
 
 /* a(lm,1) = a(lm,nz-C) = 0.  Replaces dirichlet, boundary_mod.f90:62-93. */

@@ -60,6 +60,7 @@ def lib():
         L.orc_fc_filter_k.argtypes = [_I, _I, _L, _D, ctypes.c_double, _D]
         L.orc_nonlinear_uxw.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double,
                                         _D, _D, _D, _D, _D, _I, _I]
+        L.orc_nonlinear_coupled.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double] + [_D] * 12 + [_I, _I]
         L.orc_derivr_mat.argtypes = [_P, _D, _I, _D]
         L.orc_invdl_setup.restype = _I
         L.orc_invdl_setup.argtypes = [_I, _I, _D, _D, _D, _D]
@@ -288,6 +289,21 @@ def nonlinear_uxw(sht, fc, rint, dz, W, Z, plane0=0, nplanes=None):
     assert W.shape == (n, sht.nlm) and Z.shape == W.shape
     out = [np.empty_like(W) for _ in range(3)]
     lib().orc_nonlinear_uxw(sht.h, fc.h, rint, dz, _d(W), _d(Z), *[_d(o) for o in out], plane0, nplanes)
+    return out
+
+
+def nonlinear_coupled(sht, fc, rint, dz, W, Z, Pb, Tb, T, plane0=0, nplanes=None):
+    """One coupled step: ((NQ, NS, NT) = u x omega, (MQ, MS, MT) = u x B,
+    NH = u . grad T), each (n, nlm) complex; inputs (n, nlm) complex."""
+    n = fc.n
+    if nplanes is None:
+        nplanes = n - fc.C
+    ins = [np.ascontiguousarray(x, np.complex128) for x in (W, Z, Pb, Tb, T)]
+    for x in ins:
+        assert x.shape == (n, sht.nlm)
+    out = [np.empty_like(ins[0]) for _ in range(7)]
+    lib().orc_nonlinear_coupled(sht.h, fc.h, rint, dz, *[_d(x) for x in ins], *[_d(o) for o in out],
+                                plane0, nplanes)
     return out
 
 

@@ -791,6 +791,103 @@ void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
     free(kz); free(z); free(ell);
 }
 
+/* (Q,S,T) of v = curl curl (P r) + curl (Q r) and of curl v, as in
+ * orc_nonlinear_uxw, on rows 0..nr-1 of freshly allocated (nlm, n) arrays
+ * (rows nr..n-1 zero): vq, vs, vt, cq, cs, ct. */
+static void curl_qst(orc_fc *fp, const double *kz, const double *z, const int *ell, long nlm,
+                     const double *P, const double *Q, cplx *out[6])
+{
+    int n = fp->n, nr = n - fp->C;
+    size_t nb = sizeof(cplx) * (size_t)nlm * n;
+    double *dP = malloc(nb), *dlP = malloc(nb), *dQ = malloc(nb);
+    orc_derivr_r(fp, kz, 1, (int)nlm, nlm, P, dP);
+    orc_dlr(fp, kz, z, ell, (int)nlm, nlm, P, dlP);
+    orc_derivr_r(fp, kz, 1, (int)nlm, nlm, Q, dQ);
+    for (int i = 0; i < 6; i++) out[i] = calloc(1, nb);
+    const cplx *cP = (const cplx *)P, *cQ = (const cplx *)Q;
+    for (int k = 0; k < nr; k++) {
+        double r1 = 1.0 / z[k], r2 = 1.0 / (z[k] * z[k]);
+        for (long c = 0; c < nlm; c++) {
+            long i = c + k * nlm;
+            double ll = (double)ell[c] * (ell[c] + 1);
+            out[0][i] = ll * cP[i] * r2;
+            out[1][i] = ((cplx *)dP)[i] * r1;
+            out[2][i] = cQ[i] * r1;
+            out[3][i] = ll * cQ[i] * r2;
+            out[4][i] = ((cplx *)dQ)[i] * r1;
+            out[5][i] = -((cplx *)dlP)[i] * r1;
+        }
+    }
+    free(dP); free(dlP); free(dQ);
+}
+
+/*
+ * One nonlinear step of the coupled state (velocity W, Z; magnetic field
+ * Pb, Tb with B = curl curl (Pb r) + curl (Tb r); temperature T):
+ *   (NQ,NS,NT) = u x omega, (MQ,MS,MT) = u x B, NH = u . grad T,
+ * with grad T = (dT/dr, (1/r) dT/dtheta, (1/(r sin)) dT/dphi), i.e. the
+ * (Q,S,T) vector (derivr(T), T/r, 0).  Same composition and layouts as
+ * orc_nonlinear_uxw; the scalar analysis is fftp2d_real_to_complex_xy.
+ */
+void orc_nonlinear_coupled(void *vsht, void *vfc, double rint, double dz,
+                           const double *W, const double *Z, const double *Pb, const double *Tb,
+                           const double *T, double *NQ, double *NS, double *NT, double *MQ, double *MS,
+                           double *MT, double *NH, int plane0, int nplanes_eval)
+{
+    orc_sht *h = vsht;
+    orc_fc *fp = vfc;
+    int n = fp->n, nr = n - fp->C, lmax = h->lmax;
+    long nlm = h->nlm;
+    size_t nb = sizeof(cplx) * (size_t)nlm * n;
+    double *kz = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+    int *ell = malloc(sizeof(int) * nlm);
+    orc_kz(n, dz, kz);
+    for (int k = 0; k < n; k++) z[k] = rint + dz * k;
+    for (int m = 0; m <= lmax; m++)
+        for (int l = m; l <= lmax; l++) ell[lm_index(lmax, l, m)] = l;
+
+    cplx *uw[6], *bj[6];
+    curl_qst(fp, kz, z, ell, nlm, W, Z, uw);
+    curl_qst(fp, kz, z, ell, nlm, Pb, Tb, bj);
+    cplx *gq = malloc(nb), *gs = calloc(1, nb), *gt = calloc(1, nb);
+    orc_derivr_r(fp, kz, 1, (int)nlm, nlm, T, (double *)gq);
+    for (int k = 0; k < nr; k++)
+        for (long c = 0; c < nlm; c++) gs[c + k * nlm] = ((const cplx *)T)[c + k * nlm] / z[k];
+
+    double *outs[7] = {NQ, NS, NT, MQ, MS, MT, NH};
+    for (int i = 0; i < 7; i++) memset(outs[i], 0, nb);
+    int k0 = plane0, k1 = plane0 + nplanes_eval;
+    if (k1 > nr) k1 = nr;
+    long np = (long)h->nlat * h->nphi;
+    #pragma omp parallel
+    {
+        sht_work w = work_alloc(h);
+        double *g = malloc(sizeof(double) * np * 16);
+        double *u = g, *om = g + 3 * np, *b = g + 6 * np, *gr = g + 9 * np, *c = g + 12 * np,
+               *d = g + 15 * np;
+        #pragma omp for schedule(dynamic)
+        for (int k = k0; k < k1; k++) {
+            long o = k * nlm;
+            shqst_to_spat_plane(h, uw[0] + o, uw[1] + o, uw[2] + o, u, u + np, u + 2 * np, &w);
+            shqst_to_spat_plane(h, uw[3] + o, uw[4] + o, uw[5] + o, om, om + np, om + 2 * np, &w);
+            shqst_to_spat_plane(h, bj[0] + o, bj[1] + o, bj[2] + o, b, b + np, b + 2 * np, &w);
+            shqst_to_spat_plane(h, gq + o, gs + o, gt + o, gr, gr + np, gr + 2 * np, &w);
+            orc_prodvec(np, u, u + np, u + 2 * np, om, om + np, om + 2 * np, c, c + np, c + 2 * np);
+            spat_to_shqst_plane(h, c, c + np, c + 2 * np, (cplx *)NQ + o, (cplx *)NS + o, (cplx *)NT + o, &w);
+            orc_prodvec(np, u, u + np, u + 2 * np, b, b + np, b + 2 * np, c, c + np, c + 2 * np);
+            spat_to_shqst_plane(h, c, c + np, c + 2 * np, (cplx *)MQ + o, (cplx *)MS + o, (cplx *)MT + o, &w);
+            for (long i = 0; i < np; i++)
+                d[i] = u[i] * gr[i] + u[np + i] * gr[np + i] + u[2 * np + i] * gr[2 * np + i];
+            spat_to_sh_plane(h, d, (cplx *)NH + o, &w);
+        }
+        free(g);
+        work_free(&w);
+    }
+    for (int i = 0; i < 6; i++) { free(uw[i]); free(bj[i]); }
+    free(gq); free(gs); free(gt);
+    free(kz); free(z); free(ell);
+}
+
 /* ------------------------------------------------------------------ */
 /* Boundary: FC-Gram derivative matrices and the per-l W recovery       */
 /* ------------------------------------------------------------------ */

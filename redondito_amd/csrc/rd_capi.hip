@@ -591,6 +591,52 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     return rc;
 }
 
+// One nonlinear step of the coupled state on the plan's six Fourier and six
+// spectral work arrays:
+//   B:      radial(Pb, Tb) -> S0..S2 (S3..S5 = curl B, unused), synthesis -> F3..F5
+//   u, w:   radial(W, Z) -> S0..S5, synthesis of u -> F0..F2
+//   u x B:  phi_cross(F0..F2, F3..F5) -> F3..F5, analysis -> MQ, MS, MT
+//   u x w:  synthesis of w -> F3..F5, phi_cross -> F3..F5, analysis -> NQ, NS, NT
+//   u.gT:   (dT/dr, T/r, 0) -> S0..S2, synthesis -> F3..F5, phi_dot -> F3,
+//           scalar analysis -> NH
+int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double *Z, const double *Pb,
+                         const double *Tb, const double *T, double *NQ, double *NS, double *NT, double *MQ,
+                         double *MS, double *MT, double *NH)
+{
+    if (bad_plan(plan) || !W || !Z || !Pb || !Tb || !T || !NQ || !NS || !NT || !MQ || !MS || !MT || !NH)
+        return RD_ERR_ARG;
+    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32 && prec != RD_PREC_BF16X3) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->nwork < 6 || p->nr < 1) return RD_ERR_STATE;
+    const int nr = p->nr;
+    double2 **S = p->S, **F = p->F;
+    const double2 *u_b[6] = {F[0], F[1], F[2], F[3], F[4], F[5]};
+    double2 *o3[3] = {F[3], F[4], F[5]};
+    int rc = launch_uxw_radial(p, (const double2 *)Pb, (const double2 *)Tb, S);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[3], F[4], F[5]);
+    if (!rc) rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, S);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[0], F[1], F[2]);
+    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3);
+    if (!rc)
+        rc = launch_analysis(p, prec, true, nr, F[3], F[4], F[5], (double2 *)MQ, (double2 *)MS, (double2 *)MT,
+                             p->nlm);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, S[3], S[4], S[5], p->nlm, F[3], F[4], F[5]);
+    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3);
+    if (!rc)
+        rc = launch_analysis(p, prec, true, nr, F[3], F[4], F[5], (double2 *)NQ, (double2 *)NS, (double2 *)NT,
+                             p->nlm);
+    if (!rc) rc = launch_derivr_r(p, 1, (const double2 *)T, S[0]);
+    if (!rc) rc = launch_multr(p, -1, (const double2 *)T, S[1]);
+    if (!rc) rc = launch_zero_rows(p, S[2], 0, p->nz);
+    if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[3], F[4], F[5]);
+    if (!rc) rc = launch_phi_dot(p, nr, u_b, F[3]);
+    if (!rc)
+        rc = launch_analysis(p, prec, false, nr, F[3], nullptr, nullptr, (double2 *)NH, nullptr, nullptr, p->nlm);
+    double *outs[7] = {NQ, NS, NT, MQ, MS, MT, NH};
+    for (int i = 0; i < 7 && !rc; i++) rc = launch_zero_rows(p, (double2 *)outs[i], nr, p->nz - nr);
+    return rc;
+}
+
 int rd_dirichlet(rd_plan *plan, double *a)
 {
     if (bad_plan(plan) || !a) return RD_ERR_ARG;

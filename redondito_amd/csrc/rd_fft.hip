@@ -236,6 +236,50 @@ struct PhiPtrs {
     double *fo[6];
 };
 
+// Workgroup sums of NV values per thread.  Lane 0 of every wave posts its
+// wave's sums to red, one barrier, then every thread adds the NW posts in the
+// same order, so all threads hold bit-identical totals.  The barrier also
+// publishes any LDS writes made before the call.
+template <int NV, int NW>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double (*red)[NV])
+{
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < NV; i++) red[threadIdx.x >> 6][i] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double s = 0.0;
+        for (int q = 0; q < NW; q++) s += red[q][i];
+        v[i] = s;
+    }
+}
+
+// Two real rows u, w packed as one complex sequence u + i a w share the FFT's
+// rounding: each comes back with an error of about eps (|u| + a |w|).  With a
+// a power of two within a factor 2 of |u| / |w| (su, sw sums of squares) the
+// packing costs nothing beyond the rounding of the rows themselves, and
+// dividing by a afterwards is exact.
+// A value every lane holds with the same bits, moved to scalar registers.
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double pack_scale(double su, double sw)
+{
+    if (!(su > 0.0) || !(sw > 0.0)) return 1.0;
+    int e;
+    (void)frexp(sqrt(su / sw), &e);
+    return ldexp(1.0, e);
+}
+
 // 4 rows (two packed complex FFTs) of one plane per workgroup;
 // blockIdx.y selects the field.  F[k][m][theta] -> f[k][phi][theta].
 __global__ __launch_bounds__(256) void phi_inverse_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
@@ -318,63 +362,140 @@ __global__ __launch_bounds__(256) void phi_forward_kernel(PhiPtrs P, int lmax, i
     }
 }
 
-// One (plane, theta) row per workgroup: Z_x = U_x + i W_x for x = r, theta,
-// phi; three inverse FFTs give u_x + i w_x on the nphi points; the pointwise
-// product N = u x w (prodvec, src/pseudo/pseudospec_hd.f90:235-248) is packed
-// as (N_r + i N_theta, N_phi) and transformed forward.  Rows of Fout may alias
-// rows of Fin: each workgroup reads its own row completely before writing it.
-__global__ __launch_bounds__(512) void phi_cross_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
-                                                        int nplanes, FftSeq seq, const double2 *tw)
+// Row (k, th) of the six Fourier fields (U_r, U_t, U_p, W_r, W_t, W_p), as the
+// Hermitian-extended sequences packed Z_x = U_x + i a W_x in A[x nphi + m]
+// (B is scratch), a = pack_scale(|U|^2, |W|^2); returns 1/a.  Ends with a
+// barrier.
+constexpr int PHI_GEN_THREADS = 512;
+
+__device__ double phi_load_pairs(const PhiPtrs &P, int k, int th, int lmax, int nlat, int nphi, double2 *A,
+                                 double2 *B, double (*red)[2])
 {
-    extern __shared__ double2 sm[];
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int k = t / nlat, th = t % nlat;
-    if (k >= nplanes) return;
     const int L = lmax + 1;
-    double2 *A = sm, *B = sm + 3 * nphi, *twl = sm + 6 * nphi;
-    const int nh = load_twiddles(twl, tw, nphi);
+    double s[2] = {0.0, 0.0};
     for (int e = threadIdx.x; e < 3 * nphi; e += blockDim.x) {
         const int x = e / nphi, m = e - x * nphi;
-        double2 z = make_double2(0.0, 0.0);
+        double2 u = make_double2(0.0, 0.0), w = u;
         int mm = -1;
         bool cj = false;
         if (m <= lmax) mm = m;
         else if (m >= nphi - lmax) { mm = nphi - m; cj = true; }
         if (mm >= 0) {
             const long o = ((long)k * L + mm) * nlat + th;
-            double2 u = P.F[x][o], w = P.F[3 + x][o];
+            u = P.F[x][o];
+            w = P.F[3 + x][o];
             if (mm == 0) { u.y = 0.0; w.y = 0.0; }
             if (cj) { u.y = -u.y; w.y = -w.y; }
-            z = make_double2(u.x - w.y, u.y + w.x);
         }
-        A[e] = z;
+        s[0] += u.x * u.x + u.y * u.y;
+        s[1] += w.x * w.x + w.y * w.y;
+        A[e] = u;
+        B[e] = w;
+    }
+    block_sums<2, PHI_GEN_THREADS / 64>(s, red);
+    const double a = pack_scale(s[0], s[1]);
+    for (int e = threadIdx.x; e < 3 * nphi; e += blockDim.x) {
+        const double2 u = A[e], w = B[e];
+        A[e] = make_double2(u.x - a * w.y, u.y + a * w.x);
     }
     __syncthreads();
+    return 1.0 / a;
+}
+
+// One (plane, theta) row per workgroup: three inverse FFTs of the balanced
+// packs give u_x + i a w_x on the nphi points; the pointwise product
+// N = u x w (prodvec, src/pseudo/pseudospec_hd.f90:235-248) is packed as
+// (N_theta + i N_phi, N_r) and transformed forward (the horizontal pair,
+// from which S and T come, apart from the radial part).  Rows of Fout may
+// alias rows of Fin: each workgroup reads its own row completely before
+// writing it.
+__global__ __launch_bounds__(PHI_GEN_THREADS) void phi_cross_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
+                                                                    int nplanes, FftSeq seq, const double2 *tw)
+{
+    extern __shared__ double2 sm[];
+    __shared__ double red[PHI_GEN_THREADS / 64][2];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    double2 *A = sm, *B = sm + 3 * nphi, *twl = sm + 6 * nphi;
+    const int nh = load_twiddles(twl, tw, nphi);
+    const double ia = phi_load_pairs(P, k, th, lmax, nlat, nphi, A, B, red);
     double2 *R = lds_fft<0>(A, B, 3, seq, +1, twl, nh);
     double2 *O = (R == A) ? B : A;
     for (int j = threadIdx.x; j < nphi; j += blockDim.x) {
         const double2 zr = R[j], zt = R[nphi + j], zp = R[2 * nphi + j];
-        // u = (zr.x, zt.x, zp.x), w = (zr.y, zt.y, zp.y)
-        const double nr = zt.x * zp.y - zp.x * zt.y;
-        const double nt = zp.x * zr.y - zr.x * zp.y;
-        const double np = zr.x * zt.y - zt.x * zr.y;
-        O[j] = make_double2(nr, nt);
-        O[nphi + j] = make_double2(np, 0.0);
+        // u = (zr.x, zt.x, zp.x), w = (zr.y, zt.y, zp.y) / a
+        const double wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
+        const double nr = zt.x * wp - zp.x * wt;
+        const double nt = zp.x * wr - zr.x * wp;
+        const double np = zr.x * wt - zt.x * wr;
+        O[j] = make_double2(nt, np);
+        O[nphi + j] = make_double2(nr, 0.0);
     }
     __syncthreads();
     double2 *Y = lds_fft<0>(O, R, 2, seq, -1, twl, nh);
     const double sc = 0.5 * (2.0 * M_PI / nphi);
     for (int m = threadIdx.x; m < L; m += blockDim.x) {
         const double2 yp = Y[m], ym = Y[m == 0 ? 0 : nphi - m];
-        double2 xr = make_double2(sc * (yp.x + ym.x), sc * (yp.y - ym.y));
-        double2 xt = make_double2(sc * (yp.y + ym.y), -sc * (yp.x - ym.x));
-        double2 xp = cscale(Y[nphi + m], 2.0 * sc);
+        double2 xt = make_double2(sc * (yp.x + ym.x), sc * (yp.y - ym.y));
+        double2 xp = make_double2(sc * (yp.y + ym.y), -sc * (yp.x - ym.x));
+        double2 xr = cscale(Y[nphi + m], 2.0 * sc);
         if (m == 0) { xr.y = 0.0; xt.y = 0.0; xp.y = 0.0; }
         const long o = ((long)k * L + m) * nlat + th;
         P.Fo[0][o] = xr;
         P.Fo[1][o] = xt;
         P.Fo[2][o] = xp;
     }
+}
+
+// One (plane, theta) row per workgroup: three inverse FFTs of the balanced
+// packs give u_x + i a g_x on the nphi points; the pointwise scalar product
+// D = u . g is transformed forward into Fo[0].  Fo[0] may alias a row of F:
+// each workgroup reads its own row completely before writing it.
+__global__ __launch_bounds__(PHI_GEN_THREADS) void phi_dot_kernel(PhiPtrs P, int lmax, int nlat, int nphi,
+                                                                  int nplanes, FftSeq seq, const double2 *tw)
+{
+    extern __shared__ double2 sm[];
+    __shared__ double red[PHI_GEN_THREADS / 64][2];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    double2 *A = sm, *B = sm + 3 * nphi, *twl = sm + 6 * nphi;
+    const int nh = load_twiddles(twl, tw, nphi);
+    const double ia = phi_load_pairs(P, k, th, lmax, nlat, nphi, A, B, red);
+    double2 *R = lds_fft<0>(A, B, 3, seq, +1, twl, nh);
+    double2 *O = (R == A) ? B : A;
+    for (int j = threadIdx.x; j < nphi; j += blockDim.x) {
+        const double2 zr = R[j], zt = R[nphi + j], zp = R[2 * nphi + j];
+        O[j] = make_double2((zr.x * zr.y + zt.x * zt.y + zp.x * zp.y) * ia, 0.0);
+    }
+    __syncthreads();
+    double2 *Y = lds_fft<0>(O, R, 1, seq, -1, twl, nh);
+    const double sc = 2.0 * M_PI / nphi;
+    for (int m = threadIdx.x; m < L; m += blockDim.x) {
+        double2 x = cscale(Y[m], sc);
+        if (m == 0) x.y = 0.0;
+        P.Fo[0][((long)k * L + m) * nlat + th] = x;
+    }
+}
+
+// Fin: (u_r, u_t, u_p, g_r, g_t, g_p) Fourier fields -> Fout = u . g.
+int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout)
+{
+    PhiPtrs P = {};
+    for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
+    P.Fo[0] = Fout;
+    size_t shm = sizeof(double2) * (6 * p->nphi + p->nphi / 2);
+    {
+        int lrc = allow_lds(phi_dot_kernel, shm);
+        if (lrc) return lrc;
+    }
+    hipLaunchKernelGGL(phi_dot_kernel, dim3(nplanes * p->nlat), dim3(PHI_GEN_THREADS), shm, p->stream, P, p->lmax,
+                       p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
 }
 
 int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f)
@@ -489,11 +610,12 @@ __device__ __forceinline__ double2 twq(const double2 *tw, int t, int sign)
 // that four rows run per CU.  Every phase reads its inputs into registers,
 // waits at a barrier, then writes its outputs over the same buffer:
 //   R3  radix-3 step of the inverse transform of the packed sequences
-//       z_x = u_x + i w_x (x = r, theta, phi): the spectrum block 256..511 is
+//       z_x = u_x + i a w_x (x = r, theta, phi; a a power of two that
+//       balances u and w): the spectrum block 256..511 is
 //       zero, so Y[3x+r][b] = (Z[b] + e^{2 pi i 512 r/768} Z[512+b]) e^{2 pi i b r/768}
 //   P1  first 16-point pass of the 256-point inverse FFTs over b = 16 b2 + b1
 //   P2  second inverse pass: grid values z_x at j = 3 (q1 + 16 q2) + r
-//   CR  N = u x w packed as (N_r + i N_theta, N_phi)
+//   CR  N = u x w packed as (N_theta + i N_phi, N_r)
 //   F1  first forward pass over n1 = 16 a2 + a1 of j = 3 n1 + n2
 //   F2  second forward pass: Yf[3s+n2][k] = 256-point spectra
 //   FIN X[m] = sum_n2 e^{-2 pi i n2 m/768} Yf[n2][m mod 256], unpacking of the
@@ -508,21 +630,30 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     constexpr int N = 768, NQ = 192;
     __shared__ double2 twl[NQ];
     __shared__ double2 B[PHI_BUF];
+    __shared__ double red[4][2];
+    __shared__ double ia_s;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int k = t / nlat, th = t % nlat;
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
     if (tid < NQ) twl[tid] = tw[tid];
-    // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax)
+    // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax),
+    // and the sums of squares of u and of w
+    double s[2] = {0.0, 0.0};
 #pragma unroll
     for (int f = 0; f < 6; f++) {
         double2 v = make_double2(0.0, 0.0);
         if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
         if (tid == 0) v.y = 0.0;
         B[f * 256 + tid] = v;
+        s[f / 3] += v.x * v.x + v.y * v.y;
     }
-    __syncthreads();
+    block_sums<2, 4>(s, red);
+    // balanced packs z_x = u_x + i a w_x; 1/a kept in LDS for CR (read there
+    // after several barriers)
+    const double a = wave_uniform(pack_scale(s[0], s[1]));
+    if (tid == 0) ia_s = 1.0 / a;
 
     // ---- R3 -> Y[3x+r][b]
     {
@@ -530,8 +661,8 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         double2 zb[3], zc[3];
 #pragma unroll
         for (int x = 0; x < 3; x++) {
-            const double2 u = B[x * 256 + b], w = B[(3 + x) * 256 + b];
-            const double2 uc = B[x * 256 + mm], wc = B[(3 + x) * 256 + mm];
+            const double2 u = B[x * 256 + b], w = cscale(B[(3 + x) * 256 + b], a);
+            const double2 uc = B[x * 256 + mm], wc = cscale(B[(3 + x) * 256 + mm], a);
             zb[x] = make_double2(u.x - w.y, u.y + w.x);
             zc[x] = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
         }
@@ -580,15 +711,16 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         }
     }
     __syncthreads();
-    // ---- CR: (N_r + i N_theta) at j, N_phi at N + j
+    // ---- CR: (N_theta + i N_phi) at j, N_r at N + j
     {
         double2 o0[3], o1[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int j = tid + 256 * i;
             const double2 zr = B[j], zt = B[N + j], zp = B[2 * N + j];
-            o0[i] = make_double2(zt.x * zp.y - zp.x * zt.y, zp.x * zr.y - zr.x * zp.y);
-            o1[i] = make_double2(zr.x * zt.y - zt.x * zr.y, 0.0);
+            const double ia = ia_s, wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
+            o0[i] = make_double2(zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr);
+            o1[i] = make_double2(zt.x * wp - zp.x * wt, 0.0);
         }
         __syncthreads();
 #pragma unroll
@@ -641,9 +773,9 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         const double2 x1 = cadd(B[768 + m], cadd(cmul(B[1024 + m], c1), cmul(B[1280 + m], c2)));
         double2 xm = cadd(B[mneg], cadd(cmul(B[256 + mneg], cconj(c1)), cmul(B[512 + mneg], cconj(c2))));
         if (m == 0) xm = xp;
-        double2 gr = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
-        double2 gt = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
-        double2 gp = cscale(x1, 2.0 * sc);
+        double2 gt = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
+        double2 gp = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
+        double2 gr = cscale(x1, 2.0 * sc);
         if (m == 0) {
             gr.y = 0.0;
             gt.y = 0.0;
@@ -672,7 +804,7 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
         int lrc = allow_lds(phi_cross_kernel, shm);
         if (lrc) return lrc;
     }
-    hipLaunchKernelGGL(phi_cross_kernel, dim3(nplanes * p->nlat), dim3(512), shm, p->stream, P,
+    hipLaunchKernelGGL(phi_cross_kernel, dim3(nplanes * p->nlat), dim3(PHI_GEN_THREADS), shm, p->stream, P,
                        p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
     RD_LAUNCH_CHECK();
     return RD_OK;

@@ -110,6 +110,8 @@ int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *
 int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f, double2 *const *F);
 // Fused: six Fourier fields (u_r,u_t,u_p,w_r,w_t,w_p) -> u x w -> three Fourier fields.
 int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout);
+// Fused: (u_r,u_t,u_p,g_r,g_t,g_p) -> u . g -> one Fourier field.
+int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout);
 
 // Radial transforms on (nlm, nz) complex, in place.
 int launch_z_forward(Plan *p, double2 *a);

@@ -62,6 +62,7 @@ def lib():
         L.rd_prodvec.argtypes = [vp, i] + [vp] * 9
         L.rd_fc_filter.argtypes = [vp, ctypes.c_char, vp]
         L.rd_nonlinear_uxw.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
         L.rd_derivr_mat.argtypes = [vp, i, vp]
@@ -296,6 +297,15 @@ def nonlinear_uxw(plan, W, Z, NQ, NS, NT, prec=RD_PREC_FP64):
     """N = u x omega, u = curl curl (W r) + curl (Z r), as (Q,S,T) in (lm, r)."""
     _check(lib().rd_nonlinear_uxw(plan.h, prec, plan._spec(W), plan._spec(Z), plan._spec(NQ),
                                   plan._spec(NS), plan._spec(NT)), "nonlinear_uxw")
+
+
+def nonlinear_coupled(plan, W, Z, Pb, Tb, T, NQ, NS, NT, MQ, MS, MT, NH, prec=RD_PREC_FP64):
+    """One coupled step: (NQ, NS, NT) = u x omega, (MQ, MS, MT) = u x B and
+    NH = u . grad T, with u from (W, Z) and B from (Pb, Tb) as poloidal /
+    toroidal potentials, all (lm, r)."""
+    _check(lib().rd_nonlinear_coupled(plan.h, prec, *[plan._spec(x) for x in (W, Z, Pb, Tb, T, NQ, NS, NT, MQ,
+                                                                                MS, MT, NH)]),
+           "nonlinear_coupled")
 
 
 # ------------------------------------------------------------ boundary

@@ -174,6 +174,21 @@ def shell_fields(lmax, nr, n, rint, rext, seed, slope=2.0):
     return out
 
 
+def coupled_fields(lmax, nr, n, rint, rext, seed):
+    """The coupled state (W, Z, Pb, Tb, T), each (n, nlm) in (lm, r): velocity
+    potentials with spectrum (l+1)^-2, magnetic potentials (l+1)^-2.5 and a
+    temperature (l+1)^-2 with its l = 0 mode, all with the sin radial profile."""
+    W, Z = shell_fields(lmax, nr, n, rint, rext, seed, slope=2.0)
+    Pb, Tb = shell_fields(lmax, nr, n, rint, rext, seed + 1, slope=2.5)
+    rng = np.random.default_rng(seed + 2)
+    r = rint + (rext - rint) * np.arange(nr) / (nr - 1)
+    c = (rng.standard_normal(nlm_of(lmax)) + 1j * rng.standard_normal(nlm_of(lmax))) * (ell_of(lmax) + 1.0) ** -2
+    c[: lmax + 1] = c[: lmax + 1].real
+    T = np.zeros((n, nlm_of(lmax)), complex)
+    T[:nr] = (1.0 + r[:, None]) * np.sin(PI * (r - rint) / (rext - rint))[:, None] * c[None, :]
+    return W, Z, Pb, Tb, T
+
+
 # ------------------------------------------------------------ boundary
 def boundary_inputs(z, ell, N):
     """The fields of src/prueba_boundary.f90:58-78 on the rows of z: Z0, the
@@ -235,12 +250,12 @@ def l_bands(lmax):
     return [(a, b) for a, b in zip(edges[:-1], edges[1:]) if a <= lmax]
 
 
-def band_errors(out, ref, lmax, nr):
-    """Relative L2 error of each output field (NQ, NS, NT) on the nr physical
-    shells, over all degrees and per degree band."""
+def band_errors(out, ref, lmax, nr, names=("NQ", "NS", "NT")):
+    """Relative L2 error of each output field (named by names) on the nr
+    physical shells, over all degrees and per degree band."""
     ell = ell_of(lmax)
     rep = {}
-    for name, o, r in zip(("NQ", "NS", "NT"), out, ref):
+    for name, o, r in zip(names, out, ref):
         o, r = o[:nr], r[:nr]
         d = {"all": rell2(o, r)}
         for a, b in l_bands(lmax):

@@ -25,7 +25,8 @@ def test_header_declares_entry_points():
     for must in ("rd_plan_create", "rd_sht_analysis", "rd_sht_synthesis", "rd_vsht_analysis",
                  "rd_vsht_synthesis", "rd_z_forward", "rd_z_inverse", "rd_3d_forward", "rd_3d_inverse",
                  "rd_derivr", "rd_dlr", "rd_prodvec", "rd_nonlinear_uxw", "rd_load_dirichlet_tables",
-                 "rd_derivr_mat", "rd_bc_setup", "rd_invdl", "rd_v_imposebc", "rd_v_imposebc_fast"):
+                 "rd_derivr_mat", "rd_bc_setup", "rd_invdl", "rd_v_imposebc", "rd_v_imposebc_fast",
+                 "rd_nonlinear_coupled", "rd_batch_create", "rd_batch_capture", "rd_batch_tune"):
         assert must in names
 
 

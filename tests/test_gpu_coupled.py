@@ -1,0 +1,152 @@
+"""The coupled nonlinear step (u x omega, u x B, u . grad T) on the GPU.
+
+FP64 runs against the CPU oracle at sizes the oracle finishes in seconds. These cover every radial-length and
+phi-kernel path:
+  * radial lengths 49 and 153 take the factorised kernel;
+  * radial length 89 (prime) takes the generic Stockham kernel;
+  * nphi = 768 takes the fused in-place cross kernel.
+Each output group is held to 1e-12 relative L2, and each field to 30x its FP64 noise floor where that is larger.
+
+At the large coupled shape (lmax = 511, nlat = 768, nphi = 1536, nr = 256, radial length 281):
+  * the FP64 step is checked against closed forms;
+  * the precision forms are checked against FP64, per field and per degree band.
+"""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+C, D, RI, RO = 25, 5, 0.35, 1.0
+NAMES = ("NQ", "NS", "NT", "MQ", "MS", "MT", "NH")
+BOUND = {1: 2e-6, 2: 5e-5}      # joint relative L2 vs FP64 per output group
+FORM = {1: "fp32", 2: "bf16x3"}
+
+
+def sphp():
+    from redondito_amd import sphp as S
+    return S
+
+
+def to_dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+
+
+def to_host(x):
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
+
+
+def fc_dir():
+    A, Q = O.load_tables(sphp().TABLE_DIR, C, D)
+    return O.fc_dir(A, Q)
+
+
+def run_gpu(plan, ins, prec=0, fill=3.0):
+    d = [to_dev(x) for x in ins]
+    out = [torch.full_like(d[0], fill) for _ in range(7)]
+    sphp().nonlinear_coupled(plan, *d, *out, prec=prec)
+    for x, h in zip(d, ins):
+        assert np.array_equal(to_host(x), h), "input modified"
+    return [to_host(o) for o in out]
+
+
+def groups(xs):
+    return xs[0:3], xs[3:6], xs[6:7]
+
+
+@pytest.mark.parametrize("lmax,nr,grid", [(15, 24, None), (31, 24, None), (63, 64, None), (47, 128, None),
+                                          (255, 12, None)])
+def test_coupled_vs_oracle(lmax, nr, grid):
+    S = sphp()
+    n = nr + C
+    dz = (RO - RI) / (nr - 1)
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    nphi = 2 * nlat
+    if grid is not None:
+        nlat, nphi = grid
+    print(f"case lmax={lmax} nr={nr} nlat={nlat} nphi={nphi}", flush=True)
+    plan = S.FCPlan(lmax, nlat, nphi, n, C, D, dz, RI)
+    ins = H.coupled_fields(lmax, nr, n, RI, RO, seed=20261203 + lmax)
+    oc = O.SHT(lmax, nlat, nphi)
+    fc = O.FC(n, C, D, fc_dir())
+    ref = O.nonlinear_coupled(oc, fc, RI, dz, *ins)
+    got = run_gpu(plan, ins)
+    floor = H.noise_floor(lambda *x: [o[:nr] for o in O.nonlinear_coupled(oc, fc, RI, dz, *x)], list(ins))
+    errs = [H.rell2(g[:nr], r[:nr]) for g, r in zip(got, ref)]
+    joint = [H.rell2_joint([g[:nr] for g in gg], [r[:nr] for r in rr]) for gg, rr in zip(groups(got), groups(ref))]
+    print(f"joint uxw {joint[0]:.2e} uxB {joint[1]:.2e} u.gT {joint[2]:.2e}; " +
+          " ".join(f"{nm} {e:.2e}/{f:.1e}" for nm, e, f in zip(NAMES, errs, floor)))
+    assert all(j < TOL for j in joint)
+    assert all(e < max(TOL, 30 * f) for e, f in zip(errs, floor))
+    assert all(float(np.max(np.abs(g[nr:]))) == 0.0 for g in got)
+
+
+@pytest.fixture(scope="module")
+def large():
+    """The large coupled shape: lmax 511, nlat 768, nphi 1536, nr 256."""
+    lmax, nlat, nphi, nr = 511, 768, 1536, 256
+    n = nr + C
+    dz = (RO - RI) / (nr - 1)
+    plan = sphp().FCPlan(lmax, nlat, nphi, n, C, D, dz, RI)
+    return dict(plan=plan, lmax=lmax, nr=nr, n=n, dz=dz)
+
+
+def test_coupled_large_closed_forms(large):
+    """u = Omega z x r and B = B0 z at the large shape:
+    u x omega = grad(Omega^2 s^2), u x B = grad(Omega B0 s^2 / 2) with
+    s = r sin(theta), and u . grad T = Omega dT/dphi (NH_lm = i m Omega T_lm)."""
+    plan, lmax, nr, n = large["plan"], large["lmax"], large["nr"], large["n"]
+    print(f"case large closed forms lmax={lmax} nr={nr}", flush=True)
+    r = RI + large["dz"] * np.arange(nr)
+    Om, B0 = 0.7, 1.3
+    W = np.zeros((n, plan.nlm), complex)
+    Z, Pb, Tb, T = (np.zeros_like(W) for _ in range(4))
+    Z[:nr, H.lm_index(lmax, 1, 0)] = Om * r ** 2 * np.sqrt(4 * np.pi / 3)
+    Pb[:nr, H.lm_index(lmax, 1, 0)] = 0.5 * B0 * r ** 2 * np.sqrt(4 * np.pi / 3)
+    T[:nr] = H.random_spectrum(lmax, nr, seed=11, slope=1.5)
+    NQ, NS, NT, MQ, MS, MT, NH = run_gpu(plan, (W, Z, Pb, Tb, T))
+    i00, i20 = H.lm_index(lmax, 0, 0), H.lm_index(lmax, 2, 0)
+
+    def grad_s2(c):
+        eQ = np.zeros((nr, plan.nlm), complex)
+        eS = np.zeros_like(eQ)
+        eQ[:, i00] = (2 / 3) * c * r * np.sqrt(4 * np.pi)
+        eQ[:, i20] = -(2 / 3) * c * r * np.sqrt(4 * np.pi / 5)
+        eS[:, i20] = -(1 / 3) * c * r * np.sqrt(4 * np.pi / 5)
+        return eQ, eS
+
+    eQ, eS = grad_s2(2 * Om ** 2)
+    fQ, fS = grad_s2(Om * B0)
+    eH = 1j * H.emm_of(lmax)[None, :] * Om * T[:nr]
+    e = dict(NQ=H.rell2(NQ[:nr], eQ), NS=H.rell2(NS[:nr], eS), MQ=H.rell2(MQ[:nr], fQ),
+             MS=H.rell2(MS[:nr], fS), NT=float(np.max(np.abs(NT[:nr]))), MT=float(np.max(np.abs(MT[:nr]))),
+             NH=H.rell2(NH[:nr], eH))
+    print("large closed forms: " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    assert e["NQ"] < 1e-9 and e["MQ"] < 1e-9
+    assert e["NS"] < 1e-11 and e["MS"] < 1e-11
+    assert e["NT"] < 1e-10 and e["MT"] < 1e-10
+    assert e["NH"] < 1e-12
+
+
+def test_coupled_large_precision_forms(large):
+    """Seeded coupled state at the large shape: the FP32 and BF16x3 Legendre
+    forms against the FP64 step, per output field and per degree band."""
+    plan, lmax, nr, n = large["plan"], large["lmax"], large["nr"], large["n"]
+    print(f"case large precision forms lmax={lmax} nr={nr}", flush=True)
+    ins = H.coupled_fields(lmax, nr, n, RI, RO, seed=20261207)
+    outs = {prec: run_gpu(plan, ins, prec=prec) for prec in (0, 1, 2)}
+    for prec in (1, 2):
+        rep = H.band_errors(outs[prec], outs[0], lmax, nr, names=NAMES)
+        joint = [H.rell2_joint([g[:nr] for g in gg], [r[:nr] for r in rr])
+                 for gg, rr in zip(groups(outs[prec]), groups(outs[0]))]
+        print(f"[{FORM[prec]} vs fp64 lmax={lmax} nr={nr}] joint uxw {joint[0]:.2e} uxB {joint[1]:.2e} "
+              f"u.gT {joint[2]:.2e} " + json.dumps(
+                  {k: {b: float(f"{x:.2e}") for b, x in v.items()} for k, v in rep.items() if k != "joint"}))
+        assert all(j < BOUND[prec] for j in joint), (prec, joint)

@@ -359,6 +359,60 @@ def test_uxw_solid_body_rotation():
     assert np.all(NQ[nr:] == 0)
 
 
+def _brute_basis(lmax, nlat, nphi):
+    """scipy harmonics without the CS phase on the Gauss grid: Y, dY/dtheta
+    (centred differences) and i m Y / sin(theta), with the quadrature."""
+    x, wg = np.polynomial.legendre.leggauss(nlat)
+    th = np.arccos(x[::-1])
+    wg = wg[::-1]
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    TH, PH = np.meshgrid(th, ph)
+    Y, dY, mY = {}, {}, {}
+    h = 1e-6
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            Y[l, m] = sp.sph_harm_y(l, m, TH, PH) * (-1) ** m
+            dY[l, m] = (sp.sph_harm_y(l, m, TH + h, PH) - sp.sph_harm_y(l, m, TH - h, PH)) * (-1) ** m / (2 * h)
+            mY[l, m] = 1j * m * Y[l, m] / np.sin(TH)
+    return dict(lmax=lmax, TH=TH, wg=wg, dphi=2 * np.pi / nphi, Y=Y, dY=dY, mY=mY)
+
+
+def _brute_re_sum(b, terms):
+    out = np.zeros_like(b["TH"])
+    for (l, m), v in terms:
+        out += v.real if m == 0 else 2 * v.real
+    return out
+
+
+def _brute_vec(b, q, s, t):
+    """(v_r, v_theta, v_phi) of the (Q, S, T) spectra q, s, t (nlm,)."""
+    lmax, Y, dY, mY = b["lmax"], b["Y"], b["dY"], b["mY"]
+    keys = [(l, m) for l in range(lmax + 1) for m in range(l + 1)]
+    i = lambda l, m: H.lm_index(lmax, l, m)
+    vr = _brute_re_sum(b, [((l, m), q[i(l, m)] * Y[l, m]) for l, m in keys])
+    vt = _brute_re_sum(b, [((l, m), s[i(l, m)] * dY[l, m] + t[i(l, m)] * mY[l, m]) for l, m in keys])
+    vp = _brute_re_sum(b, [((l, m), s[i(l, m)] * mY[l, m] - t[i(l, m)] * dY[l, m]) for l, m in keys])
+    return vr, vt, vp
+
+
+def _brute_vanal(b, l, m, Nr, Nt, Np):
+    """(Q, S, T) coefficients of degree l, order m by direct quadrature."""
+    Y, dY, mY, wg, dphi = b["Y"][l, m], b["dY"][l, m], b["mY"][l, m], b["wg"], b["dphi"]
+    q = np.sum(wg[None] * Nr * np.conj(Y)) * dphi
+    s = np.sum(wg[None] * (Nt * np.conj(dY) + Np * np.conj(mY))) * dphi / (l * (l + 1))
+    t = np.sum(wg[None] * (Nt * np.conj(mY) - Np * np.conj(dY))) * dphi / (l * (l + 1))
+    return q, s, t
+
+
+def _curl_qst(fc, kz, z, ell, P, Q, k):
+    """(Q,S,T) of v = curl curl (P r) + curl (Q r) and of curl v on shell k."""
+    dP, d2P, dQ = fc.derivr_r(P, kz, 1), fc.derivr_r(P, kz, 2), fc.derivr_r(Q, kz, 1)
+    r, ll = z[k], ell * (ell + 1.0)
+    v = (ll * P[k] / r ** 2, dP[k] / r, Q[k] / r)
+    c = (ll * Q[k] / r ** 2, dQ[k] / r, -(d2P[k] - ll * P[k] / r ** 2) / r)
+    return v, c
+
+
 def test_uxw_brute_force_small():
     """Independent numpy restatement on a tiny shell: u and omega evaluated on
     the grid from scipy harmonics and the oracle's radial derivatives, the cross
@@ -373,57 +427,102 @@ def test_uxw_brute_force_small():
     NQ, NS, NT = O.nonlinear_uxw(S, fc, ri, dz, W, Z)
     kz = O.kz(n, dz)
     z = ri + dz * np.arange(n)
-    ell = H.ell_of(lmax)
-    dW = fc.derivr_r(W, kz, 1)
-    d2W = fc.derivr_r(W, kz, 2)
-    dZ = fc.derivr_r(Z, kz, 1)
-    x, wg = np.polynomial.legendre.leggauss(nlat)
-    th = np.arccos(x[::-1])
-    wg = wg[::-1]
-    ph = 2 * np.pi * np.arange(nphi) / nphi
-    TH, PH = np.meshgrid(th, ph)
     k = 7                                   # one shell
-    r = z[k]
-    ll = ell * (ell + 1.0)
-    uq, us, ut = ll * W[k] / r ** 2, dW[k] / r, Z[k] / r
-    wq, ws, wt = ll * Z[k] / r ** 2, dZ[k] / r, -(d2W[k] - ll * W[k] / r ** 2) / r
-    Y, dY, mY = {}, {}, {}
-    h = 1e-6
-    for l in range(lmax + 1):
-        for m in range(l + 1):
-            Y[l, m] = sp.sph_harm_y(l, m, TH, PH) * (-1) ** m
-            dY[l, m] = (sp.sph_harm_y(l, m, TH + h, PH) - sp.sph_harm_y(l, m, TH - h, PH)) * (-1) ** m / (2 * h)
-            mY[l, m] = 1j * m * Y[l, m] / np.sin(TH)
-
-    def re_sum(terms):
-        out = np.zeros_like(TH)
-        for (l, m), v in terms:
-            out += v.real if m == 0 else 2 * v.real
-        return out
-
-    def vec(q, s, t):
-        keys = [(l, m) for l in range(lmax + 1) for m in range(l + 1)]
-        i = lambda l, m: H.lm_index(lmax, l, m)
-        vr = re_sum([((l, m), q[i(l, m)] * Y[l, m]) for l, m in keys])
-        vt = re_sum([((l, m), s[i(l, m)] * dY[l, m] + t[i(l, m)] * mY[l, m]) for l, m in keys])
-        vp = re_sum([((l, m), s[i(l, m)] * mY[l, m] - t[i(l, m)] * dY[l, m]) for l, m in keys])
-        return vr, vt, vp
-
-    u = vec(uq, us, ut)
-    w = vec(wq, ws, wt)
-    N = np.cross(np.stack(u, -1), np.stack(w, -1))
-    Nr, Nt, Np = N[..., 0], N[..., 1], N[..., 2]
-    dphi = 2 * np.pi / nphi
+    uqst, wqst = _curl_qst(fc, kz, z, H.ell_of(lmax), W, Z, k)
+    b = _brute_basis(lmax, nlat, nphi)
+    N = np.cross(np.stack(_brute_vec(b, *uqst), -1), np.stack(_brute_vec(b, *wqst), -1))
     for l in range(1, lmax + 1):
         for m in range(l + 1):
             i = H.lm_index(lmax, l, m)
-            yc = np.conj(Y[l, m])
-            q = np.sum(wg[None] * Nr * yc) * dphi
-            s = np.sum(wg[None] * (Nt * np.conj(dY[l, m]) + Np * np.conj(mY[l, m]))) * dphi / (l * (l + 1))
-            t = np.sum(wg[None] * (Nt * np.conj(mY[l, m]) - Np * np.conj(dY[l, m]))) * dphi / (l * (l + 1))
-            assert abs(q - NQ[k, i]) < 1e-7 * (1 + abs(q)), (l, m)
-            assert abs(s - NS[k, i]) < 1e-7 * (1 + abs(s)), (l, m)
-            assert abs(t - NT[k, i]) < 1e-7 * (1 + abs(t)), (l, m)
+            for e, got in zip(_brute_vanal(b, l, m, N[..., 0], N[..., 1], N[..., 2]), (NQ, NS, NT)):
+                assert abs(e - got[k, i]) < 1e-7 * (1 + abs(e)), (l, m)
+
+
+def test_coupled_brute_force_small():
+    """The coupled step against the same independent restatement: u, omega,
+    B and grad T = (dT/dr, (1/r) dT/dtheta, (1/(r sin)) dT/dphi) on the grid
+    from scipy harmonics, u x B and u . grad T analysed by direct quadrature
+    (the scalar with conj(Y_lm) weights, fftp2d_real_to_complex_xy)."""
+    lmax, nr, C, d, ri, ro = 5, 24, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat, nphi = 8, 16
+    S = O.SHT(lmax, nlat, nphi)
+    fc = O.FC(n, C, d, fcdir())
+    W, Z, Pb, Tb, T = H.coupled_fields(lmax, nr, n, ri, ro, seed=9)
+    out = O.nonlinear_coupled(S, fc, ri, dz, W, Z, Pb, Tb, T)
+    kz = O.kz(n, dz)
+    z = ri + dz * np.arange(n)
+    ell = H.ell_of(lmax)
+    k = 11
+    uqst, _ = _curl_qst(fc, kz, z, ell, W, Z, k)
+    bqst, _ = _curl_qst(fc, kz, z, ell, Pb, Tb, k)
+    gqst = (fc.derivr_r(T, kz, 1)[k], T[k] / z[k], np.zeros_like(T[k]))
+    b = _brute_basis(lmax, nlat, nphi)
+    u = np.stack(_brute_vec(b, *uqst), -1)
+    M = np.cross(u, np.stack(_brute_vec(b, *bqst), -1))
+    D = np.sum(u * np.stack(_brute_vec(b, *gqst), -1), -1)
+    MQ, MS, MT, NH = out[3], out[4], out[5], out[6]
+    for l in range(0, lmax + 1):
+        for m in range(l + 1):
+            i = H.lm_index(lmax, l, m)
+            h = np.sum(b["wg"][None] * D * np.conj(b["Y"][l, m])) * b["dphi"]
+            assert abs(h - NH[k, i]) < 1e-7 * (1 + abs(h)), (l, m)
+            if l == 0:
+                continue
+            for e, got in zip(_brute_vanal(b, l, m, M[..., 0], M[..., 1], M[..., 2]), (MQ, MS, MT)):
+                assert abs(e - got[k, i]) < 1e-7 * (1 + abs(e)), (l, m)
+
+
+def test_coupled_solid_body_uniform_field():
+    """u = Omega z x r (Z_10 = Omega r^2 sqrt(4 pi/3)) in a uniform field
+    B = B0 z (Pb_10 = (B0/2) sqrt(4 pi/3) r^2):
+      u x B = Omega B0 s s^ = grad(Omega B0 r^2 sin^2 theta / 2),
+      u . grad T = Omega dT/dphi, i.e. NH_lm = i m Omega T_lm for any T,
+    and the u x omega part is the u x omega oracle's output."""
+    lmax, nr, C, d, ri, ro = 15, 64, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat = O.grid_nlat(lmax)
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    fc = O.FC(n, C, d, fcdir())
+    r = ri + dz * np.arange(nr)
+    Om, B0 = 0.7, 1.3
+    W = np.zeros((n, S.nlm), complex)
+    Z, Pb, Tb = np.zeros_like(W), np.zeros_like(W), np.zeros_like(W)
+    Z[:nr, H.lm_index(lmax, 1, 0)] = Om * r ** 2 * np.sqrt(4 * np.pi / 3)
+    Pb[:nr, H.lm_index(lmax, 1, 0)] = 0.5 * B0 * r ** 2 * np.sqrt(4 * np.pi / 3)
+    T = np.zeros_like(W)
+    T[:nr] = H.random_spectrum(lmax, nr, seed=4, slope=1.5)
+    NQ, NS, NT, MQ, MS, MT, NH = O.nonlinear_coupled(S, fc, ri, dz, W, Z, Pb, Tb, T)
+    for a, e in zip((NQ, NS, NT), O.nonlinear_uxw(S, fc, ri, dz, W, Z)):
+        assert np.array_equal(a, e)
+    eQ = np.zeros((nr, S.nlm), complex)
+    eS = np.zeros_like(eQ)
+    eQ[:, H.lm_index(lmax, 0, 0)] = (2 / 3) * Om * B0 * r * np.sqrt(4 * np.pi)
+    eQ[:, H.lm_index(lmax, 2, 0)] = -(2 / 3) * Om * B0 * r * np.sqrt(4 * np.pi / 5)
+    eS[:, H.lm_index(lmax, 2, 0)] = -(1 / 3) * Om * B0 * r * np.sqrt(4 * np.pi / 5)
+    assert H.rell2(MQ[:nr], eQ) < 1e-9
+    assert H.rell2(MS[:nr], eS) < 1e-12
+    assert np.max(np.abs(MT[:nr])) < 1e-12
+    eH = 1j * H.emm_of(lmax)[None, :] * Om * T[:nr]
+    assert H.rell2(NH[:nr], eH) < 1e-13
+    for a in (MQ, MS, MT, NH):
+        assert np.all(a[nr:] == 0)
+
+
+def test_coupled_self_field_has_no_lorentz_term():
+    """With B built from the velocity's own potentials, u x B = u x u = 0."""
+    lmax, nr, C, d, ri, ro = 15, 24, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat = O.grid_nlat(lmax)
+    S = O.SHT(lmax, nlat, 2 * nlat)
+    fc = O.FC(n, C, d, fcdir())
+    W, Z, _, _, T = H.coupled_fields(lmax, nr, n, ri, ro, seed=3)
+    out = O.nonlinear_coupled(S, fc, ri, dz, W, Z, W, Z, T)
+    scale = np.sqrt(sum(np.sum(np.abs(x[:nr]) ** 2) for x in out[:3]))
+    assert np.sqrt(sum(np.sum(np.abs(x[:nr]) ** 2) for x in out[3:6])) < 1e-14 * scale
 
 
 # -------------------------------------------------------------- boundary
