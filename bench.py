@@ -13,6 +13,13 @@ the same number of steps and its error against the FP64 output is reported
 per field and per degree band.  The headline value is the fastest form whose
 joint relative L2 error against FP64 is at most 1e-9.
 
+Also reported:
+  * absolute scalar transforms/s and the wall solves at the same size;
+  * the batching form: 48 uneven problems, persistent launches, hipGraph;
+  * the large coupled step at lmax = 511, nr = 256 (u x omega, u x B and
+    u . grad T) in all three forms, with per-field and per-band errors;
+  * the CPU oracle timed on the host cores.
+
 Prints one JSON line.  --gpus above 1 is refused: the path is single-GPU.
 """
 import argparse
@@ -232,6 +239,58 @@ def batch_bench(reps):
     }
 
 
+CP_LMAX, CP_NLAT, CP_NPHI, CP_NR = 511, 768, 1536, 256
+CP_NAMES = ("NQ", "NS", "NT", "MQ", "MS", "MT", "NH")
+
+
+def coupled_bench(reps):
+    """The large coupled state: lmax = 511, nlat = 768, nphi = 1536, nr = 256
+    (radial length 281), velocity W, Z ~ (l+1)^-2, magnetic Pb, Tb ~ (l+1)^-2.5,
+    temperature T ~ (l+1)^-2.  One step = u x omega, u x B and u . grad T.
+    The three Legendre forms are timed on the same inputs and their error
+    against the FP64 step reported per output field and degree band; the FP64
+    wall solves are timed at the same size."""
+    import torch
+    import redondito_amd.sphp as S
+    from tests import helpers as H
+    dev = torch.device("cuda:0")
+    n = CP_NR + C
+    dz = (RO - RI) / (CP_NR - 1)
+    plan = S.FCPlan(CP_LMAX, CP_NLAT, CP_NPHI, n, C, D, dz, RI, device=dev)
+    ins = [torch.from_numpy(x).to(dev) for x in H.coupled_fields(CP_LMAX, CP_NR, n, RI, RO, seed=SEED + 7)]
+    forms, host = {}, {}
+    for name, prec in FORMS:
+        out = [torch.zeros_like(ins[0]) for _ in range(7)]
+        ms = time_calls(lambda: S.nonlinear_coupled(plan, *ins, *out, prec=prec), reps)
+        host[name] = [o.cpu().numpy() for o in out]
+        del out
+        forms[name] = {"ms_per_step": round(ms, 3), "steps_per_s": round(1e3 / ms, 2)}
+    for name in forms:
+        rep = H.band_errors(host[name], host["fp64"], CP_LMAX, CP_NR, names=CP_NAMES)
+        grp = [H.rell2_joint([x[:CP_NR] for x in host[name][a:b]], [x[:CP_NR] for x in host["fp64"][a:b]])
+               for a, b in ((0, 3), (3, 6), (6, 7))]
+        forms[name]["rel_err_vs_fp64"] = {
+            "u_x_omega": float(f"{grp[0]:.3e}"), "u_x_B": float(f"{grp[1]:.3e}"), "u_grad_T": float(f"{grp[2]:.3e}"),
+            **{f: {b: float(f"{e:.3e}") for b, e in rep[f].items()} for f in CP_NAMES}}
+        forms[name]["speedup_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / forms[name]["ms_per_step"], 3)
+    host.clear()
+    bcp = S.BCPlan()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.setup_bc(bcp, plan, ("noslip", "noslip"), "v")
+    torch.cuda.synchronize()
+    setup = (time.perf_counter() - t0) * 1e3
+    tor, phi, pol = ins[1].clone(), ins[0].clone(), torch.zeros_like(ins[0])
+    inv = time_calls(lambda: S.invDl(plan, phi, pol), reps)
+    fast = time_calls(lambda: S.v_imposebc_fast(tor, phi, pol, plan), reps)
+    plan.close()
+    return {
+        "lmax": CP_LMAX, "nlat": CP_NLAT, "nphi": CP_NPHI, "nr": CP_NR, "radial_fft_len": n,
+        "step": "u x omega, u x B, u . grad T", "forms": forms,
+        "bc_setup_ms": round(setup, 2), "invDl_ms": round(inv, 4), "v_imposebc_fast_ms": round(fast, 4),
+    }
+
+
 def roofline(form, per_stage, nlat, nlm):
     """Dominant kernel of a form against its bound, algorithmic work per launch."""
     L = LMAX + 1
@@ -264,6 +323,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 form (for profiler runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
+    ap.add_argument("--no-coupled", action="store_true", help="skip the lmax = 511 coupled-step measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -315,6 +375,7 @@ def main():
               + max(leg, (3 * Fm + 3 * Sb) / bw))
     aux = aux_timings(S, plan, Wd)
     batch = None if (args.no_batch or args.fp64_only) else batch_bench(max(2, min(args.steps, 10)))
+    coupled = None if (args.no_coupled or args.fp64_only) else coupled_bench(max(2, min(args.steps, 5)))
 
     cpu = parity = None
     if not args.no_cpu:
@@ -343,6 +404,7 @@ def main():
         "forms": forms,
         "aux": aux,
         "batch": batch,
+        "coupled": coupled,
         "rel_l2_vs_oracle": parity,
         "cpu_baseline": cpu,
     }

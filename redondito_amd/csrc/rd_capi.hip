@@ -28,6 +28,98 @@ int upload_twiddles(Plan *p, int n, double2 **dst)
     return RD_OK;
 }
 
+bool is_prime(int n)
+{
+    if (n < 2) return false;
+    for (int f = 2; (long)f * f <= n; f++)
+        if (n % f == 0) return false;
+    return true;
+}
+
+int largest_prime_factor(int n)
+{
+    int lp = 1;
+    for (int f = 2; n > 1; f++)
+        while (n % f == 0) {
+            lp = f;
+            n /= f;
+        }
+    return lp;
+}
+
+// Smallest primitive root of a prime n: g^((n-1)/f) != 1 for every prime f | n-1.
+int primitive_root(int n)
+{
+    std::vector<int> fs;
+    for (int m = n - 1, f = 2; m > 1; f++)
+        if (m % f == 0) {
+            fs.push_back(f);
+            while (m % f == 0) m /= f;
+        }
+    auto powmod = [n](long b, long e) {
+        long r = 1;
+        b %= n;
+        for (; e; e >>= 1, b = b * b % n)
+            if (e & 1) r = r * b % n;
+        return r;
+    };
+    for (int g = 2; g < n; g++) {
+        bool ok = true;
+        for (int f : fs) ok = ok && powmod(g, (n - 1) / f) != 1;
+        if (ok) return g;
+    }
+    return 0;
+}
+
+// The radial FFT of length nz: direct Stockham stages, or the Rader form for
+// a prime nz > 19 whose nz - 1 is 13-smooth (the direct stage of a large
+// prime radix is O(n^2) per sequence).  The Rader spectra are computed in
+// long double by a direct DFT and rounded once.
+int setup_zfft(Plan *p)
+{
+    const int n = p->nz;
+    ZFft &Z = p->zfft;
+    Z = ZFft{};
+    Z.n = n;
+    Z.rader = is_prime(n) && n > 19 && largest_prime_factor(n - 1) <= 13;
+    if (!Z.rader) {
+        Z.seq = make_fft_seq(n);
+        return upload_twiddles(p, n, &p->tw_z);
+    }
+    const int n1 = n - 1, g = primitive_root(n);
+    std::vector<int> perm(2 * n1);
+    long gp = 1;
+    for (int q = 0; q < n1; q++, gp = gp * g % n) perm[q] = (int)gp;            // g^q
+    for (int q = 0; q < n1; q++) perm[n1 + q] = perm[(n1 - q) % n1];              // g^-q
+    std::vector<double2> cf(2 * n1);
+    const long double tp = 2.0L * 3.14159265358979323846264338327950288L;
+    for (int si = 0; si < 2; si++) {
+        const long double sg = si == 0 ? -1.0L : 1.0L;
+        for (int k = 0; k < n1; k++) {
+            long double re = 0.0L, im = 0.0L;
+            for (int m = 0; m < n1; m++) {
+                // c_m e^{-2 pi i m k / n1}, c_m = e^{s 2 pi i g^-m / n}
+                const long double a = sg * tp * (long double)perm[n1 + m] / n
+                                      - tp * (long double)(((long)m * k) % n1) / n1;
+                re += cosl(a);
+                im += sinl(a);
+            }
+            cf[si * n1 + k] = make_double2((double)(re / n1), (double)(im / n1));
+        }
+    }
+    RD_HIP(hipMalloc(&p->z_perm, sizeof(int) * 2 * n1));
+    RD_HIP(hipMalloc(&p->z_cf, sizeof(double2) * 2 * n1));
+    RD_HIP(hipMemcpyAsync(p->z_perm, perm.data(), sizeof(int) * 2 * n1, hipMemcpyHostToDevice, p->stream));
+    RD_HIP(hipMemcpyAsync(p->z_cf, cf.data(), sizeof(double2) * 2 * n1, hipMemcpyHostToDevice, p->stream));
+    RD_HIP(hipStreamSynchronize(p->stream));
+    Z.seq = make_fft_seq(n1);
+    Z.pin = p->z_perm;
+    Z.pout = p->z_perm + n1;
+    Z.cf[0] = p->z_cf;
+    Z.cf[1] = p->z_cf + n1;
+    return upload_twiddles(p, n1, &p->tw_z);
+}
+
 void free_plan(Plan *p)
 {
     if (!p) return;
@@ -35,7 +127,8 @@ void free_plan(Plan *p)
     free_precision_tables(p);
     void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
                    p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
-                   p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z};
+                   p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z,
+                   p->z_perm, p->z_cf};
     for (void *d : dev)
         if (d) (void)hipFree(d);
     for (int i = 0; i < 6; i++) {
@@ -163,7 +256,6 @@ static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, in
     p->dz = dz;
     p->rint = rint;
     p->seq_phi = make_fft_seq(nphi);
-    p->seq_z = make_fft_seq(nz);
 
     int rc = RD_OK;
 #define TRY(x)                  \
@@ -224,7 +316,7 @@ static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, in
         TRYH(hipMemcpyAsync(p->d_dir, dir, sizeof(double) * C * d, hipMemcpyHostToDevice, p->stream));
     }
     TRY(upload_twiddles(p, nphi, &p->tw_phi));
-    TRY(upload_twiddles(p, nz, &p->tw_z));
+    TRY(setup_zfft(p));
     TRY(build_legendre_tables(p));
 
     p->fsize = (long)nz * p->L * nlat;

@@ -811,6 +811,44 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
 }
 
 // ----------------------------------------------------------------- z
+// Radial FFT of nb sequences of length Z.n held at a + s n (b is scratch of
+// the same size, xs kZScalars double2 of LDS scratch); returns the buffer
+// holding the result, in the same layout.  The Rader form gathers
+// a_p = x[g^p] into packed sequences of length n - 1, convolves them with c by
+// two FFTs and the spectrum cf[s], and scatters X[g^-q] = x_0 + conv[q],
+// X[0] = x_0 + sum_p a_p (the first forward coefficient).
+template <int RB>
+__device__ double2 *z_fft(double2 *a, double2 *b, int nb, const ZFft &Z, int sign, const double2 *tw, int nh,
+                          double2 *xs)
+{
+    if (!Z.rader) return lds_fft<RB>(a, b, nb, Z.seq, sign, tw, nh);
+    const int n = Z.n, n1 = n - 1;
+    for (int e = threadIdx.x; e < nb * n1; e += blockDim.x) {
+        const int q = e / n1, p = e - q * n1;
+        b[e] = a[q * n + Z.pin[p]];
+    }
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) xs[q] = a[q * n];
+    __syncthreads();
+    double2 *F = lds_fft<RB>(b, a, nb, Z.seq, -1, tw, nh);
+    const double2 *cf = Z.cf[sign > 0 ? 1 : 0];
+    for (int e = threadIdx.x; e < nb * n1; e += blockDim.x) {
+        const int q = e / n1, k = e - q * n1;
+        const double2 v = F[e];
+        if (k == 0) xs[nb + q] = v;
+        F[e] = cmul(v, cf[k]);
+    }
+    __syncthreads();
+    double2 *G = lds_fft<RB>(F, F == a ? b : a, nb, Z.seq, +1, tw, nh);
+    double2 *X = G == a ? b : a;
+    for (int e = threadIdx.x; e < nb * n1; e += blockDim.x) {
+        const int q = e / n1, r = e - q * n1;
+        X[q * n + Z.pout[r]] = cadd(xs[q], G[e]);
+    }
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) X[q * n] = cadd(xs[q], xs[nb + q]);
+    __syncthreads();
+    return X;
+}
+
 constexpr int ZTC = 8;   // lm columns per workgroup
 
 struct ZArgs {
@@ -861,18 +899,18 @@ __device__ void z_store(const double2 *X, double2 *a, long ld, int c0, int ncols
 // mode 0: forward (continuation, FFT -1, /n); 1: inverse (FFT +1);
 // 2: derivative of order `order` in (lm,r), output to a from ain.
 template <int RB>
-__global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, FftSeq seq, const double2 *tw)
+__global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, ZFft Z, const double2 *tw)
 {
     extern __shared__ double2 sm[];
     const int n = A.n, c0 = blockIdx.x * ZTC;
-    double2 *X = sm, *Y = sm + ZTC * n, *twl = sm + 2 * ZTC * n;
-    const int nh = load_twiddles(twl, tw, n);
+    double2 *X = sm, *Y = sm + ZTC * n, *twl = sm + 2 * ZTC * n, *xs = sm + (2 * ZTC + 1) * n;
+    const int nh = load_twiddles(twl, tw, Z.seq.n);
     const double2 *src = (mode == 2) ? A.ain : A.a;
     const int nr = n - A.C;
     if (mode == 1) {
         z_load(X, src, A.ld, c0, A.ncols, n, n);
         __syncthreads();
-        double2 *R = lds_fft<RB>(X, Y, ZTC, seq, +1, twl, nh);
+        double2 *R = z_fft<RB>(X, Y, ZTC, Z, +1, twl, nh, xs);
         z_store(R, A.a, A.ld, c0, A.ncols, n, n, 1.0);
         return;
     }
@@ -882,7 +920,7 @@ __global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, FftSeq seq, c
         fc_continue(X, ZTC, n, A.C, A.d, A.dir);
         __syncthreads();
     }
-    double2 *R = lds_fft<RB>(X, Y, ZTC, seq, -1, twl, nh);
+    double2 *R = z_fft<RB>(X, Y, ZTC, Z, -1, twl, nh, xs);
     if (mode == 0) {
         z_store(R, A.a, A.ld, c0, A.ncols, n, n, 1.0 / n);
         return;
@@ -902,7 +940,7 @@ __global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, FftSeq seq, c
     }
     __syncthreads();
     double2 *O = (R == X) ? Y : X;
-    double2 *R2 = lds_fft<RB>(R, O, ZTC, seq, +1, twl, nh);
+    double2 *R2 = z_fft<RB>(R, O, ZTC, Z, +1, twl, nh, xs);
     z_store(R2, A.a, A.ld, c0, A.ncols, n, n, 1.0);
 }
 
@@ -919,16 +957,16 @@ static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order
     A.C = p->C;
     A.d = p->d;
     A.order = order;
-    size_t shm = sizeof(double2) * (2 * ZTC + 1) * p->nz;
+    size_t shm = sizeof(double2) * ((2 * ZTC + 1) * p->nz + kZScalars);
     const dim3 grid((ncols + ZTC - 1) / ZTC), block(512);
 #define RD_Z_CASE(RBV)                                                                         \
     case RBV: {                                                                                \
         int lrc = allow_lds(z_kernel<RBV>, shm);                                               \
         if (lrc) return lrc;                                                                   \
-        hipLaunchKernelGGL(z_kernel<RBV>, grid, block, shm, p->stream, A, mode, p->seq_z, p->tw_z); \
+        hipLaunchKernelGGL(z_kernel<RBV>, grid, block, shm, p->stream, A, mode, p->zfft, p->tw_z); \
         break;                                                                                 \
     }
-    switch (big_radix(p->seq_z)) {
+    switch (big_radix(p->zfft.seq)) {
         RD_Z_CASE(0)
         RD_Z_CASE(5)
         RD_Z_CASE(7)
@@ -985,13 +1023,13 @@ __device__ void z_load_cols(double2 *X, const double2 *a, long ld, int c0, int n
 // Both forward transforms (W and Z columns) run as one batch, the three
 // derivative spectra (i kz W^, -kz^2 W^, i kz Z^) as one inverse batch.
 template <int RB>
-__global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq seq, const double2 *tw)
+__global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, ZFft Z, const double2 *tw)
 {
     extern __shared__ double2 sm[];
     const int n = A.n, nr = n - A.C;
     const int c0 = xcd_tile(blockIdx.x, gridDim.x) * UTC;
-    double2 *P = sm, *Q = sm + 3 * UTC * n, *twl = sm + 6 * UTC * n;
-    const int nh = load_twiddles(twl, tw, n);
+    double2 *P = sm, *Q = sm + 3 * UTC * n, *twl = sm + 6 * UTC * n, *xs = sm + (6 * UTC + 1) * n;
+    const int nh = load_twiddles(twl, tw, Z.seq.n);
     const int rows = A.C > 0 ? nr : n;
     z_load_cols(P, A.W, A.ld, c0, A.ncols, n, rows, UTC);
     z_load_cols(P + UTC * n, A.Z, A.ld, c0, A.ncols, n, rows, UTC);
@@ -1000,7 +1038,7 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq se
         fc_continue(P, 2 * UTC, n, A.C, A.d, A.dir);
         __syncthreads();
     }
-    double2 *R = lds_fft<RB>(P, Q, 2 * UTC, seq, -1, twl, nh);
+    double2 *R = z_fft<RB>(P, Q, 2 * UTC, Z, -1, twl, nh, xs);
     double2 *I = (R == P) ? Q : P;
     // spectra / n times (i kz)^p: sequences [0,UTC) dW, [UTC,2UTC) d2W, [2UTC,3UTC) dZ
     const double inv_n = 1.0 / n;
@@ -1018,7 +1056,7 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, FftSeq se
         I[e] = v;
     }
     __syncthreads();
-    double2 *D = lds_fft<RB>(I, R, 3 * UTC, seq, +1, twl, nh);
+    double2 *D = z_fft<RB>(I, R, 3 * UTC, Z, +1, twl, nh, xs);
     // u:     Q = l(l+1) W / r^2, S = dW / r, T = Z / r
     // omega: Q = l(l+1) Z / r^2, S = dZ / r, T = -(d2W - l(l+1) W / r^2) / r
     for (int e = threadIdx.x; e < nr * UTC; e += blockDim.x) {
@@ -1341,16 +1379,16 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
 #undef RD_F_CASE
     default: break;
     }
-    size_t shm = sizeof(double2) * (6 * UTC + 1) * p->nz;
+    size_t shm = sizeof(double2) * ((6 * UTC + 1) * p->nz + kZScalars);
     const dim3 grid((p->nlm + UTC - 1) / UTC), block(256);
 #define RD_U_CASE(RBV)                                                                        \
     case RBV: {                                                                                 \
         int lrc = allow_lds(uxw_radial_kernel<RBV>, shm);                                       \
         if (lrc) return lrc;                                                                    \
-        hipLaunchKernelGGL(uxw_radial_kernel<RBV>, grid, block, shm, p->stream, A, p->seq_z, p->tw_z); \
+        hipLaunchKernelGGL(uxw_radial_kernel<RBV>, grid, block, shm, p->stream, A, p->zfft, p->tw_z); \
         break;                                                                                  \
     }
-    switch (big_radix(p->seq_z)) {
+    switch (big_radix(p->zfft.seq)) {
         RD_U_CASE(0)
         RD_U_CASE(5)
         RD_U_CASE(7)

@@ -20,6 +20,20 @@ struct FftSeq {
 
 FftSeq make_fft_seq(int n);
 
+// The radial FFT of length n as the kernels run it.  A prime n whose n - 1
+// has no prime factor above 13 runs in Rader form: X[0] = sum_j x_j and
+// X[g^-q] = x_0 + sum_p x[g^p] c[q - p] (g a primitive root mod n, c_m =
+// e^{s 2 pi i g^-m / n}), the cyclic convolution by two FFTs of length n - 1
+// (seq) and the precomputed spectrum cf[s] of c divided by n - 1.  Otherwise
+// seq is of length n itself.
+struct ZFft {
+    FftSeq seq;
+    int n, rader;
+    const int *pin, *pout;          // g^p mod n, g^-q mod n
+    const double2 *cf[2];           // s = -1, s = +1
+};
+constexpr int kZScalars = 32;       // LDS scratch (double2) the Rader form needs
+
 // Workgroup index remapped so that consecutive logical tiles run on the same
 // XCD (the hardware deals workgroups round-robin over the 8 XCDs, each with
 // its own L2): tiles that share operands then share one L2.  Only for grids
@@ -61,9 +75,12 @@ struct Plan {
     // empty): synthesis entries m | tt<<16, analysis entries m | lt<<16.
     int *d_syn_q, *d_ana_q;
     int syn_qlen, ana_qlen;
-    // FFTs
-    FftSeq seq_phi, seq_z;
+    // FFTs; tw_z is of the length of zfft.seq (n - 1 in Rader form)
+    FftSeq seq_phi;
+    ZFft zfft;
     double2 *tw_phi, *tw_z;        // exp(-2 pi i t / n)
+    int *z_perm;                   // Rader index tables (2 (n - 1) ints)
+    double2 *z_cf;                 // Rader spectra (2 (n - 1))
     // no-slip wall solves, built by rd_bc_setup (column major, N = nr):
     // d1, d2 N x N; binv N x (N-4) per l = 1..lmax; d2bor 4 x N
     int bc_ready;

@@ -19,6 +19,7 @@
       CALL casos_vec()
       CALL caso_z()
       CALL caso_uxw()
+      CALL caso_coupled()
       CALL caso_boundary()
       IF (nfail .gt. 0) THEN
          PRINT '(A,I0,A)', 'prueba_rd: ', nfail, ' failed'
@@ -270,6 +271,80 @@
       CALL gdev_free(C_LOC(nq)); CALL gdev_free(C_LOC(ns)); CALL gdev_free(C_LOC(nt))
       CALL fcgram_destroy_plan(planfc)
       END SUBROUTINE caso_uxw
+
+! The coupled step for a solid-body rotation in a uniform field B0 z:
+! u x B = grad(Omega B0 r^2 sin^2 / 2) (MQ_00, MQ_20, MS_20) and
+! u . grad T = Omega dT/dphi (NH_lm = i m Omega T_lm) for a deterministic T.
+      SUBROUTINE caso_coupled()
+      INTEGER, PARAMETER :: lmax = 15, nr = 64, C = 25, od = 5, nz = nr + C
+      INTEGER, PARAMETER :: nlat = 24, nphi = 48, nlm = (lmax+1)*(lmax+2)/2
+      DOUBLE PRECISION, PARAMETER :: ri = 0.35D0, ro = 1.0D0, om = 0.7D0, b0 = 1.3D0
+      DOUBLE COMPLEX, TARGET :: w(nlm,nz), z(nlm,nz), pb(nlm,nz), tb(nlm,nz), t(nlm,nz)
+      DOUBLE COMPLEX, TARGET :: nq(nlm,nz), ns(nlm,nz), nt(nlm,nz), mq(nlm,nz), ms(nlm,nz)
+      DOUBLE COMPLEX, TARGET :: mt(nlm,nz), nh(nlm,nz)
+      DOUBLE PRECISION :: dz, r, eq, es, et, eh, sc, sh
+      INTEGER :: k, l, m
+      CHARACTER(len=256) :: tdir
+      CALL GET_ENVIRONMENT_VARIABLE('RD_TABLE_DIR',tdir)
+      dz = (ro-ri)/(nr-1)
+      CALL fcgram_create_plan(planfc,(/nlat,nphi,nz/),(/0,0,C/),(/0,0,od/), &
+           (/1.0D0,1.0D0,dz/),TRIM(tdir),0,lmax,ri)
+      w = 0; z = 0; pb = 0; tb = 0; t = 0
+      DO k = 1,nr
+         r = ri+(k-1)*dz
+         z(lm(lmax,1,0),k) = om*r**2*SQRT(4*pi/3)
+         pb(lm(lmax,1,0),k) = 0.5D0*b0*r**2*SQRT(4*pi/3)
+         DO m = 0,lmax
+            DO l = m,lmax
+               t(lm(lmax,l,m),k) = SIN(pi*(r-ri)/(ro-ri))/(l+1.0D0)**1.5D0 * &
+                    DCMPLX(COS(1.0D0*(l+m)),MERGE(0.0D0,SIN(1.0D0*l*m),m.eq.0))
+            END DO
+         END DO
+      END DO
+      CALL dev_all(w); CALL dev_all(z); CALL dev_all(pb); CALL dev_all(tb); CALL dev_all(t)
+      CALL dev_all(nq); CALL dev_all(ns); CALL dev_all(nt); CALL dev_all(mq); CALL dev_all(ms)
+      CALL dev_all(mt); CALL dev_all(nh)
+      CALL gdev_update_to(C_LOC(w),INT(16*SIZE(w),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(z),INT(16*SIZE(z),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(pb),INT(16*SIZE(pb),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(tb),INT(16*SIZE(tb),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(t),INT(16*SIZE(t),C_SIZE_T))
+      CALL nonlinear_coupled(w,z,pb,tb,t,nq,ns,nt,mq,ms,mt,nh,0)
+      CALL gdev_update_from(C_LOC(mq),INT(16*SIZE(mq),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(ms),INT(16*SIZE(ms),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(mt),INT(16*SIZE(mt),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(nh),INT(16*SIZE(nh),C_SIZE_T))
+      eq = 0; es = 0; et = 0; eh = 0; sc = 0; sh = 0
+      DO k = 1,nr
+         r = ri+(k-1)*dz
+         sc = MAX(sc,(2.0D0/3)*om*b0*r*SQRT(4*pi))
+         eq = MAX(eq,ABS(mq(lm(lmax,0,0),k)-(2.0D0/3)*om*b0*r*SQRT(4*pi)), &
+                  ABS(mq(lm(lmax,2,0),k)+(2.0D0/3)*om*b0*r*SQRT(4*pi/5)))
+         es = MAX(es,ABS(ms(lm(lmax,2,0),k)+(1.0D0/3)*om*b0*r*SQRT(4*pi/5)))
+         et = MAX(et,MAXVAL(ABS(mt(:,k))))
+         DO m = 0,lmax
+            DO l = m,lmax
+               sh = MAX(sh,ABS(DCMPLX(0.0D0,m*om)*t(lm(lmax,l,m),k)))
+               eh = MAX(eh,ABS(nh(lm(lmax,l,m),k)-DCMPLX(0.0D0,m*om)*t(lm(lmax,l,m),k)))
+            END DO
+         END DO
+      END DO
+      CALL report('u x B solid body in B0 z MQ (rel)',eq/sc,1.0D-9)
+      CALL report('u x B solid body in B0 z MS (rel)',es/sc,1.0D-12)
+      CALL report('u x B solid body in B0 z MT (abs)',et,1.0D-12)
+      CALL report('u . grad T = Omega dT/dphi (rel)',eh/sh,1.0D-12)
+      CALL gdev_free(C_LOC(w)); CALL gdev_free(C_LOC(z)); CALL gdev_free(C_LOC(pb))
+      CALL gdev_free(C_LOC(tb)); CALL gdev_free(C_LOC(t)); CALL gdev_free(C_LOC(nq))
+      CALL gdev_free(C_LOC(ns)); CALL gdev_free(C_LOC(nt)); CALL gdev_free(C_LOC(mq))
+      CALL gdev_free(C_LOC(ms)); CALL gdev_free(C_LOC(mt)); CALL gdev_free(C_LOC(nh))
+      CALL fcgram_destroy_plan(planfc)
+      END SUBROUTINE caso_coupled
+
+! Device copy of a spectral host array (registered by its address).
+      SUBROUTINE dev_all(a)
+      DOUBLE COMPLEX, TARGET :: a(:,:)
+      CALL gdev_alloc(C_LOC(a),INT(16*SIZE(a),C_SIZE_T))
+      END SUBROUTINE dev_all
 
 ! prueba_boundary (src/prueba_boundary.f90:58-128): lmax = 7, nz = 128,
 ! rint = 0.5, rext = 1.5, through the boundary module, at the levels

@@ -145,6 +145,12 @@
           TYPE(C_PTR), VALUE :: plan, w, z, nq, ns, nt
           INTEGER(C_INT), VALUE :: prec
         END FUNCTION
+        INTEGER(C_INT) FUNCTION rd_nonlinear_coupled(plan,prec,w,z,pb,tb,t,nq,ns,nt,mq,ms,mt,nh) &
+                BIND(C,NAME='rd_nonlinear_coupled')
+          IMPORT :: C_INT, C_PTR
+          TYPE(C_PTR), VALUE :: plan, w, z, pb, tb, t, nq, ns, nt, mq, ms, mt, nh
+          INTEGER(C_INT), VALUE :: prec
+        END FUNCTION
         INTEGER(C_INT) FUNCTION rd_derivr_mat(plan,p,d) BIND(C,NAME='rd_derivr_mat')
           IMPORT :: C_INT, C_PTR
           TYPE(C_PTR), VALUE :: plan, d

@@ -226,3 +226,20 @@
            rd_dptr(C_LOC(nt),'nt')),'nonlinear_uxw')
       CALL fc_sync(planfc)
       END SUBROUTINE nonlinear_uxw
+
+! One nonlinear step of the coupled state (no reference routine): velocity
+! W, Z, magnetic field Pb, Tb (poloidal, toroidal) and temperature T in
+! (lm,r) -> u x omega (NQ,NS,NT), u x B (MQ,MS,MT), u . grad T (NH).
+      SUBROUTINE nonlinear_coupled(w,z,pb,tb,t,nq,ns,nt,mq,ms,mt,nh,prec)
+      USE fft
+      IMPLICIT NONE
+      DOUBLE COMPLEX, INTENT(IN), TARGET :: w(*),z(*),pb(*),tb(*),t(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET :: nq(*),ns(*),nt(*),mq(*),ms(*),mt(*),nh(*)
+      INTEGER, INTENT(IN) :: prec
+      CALL rd_check(rd_nonlinear_coupled(planfc%h,INT(prec,C_INT),rd_dptr(C_LOC(w),'w'), &
+           rd_dptr(C_LOC(z),'z'),rd_dptr(C_LOC(pb),'pb'),rd_dptr(C_LOC(tb),'tb'), &
+           rd_dptr(C_LOC(t),'t'),rd_dptr(C_LOC(nq),'nq'),rd_dptr(C_LOC(ns),'ns'), &
+           rd_dptr(C_LOC(nt),'nt'),rd_dptr(C_LOC(mq),'mq'),rd_dptr(C_LOC(ms),'ms'), &
+           rd_dptr(C_LOC(mt),'mt'),rd_dptr(C_LOC(nh),'nh')),'nonlinear_coupled')
+      CALL fc_sync(planfc)
+      END SUBROUTINE nonlinear_coupled

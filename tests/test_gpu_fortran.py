@@ -19,5 +19,5 @@ def test_prueba_rd_known_answers():
     print(r.stdout)
     print(r.stderr[-2000:])
     checks = [l for l in r.stdout.splitlines() if l.startswith("CHECK")]
-    assert len(checks) >= 23
+    assert len(checks) >= 28
     assert r.returncode == 0 and all(l.rstrip().endswith("PASS") for l in checks)

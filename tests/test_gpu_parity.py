@@ -170,7 +170,10 @@ def test_prueba_z_periodic():
     assert e1 < 1e-15 and e2 < 1e-14
 
 
-@pytest.mark.parametrize("nr,lmax", [(24, 31), (39, 7), (96, 15), (128, 15), (231, 7)])
+# radial lengths 49, 64, 121, 153, 256; the primes 89 and 281 (Rader form:
+# 88 = 8 x 11, 280 = 8 x 5 x 7) and 47 (46 = 2 x 23, direct prime stage)
+@pytest.mark.parametrize("nr,lmax", [(24, 31), (39, 7), (96, 15), (128, 15), (231, 7), (64, 15), (256, 7),
+                                     (22, 15)])
 def test_z_fcgram_vs_oracle(nr, lmax):
     S = sphp()
     C, d = 25, 5
@@ -188,8 +191,12 @@ def test_z_fcgram_vs_oracle(nr, lmax):
     ref2 = fc.inverse(ref.copy())
     S.fftp1d_complex_to_real_z(plan, x)
     e2 = H.rell2(to_host(x), ref2)
-    print(f"n={n}: forward {e1:.2e} inverse {e2:.2e}")
-    assert e1 < TOL and e2 < TOL
+    d1 = torch.zeros_like(x)
+    S.derivr(plan, to_dev(a), d1, 1, "r")
+    kz = O.kz(n, dz)
+    e3 = H.rell2(to_host(d1)[:nr], fc.derivr_r(a.copy(), kz, 1)[:nr])
+    print(f"n={n}: forward {e1:.2e} inverse {e2:.2e} derivr {e3:.2e}")
+    assert e1 < TOL and e2 < TOL and e3 < TOL
 
 
 @pytest.mark.parametrize("nz", [64, 128, 256])
