@@ -314,9 +314,13 @@ struct SynArgs {
 // One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
 // T = double: the FP64 path; T = float: the FP32 precision form (tables and
 // operands rounded to FP32, FP32 accumulation, FP64 output).
+// The spectral operand is staged through two LDS buffers: while chunk c is
+// multiplied out of one, chunk c+1 is loaded in two halves and written to the
+// other (no wave reads it until the barrier that ends chunk c), so each chunk
+// costs one barrier and its global-load latency overlaps the MFMA work.
 template <typename T, bool VEC, int JBV>
 __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int jt,
-                                         T (&Bs)[SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
+                                         T (&Bs)[2][SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
 {
     using Cfg = SynCfg<VEC, JBV>;
     using V = typename Mf<T>::V;
@@ -340,22 +344,49 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
     const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
     const long lm0 = lm_index(A.lmax, m, m);
 
-    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
-        // stage the spectral chunk: NF x (JT/2 planes) x SKC l's
-        constexpr int PER = (JT / 2) * SKC;
-        for (int e = tid; e < NF * PER; e += 256) {
-            int f = e / PER, r = e % PER;
-            int pl = r / SKC, ll = r % SKC;
-            int l = lbase + ll, k = k0 + pl;
-            double2 v = make_double2(0.0, 0.0);
-            if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-            Bs[f][ll][2 * pl] = (T)v.x;
-            Bs[f][ll][2 * pl + 1] = (T)v.y;
+    // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads;
+    // each thread moves NLD double2 of it, in two halves of NH
+    constexpr int PER = (JT / 2) * SKC;
+    constexpr int NLD = NF * PER / 256, NH = NLD / 2;
+    static_assert(NF * PER == 256 * NLD && NLD % 2 == 0, "chunk must split evenly over the workgroup");
+    double2 r[NH];
+    auto load_half = [&](int lbase, int h) {
+#pragma unroll
+        for (int i = 0; i < NH; i++) {
+            const int e = tid + 256 * (h * NH + i);
+            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+            const int l = lbase + ll, k = k0 + pl;
+            r[i] = make_double2(0.0, 0.0);
+            if (l <= A.lmax && k < A.nplanes) r[i] = A.in[f][lm0 + (l - m) + k * A.ld];
         }
-        __syncthreads();
+    };
+    auto store_half = [&](int buf, int h) {
+#pragma unroll
+        for (int i = 0; i < NH; i++) {
+            const int e = tid + 256 * (h * NH + i);
+            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+            Bs[buf][f][ll][2 * pl] = (T)r[i].x;
+            Bs[buf][f][ll][2 * pl + 1] = (T)r[i].y;
+        }
+    };
+    load_half(m, 0);
+    store_half(0, 0);
+    load_half(m, 1);
+    store_half(0, 1);
+    __syncthreads();
+
+    int buf = 0;
+    for (int lbase = m; lbase <= A.lmax; lbase += SKC, buf ^= 1) {
+        const bool more = lbase + SKC <= A.lmax;
+        if (more) load_half(lbase + SKC, 0);
         const int rowbase = (lbase - m) >> 1;
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++) {
+            if (st == 1 && more) {
+                store_half(buf ^ 1, 0);
+                load_half(lbase + SKC, 1);
+            }
+            if (st == 3 && more) store_half(buf ^ 1, 1);
             const int row = rowbase + 4 * st + kk;
             const long ie = offe + (long)row * A.nthp + th0 + col;
             const long io = offo + (long)row * A.nthp + th0 + col;
@@ -371,13 +402,13 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = b * 16 + col;
-                acc[0][0][b] = Mf<T>::mma(Pe, Bs[0][lle][j], acc[0][0][b]);
-                acc[0][1][b] = Mf<T>::mma(Po, Bs[0][llo][j], acc[0][1][b]);
+                acc[0][0][b] = Mf<T>::mma(Pe, Bs[buf][0][lle][j], acc[0][0][b]);
+                acc[0][1][b] = Mf<T>::mma(Po, Bs[buf][0][llo][j], acc[0][1][b]);
                 if (VEC) {
-                    const T Se = Bs[1][lle][j], So = Bs[1][llo][j];
-                    const T Te = Bs[2][lle][j], To = Bs[2][llo][j];
-                    const T iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
-                    const T iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
+                    const T Se = Bs[buf][1][lle][j], So = Bs[buf][1][llo][j];
+                    const T Te = Bs[buf][2][lle][j], To = Bs[buf][2][llo][j];
+                    const T iSe = sgn_i * Bs[buf][1][lle][j ^ 1], iSo = sgn_i * Bs[buf][1][llo][j ^ 1];
+                    const T iTe = sgn_i * Bs[buf][2][lle][j ^ 1], iTo = sgn_i * Bs[buf][2][llo][j ^ 1];
                     // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
                     acc[1][0][b] = Mf<T>::mma(Do, So, acc[1][0][b]);
                     acc[1][0][b] = Mf<T>::mma(Me, iTe, acc[1][0][b]);
@@ -417,10 +448,10 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
 }
 
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void synthesis_kernel(SynArgs<T> A)
+__global__ __launch_bounds__(256, 2) void synthesis_kernel(SynArgs<T> A)
 {
     using Cfg = SynCfg<VEC>;
-    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
+    __shared__ T Bs[2][Cfg::NF][SKC][Cfg::JTP];
     // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
     // back to back on one XCD, so its table rows are read from HBM once
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
@@ -696,7 +727,7 @@ __global__ __launch_bounds__(256) void batch_synthesis_kernel(const SynArgs<T> *
                                                               int nitems, int *counter)
 {
     using Cfg = SynCfg<false, JBV>;
-    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
+    __shared__ T Bs[2][Cfg::NF][SKC][Cfg::JTP];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
