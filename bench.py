@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 form (for profiler runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
     ap.add_argument("--no-coupled", action="store_true", help="skip the lmax = 511 coupled-step measurement")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="only run the lmax = 511 coupled-step measurement (for profiler runs) and print it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -332,6 +334,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.coupled_only:
+        print(json.dumps({"coupled": coupled_bench(max(2, args.steps))}))
         return
 
     import torch

@@ -195,6 +195,15 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int nb, const FftSeq &seq, i
     return a;
 }
 
+// The register-DFT kernel variant for the radial FFT: in Rader form the
+// inner length is short and only radices 5 and 7 pay for their registers.
+int big_radix(const FftSeq &seq);
+static int z_radix(const ZFft &Z)
+{
+    const int rb = big_radix(Z.seq);
+    return (Z.rader && rb > 7) ? 0 : rb;
+}
+
 // The register-DFT radix of a length: its largest prime factor if that is
 // one of 5, 7, 11, 13, else 0 (the generic stage is faster for 17 and 19,
 // whose register DFTs need too many VGPRs).
@@ -966,7 +975,7 @@ static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order
         hipLaunchKernelGGL(z_kernel<RBV>, grid, block, shm, p->stream, A, mode, p->zfft, p->tw_z); \
         break;                                                                                 \
     }
-    switch (big_radix(p->zfft.seq)) {
+    switch (z_radix(p->zfft)) {
         RD_Z_CASE(0)
         RD_Z_CASE(5)
         RD_Z_CASE(7)
@@ -1388,7 +1397,7 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
         hipLaunchKernelGGL(uxw_radial_kernel<RBV>, grid, block, shm, p->stream, A, p->zfft, p->tw_z); \
         break;                                                                                  \
     }
-    switch (big_radix(p->zfft.seq)) {
+    switch (z_radix(p->zfft)) {
         RD_U_CASE(0)
         RD_U_CASE(5)
         RD_U_CASE(7)

@@ -314,13 +314,22 @@ struct SynArgs {
 // One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
 // T = double: the FP64 path; T = float: the FP32 precision form (tables and
 // operands rounded to FP32, FP32 accumulation, FP64 output).
-// The spectral operand is staged through two LDS buffers: while chunk c is
-// multiplied out of one, chunk c+1 is loaded in two halves and written to the
-// other (no wave reads it until the barrier that ends chunk c), so each chunk
-// costs one barrier and its global-load latency overlaps the MFMA work.
+// FP64 vector tiles stage the spectral operand through two LDS buffers:
+// while chunk c is multiplied out of one, chunk c+1 is loaded in two halves
+// and written to the other (no wave reads it until the barrier that ends
+// chunk c), so each chunk costs one barrier and its global-load latency
+// overlaps the MFMA work.  The other forms keep one buffer: their smaller
+// register footprint buys more waves per SIMD, which hides the latency better.
+template <typename T, bool VEC>
+constexpr int syn_nbuf()
+{
+    return (sizeof(T) == 8 && VEC) ? 2 : 1;
+}
+
 template <typename T, bool VEC, int JBV>
-__device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int jt,
-                                         T (&Bs)[2][SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
+__device__ __forceinline__ void syn_tile(
+    const SynArgs<T> &A, int m, int tt, int jt,
+    T (&Bs)[syn_nbuf<T, VEC>()][SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
 {
     using Cfg = SynCfg<VEC, JBV>;
     using V = typename Mf<T>::V;
@@ -369,24 +378,39 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
             Bs[buf][f][ll][2 * pl + 1] = (T)r[i].y;
         }
     };
-    load_half(m, 0);
-    store_half(0, 0);
-    load_half(m, 1);
-    store_half(0, 1);
-    __syncthreads();
+    constexpr bool DB = syn_nbuf<T, VEC>() == 2;
+    if (DB) {
+        load_half(m, 0);
+        store_half(0, 0);
+        load_half(m, 1);
+        store_half(0, 1);
+        __syncthreads();
+    }
 
     int buf = 0;
-    for (int lbase = m; lbase <= A.lmax; lbase += SKC, buf ^= 1) {
+    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
         const bool more = lbase + SKC <= A.lmax;
-        if (more) load_half(lbase + SKC, 0);
+        if (DB) {
+            if (more) load_half(lbase + SKC, 0);
+        } else {
+            for (int e = tid; e < NF * PER; e += 256) {
+                const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+                const int l = lbase + ll, k = k0 + pl;
+                double2 v = make_double2(0.0, 0.0);
+                if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
+                Bs[0][f][ll][2 * pl] = (T)v.x;
+                Bs[0][f][ll][2 * pl + 1] = (T)v.y;
+            }
+            __syncthreads();
+        }
         const int rowbase = (lbase - m) >> 1;
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++) {
-            if (st == 1 && more) {
+            if (DB && st == 1 && more) {
                 store_half(buf ^ 1, 0);
                 load_half(lbase + SKC, 1);
             }
-            if (st == 3 && more) store_half(buf ^ 1, 1);
+            if (DB && st == 3 && more) store_half(buf ^ 1, 1);
             const int row = rowbase + 4 * st + kk;
             const long ie = offe + (long)row * A.nthp + th0 + col;
             const long io = offo + (long)row * A.nthp + th0 + col;
@@ -423,6 +447,7 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
             }
         }
         __syncthreads();
+        if (DB) buf ^= 1;
     }
 
     // north = sym + anti, south = sym - anti
@@ -448,10 +473,10 @@ __device__ __forceinline__ void syn_tile(const SynArgs<T> &A, int m, int tt, int
 }
 
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256, 2) void synthesis_kernel(SynArgs<T> A)
+__global__ __launch_bounds__(256, (syn_nbuf<T, VEC>() == 2 ? 2 : 4)) void synthesis_kernel(SynArgs<T> A)
 {
     using Cfg = SynCfg<VEC>;
-    __shared__ T Bs[2][Cfg::NF][SKC][Cfg::JTP];
+    __shared__ T Bs[syn_nbuf<T, VEC>()][Cfg::NF][SKC][Cfg::JTP];
     // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
     // back to back on one XCD, so its table rows are read from HBM once
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
@@ -727,7 +752,7 @@ __global__ __launch_bounds__(256) void batch_synthesis_kernel(const SynArgs<T> *
                                                               int nitems, int *counter)
 {
     using Cfg = SynCfg<false, JBV>;
-    __shared__ T Bs[2][Cfg::NF][SKC][Cfg::JTP];
+    __shared__ T Bs[syn_nbuf<T, false>()][Cfg::NF][SKC][Cfg::JTP];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
