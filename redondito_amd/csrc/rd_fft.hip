@@ -758,7 +758,11 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     __syncthreads();
     // ---- F2: Yf[3s+n2][k1 + 16 k2]
     {
-        const int s = tid / 48, rem = tid % 48, n2 = rem >> 4, k1 = rem & 15;
+        // indices recomputed from an opaque copy of tid: shared with F1 they
+        // stay live across the barrier and cost a spill
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        const int s = t2 / 48, rem = t2 % 48, n2 = rem >> 4, k1 = rem & 15;
         double2 v[16];
         if (tid < 96) {
 #pragma unroll
