@@ -7,8 +7,9 @@ are seeded complex normal with spectrum (l+1)^-2 and radial profile
 sin(pi (r - ri)/(ro - ri)).  One step = one full evaluation
 (W, Z) -> (NQ, NS, NT) with all inputs resident in HBM.
 
-Three forms run on the same inputs: FP64, and the precision forms with the
-Legendre contractions on FP32 MFMA and as BF16x3 splits.  Each is timed over
+Four forms run on the same inputs: FP64, and the precision forms with the
+Legendre contractions on FP32 MFMA, as BF16x3 splits, and as int8 digit slices
+of the FP64 operands with exact int32 accumulation (the Ozaki form).  Each is timed over
 the same number of steps and its error against the FP64 output is reported
 per field and per degree band.  The headline value is the fastest form whose
 joint relative L2 error against FP64 is at most 1e-9.
@@ -37,8 +38,10 @@ LMAX, NR, C, D = 255, 128, 25, 5
 RI, RO = 0.35, 1.0
 SEED = 20261203
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
-FORMS = (("fp64", 0), ("fp32", 1), ("bf16x3", 2))
+# dense MFMA peaks; the int8 digit form spends 28 int8 products per FP64 product,
+# so its FP64-equivalent peak is the 5.0 POP/s int8 peak over 28
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "ozaki": 5000.0 / 28}
+FORMS = (("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("ozaki", 3))
 ERR_BAR = 1e-9
 
 
@@ -400,7 +403,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 elsewhere",
-                  "bf16x3": "bf16x3 Legendre, f64 elsewhere"}[head],
+                  "bf16x3": "bf16x3 Legendre, f64 elsewhere",
+                  "ozaki": "int8 digit-sliced f64 Legendre (exact int32 accumulation), f64 elsewhere"}[head],
         "data": "synthetic seeded W,Z: complex normal, (l+1)^-2, sin radial profile",
         "config": {"workload": "u x omega nonlinear term", "form": head, "lmax": LMAX, "nlat": nlat,
                    "nphi": nphi, "nr": NR, "fcgram_C": C, "fcgram_d": D, "radial_fft_len": n},

@@ -35,6 +35,8 @@ extern "C" {
 #define RD_PREC_FP64 0      /* plain FP64 path (MFMA F64 Legendre)          */
 #define RD_PREC_FP32 1      /* Legendre contractions on FP32 MFMA           */
 #define RD_PREC_BF16X3 2    /* BF16 3-term split, FP32 accumulation         */
+#define RD_PREC_OZAKI 3     /* FP64 operands as int8 digit slices, exact    */
+                            /* int32 accumulation, FP64 recombination       */
 
 typedef struct rd_plan rd_plan;
 

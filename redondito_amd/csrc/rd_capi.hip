@@ -125,6 +125,7 @@ void free_plan(Plan *p)
     if (!p) return;
     free_bc(p);
     free_precision_tables(p);
+    free_oz_tables(p);
     void *dev[] = {p->d_ct, p->d_st, p->d_wt, p->d_z, p->d_kz, p->d_dir, p->d_ell,
                    p->syn_P, p->syn_D, p->syn_M, p->ana_P, p->ana_D, p->ana_M, p->d_syn_off,
                    p->d_ana_off, p->d_ana_ncol, p->d_syn_q, p->d_ana_q, p->tw_phi, p->tw_z,
@@ -642,7 +643,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
                      double *NT)
 {
     if (bad_plan(plan) || !W || !Z || !NQ || !NS || !NT) return RD_ERR_ARG;
-    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32 && prec != RD_PREC_BF16X3) return RD_ERR_ARG;
+    if (prec < RD_PREC_FP64 || prec > RD_PREC_OZAKI) return RD_ERR_ARG;
     Plan *p = P(plan);
     if (p->nwork < 6) return RD_ERR_STATE;
     const int nr = p->nr;
@@ -697,7 +698,7 @@ int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double 
 {
     if (bad_plan(plan) || !W || !Z || !Pb || !Tb || !T || !NQ || !NS || !NT || !MQ || !MS || !MT || !NH)
         return RD_ERR_ARG;
-    if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32 && prec != RD_PREC_BF16X3) return RD_ERR_ARG;
+    if (prec < RD_PREC_FP64 || prec > RD_PREC_OZAKI) return RD_ERR_ARG;
     Plan *p = P(plan);
     if (p->nwork < 6 || p->nr < 1) return RD_ERR_STATE;
     const int nr = p->nr;

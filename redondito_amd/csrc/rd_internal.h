@@ -68,6 +68,7 @@ struct Plan {
     // (synthesis [m,p][row/4][theta][4], analysis [m,p][theta/4][col][4])
     float *syn32_P, *syn32_D, *syn32_M, *ana32_P, *ana32_D, *ana32_M;
     uint16_t *synb_hi[3], *synb_lo[3], *anab_hi[3], *anab_lo[3];
+    void *oz;                      // int8 digit tables and operand scratch of the Ozaki form
     long *d_syn_off, *d_ana_off;   // per (m,p)
     int *d_ana_ncol;               // per m
     long syn_rows, ana_elems;
@@ -109,7 +110,7 @@ enum UxwStage { ST_RADIAL = 0, ST_SYN_U, ST_SYN_W, ST_PHI_CROSS, ST_ANALYSIS, ST
 
 // ---------------------------------------------------------------- kernels
 // Legendre contractions over all planes (Fourier layout F[k][m][theta]);
-// prec is RD_PREC_FP64, RD_PREC_FP32 or RD_PREC_BF16X3.
+// prec is RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3 or RD_PREC_OZAKI.
 int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q, const double2 *s,
                      const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2);
 int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0, const double2 *g1,
@@ -119,6 +120,12 @@ int ensure_precision_tables(Plan *p, int prec);
 void free_precision_tables(Plan *p);
 int launch_synthesis_bf3(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
                          long ld, double2 *f0, double2 *f1, double2 *f2);
+int ensure_oz_tables(Plan *p);
+void free_oz_tables(Plan *p);
+int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
+                        long ld, double2 *f0, double2 *f1, double2 *f2);
+int launch_analysis_oz(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1, const double2 *g2,
+                       double2 *q, double2 *s, double2 *t, long ld);
 int launch_analysis_bf3(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1,
                         const double2 *g2, double2 *q, double2 *s, double2 *t, long ld);
 

@@ -526,6 +526,7 @@ int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q,
     if (prec == RD_PREC_FP64)
         return launch_synthesis_t<double>(p, p->syn_P, p->syn_D, p->syn_M, vec, nplanes, q, s, t, ld, f0, f1,
                                           f2);
+    if (prec == RD_PREC_OZAKI) return launch_synthesis_oz(p, vec, nplanes, q, s, t, ld, f0, f1, f2);
     int rc = ensure_precision_tables(p, prec);
     if (rc) return rc;
     if (prec == RD_PREC_FP32)
@@ -729,6 +730,7 @@ int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0,
     if (prec == RD_PREC_FP64)
         return launch_analysis_t<double>(p, p->ana_P, p->ana_D, p->ana_M, vec, nplanes, g0, g1, g2, q, s, t,
                                          ld);
+    if (prec == RD_PREC_OZAKI) return launch_analysis_oz(p, vec, nplanes, g0, g1, g2, q, s, t, ld);
     int rc = ensure_precision_tables(p, prec);
     if (rc) return rc;
     if (prec == RD_PREC_FP32)

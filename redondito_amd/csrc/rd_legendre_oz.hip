@@ -1,0 +1,767 @@
+// Exact-accumulation form of the Legendre contractions (RD_PREC_OZAKI):
+// Ozaki-style splitting of FP64 operands into int8 digits, products on the
+// int8 MFMA with exact int32 accumulation, recombination in FP64.
+//
+// Every contraction of one order m is written as real GEMMs C = A B:
+//   synthesis  (rows theta, K over l, columns = plane x {re, im})
+//     f0: sym  Pe Q_e                 anti Po Q_o
+//     f1: sym [Do|Me][S_o; iT_e]      anti [De|Mo][S_e; iT_o]
+//    -f2: sym [Do|Me][T_o; -iS_e]     anti [De|Mo][T_e; -iS_o]
+//   north = sym + anti, south = sym - anti (the FP64 kernels' symmetric and
+//   antisymmetric accumulators);
+//   analysis   (rows l, K over the northern thetas, columns as above)
+//     Q_e = Pe r+, Q_o = Po r-, S_e = [De|Me][t-; -ip+], S_o = [Do|Mo][t+; -ip-],
+//     T_e = [De|Me][-p-; -it+], T_o = [Do|Mo][-p+; -it-]
+//   with x+- = x_north +- x_south.
+// So each direction has four distinct A matrices per m (tables, sliced once
+// per plan) and six B matrices (sliced per call by a memory-bound kernel).
+//
+// Splitting: a row of A (or a column of B) with largest magnitude M gets the
+// exponent e = ilogb(M) + 2, so |x| 2^-e < 1/2, and OZ_S base-128 digits
+// d_p = rint(128 r), r <- 128 r - d_p, all in [-64, 64] (negation is safe in
+// int8).  The digit products d_p(A) d_q(B) with p + q <= OZ_S - 1 (0-based)
+// are accumulated exactly in int32 per level p + q, and
+//   C = 2^(eA + eB) sum_L 128^-(L+2) acc_L.
+// The truncation of each operand at OZ_S digits bounds the error by
+// 2^-(7 OZ_S + 1) of the row or column maximum per element.
+#include "rd_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace rd {
+
+namespace {
+
+constexpr int OZ_S = 7;        // digits per operand
+constexpr int OZ_CT = 64;      // columns per workgroup tile
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int oz_exponent(double mx)
+{
+    return mx > 0.0 ? ilogb(mx) + 2 : 0;
+}
+
+// Digits of x 2^-e (|x 2^-e| < 1/2) into d[0..OZ_S).
+__device__ __forceinline__ void oz_digits(double x, int e, int8_t (&d)[OZ_S])
+{
+    double r = ldexp(x, -e);
+#pragma unroll
+    for (int p = 0; p < OZ_S; p++) {
+        r *= 128.0;
+        const double q = rint(r);
+        r -= q;
+        d[p] = (int8_t)(int)q;
+    }
+}
+
+// Slices of one matrix: [OZ_S][rows][K] int8 from off, K a multiple of 64,
+// split = start of the second part of a concatenated K (or K).
+struct OzMat {
+    long off;
+    int K, split, rows;
+};
+
+__host__ __device__ inline int rup(int a, int b) { return (a + b - 1) / b * b; }
+
+// ------------------------------------------------------------ tables
+// synthesis A: rows theta (thp), K over l_p
+__global__ void syn_tab_oz_kernel(const double *tP, const double *tD, const double *tM, const long *syn_off,
+                                  int lmax, int nth, int nthp, const OzMat *am, int8_t *A, int *eA, int ra)
+{
+    const int th = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y, m = blockIdx.z;
+    const OzMat M = am[m * 4 + a];
+    if (th >= M.rows) return;
+    const int L = lmax + 1, re = (L - m + 1) / 2, ro = (L - m) / 2;
+    auto tab = [&](const double *t, int par, int row) -> double {
+        const int nrow = par ? ro : re;
+        if (row < 0 || row >= nrow || th >= nth) return 0.0;
+        return t[(syn_off[2 * m + par] + row) * nthp + th];
+    };
+    auto val = [&](int k) -> double {
+        switch (a) {
+        case 0: return tab(tP, 0, k);
+        case 1: return tab(tP, 1, k);
+        case 2: return k < M.split ? tab(tD, 1, k) : tab(tM, 0, k - M.split);
+        default: return k < M.split ? tab(tD, 0, k) : tab(tM, 1, k - M.split);
+        }
+    };
+    double mx = 0.0;
+    for (int k = 0; k < M.K; k++) mx = fmax(mx, fabs(val(k)));
+    const int e = oz_exponent(mx);
+    eA[(m * 4 + a) * ra + th] = e;
+    for (int k = 0; k < M.K; k++) {
+        int8_t d[OZ_S];
+        oz_digits(val(k), e, d);
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++) A[M.off + ((long)p * M.rows + th) * M.K + k] = d[p];
+    }
+}
+
+// analysis A: rows l_p, K over the northern thetas (twice for [D|M])
+__global__ void ana_tab_oz_kernel(const double *tP, const double *tD, const double *tM, const long *ana_off,
+                                  const int *ana_ncol, int lmax, int nth, int kth, const OzMat *am, int8_t *A,
+                                  int *eA, int ra)
+{
+    const int lp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y, m = blockIdx.z;
+    const OzMat M = am[m * 4 + a];
+    if (lp >= M.rows) return;
+    const int L = lmax + 1, par = a & 1, nrow = par ? (L - m) / 2 : (L - m + 1) / 2;
+    const int ncol = ana_ncol[m];
+    auto tab = [&](const double *t, int th) -> double {
+        if (lp >= nrow || th < 0 || th >= nth) return 0.0;
+        return t[ana_off[2 * m + par] + (long)th * ncol + lp];
+    };
+    auto val = [&](int k) -> double {
+        if (a < 2) return tab(tP, k);
+        return k < kth ? tab(tD, k) : tab(tM, k - kth);
+    };
+    double mx = 0.0;
+    for (int k = 0; k < M.K; k++) mx = fmax(mx, fabs(val(k)));
+    const int e = oz_exponent(mx);
+    eA[(m * 4 + a) * ra + lp] = e;
+    for (int k = 0; k < M.K; k++) {
+        int8_t d[OZ_S];
+        oz_digits(val(k), e, d);
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++) A[M.off + ((long)p * M.rows + lp) * M.K + k] = d[p];
+    }
+}
+
+// ------------------------------------------------------------ operands
+// Wave maxima of 12 values (6 matrices x {re, im}); every lane gets them.
+__device__ __forceinline__ void wave_max12(double (&v)[12])
+{
+#pragma unroll
+    for (int i = 0; i < 12; i++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] = fmax(v[i], __shfl_xor(v[i], off));
+}
+
+struct OzBArgs {
+    const double2 *in[3];
+    int8_t *B;
+    int *eB;
+    const OzMat *bm;
+    long ld;
+    int lmax, nlat, nth, kth, nplanes, ncols, nmat;
+};
+
+__device__ __forceinline__ double2 mul_i(double2 x) { return make_double2(-x.y, x.x); }
+__device__ __forceinline__ double2 mul_mi(double2 x) { return make_double2(x.y, -x.x); }
+__device__ __forceinline__ double2 neg2(double2 x) { return make_double2(-x.x, -x.y); }
+
+// Writes the digits of four consecutive values v[0..3] (complex: columns 2k,
+// 2k+1) of matrix b at K indices kidx..kidx+3, scaled by the column exponents
+// e (re, im); one packed 4-byte store per digit plane and column.
+__device__ __forceinline__ void oz_put4(const OzBArgs &A, int m, int b, int kidx, int k, const double2 (&v)[4],
+                                        const int *e)
+{
+    const OzMat M = A.bm[m * 6 + b];
+    unsigned wr[OZ_S], wi[OZ_S];
+#pragma unroll
+    for (int p = 0; p < OZ_S; p++) wr[p] = wi[p] = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int8_t dr[OZ_S], di[OZ_S];
+        oz_digits(v[i].x, e[0], dr);
+        oz_digits(v[i].y, e[1], di);
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++) {
+            wr[p] |= (unsigned)(uint8_t)dr[p] << (8 * i);
+            wi[p] |= (unsigned)(uint8_t)di[p] << (8 * i);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < OZ_S; p++) {
+        *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k) * M.K + kidx) = wr[p];
+        *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k + 1) * M.K + kidx) = wi[p];
+    }
+}
+
+__device__ __forceinline__ void oz_max4(double *mx, int b, const double2 (&v)[4], bool swap)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double x = fabs(swap ? v[i].y : v[i].x), y = fabs(swap ? v[i].x : v[i].y);
+        mx[2 * b] = fmax(mx[2 * b], x);
+        mx[2 * b + 1] = fmax(mx[2 * b + 1], y);
+    }
+}
+
+// synthesis operands: one wave per (m, plane k), groups of four rows t of the
+// even and odd l per lane.
+template <bool VEC>
+__global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
+{
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
+    if (k >= A.nplanes) return;
+    const int L = A.lmax + 1, re = (L - m + 1) / 2, ro = (L - m) / 2;
+    const int re16 = rup(re, 16), ro16 = rup(ro, 16);
+    const long lm0 = (long)m * (2 * A.lmax + 3 - m) / 2 + k * A.ld;   // lm_index(m, m) + plane offset
+    auto load4 = [&](int f, int par, int t0, double2 (&v)[4]) {
+        const int nrow = par ? ro : re;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = make_double2(0.0, 0.0);
+            if (t0 + i < nrow) v[i] = A.in[f][lm0 + 2 * (t0 + i) + par];
+            if (m == 0) v[i].y = 0.0;
+        }
+    };
+    auto rot = [](const double2 (&v)[4], double2 (&o)[4], bool plus_i) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) o[i] = plus_i ? mul_i(v[i]) : mul_mi(v[i]);
+    };
+    double mx[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) mx[i] = 0.0;
+    for (int t0 = 4 * tid; t0 < re; t0 += 256) {
+        double2 qe[4], qo[4];
+        load4(0, 0, t0, qe);
+        load4(0, 1, t0, qo);
+        oz_max4(mx, 0, qe, false);
+        oz_max4(mx, 1, qo, false);
+        if (VEC) {
+            double2 se[4], so[4], te[4], to[4];
+            load4(1, 0, t0, se);
+            load4(1, 1, t0, so);
+            load4(2, 0, t0, te);
+            load4(2, 1, t0, to);
+            // |(iX).re| = |X.im|, |(iX).im| = |X.re|, likewise for -iX
+            oz_max4(mx, 2, so, false); oz_max4(mx, 2, te, true);
+            oz_max4(mx, 3, se, false); oz_max4(mx, 3, to, true);
+            oz_max4(mx, 4, to, false); oz_max4(mx, 4, se, true);
+            oz_max4(mx, 5, te, false); oz_max4(mx, 5, so, true);
+        }
+    }
+    wave_max12(mx);
+    int es[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
+    for (int t0 = 4 * tid; t0 < re; t0 += 256) {
+        double2 qe[4], qo[4];
+        load4(0, 0, t0, qe);
+        load4(0, 1, t0, qo);
+        oz_put4(A, m, 0, t0, k, qe, es + 0);
+        if (t0 < ro) oz_put4(A, m, 1, t0, k, qo, es + 2);
+        if (VEC) {
+            double2 se[4], so[4], te[4], to[4], w[4];
+            load4(1, 0, t0, se);
+            load4(1, 1, t0, so);
+            load4(2, 0, t0, te);
+            load4(2, 1, t0, to);
+            if (t0 < ro) {
+                oz_put4(A, m, 2, t0, k, so, es + 4);
+                oz_put4(A, m, 4, t0, k, to, es + 8);
+                rot(to, w, true);
+                oz_put4(A, m, 3, re16 + t0, k, w, es + 6);
+                rot(so, w, false);
+                oz_put4(A, m, 5, re16 + t0, k, w, es + 10);
+            }
+            rot(te, w, true);
+            oz_put4(A, m, 2, ro16 + t0, k, w, es + 4);
+            rot(se, w, false);
+            oz_put4(A, m, 4, ro16 + t0, k, w, es + 8);
+            oz_put4(A, m, 3, t0, k, se, es + 6);
+            oz_put4(A, m, 5, t0, k, te, es + 10);
+        }
+    }
+    if (tid == 0)
+        for (int i = 0; i < 2 * A.nmat; i++) A.eB[(m * 6 + i / 2) * A.ncols + 2 * k + (i & 1)] = es[i];
+}
+
+// analysis operands: one wave per (m, plane k), groups of four northern
+// thetas t per lane.
+template <bool VEC>
+__global__ __launch_bounds__(256) void ana_b_oz_kernel(OzBArgs A)
+{
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
+    if (k >= A.nplanes) return;
+    const int L = A.lmax + 1;
+    const long base = ((long)k * L + m) * A.nlat;
+    // x+ and x- of field f at thetas t0..t0+3
+    auto load4 = [&](int f, int t0, double2 (&sp)[4], double2 (&sm)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double2 n = make_double2(0.0, 0.0), s = n;
+            if (t0 + i < A.nth) {
+                n = A.in[f][base + t0 + i];
+                s = A.in[f][base + A.nlat - 1 - (t0 + i)];
+            }
+            if (m == 0) { n.y = 0.0; s.y = 0.0; }
+            sp[i] = make_double2(n.x + s.x, n.y + s.y);
+            sm[i] = make_double2(n.x - s.x, n.y - s.y);
+        }
+    };
+    auto map = [](const double2 (&v)[4], double2 (&o)[4], int how) {   // 0: -i x, 1: -x
+#pragma unroll
+        for (int i = 0; i < 4; i++) o[i] = how == 0 ? mul_mi(v[i]) : neg2(v[i]);
+    };
+    double mx[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) mx[i] = 0.0;
+    for (int t0 = 4 * tid; t0 < A.nth; t0 += 256) {
+        double2 rp[4], rm[4];
+        load4(0, t0, rp, rm);
+        oz_max4(mx, 0, rp, false);
+        oz_max4(mx, 1, rm, false);
+        if (VEC) {
+            double2 tp[4], tm[4], pp[4], pm[4];
+            load4(1, t0, tp, tm);
+            load4(2, t0, pp, pm);
+            oz_max4(mx, 2, tm, false); oz_max4(mx, 2, pp, true);
+            oz_max4(mx, 3, tp, false); oz_max4(mx, 3, pm, true);
+            oz_max4(mx, 4, pm, false); oz_max4(mx, 4, tp, true);
+            oz_max4(mx, 5, pp, false); oz_max4(mx, 5, tm, true);
+        }
+    }
+    wave_max12(mx);
+    int es[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
+    for (int t0 = 4 * tid; t0 < A.nth; t0 += 256) {
+        double2 rp[4], rm[4];
+        load4(0, t0, rp, rm);
+        oz_put4(A, m, 0, t0, k, rp, es + 0);
+        oz_put4(A, m, 1, t0, k, rm, es + 2);
+        if (VEC) {
+            double2 tp[4], tm[4], pp[4], pm[4], w[4];
+            load4(1, t0, tp, tm);
+            load4(2, t0, pp, pm);
+            oz_put4(A, m, 2, t0, k, tm, es + 4);
+            map(pp, w, 0);
+            oz_put4(A, m, 2, A.kth + t0, k, w, es + 4);
+            oz_put4(A, m, 3, t0, k, tp, es + 6);
+            map(pm, w, 0);
+            oz_put4(A, m, 3, A.kth + t0, k, w, es + 6);
+            map(pm, w, 1);
+            oz_put4(A, m, 4, t0, k, w, es + 8);
+            map(tp, w, 0);
+            oz_put4(A, m, 4, A.kth + t0, k, w, es + 8);
+            map(pp, w, 1);
+            oz_put4(A, m, 5, t0, k, w, es + 10);
+            map(tm, w, 0);
+            oz_put4(A, m, 5, A.kth + t0, k, w, es + 10);
+        }
+    }
+    if (tid == 0)
+        for (int i = 0; i < 2 * A.nmat; i++) A.eB[(m * 6 + i / 2) * A.ncols + 2 * k + (i & 1)] = es[i];
+}
+
+// ------------------------------------------------------------ GEMM
+struct OzGArgs {
+    const int8_t *A, *B;
+    const OzMat *am, *bm;
+    const int *eA, *eB;
+    const int2 *items;          // (m, row tile)
+    double2 *out[3];
+    long ld;
+    int ra, ncols, n_jt;
+    int lmax, nlat, nth, nplanes;
+};
+
+// LDS-staged digit GEMM.  A workgroup owns a 64-row x 64-column output tile;
+// per K step of 64 it copies the OZ_S digit planes of the A rows and of the B
+// columns to LDS (rows at a pitch of 80 bytes: the 16-byte fragment reads of
+// 16 lanes then cover all 64 banks), and each wave multiplies its 16 rows by
+// the 64 columns: acc[p + q][j] += A_p B_q[j] for p + q <= OZ_S - 1.
+constexpr int OZ_PITCH = 80;
+constexpr int OZ_TILE = OZ_S * 64 * OZ_PITCH;     // bytes of one operand tile
+
+__device__ __forceinline__ void oz_stage(const int8_t *src, int nrows_total, int r0, int K, int kb,
+                                         int8_t *dst)
+{
+    // OZ_S planes x 64 rows x 64 bytes, 16 bytes per thread and step
+    for (int c = threadIdx.x; c < OZ_S * 64 * 4; c += 256) {
+        const int p = c >> 8, rem = c & 255, row = rem >> 2, q = rem & 3;
+        const v4i v = *(const v4i *)(src + ((long)p * nrows_total + r0 + row) * K + kb + 16 * q);
+        *(v4i *)(dst + (p * 64 + row) * OZ_PITCH + 16 * q) = v;
+    }
+}
+
+__device__ __forceinline__ void oz_gemm_lds(const int8_t *Ab, int arows, int arow0, const int8_t *Bb,
+                                            int ncols, int col0, int K, int8_t *As, int8_t *Bs,
+                                            v4i (&acc)[OZ_S][4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    for (int kb = 0; kb < K; kb += 64) {
+        oz_stage(Ab, arows, arow0, K, kb, As);
+        oz_stage(Bb, ncols, col0, K, kb, Bs);
+        __syncthreads();
+        v4i a[OZ_S];
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++) a[p] = *(const v4i *)(As + (p * 64 + wave * 16 + r) * OZ_PITCH + 16 * g);
+#pragma unroll
+        for (int q = 0; q < OZ_S; q++) {
+            v4i bq[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bq[j] = *(const v4i *)(Bs + (q * 64 + 16 * j + r) * OZ_PITCH + 16 * g);
+#pragma unroll
+            for (int p = 0; p < OZ_S - q; p++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], bq[j], acc[p + q][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+}
+
+// FP64 value of output element (v, j): the levels from the smallest weight up.
+__device__ __forceinline__ double oz_value(const v4i (&acc)[OZ_S][4], int j, int v, int e)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int L = OZ_S - 1; L >= 0; L--) s += ldexp((double)acc[L][j][v], -7 * (L + 2));
+    return ldexp(s, e);
+}
+
+__device__ __forceinline__ void oz_zero(v4i (&acc)[OZ_S][4])
+{
+#pragma unroll
+    for (int L = 0; L < OZ_S; L++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[L][j] = (v4i){0, 0, 0, 0};
+}
+
+// Register digit GEMM for the short-K synthesis tiles: each wave loads its
+// A rows and B columns straight from L2 (no staging barrier per K step) and
+// multiplies 16 rows by 32 columns.
+constexpr int OZ_SJB = 2;
+
+__device__ __forceinline__ void oz_gemm_reg(const int8_t *Ab, int arows, int row0, const int8_t *Bb, int ncols,
+                                            int col0, int K, v4i (&acc)[OZ_S][4])
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (int kb = 0; kb < K; kb += 64) {
+        v4i a[OZ_S], b[OZ_S][OZ_SJB];
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++)
+            a[p] = *(const v4i *)(Ab + ((long)p * arows + row0 + r) * K + kb + 16 * g);
+#pragma unroll
+        for (int q = 0; q < OZ_S; q++)
+#pragma unroll
+            for (int j = 0; j < OZ_SJB; j++)
+                b[q][j] = *(const v4i *)(Bb + ((long)q * ncols + col0 + 16 * j + r) * K + kb + 16 * g);
+#pragma unroll
+        for (int p = 0; p < OZ_S; p++)
+#pragma unroll
+            for (int q = 0; q < OZ_S - p; q++)
+#pragma unroll
+                for (int j = 0; j < OZ_SJB; j++)
+                    acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[q][j], acc[p + q][j], 0, 0, 0);
+    }
+}
+
+// synthesis: workgroup = 64 thetas (16 per wave) x 32 columns, one field,
+// both parities
+__global__ __launch_bounds__(256) void syn_oz_kernel(OzGArgs A)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int jt = blockIdx.x % A.n_jt;
+    const int2 it = A.items[blockIdx.x / A.n_jt];
+    const int m = it.x, row0 = it.y * 64 + wave * 16, col0 = jt * 16 * OZ_SJB;
+    const int f = blockIdx.y;
+    const int L = A.lmax + 1;
+    const int am[2] = {f == 0 ? 0 : 2, f == 0 ? 1 : 3};
+    const int bmx[2] = {f == 0 ? 0 : (f == 1 ? 2 : 4), f == 0 ? 1 : (f == 1 ? 3 : 5)};
+    double res[2][OZ_SJB][4];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const OzMat Ma = A.am[m * 4 + am[s]], Mb = A.bm[m * 6 + bmx[s]];
+        v4i acc[OZ_S][4];
+        oz_zero(acc);
+        if (Ma.K > 0) oz_gemm_reg(A.A + Ma.off, Ma.rows, row0, A.B + Mb.off, A.ncols, col0, Ma.K, acc);
+#pragma unroll
+        for (int j = 0; j < OZ_SJB; j++) {
+            const int col = col0 + 16 * j + r;
+            const int eb = A.eB[(m * 6 + bmx[s]) * A.ncols + col];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                res[s][j][v] = oz_value(acc, j, v, A.eA[(m * 4 + am[s]) * A.ra + row0 + 4 * g + v] + eb);
+        }
+    }
+    const double sg = f == 2 ? -1.0 : 1.0;
+    double *out = (double *)A.out[f];
+#pragma unroll
+    for (int j = 0; j < OZ_SJB; j++) {
+        const int col = col0 + 16 * j + r, k = col >> 1, c = col & 1;
+        if (k >= A.nplanes) continue;
+        const long base = ((long)k * L + m) * A.nlat;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int th = row0 + 4 * g + v;
+            if (th >= A.nth) continue;
+            out[2 * (base + th) + c] = sg * (res[0][j][v] + res[1][j][v]);
+            out[2 * (base + A.nlat - 1 - th) + c] = sg * (res[0][j][v] - res[1][j][v]);
+        }
+    }
+}
+
+// analysis: workgroup = 64 rows l_p (16 per wave) x 64 columns, one field,
+// both parities
+__global__ __launch_bounds__(256) void ana_oz_kernel(OzGArgs A)
+{
+    extern __shared__ __align__(16) int8_t oz_sm[];
+    int8_t *As = oz_sm, *Bs = oz_sm + OZ_TILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int jt = blockIdx.x % A.n_jt;
+    const int2 it = A.items[blockIdx.x / A.n_jt];
+    const int m = it.x, arow0 = it.y * 64, row0 = arow0 + wave * 16, col0 = jt * 64;
+    const int f = blockIdx.y;
+    const long lm0 = (long)m * (2 * A.lmax + 3 - m) / 2;
+    double *out = (double *)A.out[f];
+#pragma unroll
+    for (int par = 0; par < 2; par++) {
+        const int a = f == 0 ? par : 2 + par;
+        const int b = f == 0 ? par : (f == 1 ? 2 + par : 4 + par);
+        const OzMat Ma = A.am[m * 4 + a], Mb = A.bm[m * 6 + b];
+        if (arow0 >= Ma.rows) continue;       // uniform over the workgroup
+        v4i acc[OZ_S][4];
+        oz_zero(acc);
+        oz_gemm_lds(A.A + Ma.off, Ma.rows, arow0, A.B + Mb.off, A.ncols, col0, Ma.K, As, Bs, acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = col0 + 16 * j + r, k = col >> 1, c = col & 1;
+            if (k >= A.nplanes) continue;
+            const int eb = A.eB[(m * 6 + b) * A.ncols + col];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int lp = row0 + 4 * g + v;
+                const int l = m + par + 2 * lp;
+                if (l > A.lmax) continue;
+                double val = oz_value(acc, j, v, A.eA[(m * 4 + a) * A.ra + lp] + eb);
+                if (m == 0 && c == 1) val = 0.0;
+                out[2 * (lm0 + (l - m) + k * A.ld) + c] = val;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host
+struct OzDir {
+    OzMat *d_am = nullptr, *d_bm = nullptr;
+    int8_t *A = nullptr, *B = nullptr;
+    int *eA = nullptr, *eB = nullptr;
+    int2 *d_items = nullptr;
+    int n_items = 0, ra = 0;
+};
+
+struct OzState {
+    OzDir syn, ana;
+    int ncols = 0;
+};
+
+static OzState *oz(Plan *p) { return (OzState *)p->oz; }
+
+static int oz_alloc_dir(Plan *p, OzDir &D, const std::vector<OzMat> &am, const std::vector<OzMat> &bm,
+                        long abytes, long bbytes, const std::vector<int2> &items, int ra, int ncols)
+{
+    const int L = p->L;
+    RD_HIP(hipMalloc(&D.d_am, sizeof(OzMat) * am.size()));
+    RD_HIP(hipMalloc(&D.d_bm, sizeof(OzMat) * bm.size()));
+    RD_HIP(hipMemcpyAsync(D.d_am, am.data(), sizeof(OzMat) * am.size(), hipMemcpyHostToDevice, p->stream));
+    RD_HIP(hipMemcpyAsync(D.d_bm, bm.data(), sizeof(OzMat) * bm.size(), hipMemcpyHostToDevice, p->stream));
+    RD_HIP(hipMalloc(&D.A, abytes));
+    RD_HIP(hipMalloc(&D.B, bbytes));
+    RD_HIP(hipMemsetAsync(D.B, 0, bbytes, p->stream));
+    RD_HIP(hipMalloc(&D.eA, sizeof(int) * 4L * L * ra));
+    RD_HIP(hipMemsetAsync(D.eA, 0, sizeof(int) * 4L * L * ra, p->stream));
+    RD_HIP(hipMalloc(&D.eB, sizeof(int) * 6L * L * ncols));
+    RD_HIP(hipMemsetAsync(D.eB, 0, sizeof(int) * 6L * L * ncols, p->stream));
+    RD_HIP(hipMalloc(&D.d_items, sizeof(int2) * items.size()));
+    RD_HIP(hipMemcpyAsync(D.d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+                          p->stream));
+    D.n_items = (int)items.size();
+    D.ra = ra;
+    return RD_OK;
+}
+
+// Builds the sliced tables and the operand scratch on first use.
+int ensure_oz_tables(Plan *p)
+{
+    if (p->oz) return RD_OK;
+    OzState *S = new OzState();
+    p->oz = S;
+    const int L = p->L, nth = p->nth;
+    S->ncols = rup(2 * p->nz, OZ_CT);
+    const int thp = rup(nth, 64), kth = rup(nth, 16);
+    // synthesis: A rows theta, K over l_p
+    {
+        std::vector<OzMat> am(4 * L), bm(6 * L);
+        std::vector<int2> items;
+        long ao = 0, bo = 0;
+        for (int m = 0; m < L; m++) {
+            const int re = (L - m + 1) / 2, ro = (L - m) / 2, re16 = rup(re, 16), ro16 = rup(ro, 16);
+            const int K[4] = {rup(re, 64), rup(ro, 64), rup(ro16 + re16, 64), rup(re16 + ro16, 64)};
+            const int sp[4] = {K[0], K[1], ro16, re16};
+            for (int a = 0; a < 4; a++) {
+                am[4 * m + a] = OzMat{ao, K[a], sp[a], thp};
+                ao += (long)OZ_S * thp * K[a];
+            }
+            const int bk[6] = {0, 1, 2, 3, 2, 3};
+            for (int b = 0; b < 6; b++) {
+                bm[6 * m + b] = OzMat{bo, K[bk[b]], sp[bk[b]], S->ncols};
+                bo += (long)OZ_S * S->ncols * K[bk[b]];
+            }
+            for (int tt = 0; tt < thp / 64; tt++) items.push_back(make_int2(m, tt));
+        }
+        int rc = oz_alloc_dir(p, S->syn, am, bm, ao, bo, items, thp, S->ncols);
+        if (rc) return rc;
+        hipLaunchKernelGGL(syn_tab_oz_kernel, dim3(thp / 64, 4, L), dim3(64), 0, p->stream, p->syn_P, p->syn_D,
+                           p->syn_M, p->d_syn_off, p->lmax, nth, p->nthp, S->syn.d_am, S->syn.A, S->syn.eA, thp);
+        RD_LAUNCH_CHECK();
+    }
+    // analysis: A rows l_p, K over the northern thetas
+    {
+        const int ra = rup((L + 1) / 2, 64);
+        std::vector<OzMat> am(4 * L), bm(6 * L);
+        std::vector<int2> items;
+        long ao = 0, bo = 0;
+        const int K1 = rup(kth, 64), K2 = rup(2 * kth, 64);
+        for (int m = 0; m < L; m++) {
+            const int re = (L - m + 1) / 2, ro = (L - m) / 2;
+            const int rows[4] = {rup(re, 64), rup(ro, 64), rup(re, 64), rup(ro, 64)};
+            const int K[4] = {K1, K1, K2, K2};
+            for (int a = 0; a < 4; a++) {
+                am[4 * m + a] = OzMat{ao, K[a], kth, rows[a]};
+                ao += (long)OZ_S * rows[a] * K[a];
+            }
+            const int bk[6] = {K1, K1, K2, K2, K2, K2};
+            for (int b = 0; b < 6; b++) {
+                bm[6 * m + b] = OzMat{bo, bk[b], kth, S->ncols};
+                bo += (long)OZ_S * S->ncols * bk[b];
+            }
+            for (int rt = 0; rt < rows[0] / 64; rt++) items.push_back(make_int2(m, rt));
+        }
+        int rc = oz_alloc_dir(p, S->ana, am, bm, ao, bo, items, ra, S->ncols);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ana_tab_oz_kernel, dim3(ra / 64, 4, L), dim3(64), 0, p->stream, p->ana_P, p->ana_D,
+                           p->ana_M, p->d_ana_off, p->d_ana_ncol, p->lmax, nth, kth, S->ana.d_am, S->ana.A,
+                           S->ana.eA, ra);
+        RD_LAUNCH_CHECK();
+    }
+    return RD_OK;
+}
+
+void free_oz_tables(Plan *p)
+{
+    OzState *S = oz(p);
+    if (!S) return;
+    for (OzDir *D : {&S->syn, &S->ana}) {
+        void *ptrs[] = {D->d_am, D->d_bm, D->A, D->B, D->eA, D->eB, D->d_items};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+    }
+    delete S;
+    p->oz = nullptr;
+}
+
+int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
+                        long ld, double2 *f0, double2 *f1, double2 *f2)
+{
+    int rc = ensure_oz_tables(p);
+    if (rc) return rc;
+    OzState *S = oz(p);
+    if (2 * nplanes > S->ncols) return RD_ERR_ARG;
+    OzBArgs B;
+    B.in[0] = q;
+    B.in[1] = vec ? s : q;
+    B.in[2] = vec ? t : q;
+    B.B = S->syn.B;
+    B.eB = S->syn.eB;
+    B.bm = S->syn.d_bm;
+    B.ld = ld;
+    B.lmax = p->lmax;
+    B.nlat = p->nlat;
+    B.nth = p->nth;
+    B.kth = 0;
+    B.nplanes = nplanes;
+    B.ncols = S->ncols;
+    B.nmat = vec ? 6 : 2;
+    if (vec)
+        hipLaunchKernelGGL(syn_b_oz_kernel<true>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    else
+        hipLaunchKernelGGL(syn_b_oz_kernel<false>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    RD_LAUNCH_CHECK();
+    OzGArgs G = {};
+    G.A = S->syn.A;
+    G.B = S->syn.B;
+    G.am = S->syn.d_am;
+    G.bm = S->syn.d_bm;
+    G.eA = S->syn.eA;
+    G.eB = S->syn.eB;
+    G.items = S->syn.d_items;
+    G.out[0] = f0;
+    G.out[1] = vec ? f1 : f0;
+    G.out[2] = vec ? f2 : f0;
+    G.ld = ld;
+    G.ra = S->syn.ra;
+    G.ncols = S->ncols;
+    G.n_jt = (2 * nplanes + 16 * OZ_SJB - 1) / (16 * OZ_SJB);
+    G.lmax = p->lmax;
+    G.nlat = p->nlat;
+    G.nth = p->nth;
+    G.nplanes = nplanes;
+    hipLaunchKernelGGL(syn_oz_kernel, dim3(S->syn.n_items * G.n_jt, vec ? 3 : 1), dim3(256), 0, p->stream, G);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+int launch_analysis_oz(Plan *p, bool vec, int nplanes, const double2 *g0, const double2 *g1, const double2 *g2,
+                       double2 *q, double2 *s, double2 *t, long ld)
+{
+    int rc = ensure_oz_tables(p);
+    if (rc) return rc;
+    OzState *S = oz(p);
+    if (2 * nplanes > S->ncols) return RD_ERR_ARG;
+    OzBArgs B;
+    B.in[0] = g0;
+    B.in[1] = vec ? g1 : g0;
+    B.in[2] = vec ? g2 : g0;
+    B.B = S->ana.B;
+    B.eB = S->ana.eB;
+    B.bm = S->ana.d_bm;
+    B.ld = ld;
+    B.lmax = p->lmax;
+    B.nlat = p->nlat;
+    B.nth = p->nth;
+    B.kth = rup(p->nth, 16);
+    B.nplanes = nplanes;
+    B.ncols = S->ncols;
+    B.nmat = vec ? 6 : 2;
+    if (vec)
+        hipLaunchKernelGGL(ana_b_oz_kernel<true>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    else
+        hipLaunchKernelGGL(ana_b_oz_kernel<false>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    RD_LAUNCH_CHECK();
+    OzGArgs G = {};
+    G.A = S->ana.A;
+    G.B = S->ana.B;
+    G.am = S->ana.d_am;
+    G.bm = S->ana.d_bm;
+    G.eA = S->ana.eA;
+    G.eB = S->ana.eB;
+    G.items = S->ana.d_items;
+    G.out[0] = q;
+    G.out[1] = vec ? s : q;
+    G.out[2] = vec ? t : q;
+    G.ld = ld;
+    G.ra = S->ana.ra;
+    G.ncols = S->ncols;
+    G.n_jt = (2 * nplanes + OZ_CT - 1) / OZ_CT;
+    G.lmax = p->lmax;
+    G.nlat = p->nlat;
+    G.nth = p->nth;
+    G.nplanes = nplanes;
+    hipLaunchKernelGGL(ana_oz_kernel, dim3(S->ana.n_items * G.n_jt, vec ? 3 : 1), dim3(256), 2 * OZ_TILE,
+                       p->stream, G);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+}  // namespace rd

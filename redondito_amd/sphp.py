@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "librdsphp.so")
 TABLE_DIR = os.path.join(_HERE, "tables")
 
-RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3 = 0, 1, 2
+RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3, RD_PREC_OZAKI = 0, 1, 2, 3
 
 _lib = None
 

@@ -1,12 +1,19 @@
-"""Precision forms of the nonlinear term: Legendre contractions on FP32 MFMA
-and as BF16x3 splits with FP32 accumulation (tables, nodes, weights and the
-radial FC-Gram step stay FP64).  Their error is reported per output field and
-per degree band against the FP64 path and the oracle.
+"""Precision forms of the nonlinear term: Legendre contractions on FP32 MFMA,
+as BF16x3 splits with FP32 accumulation, and as int8 digit slices with exact
+int32 accumulation (tables, nodes, weights and the radial FC-Gram step stay
+FP64).  Their error is reported per output field and per degree band against
+the FP64 path and the oracle.
 
-Expected levels: FP32 rounding of tables and operands gives about 1e-7
-relative per contraction; BF16x3 keeps about 17 significant bits per operand
-(2^-17 = 8e-6).  The bounds below are those levels with headroom: the forms
-are reported, not held to the FP64 bar.
+Expected levels:
+  * FP32 rounding of tables and operands gives about 1e-7 relative per contraction.
+  * BF16x3 keeps about 17 significant bits per operand (2^-17 = 8e-6).
+  * The int8 form cuts each operand at seven base-128 digits after a power-of-two
+    scale per table row and per operand column, so each element is exact to
+    2^-50 of its row or column maximum; the contraction then carries about
+    1e-11 of the largest products, and the evaluation about 1e-10.
+The bounds below are those levels with headroom: the FP32 and BF16x3 forms are
+reported, not held to the FP64 bar; the int8 form is held to the 1e-9 bar of
+a mixed form.
 """
 import json
 
@@ -19,8 +26,8 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-BOUND = {1: 2e-6, 2: 5e-5}     # joint relative L2 vs FP64
-NAMES = {1: "fp32", 2: "bf16x3"}
+BOUND = {1: 2e-6, 2: 5e-5, 3: 1e-9}     # joint relative L2 vs FP64
+NAMES = {1: "fp32", 2: "bf16x3", 3: "ozaki int8x7"}
 
 
 def _run(lmax, nr, seed):
@@ -33,7 +40,7 @@ def _run(lmax, nr, seed):
     W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=seed, slope=2.0)
     Wd, Zd = (torch.from_numpy(x).to("cuda:0") for x in (W, Z))
     outs = {}
-    for prec in (0, 1, 2):
+    for prec in (0, 1, 2, 3):
         o = [torch.zeros_like(Wd) for _ in range(3)]
         S.nonlinear_uxw(plan, Wd, Zd, *o, prec=prec)
         torch.cuda.synchronize()
@@ -44,7 +51,7 @@ def _run(lmax, nr, seed):
 @pytest.mark.parametrize("lmax,nr", [(31, 24), (127, 96), (255, 128)])
 def test_precision_forms_vs_fp64(lmax, nr):
     plan, W, Z, outs, dz, nlat = _run(lmax, nr, seed=20261203)
-    for prec in (1, 2):
+    for prec in (1, 2, 3):
         rep = H.band_errors(outs[prec], outs[0], lmax, nr)
         print(f"[{NAMES[prec]} vs fp64 lmax={lmax} nr={nr}] " + json.dumps(
             {k: (v if isinstance(v, float) else {b: float(f"{e:.2e}") for b, e in v.items()})
@@ -68,7 +75,7 @@ def test_precision_forms_vs_oracle_small():
     fc = O.FC(nr + 25, 25, 5, O.fc_dir(A, Q))
     ref = O.nonlinear_uxw(sht, fc, 0.35, dz, W, Z)
     assert H.band_errors(outs[0], ref, lmax, nr)["joint"] < 1e-12
-    for prec in (1, 2):
+    for prec in (1, 2, 3):
         e = H.band_errors(outs[prec], ref, lmax, nr)["joint"]
         print(f"{NAMES[prec]} vs oracle: {e:.2e}")
         assert e < BOUND[prec]
@@ -79,5 +86,6 @@ def test_bad_precision_rejected():
     plan = S.FCPlan(7, 12, 24, 40, 25, 5, 0.1, 0.5)
     W = torch.zeros((40, plan.nlm), dtype=torch.complex128, device="cuda:0")
     o = [torch.zeros_like(W) for _ in range(3)]
-    with pytest.raises(S.RdError):
-        S.nonlinear_uxw(plan, W, W, *o, prec=3)
+    for bad in (-1, 4):
+        with pytest.raises(S.RdError):
+            S.nonlinear_uxw(plan, W, W, *o, prec=bad)
