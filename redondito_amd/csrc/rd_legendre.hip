@@ -292,6 +292,17 @@ int build_legendre_tables(Plan *p)
 // (SJT/2 planes).  The l loop runs in chunks of 32 (16 per parity).
 constexpr int SKC = 32;
 
+// 16-column blocks per wave of the one-problem kernels.  FP64 vector tiles
+// keep one block: the accumulators then fit 108 VGPRs, which buys 4 waves
+// per SIMD, and at this occupancy the global-load latency of the staging
+// loop hides better than with two blocks at 2 waves and a double-buffered
+// stage (0.54 against 0.59 ms at lmax 255, 256 planes).
+template <typename T, bool VEC>
+constexpr int syn_jb()
+{
+    return (VEC && sizeof(T) == 8) ? 1 : (VEC ? 2 : 4);
+}
+
 template <bool VEC, int JBV = (VEC ? 2 : 4)>
 struct SynCfg {
     static constexpr int NF = VEC ? 3 : 1;
@@ -314,22 +325,10 @@ struct SynArgs {
 // One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
 // T = double: the FP64 path; T = float: the FP32 precision form (tables and
 // operands rounded to FP32, FP32 accumulation, FP64 output).
-// FP64 vector tiles stage the spectral operand through two LDS buffers:
-// while chunk c is multiplied out of one, chunk c+1 is loaded in two halves
-// and written to the other (no wave reads it until the barrier that ends
-// chunk c), so each chunk costs one barrier and its global-load latency
-// overlaps the MFMA work.  The other forms keep one buffer: their smaller
-// register footprint buys more waves per SIMD, which hides the latency better.
-template <typename T, bool VEC>
-constexpr int syn_nbuf()
-{
-    return (sizeof(T) == 8 && VEC) ? 2 : 1;
-}
-
 template <typename T, bool VEC, int JBV>
 __device__ __forceinline__ void syn_tile(
     const SynArgs<T> &A, int m, int tt, int jt,
-    T (&Bs)[syn_nbuf<T, VEC>()][SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
+    T (&Bs)[SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
 {
     using Cfg = SynCfg<VEC, JBV>;
     using V = typename Mf<T>::V;
@@ -353,64 +352,21 @@ __device__ __forceinline__ void syn_tile(
     const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
     const long lm0 = lm_index(A.lmax, m, m);
 
-    // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads;
-    // each thread moves NLD double2 of it, in two halves of NH
+    // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads
     constexpr int PER = (JT / 2) * SKC;
-    constexpr int NLD = NF * PER / 256, NH = NLD / 2;
-    static_assert(NF * PER == 256 * NLD && NLD % 2 == 0, "chunk must split evenly over the workgroup");
-    double2 r[NH];
-    auto load_half = [&](int lbase, int h) {
-#pragma unroll
-        for (int i = 0; i < NH; i++) {
-            const int e = tid + 256 * (h * NH + i);
+    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
+        for (int e = tid; e < NF * PER; e += 256) {
             const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
             const int l = lbase + ll, k = k0 + pl;
-            r[i] = make_double2(0.0, 0.0);
-            if (l <= A.lmax && k < A.nplanes) r[i] = A.in[f][lm0 + (l - m) + k * A.ld];
+            double2 v = make_double2(0.0, 0.0);
+            if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
+            Bs[f][ll][2 * pl] = (T)v.x;
+            Bs[f][ll][2 * pl + 1] = (T)v.y;
         }
-    };
-    auto store_half = [&](int buf, int h) {
-#pragma unroll
-        for (int i = 0; i < NH; i++) {
-            const int e = tid + 256 * (h * NH + i);
-            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
-            Bs[buf][f][ll][2 * pl] = (T)r[i].x;
-            Bs[buf][f][ll][2 * pl + 1] = (T)r[i].y;
-        }
-    };
-    constexpr bool DB = syn_nbuf<T, VEC>() == 2;
-    if (DB) {
-        load_half(m, 0);
-        store_half(0, 0);
-        load_half(m, 1);
-        store_half(0, 1);
         __syncthreads();
-    }
-
-    int buf = 0;
-    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
-        const bool more = lbase + SKC <= A.lmax;
-        if (DB) {
-            if (more) load_half(lbase + SKC, 0);
-        } else {
-            for (int e = tid; e < NF * PER; e += 256) {
-                const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
-                const int l = lbase + ll, k = k0 + pl;
-                double2 v = make_double2(0.0, 0.0);
-                if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-                Bs[0][f][ll][2 * pl] = (T)v.x;
-                Bs[0][f][ll][2 * pl + 1] = (T)v.y;
-            }
-            __syncthreads();
-        }
         const int rowbase = (lbase - m) >> 1;
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++) {
-            if (DB && st == 1 && more) {
-                store_half(buf ^ 1, 0);
-                load_half(lbase + SKC, 1);
-            }
-            if (DB && st == 3 && more) store_half(buf ^ 1, 1);
             const int row = rowbase + 4 * st + kk;
             const long ie = offe + (long)row * A.nthp + th0 + col;
             const long io = offo + (long)row * A.nthp + th0 + col;
@@ -426,13 +382,13 @@ __device__ __forceinline__ void syn_tile(
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = b * 16 + col;
-                acc[0][0][b] = Mf<T>::mma(Pe, Bs[buf][0][lle][j], acc[0][0][b]);
-                acc[0][1][b] = Mf<T>::mma(Po, Bs[buf][0][llo][j], acc[0][1][b]);
+                acc[0][0][b] = Mf<T>::mma(Pe, Bs[0][lle][j], acc[0][0][b]);
+                acc[0][1][b] = Mf<T>::mma(Po, Bs[0][llo][j], acc[0][1][b]);
                 if (VEC) {
-                    const T Se = Bs[buf][1][lle][j], So = Bs[buf][1][llo][j];
-                    const T Te = Bs[buf][2][lle][j], To = Bs[buf][2][llo][j];
-                    const T iSe = sgn_i * Bs[buf][1][lle][j ^ 1], iSo = sgn_i * Bs[buf][1][llo][j ^ 1];
-                    const T iTe = sgn_i * Bs[buf][2][lle][j ^ 1], iTo = sgn_i * Bs[buf][2][llo][j ^ 1];
+                    const T Se = Bs[1][lle][j], So = Bs[1][llo][j];
+                    const T Te = Bs[2][lle][j], To = Bs[2][llo][j];
+                    const T iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
+                    const T iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
                     // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
                     acc[1][0][b] = Mf<T>::mma(Do, So, acc[1][0][b]);
                     acc[1][0][b] = Mf<T>::mma(Me, iTe, acc[1][0][b]);
@@ -447,7 +403,6 @@ __device__ __forceinline__ void syn_tile(
             }
         }
         __syncthreads();
-        if (DB) buf ^= 1;
     }
 
     // north = sym + anti, south = sym - anti
@@ -473,10 +428,10 @@ __device__ __forceinline__ void syn_tile(
 }
 
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256, (syn_nbuf<T, VEC>() == 2 ? 2 : 4)) void synthesis_kernel(SynArgs<T> A)
+__global__ __launch_bounds__(256, 4) void synthesis_kernel(SynArgs<T> A)
 {
-    using Cfg = SynCfg<VEC>;
-    __shared__ T Bs[syn_nbuf<T, VEC>()][Cfg::NF][SKC][Cfg::JTP];
+    using Cfg = SynCfg<VEC, syn_jb<T, VEC>()>;
+    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
     // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
     // back to back on one XCD, so its table rows are read from HBM once
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
@@ -509,7 +464,7 @@ static int launch_synthesis_t(Plan *p, const T *tP, const T *tD, const T *tM, bo
     A.nthp = p->nthp;
     A.nplanes = nplanes;
     A.queue = p->d_syn_q;
-    const int JT = vec ? SynCfg<true>::JT : SynCfg<false>::JT;
+    const int JT = vec ? SynCfg<true, syn_jb<T, true>()>::JT : SynCfg<false, syn_jb<T, false>()>::JT;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
     if (vec)
@@ -540,6 +495,17 @@ int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q,
 // columns (2 waves x JB x 16).  The theta loop runs over the northern nodes
 // in chunks of 16.
 constexpr int ATC = 16;
+
+// 16-column blocks per wave of the one-problem kernels.  Vector tiles keep
+// one block: FP64 then fits 128 VGPRs at 4 waves per SIMD (0.54 against
+// 0.66 ms with two blocks at 2 waves, lmax 255, 256 planes) and FP32 gains
+// in the same way (0.45 against 0.50 ms); the table rows are read twice as
+// often from L2, which costs less than the latency hiding it buys.
+template <typename T, bool VEC>
+constexpr int ana_jb()
+{
+    return VEC ? 1 : 4;
+}
 
 template <bool VEC, int JBV = (VEC ? 2 : 4)>
 struct AnaCfg {
@@ -675,9 +641,9 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
 }
 
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void analysis_kernel(AnaArgs<T> A)
+__global__ __launch_bounds__(256, 4) void analysis_kernel(AnaArgs<T> A)
 {
-    using Cfg = AnaCfg<VEC>;
+    using Cfg = AnaCfg<VEC, ana_jb<T, VEC>()>;
     __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
     __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
     // per-XCD queue entry (m, l tile); the plane tiles of one entry run back
@@ -713,7 +679,7 @@ static int launch_analysis_t(Plan *p, const T *tP, const T *tD, const T *tM, boo
     A.nth = p->nth;
     A.nthp16 = p->nthp16;
     A.nplanes = nplanes;
-    const int JT = vec ? AnaCfg<true>::JT : AnaCfg<false>::JT;
+    const int JT = vec ? AnaCfg<true, ana_jb<T, true>()>::JT : AnaCfg<false, ana_jb<T, false>()>::JT;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->ana_qlen * A.n_jt));
     if (vec)
@@ -754,7 +720,7 @@ __global__ __launch_bounds__(256) void batch_synthesis_kernel(const SynArgs<T> *
                                                               int nitems, int *counter)
 {
     using Cfg = SynCfg<false, JBV>;
-    __shared__ T Bs[syn_nbuf<T, false>()][Cfg::NF][SKC][Cfg::JTP];
+    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);

@@ -152,7 +152,9 @@ void free_precision_tables(Plan *p)
 // ------------------------------------------------------------- synthesis
 // Same tiling as synthesis_kernel: one m, 64 northern nodes (16 per wave),
 // JT real columns; the l loop in chunks of 32 (one K=16 MFMA per parity).
-// LDS holds the chunk as [field][parity][column][l_p], hi and lo.
+// LDS holds the chunk as [field][parity][column][l_p], hi and lo.  Vector
+// tiles take one 16-column block per wave: 90 VGPRs and 5 waves per SIMD
+// (0.45 against 0.55 ms with two blocks at 2 waves, lmax 255, 256 planes).
 constexpr int BP = 20;   // column pitch in BF16 elements (16 + 4, 8-byte aligned)
 
 struct Bf3Args {
@@ -167,9 +169,9 @@ struct Bf3Args {
 };
 
 template <bool VEC>
-__global__ __launch_bounds__(256) void synthesis_bf3_kernel(Bf3Args A)
+__global__ __launch_bounds__(256, VEC ? 4 : 1) void synthesis_bf3_kernel(Bf3Args A)
 {
-    constexpr int NF = VEC ? 3 : 1, JB = VEC ? 2 : 4, JT = 16 * JB;
+    constexpr int NF = VEC ? 3 : 1, JB = VEC ? 1 : 4, JT = 16 * JB;
     __shared__ __align__(16) unsigned short Bh[NF][2][JT][BP];
     __shared__ __align__(16) unsigned short Bl[NF][2][JT][BP];
 
@@ -287,10 +289,12 @@ __global__ __launch_bounds__(256) void synthesis_bf3_kernel(Bf3Args A)
 // Same tiling as analysis_kernel: one m, 32 rows l_p (2 waves x 16), JT real
 // columns; the theta loop in chunks of 16 (one K=16 MFMA).  LDS holds the
 // north +- south combinations as [field][sum/dif][column][theta], hi and lo.
+// Vector tiles take one 16-column block per wave, as in synthesis (0.39
+// against 0.54 ms with two blocks).
 template <bool VEC>
-__global__ __launch_bounds__(256) void analysis_bf3_kernel(Bf3Args A)
+__global__ __launch_bounds__(256, VEC ? 4 : 1) void analysis_bf3_kernel(Bf3Args A)
 {
-    constexpr int NF = VEC ? 3 : 1, JB = VEC ? 2 : 4, JT = 2 * 16 * JB;
+    constexpr int NF = VEC ? 3 : 1, JB = VEC ? 1 : 4, JT = 2 * 16 * JB;
     __shared__ __align__(16) unsigned short Gh[NF][2][JT][BP];
     __shared__ __align__(16) unsigned short Gl[NF][2][JT][BP];
 
@@ -449,7 +453,7 @@ int launch_synthesis_bf3(Plan *p, bool vec, int nplanes, const double2 *q, const
                          long ld, double2 *f0, double2 *f1, double2 *f2)
 {
     Bf3Args A = bf3_args(p, true, vec, nplanes, q, s, t, f0, f1, f2, ld);
-    const int JT = vec ? 32 : 64;
+    const int JT = vec ? 16 : 64;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
     if (vec)
@@ -464,7 +468,7 @@ int launch_analysis_bf3(Plan *p, bool vec, int nplanes, const double2 *g0, const
                         const double2 *g2, double2 *q, double2 *s, double2 *t, long ld)
 {
     Bf3Args A = bf3_args(p, false, vec, nplanes, g0, g1, g2, q, s, t, ld);
-    const int JT = vec ? 64 : 128;
+    const int JT = vec ? 32 : 128;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
     dim3 grid((unsigned)(kXcds * p->ana_qlen * A.n_jt));
     if (vec)
