@@ -234,6 +234,8 @@ int build_legendre_tables(Plan *p)
         ana_off[2 * m + 1] = elems + (long)p->nthp16 * ncol;
         elems += 2L * p->nthp16 * ncol;
     }
+    // the kernels index the tables with 32-bit element offsets
+    if (rows * (long)p->nthp >= (1L << 32) || elems >= (1L << 32)) return RD_ERR_ARG;
     p->syn_rows = rows;
     p->ana_elems = elems;
     std::vector<int> synq = synthesis_queues(L, p->nthp / 64, &p->syn_qlen);
@@ -352,24 +354,48 @@ __device__ __forceinline__ void syn_tile(
     const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
     const long lm0 = lm_index(A.lmax, m, m);
 
-    // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads
+    // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads;
+    // each thread moves NLD double2 of it.  The next chunk is loaded into
+    // registers while the current one is multiplied out of LDS.
     constexpr int PER = (JT / 2) * SKC;
-    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
-        for (int e = tid; e < NF * PER; e += 256) {
+    constexpr int NLD = NF * PER / 256;
+    static_assert(NF * PER == 256 * NLD, "chunk must split evenly over the workgroup");
+    struct { T x, y; } r[NLD];   // held in T: the FP32 form keeps half the registers
+    auto load = [&](int lbase) {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e = tid + 256 * i;
             const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
             const int l = lbase + ll, k = k0 + pl;
             double2 v = make_double2(0.0, 0.0);
             if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-            Bs[f][ll][2 * pl] = (T)v.x;
-            Bs[f][ll][2 * pl + 1] = (T)v.y;
+            r[i].x = (T)v.x;
+            r[i].y = (T)v.y;
         }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e = tid + 256 * i;
+            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+            Bs[f][ll][2 * pl] = r[i].x;
+            Bs[f][ll][2 * pl + 1] = r[i].y;
+        }
+    };
+    load(m);
+    for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
+        store();
         __syncthreads();
+        if (lbase + SKC <= A.lmax) load(lbase + SKC);
         const int rowbase = (lbase - m) >> 1;
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++) {
             const int row = rowbase + 4 * st + kk;
-            const long ie = offe + (long)row * A.nthp + th0 + col;
-            const long io = offo + (long)row * A.nthp + th0 + col;
+            // 32-bit element offsets from the uniform table bases (tables
+            // stay below 4 GB): one VGPR per address instead of a 64-bit
+            // pointer per table and step
+            const unsigned ie = (unsigned)offe + (unsigned)(row * A.nthp + th0 + col);
+            const unsigned io = (unsigned)offo + (unsigned)(row * A.nthp + th0 + col);
             const T Pe = A.tP[ie], Po = A.tP[io];
             T De = 0, Do = 0, Me = 0, Mo = 0;
             if (VEC) {
@@ -578,14 +604,17 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
 #pragma unroll
         for (int st = 0; st < ATC / 4; st++) {
             const int tl = 4 * st + kk;
-            const long rowoff = (long)(tc + tl) * ncol + lp0 + col;
-            const T Pe = A.tP[offe + rowoff], Po = A.tP[offo + rowoff];
+            // 32-bit element offsets from the uniform table bases, as in
+            // syn_tile
+            const unsigned rowoff = (unsigned)((tc + tl) * ncol + lp0 + col);
+            const unsigned ie = (unsigned)offe + rowoff, io = (unsigned)offo + rowoff;
+            const T Pe = A.tP[ie], Po = A.tP[io];
             T De = 0, Do = 0, Me = 0, Mo = 0;
             if (VEC) {
-                De = A.tD[offe + rowoff];
-                Do = A.tD[offo + rowoff];
-                Me = A.tM[offe + rowoff];
-                Mo = A.tM[offo + rowoff];
+                De = A.tD[ie];
+                Do = A.tD[io];
+                Me = A.tM[ie];
+                Mo = A.tM[io];
             }
 #pragma unroll
             for (int b = 0; b < JB; b++) {

@@ -199,27 +199,43 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void synthesis_bf3_kernel(Bf3Args
     const long offe = A.off[2 * m] * A.nthp, offo = A.off[2 * m + 1] * A.nthp;
     const long lm0 = lm_idx(A.lmax, m, m);
 
-    for (int lbase = m; lbase <= A.lmax; lbase += 32) {
-        constexpr int PER = (JT / 2) * 32;
-        for (int e = tid; e < NF * PER; e += 256) {
-            const int f = e / PER, r = e % PER;
-            const int pl = r / 32, ll = r % 32;
+    // a chunk is NF x (JT/2 planes) x 32 l's, l fastest across threads; each
+    // thread moves NLD double2 of it, and the next chunk is loaded into
+    // registers while the current one is multiplied out of LDS
+    constexpr int PER = (JT / 2) * 32, NLD = NF * PER / 256;
+    static_assert(NF * PER == 256 * NLD, "chunk must split evenly over the workgroup");
+    double2 rv[NLD];
+    auto load = [&](int lbase) {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e = tid + 256 * i;
+            const int f = e / PER, r = e % PER, pl = r / 32, ll = r % 32;
             const int l = lbase + ll, k = k0 + pl;
-            double2 v = make_double2(0.0, 0.0);
-            if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
+            rv[i] = make_double2(0.0, 0.0);
+            if (l <= A.lmax && k < A.nplanes) rv[i] = A.in[f][lm0 + (l - m) + k * A.ld];
+        }
+    };
+    load(m);
+    for (int lbase = m; lbase <= A.lmax; lbase += 32) {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e = tid + 256 * i;
+            const int f = e / PER, r = e % PER, pl = r / 32, ll = r % 32;
             const int par = ll & 1, lp = ll >> 1;
             unsigned short h, lo;
-            split_bf16(v.x, h, lo);
+            split_bf16(rv[i].x, h, lo);
             Bh[f][par][2 * pl][lp] = h;
             Bl[f][par][2 * pl][lp] = lo;
-            split_bf16(v.y, h, lo);
+            split_bf16(rv[i].y, h, lo);
             Bh[f][par][2 * pl + 1][lp] = h;
             Bl[f][par][2 * pl + 1][lp] = lo;
         }
         __syncthreads();
-        const long rg = (((lbase - m) >> 1) >> 2) + kk;   // group of 4 rows
-        const long ie = offe + rg * 4L * A.nthp + 4L * (th0 + col);
-        const long io = offo + rg * 4L * A.nthp + 4L * (th0 + col);
+        if (lbase + 32 <= A.lmax) load(lbase + 32);
+        // 32-bit element offsets from the uniform table bases
+        const int rg = (((lbase - m) >> 1) >> 2) + kk;   // group of 4 rows
+        const unsigned ie = (unsigned)offe + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
+        const unsigned io = (unsigned)offo + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
         const v4s Peh = *(const v4s *)(A.th_[0] + ie), Pel = *(const v4s *)(A.tl_[0] + ie);
         const v4s Poh = *(const v4s *)(A.th_[0] + io), Pol = *(const v4s *)(A.tl_[0] + io);
         v4s Deh = {}, Del = {}, Doh = {}, Dol = {}, Meh = {}, Mel = {}, Moh = {}, Mol = {};
