@@ -633,6 +633,16 @@ __device__ __forceinline__ double2 twq(const double2 *tw, int t, int sign)
 // padding; twiddles come from a quarter table of the 768 roots.
 constexpr int PHI_BUF = 9 * 256;
 
+// Orders LDS accesses among the lanes of one wave: a wave's LDS instructions
+// complete in issue order, so a compiler fence at wavefront scope is enough
+// when every lane that reads a slot is in the wave that wrote it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                               const double2 *tw)
 {
@@ -687,33 +697,40 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     }
     __syncthreads();
     // ---- P1: T[xr][b1][q1] = e^{2 pi i b1 q1/256} sum_b2 Y[xr][16 b2 + b1] e^{2 pi i b2 q1/16}
+    // The three sequences xr = 3x + r of field x are worked by lanes 0..47 of
+    // wave x, and P1 and P2 of them touch only slots x N .. x N + 767
+    // (P2 writes the grid order j = 3 (q1 + 16 q2) + r): wave-local ordering
+    // suffices until CR.
     {
-        const int xr = tid >> 4, b1 = tid & 15;
+        const int x = tid >> 6, r = (tid & 63) >> 4, xr = 3 * x + r, b1 = tid & 15;
+        const bool act = tid < 192 && r < 3;
         double2 v[16];
-        if (tid < 144) {
+        if (act) {
 #pragma unroll
             for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
         }
-        __syncthreads();
-        if (tid < 144) {
+        wave_lds_sync();
+        if (act) {
             dft16(v, +1);
 #pragma unroll
             for (int q1 = 0; q1 < 16; q1++)
                 B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // ---- P2: grid values z_x at j = 3 (q1 + 16 q2) + r, stored at x N + j
     {
-        const int xr = tid >> 4, q1 = tid & 15;
-        const int x = xr / 3, r = xr - 3 * x;
+        int t1 = tid;   // opaque copy, as in F2
+        asm volatile("" : "+v"(t1));
+        const int x = t1 >> 6, r = (t1 & 63) >> 4, xr = 3 * x + r, q1 = t1 & 15;
+        const bool act = t1 < 192 && r < 3;
         double2 v[16];
-        if (tid < 144) {
+        if (act) {
 #pragma unroll
             for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
         }
-        __syncthreads();
-        if (tid < 144) {
+        wave_lds_sync();
+        if (act) {
             dft16(v, +1);
 #pragma unroll
             for (int q2 = 0; q2 < 16; q2++) B[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
@@ -731,7 +748,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
             o0[i] = make_double2(zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr);
             o1[i] = make_double2(zt.x * wp - zp.x * wt, 0.0);
         }
-        __syncthreads();
+        // each thread overwrites only the slots it has just read
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             B[tid + 256 * i] = o0[i];
@@ -740,36 +757,41 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     }
     __syncthreads();
     // ---- F1: F[3s+n2][a1][k1] = e^{-2 pi i a1 k1/256} sum_a2 P_s[3 (16 a2 + a1) + n2] e^{-2 pi i a2 k1/16}
+    // Packed sequence s is worked by lanes 0..47 of wave s, and F1 and F2 of
+    // it touch only slots s N .. s N + 767: wave-local ordering suffices
+    // until FIN.
     {
-        const int s = tid / 48, rem = tid % 48, n2 = rem >> 4, a1 = rem & 15;
+        const int s = tid >> 6, rem = tid & 63, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = tid < 128 && rem < 48;
         double2 v[16];
-        if (tid < 96) {
+        if (act) {
 #pragma unroll
             for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 3 * (16 * a2 + a1) + n2];
         }
-        __syncthreads();
-        if (tid < 96) {
+        wave_lds_sync();
+        if (act) {
             dft16(v, -1);
 #pragma unroll
             for (int k1 = 0; k1 < 16; k1++)
                 B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // ---- F2: Yf[3s+n2][k1 + 16 k2]
     {
         // indices recomputed from an opaque copy of tid: shared with F1 they
         // stay live across the barrier and cost a spill
         int t2 = tid;
         asm volatile("" : "+v"(t2));
-        const int s = t2 / 48, rem = t2 % 48, n2 = rem >> 4, k1 = rem & 15;
+        const int s = t2 >> 6, rem = t2 & 63, n2 = rem >> 4, k1 = rem & 15;
+        const bool act = t2 < 128 && rem < 48;
         double2 v[16];
-        if (tid < 96) {
+        if (act) {
 #pragma unroll
             for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
         }
-        __syncthreads();
-        if (tid < 96) {
+        wave_lds_sync();
+        if (act) {
             dft16(v, -1);
 #pragma unroll
             for (int k2 = 0; k2 < 16; k2++) B[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
