@@ -302,7 +302,7 @@ constexpr int SKC = 32;
 template <typename T, bool VEC>
 constexpr int syn_jb()
 {
-    return (VEC && sizeof(T) == 8) ? 1 : (VEC ? 2 : 4);
+    return VEC ? 1 : 4;
 }
 
 template <bool VEC, int JBV = (VEC ? 2 : 4)>
@@ -382,28 +382,42 @@ __device__ __forceinline__ void syn_tile(
             Bs[f][ll][2 * pl + 1] = r[i].y;
         }
     };
+    // The table values of step g (rows 4 g + kk of both parities) are
+    // loaded one step ahead, across chunk boundaries, so that their latency
+    // overlaps the MFMAs of step g - 1.  32-bit element offsets from the
+    // uniform table bases (tables stay below 4 GB): one VGPR per address
+    // instead of a 64-bit pointer per table and step.
+    struct Tab {
+        T Pe, Po, De, Do, Me, Mo;
+    };
+    auto ltab = [&](int row) {
+        Tab t;
+        const unsigned ie = (unsigned)offe + (unsigned)(row * A.nthp + th0 + col);
+        const unsigned io = (unsigned)offo + (unsigned)(row * A.nthp + th0 + col);
+        t.Pe = A.tP[ie];
+        t.Po = A.tP[io];
+        t.De = t.Do = t.Me = t.Mo = T(0);
+        if (VEC) {
+            t.De = A.tD[ie];
+            t.Do = A.tD[io];
+            t.Me = A.tM[ie];
+            t.Mo = A.tM[io];
+        }
+        return t;
+    };
+    const int nsteps = ((A.lmax - m) / SKC + 1) * (SKC / 8);
+    int g = 0;
+    Tab cur = ltab(kk);
     load(m);
     for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
         store();
         __syncthreads();
         if (lbase + SKC <= A.lmax) load(lbase + SKC);
-        const int rowbase = (lbase - m) >> 1;
 #pragma unroll
-        for (int st = 0; st < SKC / 8; st++) {
-            const int row = rowbase + 4 * st + kk;
-            // 32-bit element offsets from the uniform table bases (tables
-            // stay below 4 GB): one VGPR per address instead of a 64-bit
-            // pointer per table and step
-            const unsigned ie = (unsigned)offe + (unsigned)(row * A.nthp + th0 + col);
-            const unsigned io = (unsigned)offo + (unsigned)(row * A.nthp + th0 + col);
-            const T Pe = A.tP[ie], Po = A.tP[io];
-            T De = 0, Do = 0, Me = 0, Mo = 0;
-            if (VEC) {
-                De = A.tD[ie];
-                Do = A.tD[io];
-                Me = A.tM[ie];
-                Mo = A.tM[io];
-            }
+        for (int st = 0; st < SKC / 8; st++, g++) {
+            Tab nxt = cur;
+            if (g + 1 < nsteps) nxt = ltab(4 * (g + 1) + kk);
+            const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
             const int lle = 8 * st + 2 * kk, llo = lle + 1;
 #pragma unroll
             for (int b = 0; b < JB; b++) {
@@ -427,6 +441,7 @@ __device__ __forceinline__ void syn_tile(
                     acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
                 }
             }
+            cur = nxt;
         }
         __syncthreads();
     }
