@@ -490,23 +490,6 @@ __global__ __launch_bounds__(PHI_GEN_THREADS) void phi_dot_kernel(PhiPtrs P, int
     }
 }
 
-// Fin: (u_r, u_t, u_p, g_r, g_t, g_p) Fourier fields -> Fout = u . g.
-int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout)
-{
-    PhiPtrs P = {};
-    for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
-    P.Fo[0] = Fout;
-    size_t shm = sizeof(double2) * (6 * p->nphi + p->nphi / 2);
-    {
-        int lrc = allow_lds(phi_dot_kernel, shm);
-        if (lrc) return lrc;
-    }
-    hipLaunchKernelGGL(phi_dot_kernel, dim3(nplanes * p->nlat), dim3(PHI_GEN_THREADS), shm, p->stream, P, p->lmax,
-                       p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
-    RD_LAUNCH_CHECK();
-    return RD_OK;
-}
-
 int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f)
 {
     PhiPtrs P = {};
@@ -823,6 +806,257 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     }
 }
 
+// ----------------------------------------- phi / cross and dot, nphi = 1536
+// The same step for nphi = 6 x 16 x 16 with at most 512 stored m:
+//   inverse: m = 256 a + b, phi index j = 6 q + r.  Blocks a = 2, 3 of the
+//   padded spectrum are zero, so the radix-6 step takes four terms, fused
+//   with the loads:
+//     Y_r[b] = (Z_0 + w6^r Z_1 + w6^{4r} Z_4 + w6^{5r} Z_5) e^{2 pi i b r/1536},
+//   Z_a = Z[256 a + b]; then z[6 q + r] = IDFT_256(Y_r)[q] as 16 x 16.
+//   forward: j = 6 n1 + n2, six 256-point DFTs per packed sequence, and only
+//   the outputs m <= lmax (and their mirrors) are formed from them.
+// One row per workgroup of 512 threads, in place in one 72 KB LDS buffer,
+// two rows per CU.  DOT: the forward part transforms the one real row u . g
+// (u + i a g packed as in the cross product) instead of the packed pair
+// (N_theta + i N_phi, N_r).  A sequence's 16 lanes lie in one wave, so the
+// exchanges inside one 256-point sequence need only wave-local ordering.
+constexpr int PHI6_BUF = 18 * 256;
+
+// exp(sign 2 pi i t / 1536) from the quarter table of exp(-2 pi i t / 1536),
+// t < 1536, using w(t + 384 q) = (-i)^q w(t).
+__device__ __forceinline__ double2 tw1536(const double2 *tw, int t, int sign)
+{
+    const int q = t / 384;
+    const double2 w = tw[t - 384 * q];
+    double2 r;
+    if (q == 0) r = w;
+    else if (q == 1) r = make_double2(w.y, -w.x);
+    else if (q == 2) r = make_double2(-w.x, -w.y);
+    else r = make_double2(-w.y, w.x);
+    return sign > 0 ? make_double2(r.x, -r.y) : r;
+}
+
+template <bool DOT>
+__global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                         const double2 *tw)
+{
+    constexpr int N = 1536, NQ = 384, NS = DOT ? 1 : 2;
+    __shared__ double2 twl[NQ];
+    __shared__ double2 B[PHI6_BUF];
+    __shared__ double red[8][2];
+    __shared__ double ia_s;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    const int tid = threadIdx.x;
+    if (tid < NQ) twl[tid] = tw[tid];
+    // the row's six coefficient vectors at B[f 512 + m], m = 0..511 (zero
+    // above lmax), and the sums of squares of u and of the second factor
+    double s[2] = {0.0, 0.0};
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
+        if (tid == 0) v.y = 0.0;
+        B[f * 512 + tid] = v;
+        s[f / 3] += v.x * v.x + v.y * v.y;
+    }
+    block_sums<2, 8>(s, red);
+    const double a = wave_uniform(pack_scale(s[0], s[1]));
+    if (tid == 0) ia_s = 1.0 / a;
+
+    // ---- R6 -> Y[6x+r][b] for the items (x, b) = tid and tid + 512
+    {
+        double2 z[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = tid + 512 * h, x = i >> 8, b = i & 255;
+            if (i < 768) {
+                // packs U + i a W at m = b, 256 + b and at the mirrors of
+                // 512 - b (zero for b = 0: m = 512 is above every lmax here)
+                // and 256 - b
+                const int ms[4] = {b, 256 + b, 512 - b, 256 - b};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    double2 zz = make_double2(0.0, 0.0);
+                    if (q != 2 || b != 0) {
+                        const double2 u = B[x * 512 + ms[q]], w = cscale(B[(3 + x) * 512 + ms[q]], a);
+                        zz = q < 2 ? make_double2(u.x - w.y, u.y + w.x) : make_double2(u.x + w.y, -u.y + w.x);
+                    }
+                    z[h][q] = zz;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = tid + 512 * h, x = i >> 8, b = i & 255;
+            if (i < 768) {
+#pragma unroll
+                for (int r = 0; r < 6; r++) {
+                    const double2 w1 = tw1536(twl, 256 * r, +1), w4 = tw1536(twl, (1024 * r) % N, +1);
+                    const double2 w5 = tw1536(twl, (1280 * r) % N, +1);
+                    const double2 y = cadd(cadd(z[h][0], cmul(z[h][1], w1)), cadd(cmul(z[h][2], w4), cmul(z[h][3], w5)));
+                    B[(6 * x + r) * 256 + b] = r == 0 ? y : cmul(y, tw1536(twl, b * r, +1));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- P1: T[xr][b1][q1] = e^{2 pi i b1 q1/256} sum_b2 Y[xr][16 b2 + b1] e^{2 pi i b2 q1/16}
+    {
+        const int xr = tid >> 4, b1 = tid & 15;
+        const bool act = tid < 288;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q1 = 0; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], tw1536(twl, 6 * b1 * q1, +1));
+        }
+    }
+    wave_lds_sync();
+    // ---- P2: grid values z_x at j = 6 (q1 + 16 q2) + r, stored at x N + j
+    {
+        int t1 = tid;   // opaque copy: shared with P1 the indices stay live and spill
+        asm volatile("" : "+v"(t1));
+        const int xr = t1 >> 4, q1 = t1 & 15, x = xr / 6, r = xr - 6 * x;
+        const bool act = t1 < 288;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
+        }
+        __syncthreads();
+        if (act) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q2 = 0; q2 < 16; q2++) B[x * N + 6 * (q1 + 16 * q2) + r] = v[q2];
+        }
+    }
+    __syncthreads();
+    // ---- CR: each thread overwrites only the slots it reads
+    {
+        const double ia = ia_s;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int j = tid + 512 * i;
+            const double2 zr = B[j], zt = B[N + j], zp = B[2 * N + j];
+            const double wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
+            if (DOT) {
+                B[j] = make_double2(zr.x * wr + zt.x * wt + zp.x * wp, 0.0);
+            } else {
+                B[j] = make_double2(zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr);
+                B[N + j] = make_double2(zt.x * wp - zp.x * wt, 0.0);
+            }
+        }
+    }
+    __syncthreads();
+    // ---- F1: F[6s+n2][a1][k1] = e^{-2 pi i a1 k1/256} sum_a2 P_s[6 (16 a2 + a1) + n2] e^{-2 pi i a2 k1/16}
+    {
+        const int s = tid / 96, rem = tid % 96, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = tid < 96 * NS;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 6 * (16 * a2 + a1) + n2];
+        }
+        __syncthreads();
+        if (act) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++)
+                B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], tw1536(twl, 6 * a1 * k1, -1));
+        }
+    }
+    wave_lds_sync();
+    // ---- F2: Yf[6s+n2][k1 + 16 k2]
+    {
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        const int s = t2 / 96, rem = t2 % 96, n2 = rem >> 4, k1 = rem & 15;
+        const bool act = t2 < 96 * NS;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k2 = 0; k2 < 16; k2++) B[(s * 6 + n2) * 256 + k1 + 16 * k2] = v[k2];
+        }
+    }
+    __syncthreads();
+    // ---- FIN: X[m] = sum_n2 e^{-2 pi i n2 m/1536} Yf[n2][m mod 256]
+    const int m = tid;
+    if (m < L) {
+        const int mm = m & 255, mneg = (256 - mm) & 255;
+        double2 c[6];
+#pragma unroll
+        for (int n2 = 0; n2 < 6; n2++) c[n2] = tw1536(twl, (n2 * m) % N, -1);
+        double2 xp = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int n2 = 0; n2 < 6; n2++) xp = cadd(xp, cmul(B[n2 * 256 + mm], c[n2]));
+        const long o = ((long)k * L + m) * nlat + th;
+        if (DOT) {
+            double2 x = cscale(xp, 2.0 * M_PI / N);
+            if (m == 0) x.y = 0.0;
+            P.Fo[0][o] = x;
+        } else {
+            const double sc = 0.5 * (2.0 * M_PI / N);
+            double2 x1 = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int n2 = 0; n2 < 6; n2++) {
+                x1 = cadd(x1, cmul(B[(6 + n2) * 256 + mm], c[n2]));
+                xm = cadd(xm, cmul(B[n2 * 256 + mneg], cconj(c[n2])));
+            }
+            if (m == 0) xm = xp;
+            double2 gt = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
+            double2 gp = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
+            double2 gr = cscale(x1, 2.0 * sc);
+            if (m == 0) {
+                gr.y = 0.0;
+                gt.y = 0.0;
+                gp.y = 0.0;
+            }
+            P.Fo[0][o] = gr;
+            P.Fo[1][o] = gt;
+            P.Fo[2][o] = gp;
+        }
+    }
+}
+
+// Fin: (u_r, u_t, u_p, g_r, g_t, g_p) Fourier fields -> Fout = u . g.
+int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout)
+{
+    PhiPtrs P = {};
+    for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
+    P.Fo[0] = Fout;
+    if (p->nphi == 1536 && p->lmax <= 511) {
+        hipLaunchKernelGGL(phi1536_kernel<true>, dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P, p->lmax,
+                           p->nlat, nplanes, p->tw_phi);
+        RD_LAUNCH_CHECK();
+        return RD_OK;
+    }
+    size_t shm = sizeof(double2) * (6 * p->nphi + p->nphi / 2);
+    {
+        int lrc = allow_lds(phi_dot_kernel, shm);
+        if (lrc) return lrc;
+    }
+    hipLaunchKernelGGL(phi_dot_kernel, dim3(nplanes * p->nlat), dim3(PHI_GEN_THREADS), shm, p->stream, P, p->lmax,
+                       p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
 int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout)
 {
     PhiPtrs P = {};
@@ -830,6 +1064,12 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
     for (int i = 0; i < 3; i++) P.Fo[i] = Fout[i];
     if (p->nphi == 768 && p->lmax <= 255) {
         hipLaunchKernelGGL(phi_cross768_kernel, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P, p->lmax,
+                           p->nlat, nplanes, p->tw_phi);
+        RD_LAUNCH_CHECK();
+        return RD_OK;
+    }
+    if (p->nphi == 1536 && p->lmax <= 511) {
+        hipLaunchKernelGGL(phi1536_kernel<false>, dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P, p->lmax,
                            p->nlat, nplanes, p->tw_phi);
         RD_LAUNCH_CHECK();
         return RD_OK;

@@ -40,6 +40,19 @@ def rell2_joint(xs, ys):
     return (num / max(den, 1e-300)) ** 0.5
 
 
+def field_bounds(refs, floors, tol, factor=30.0):
+    """Per-field relative-L2 bounds for outputs refs (one group) against an
+    FP64 reference: tol, or factor x the field's noise floor, or the relative
+    error at which the field's absolute error reaches a tenth of the group's
+    joint tolerance, whichever is largest.  The last term covers fields far
+    smaller than their group (the toroidal part of a near-gradient u x w is
+    1e4 times smaller than the poloidal part at lmax 300): their rounding
+    noise is that of the group, which one-ulp input noise does not show."""
+    norms = [float(np.linalg.norm(np.asarray(r).ravel())) for r in refs]
+    group = sum(x * x for x in norms) ** 0.5
+    return [max(tol, factor * f, 0.1 * tol * group / max(nm, 1e-300)) for nm, f in zip(norms, floors)]
+
+
 def noise_floor(fn, inputs, seed=0, rel=2.0 ** -52):
     """FP64 conditioning of fn at these inputs: the relative L2 change of each
     output when every input gets independent relative noise of one ulp.  Two
