@@ -1282,7 +1282,11 @@ struct UxwRadArgs {
     int ncols, n, C, d;
 };
 
-constexpr int UTC = 4;   // lm columns per workgroup in the fused radial step
+// lm columns per workgroup in the fused radial step.  One column keeps the
+// workgroup's LDS at 7 n double2 (31 KB at n = 281), so five workgroups fit
+// on a CU; four columns (112 KB) allowed one, and the lmax = 511 coupled step
+// took 92 ms instead of 74.
+constexpr int UTC = 1;
 
 // Columns c of a batch held in LDS at X + c*n: rows nr..n-1 from the FC-Gram
 // continuation of rows 0..d-1 and nr-d..nr-1.

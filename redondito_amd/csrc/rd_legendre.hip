@@ -327,7 +327,11 @@ struct SynArgs {
 // One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
 // T = double: the FP64 path; T = float: the FP32 precision form (tables and
 // operands rounded to FP32, FP32 accumulation, FP64 output).
-template <typename T, bool VEC, int JBV>
+// PF: prefetch the next spectral chunk and the next step's table values
+// into registers (the one-problem kernels).  The persistent batch kernels
+// keep PF off: the extra registers cost them more occupancy than the
+// prefetch gains (round trip 13.4 against 14.2 ms).
+template <typename T, bool VEC, int JBV, bool PF = true>
 __device__ __forceinline__ void syn_tile(
     const SynArgs<T> &A, int m, int tt, int jt,
     T (&Bs)[SynCfg<VEC, JBV>::NF][SKC][SynCfg<VEC, JBV>::JTP])
@@ -407,16 +411,44 @@ __device__ __forceinline__ void syn_tile(
     };
     const int nsteps = ((A.lmax - m) / SKC + 1) * (SKC / 8);
     int g = 0;
-    Tab cur = ltab(kk);
-    load(m);
+    Tab cur = {};
+    if constexpr (PF) {
+        cur = ltab(kk);
+        load(m);
+    }
     for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
-        store();
-        __syncthreads();
-        if (lbase + SKC <= A.lmax) load(lbase + SKC);
+        if constexpr (PF) {
+            store();
+            __syncthreads();
+            if (lbase + SKC <= A.lmax) load(lbase + SKC);
+        } else {
+            for (int e = tid; e < NF * PER; e += 256) {
+                const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+                const int l = lbase + ll, k = k0 + pl;
+                double2 v = make_double2(0.0, 0.0);
+                if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
+                Bs[f][ll][2 * pl] = (T)v.x;
+                Bs[f][ll][2 * pl + 1] = (T)v.y;
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++, g++) {
             Tab nxt = cur;
-            if (g + 1 < nsteps) nxt = ltab(4 * (g + 1) + kk);
+            if constexpr (PF) {
+                if (g + 1 < nsteps) nxt = ltab(4 * (g + 1) + kk);
+            } else {
+                const long ie = offe + (long)(4 * g + kk) * A.nthp + th0 + col;
+                const long io = offo + (long)(4 * g + kk) * A.nthp + th0 + col;
+                cur.Pe = A.tP[ie];
+                cur.Po = A.tP[io];
+                if (VEC) {
+                    cur.De = A.tD[ie];
+                    cur.Do = A.tD[io];
+                    cur.Me = A.tM[ie];
+                    cur.Mo = A.tM[io];
+                }
+            }
             const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
             const int lle = 8 * st + 2 * kk, llo = lle + 1;
 #pragma unroll
@@ -441,7 +473,7 @@ __device__ __forceinline__ void syn_tile(
                     acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
                 }
             }
-            cur = nxt;
+            if constexpr (PF) cur = nxt;
         }
         __syncthreads();
     }
@@ -774,7 +806,7 @@ __global__ __launch_bounds__(256) void batch_synthesis_kernel(const SynArgs<T> *
         if (it >= nitems) return;
         const int4 w = items[it];
         const SynArgs<T> A = probs[w.x];
-        syn_tile<T, false, JBV>(A, w.y, w.z, w.w, Bs);
+        syn_tile<T, false, JBV, false>(A, w.y, w.z, w.w, Bs);
     }
 }
 
