@@ -410,10 +410,13 @@ __device__ __forceinline__ void syn_tile(
         return t;
     };
     const int nsteps = ((A.lmax - m) / SKC + 1) * (SKC / 8);
+    // two register sets, selected by the parity of the unrolled step (four
+    // steps per chunk), so that no copy of a set in flight is needed: a copy
+    // would wait for the loads in the step that issues them
     int g = 0;
-    Tab cur = {};
+    Tab tb[2] = {};
     if constexpr (PF) {
-        cur = ltab(kk);
+        tb[0] = ltab(kk);
         load(m);
     }
     for (int lbase = m; lbase <= A.lmax; lbase += SKC) {
@@ -434,9 +437,12 @@ __device__ __forceinline__ void syn_tile(
         }
 #pragma unroll
         for (int st = 0; st < SKC / 8; st++, g++) {
-            Tab nxt = cur;
+            Tab cur = {};
             if constexpr (PF) {
-                if (g + 1 < nsteps) nxt = ltab(4 * (g + 1) + kk);
+                // the last step reloads row kk, which every table holds
+                const int gn = (g + 1 < nsteps) ? g + 1 : 0;
+                tb[(st + 1) & 1] = ltab(4 * gn + kk);
+                cur = tb[st & 1];
             } else {
                 const long ie = offe + (long)(4 * g + kk) * A.nthp + th0 + col;
                 const long io = offo + (long)(4 * g + kk) * A.nthp + th0 + col;
@@ -473,7 +479,6 @@ __device__ __forceinline__ void syn_tile(
                     acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
                 }
             }
-            if constexpr (PF) cur = nxt;
         }
         __syncthreads();
     }
@@ -602,7 +607,9 @@ struct AnaArgs {
 
 // One analysis tile (m, l tile lt, plane tile jt) of one problem; T as in
 // syn_tile.
-template <typename T, bool VEC, int JBV>
+// PF: load the next step's table values one step ahead into the second of
+// two register sets (the one-problem kernels; off for the batch kernels).
+template <typename T, bool VEC, int JBV, bool PF = true>
 __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int jt,
                                          T (&Gn)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP],
                                          T (&Gs)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP])
@@ -629,6 +636,30 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
             for (int b = 0; b < JB; b++) acc[f][s][b] = (V){0, 0, 0, 0};
 
     const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
+    struct Tab {
+        T Pe, Po, De, Do, Me, Mo;
+    };
+    // table values at theta row t (32-bit element offsets from the uniform
+    // table bases, as in syn_tile)
+    auto ltab = [&](int t) {
+        Tab v;
+        const unsigned rowoff = (unsigned)(t * ncol + lp0 + col);
+        const unsigned ie = (unsigned)offe + rowoff, io = (unsigned)offo + rowoff;
+        v.Pe = A.tP[ie];
+        v.Po = A.tP[io];
+        v.De = v.Do = v.Me = v.Mo = T(0);
+        if (VEC) {
+            v.De = A.tD[ie];
+            v.Do = A.tD[io];
+            v.Me = A.tM[ie];
+            v.Mo = A.tM[io];
+        }
+        return v;
+    };
+    const int nsteps = ((A.nth + ATC - 1) / ATC) * (ATC / 4);
+    int g = 0;
+    Tab tb[2] = {};
+    if constexpr (PF) tb[0] = ltab(kk);
 
     for (int tc = 0; tc < A.nth; tc += ATC) {
         constexpr int PER = (JT / 2) * ATC;
@@ -649,20 +680,17 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
         }
         __syncthreads();
 #pragma unroll
-        for (int st = 0; st < ATC / 4; st++) {
+        for (int st = 0; st < ATC / 4; st++, g++) {
             const int tl = 4 * st + kk;
-            // 32-bit element offsets from the uniform table bases, as in
-            // syn_tile
-            const unsigned rowoff = (unsigned)((tc + tl) * ncol + lp0 + col);
-            const unsigned ie = (unsigned)offe + rowoff, io = (unsigned)offo + rowoff;
-            const T Pe = A.tP[ie], Po = A.tP[io];
-            T De = 0, Do = 0, Me = 0, Mo = 0;
-            if (VEC) {
-                De = A.tD[ie];
-                Do = A.tD[io];
-                Me = A.tM[ie];
-                Mo = A.tM[io];
+            Tab cur;
+            if constexpr (PF) {
+                const int gn = (g + 1 < nsteps) ? g + 1 : 0;   // the last step reloads row kk
+                tb[(st + 1) & 1] = ltab(4 * gn + kk);
+                cur = tb[st & 1];
+            } else {
+                cur = ltab(tc + tl);
             }
+            const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = (wj * JB + b) * 16 + col;
@@ -826,7 +854,7 @@ __global__ __launch_bounds__(256) void batch_analysis_kernel(const AnaArgs<T> *p
         if (it >= nitems) return;
         const int4 w = items[it];
         const AnaArgs<T> A = probs[w.x];
-        ana_tile<T, false, JBV>(A, w.y, w.z, w.w, Gn, Gs);
+        ana_tile<T, false, JBV, false>(A, w.y, w.z, w.w, Gn, Gs);
     }
 }
 
