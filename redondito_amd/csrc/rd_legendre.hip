@@ -457,26 +457,29 @@ __device__ __forceinline__ void syn_tile(
             }
             const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
             const int lle = 8 * st + 2 * kk, llo = lle + 1;
+            // steps whose eight l all lie above lmax hold only padding
+            if (lbase + 8 * st <= A.lmax) {
 #pragma unroll
-            for (int b = 0; b < JB; b++) {
-                const int j = b * 16 + col;
-                acc[0][0][b] = Mf<T>::mma(Pe, Bs[0][lle][j], acc[0][0][b]);
-                acc[0][1][b] = Mf<T>::mma(Po, Bs[0][llo][j], acc[0][1][b]);
-                if (VEC) {
-                    const T Se = Bs[1][lle][j], So = Bs[1][llo][j];
-                    const T Te = Bs[2][lle][j], To = Bs[2][llo][j];
-                    const T iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
-                    const T iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
-                    // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
-                    acc[1][0][b] = Mf<T>::mma(Do, So, acc[1][0][b]);
-                    acc[1][0][b] = Mf<T>::mma(Me, iTe, acc[1][0][b]);
-                    acc[1][1][b] = Mf<T>::mma(De, Se, acc[1][1][b]);
-                    acc[1][1][b] = Mf<T>::mma(Mo, iTo, acc[1][1][b]);
-                    // v_phi: symmetric = mP_e (iS)_e - dP_o T_o, antisymmetric = mP_o (iS)_o - dP_e T_e
-                    acc[2][0][b] = Mf<T>::mma(Me, iSe, acc[2][0][b]);
-                    acc[2][0][b] = Mf<T>::mma(Do, -To, acc[2][0][b]);
-                    acc[2][1][b] = Mf<T>::mma(Mo, iSo, acc[2][1][b]);
-                    acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
+                for (int b = 0; b < JB; b++) {
+                    const int j = b * 16 + col;
+                    acc[0][0][b] = Mf<T>::mma(Pe, Bs[0][lle][j], acc[0][0][b]);
+                    acc[0][1][b] = Mf<T>::mma(Po, Bs[0][llo][j], acc[0][1][b]);
+                    if (VEC) {
+                        const T Se = Bs[1][lle][j], So = Bs[1][llo][j];
+                        const T Te = Bs[2][lle][j], To = Bs[2][llo][j];
+                        const T iSe = sgn_i * Bs[1][lle][j ^ 1], iSo = sgn_i * Bs[1][llo][j ^ 1];
+                        const T iTe = sgn_i * Bs[2][lle][j ^ 1], iTo = sgn_i * Bs[2][llo][j ^ 1];
+                        // v_theta: symmetric = dP_o S_o + mP_e (iT)_e, antisymmetric = dP_e S_e + mP_o (iT)_o
+                        acc[1][0][b] = Mf<T>::mma(Do, So, acc[1][0][b]);
+                        acc[1][0][b] = Mf<T>::mma(Me, iTe, acc[1][0][b]);
+                        acc[1][1][b] = Mf<T>::mma(De, Se, acc[1][1][b]);
+                        acc[1][1][b] = Mf<T>::mma(Mo, iTo, acc[1][1][b]);
+                        // v_phi: symmetric = mP_e (iS)_e - dP_o T_o, antisymmetric = mP_o (iS)_o - dP_e T_e
+                        acc[2][0][b] = Mf<T>::mma(Me, iSe, acc[2][0][b]);
+                        acc[2][0][b] = Mf<T>::mma(Do, -To, acc[2][0][b]);
+                        acc[2][1][b] = Mf<T>::mma(Mo, iSo, acc[2][1][b]);
+                        acc[2][1][b] = Mf<T>::mma(De, -Te, acc[2][1][b]);
+                    }
                 }
             }
         }
@@ -626,6 +629,10 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
     const T sgn_mi = (col & 1) ? T(-1) : T(1);   // (-i X)_j = sgn_mi * X_{j^1}
     const int ncol = A.ncol[m];
     const int lp0 = lt * 32 + wl * 16;
+    // a wave whose 16 rows l_p lie above both parities' row counts (the
+    // even count (L - m + 1) / 2 is the larger) holds only padding: it stages
+    // and waits at the barriers but skips the MFMAs
+    const bool live = lp0 < (L - m + 1) / 2;
 
     V acc[NF][2][JB];
 #pragma unroll
@@ -691,29 +698,31 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
                 cur = ltab(tc + tl);
             }
             const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
+            if (live) {
 #pragma unroll
-            for (int b = 0; b < JB; b++) {
-                const int j = (wj * JB + b) * 16 + col;
-                const T rn = Gn[0][tl][j], rs = Gs[0][tl][j];
-                acc[0][0][b] = Mf<T>::mma(Pe, rn + rs, acc[0][0][b]);
-                acc[0][1][b] = Mf<T>::mma(Po, rn - rs, acc[0][1][b]);
-                if (VEC) {
-                    const T tn = Gn[1][tl][j], ts = Gs[1][tl][j];
-                    const T pn = Gn[2][tl][j], ps = Gs[2][tl][j];
-                    const T mtn = sgn_mi * Gn[1][tl][j ^ 1], mts = sgn_mi * Gs[1][tl][j ^ 1];
-                    const T mpn = sgn_mi * Gn[2][tl][j ^ 1], mps = sgn_mi * Gs[2][tl][j ^ 1];
-                    // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
-                    acc[1][0][b] = Mf<T>::mma(De, tn - ts, acc[1][0][b]);
-                    acc[1][0][b] = Mf<T>::mma(Me, mpn + mps, acc[1][0][b]);
-                    // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
-                    acc[1][1][b] = Mf<T>::mma(Do, tn + ts, acc[1][1][b]);
-                    acc[1][1][b] = Mf<T>::mma(Mo, mpn - mps, acc[1][1][b]);
-                    // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
-                    acc[2][0][b] = Mf<T>::mma(Me, mtn + mts, acc[2][0][b]);
-                    acc[2][0][b] = Mf<T>::mma(De, ps - pn, acc[2][0][b]);
-                    // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
-                    acc[2][1][b] = Mf<T>::mma(Mo, mtn - mts, acc[2][1][b]);
-                    acc[2][1][b] = Mf<T>::mma(Do, -(pn + ps), acc[2][1][b]);
+                for (int b = 0; b < JB; b++) {
+                    const int j = (wj * JB + b) * 16 + col;
+                    const T rn = Gn[0][tl][j], rs = Gs[0][tl][j];
+                    acc[0][0][b] = Mf<T>::mma(Pe, rn + rs, acc[0][0][b]);
+                    acc[0][1][b] = Mf<T>::mma(Po, rn - rs, acc[0][1][b]);
+                    if (VEC) {
+                        const T tn = Gn[1][tl][j], ts = Gs[1][tl][j];
+                        const T pn = Gn[2][tl][j], ps = Gs[2][tl][j];
+                        const T mtn = sgn_mi * Gn[1][tl][j ^ 1], mts = sgn_mi * Gs[1][tl][j ^ 1];
+                        const T mpn = sgn_mi * Gn[2][tl][j ^ 1], mps = sgn_mi * Gs[2][tl][j ^ 1];
+                        // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
+                        acc[1][0][b] = Mf<T>::mma(De, tn - ts, acc[1][0][b]);
+                        acc[1][0][b] = Mf<T>::mma(Me, mpn + mps, acc[1][0][b]);
+                        // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
+                        acc[1][1][b] = Mf<T>::mma(Do, tn + ts, acc[1][1][b]);
+                        acc[1][1][b] = Mf<T>::mma(Mo, mpn - mps, acc[1][1][b]);
+                        // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
+                        acc[2][0][b] = Mf<T>::mma(Me, mtn + mts, acc[2][0][b]);
+                        acc[2][0][b] = Mf<T>::mma(De, ps - pn, acc[2][0][b]);
+                        // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
+                        acc[2][1][b] = Mf<T>::mma(Mo, mtn - mts, acc[2][1][b]);
+                        acc[2][1][b] = Mf<T>::mma(Do, -(pn + ps), acc[2][1][b]);
+                    }
                 }
             }
         }

@@ -328,6 +328,9 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void analysis_bf3_kernel(Bf3Args 
     const unsigned long long mmi = (col & 1) ? kSigns : 0ull;
     const int ncol = A.ncol[m];
     const int lp0 = lt * 32 + wl * 16;
+    // a wave whose 16 rows l_p are all padding stages and waits at the
+    // barriers but skips the table loads and MFMAs
+    const bool live = lp0 < (L - m + 1) / 2;
 
     v4f acc[NF][2][JB];
 #pragma unroll
@@ -366,49 +369,51 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void analysis_bf3_kernel(Bf3Args 
             Gl[f][1][2 * pl + 1][tl] = lo;
         }
         __syncthreads();
-        const long tg = (tc >> 2) + kk;   // group of 4 nodes
-        const long ie = offe + tg * 4L * ncol + 4L * (lp0 + col);
-        const long io = offo + tg * 4L * ncol + 4L * (lp0 + col);
-        const v4s Peh = *(const v4s *)(A.th_[0] + ie), Pel = *(const v4s *)(A.tl_[0] + ie);
-        const v4s Poh = *(const v4s *)(A.th_[0] + io), Pol = *(const v4s *)(A.tl_[0] + io);
-        v4s Deh = {}, Del = {}, Doh = {}, Dol = {}, Meh = {}, Mel = {}, Moh = {}, Mol = {};
-        if (VEC) {
-            Deh = *(const v4s *)(A.th_[1] + ie);
-            Del = *(const v4s *)(A.tl_[1] + ie);
-            Doh = *(const v4s *)(A.th_[1] + io);
-            Dol = *(const v4s *)(A.tl_[1] + io);
-            Meh = *(const v4s *)(A.th_[2] + ie);
-            Mel = *(const v4s *)(A.tl_[2] + ie);
-            Moh = *(const v4s *)(A.th_[2] + io);
-            Mol = *(const v4s *)(A.tl_[2] + io);
-        }
-#pragma unroll
-        for (int b = 0; b < JB; b++) {
-            const int j = (wj * JB + b) * 16 + col;
-#define LDG(X, f, sd, jj) (*(const v4s *)&X[f][sd][jj][4 * kk])
-            acc[0][0][b] = mma3(Peh, Pel, LDG(Gh, 0, 0, j), LDG(Gl, 0, 0, j), acc[0][0][b]);
-            acc[0][1][b] = mma3(Poh, Pol, LDG(Gh, 0, 1, j), LDG(Gl, 0, 1, j), acc[0][1][b]);
+        if (live) {
+            const long tg = (tc >> 2) + kk;   // group of 4 nodes
+            const long ie = offe + tg * 4L * ncol + 4L * (lp0 + col);
+            const long io = offo + tg * 4L * ncol + 4L * (lp0 + col);
+            const v4s Peh = *(const v4s *)(A.th_[0] + ie), Pel = *(const v4s *)(A.tl_[0] + ie);
+            const v4s Poh = *(const v4s *)(A.th_[0] + io), Pol = *(const v4s *)(A.tl_[0] + io);
+            v4s Deh = {}, Del = {}, Doh = {}, Dol = {}, Meh = {}, Mel = {}, Moh = {}, Mol = {};
             if (VEC) {
-                const v4s s1h = LDG(Gh, 1, 0, j), s1l = LDG(Gl, 1, 0, j);
-                const v4s d1h = LDG(Gh, 1, 1, j), d1l = LDG(Gl, 1, 1, j);
-                const v4s ns2h = flip(LDG(Gh, 2, 0, j), kSigns), ns2l = flip(LDG(Gl, 2, 0, j), kSigns);
-                const v4s nd2h = flip(LDG(Gh, 2, 1, j), kSigns), nd2l = flip(LDG(Gl, 2, 1, j), kSigns);
-                const v4s rs1h = flip(LDG(Gh, 1, 0, j ^ 1), mmi), rs1l = flip(LDG(Gl, 1, 0, j ^ 1), mmi);
-                const v4s rd1h = flip(LDG(Gh, 1, 1, j ^ 1), mmi), rd1l = flip(LDG(Gl, 1, 1, j ^ 1), mmi);
-                const v4s rs2h = flip(LDG(Gh, 2, 0, j ^ 1), mmi), rs2l = flip(LDG(Gl, 2, 0, j ^ 1), mmi);
-                const v4s rd2h = flip(LDG(Gh, 2, 1, j ^ 1), mmi), rd2l = flip(LDG(Gl, 2, 1, j ^ 1), mmi);
-                // S_e = dP_e dif_t + mP_e (-i) sum_p ; S_o = dP_o sum_t + mP_o (-i) dif_p
-                acc[1][0][b] = mma3(Deh, Del, d1h, d1l, acc[1][0][b]);
-                acc[1][0][b] = mma3(Meh, Mel, rs2h, rs2l, acc[1][0][b]);
-                acc[1][1][b] = mma3(Doh, Dol, s1h, s1l, acc[1][1][b]);
-                acc[1][1][b] = mma3(Moh, Mol, rd2h, rd2l, acc[1][1][b]);
-                // T_e = mP_e (-i) sum_t - dP_e dif_p ; T_o = mP_o (-i) dif_t - dP_o sum_p
-                acc[2][0][b] = mma3(Meh, Mel, rs1h, rs1l, acc[2][0][b]);
-                acc[2][0][b] = mma3(Deh, Del, nd2h, nd2l, acc[2][0][b]);
-                acc[2][1][b] = mma3(Moh, Mol, rd1h, rd1l, acc[2][1][b]);
-                acc[2][1][b] = mma3(Doh, Dol, ns2h, ns2l, acc[2][1][b]);
+                Deh = *(const v4s *)(A.th_[1] + ie);
+                Del = *(const v4s *)(A.tl_[1] + ie);
+                Doh = *(const v4s *)(A.th_[1] + io);
+                Dol = *(const v4s *)(A.tl_[1] + io);
+                Meh = *(const v4s *)(A.th_[2] + ie);
+                Mel = *(const v4s *)(A.tl_[2] + ie);
+                Moh = *(const v4s *)(A.th_[2] + io);
+                Mol = *(const v4s *)(A.tl_[2] + io);
             }
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const int j = (wj * JB + b) * 16 + col;
+#define LDG(X, f, sd, jj) (*(const v4s *)&X[f][sd][jj][4 * kk])
+                acc[0][0][b] = mma3(Peh, Pel, LDG(Gh, 0, 0, j), LDG(Gl, 0, 0, j), acc[0][0][b]);
+                acc[0][1][b] = mma3(Poh, Pol, LDG(Gh, 0, 1, j), LDG(Gl, 0, 1, j), acc[0][1][b]);
+                if (VEC) {
+                    const v4s s1h = LDG(Gh, 1, 0, j), s1l = LDG(Gl, 1, 0, j);
+                    const v4s d1h = LDG(Gh, 1, 1, j), d1l = LDG(Gl, 1, 1, j);
+                    const v4s ns2h = flip(LDG(Gh, 2, 0, j), kSigns), ns2l = flip(LDG(Gl, 2, 0, j), kSigns);
+                    const v4s nd2h = flip(LDG(Gh, 2, 1, j), kSigns), nd2l = flip(LDG(Gl, 2, 1, j), kSigns);
+                    const v4s rs1h = flip(LDG(Gh, 1, 0, j ^ 1), mmi), rs1l = flip(LDG(Gl, 1, 0, j ^ 1), mmi);
+                    const v4s rd1h = flip(LDG(Gh, 1, 1, j ^ 1), mmi), rd1l = flip(LDG(Gl, 1, 1, j ^ 1), mmi);
+                    const v4s rs2h = flip(LDG(Gh, 2, 0, j ^ 1), mmi), rs2l = flip(LDG(Gl, 2, 0, j ^ 1), mmi);
+                    const v4s rd2h = flip(LDG(Gh, 2, 1, j ^ 1), mmi), rd2l = flip(LDG(Gl, 2, 1, j ^ 1), mmi);
+                    // S_e = dP_e dif_t + mP_e (-i) sum_p ; S_o = dP_o sum_t + mP_o (-i) dif_p
+                    acc[1][0][b] = mma3(Deh, Del, d1h, d1l, acc[1][0][b]);
+                    acc[1][0][b] = mma3(Meh, Mel, rs2h, rs2l, acc[1][0][b]);
+                    acc[1][1][b] = mma3(Doh, Dol, s1h, s1l, acc[1][1][b]);
+                    acc[1][1][b] = mma3(Moh, Mol, rd2h, rd2l, acc[1][1][b]);
+                    // T_e = mP_e (-i) sum_t - dP_e dif_p ; T_o = mP_o (-i) dif_t - dP_o sum_p
+                    acc[2][0][b] = mma3(Meh, Mel, rs1h, rs1l, acc[2][0][b]);
+                    acc[2][0][b] = mma3(Deh, Del, nd2h, nd2l, acc[2][0][b]);
+                    acc[2][1][b] = mma3(Moh, Mol, rd1h, rd1l, acc[2][1][b]);
+                    acc[2][1][b] = mma3(Doh, Dol, ns2h, ns2l, acc[2][1][b]);
+                }
 #undef LDG
+            }
         }
         __syncthreads();
     }
