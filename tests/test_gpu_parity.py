@@ -1,8 +1,8 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle.
 
 Tolerances: the FP64 path must match the oracle to 1e-12 relative L2 (the
-north-star bound for the plain form); known-answer fields of the reference's
-tests are held to the levels LEEME.md records for its CPU path.
+bound set for the plain form); known-answer fields of the reference's tests
+are held to the levels LEEME.md records for its CPU path.
 """
 import numpy as np
 import pytest
