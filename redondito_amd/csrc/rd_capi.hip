@@ -669,7 +669,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     if (!rc) {
         const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
         double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
-        rc = launch_phi_cross(p, nr, Fin, Fout);
+        rc = launch_phi_cross(p, nr, Fin, Fout, prec);
     }
     MARK(4);
     if (!rc)
@@ -709,12 +709,12 @@ int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double 
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[3], F[4], F[5]);
     if (!rc) rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, S);
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[0], F[1], F[2]);
-    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3);
+    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3, prec);
     if (!rc)
         rc = launch_analysis(p, prec, true, nr, F[3], F[4], F[5], (double2 *)MQ, (double2 *)MS, (double2 *)MT,
                              p->nlm);
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[3], S[4], S[5], p->nlm, F[3], F[4], F[5]);
-    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3);
+    if (!rc) rc = launch_phi_cross(p, nr, u_b, o3, prec);
     if (!rc)
         rc = launch_analysis(p, prec, true, nr, F[3], F[4], F[5], (double2 *)NQ, (double2 *)NS, (double2 *)NT,
                              p->nlm);
@@ -722,7 +722,7 @@ int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double 
     if (!rc) rc = launch_multr(p, -1, (const double2 *)T, S[1]);
     if (!rc) rc = launch_zero_rows(p, S[2], 0, p->nz);
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[3], F[4], F[5]);
-    if (!rc) rc = launch_phi_dot(p, nr, u_b, F[3]);
+    if (!rc) rc = launch_phi_dot(p, nr, u_b, F[3], prec);
     if (!rc)
         rc = launch_analysis(p, prec, false, nr, F[3], nullptr, nullptr, (double2 *)NH, nullptr, nullptr, p->nlm);
     double *outs[7] = {NQ, NS, NT, MQ, MS, MT, NH};

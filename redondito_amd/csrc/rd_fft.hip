@@ -50,6 +50,29 @@ __device__ __forceinline__ double2 cmuli(double2 a, int sign)
 {
     return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
+// the same in FP32 (the FP32 phi kernels of the precision forms)
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cmuli(float2 a, int sign)
+{
+    return sign > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// scalar type and constructor of a complex type
+template <typename V> struct Cx;
+template <> struct Cx<double2> {
+    using S = double;
+    static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <> struct Cx<float2> {
+    using S = float;
+    static __device__ __forceinline__ float2 make(double x, double y) { return make_float2((float)x, (float)y); }
+};
 // exp(sign 2 pi i t / n) from a table of the first nh entries of
 // exp(-2 pi i t / n); nh = n/2 (even n) uses exp(-i pi) = -1 for the rest.
 __device__ __forceinline__ double2 twid(const double2 *tw, int t, int nh, int sign)
@@ -542,17 +565,19 @@ int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f
 
 // Length-16 DFT in registers, out[k] = sum_j v[j] exp(sign 2 pi i j k / 16),
 // as 4 x 4 with the w16 twiddles as constants.
-__device__ __forceinline__ void dft4_inplace(double2 &a, double2 &b, double2 &c, double2 &d, int sign)
+template <typename V>
+__device__ __forceinline__ void dft4_inplace(V &a, V &b, V &c, V &d, int sign)
 {
-    const double2 s02 = cadd(a, c), d02 = csub(a, c);
-    const double2 s13 = cadd(b, d), d13 = cmuli(csub(b, d), sign);
+    const V s02 = cadd(a, c), d02 = csub(a, c);
+    const V s13 = cadd(b, d), d13 = cmuli(csub(b, d), sign);
     a = cadd(s02, s13);
     c = csub(s02, s13);
     b = cadd(d02, d13);
     d = csub(d02, d13);
 }
 
-__device__ __forceinline__ void dft16(double2 (&v)[16], int sign)
+template <typename V>
+__device__ __forceinline__ void dft16(V (&v)[16], int sign)
 {
     // step 1: 4-point DFTs over j1 for each j0 (elements v[4 j1 + j0])
 #pragma unroll
@@ -560,8 +585,8 @@ __device__ __forceinline__ void dft16(double2 (&v)[16], int sign)
     // after step 1, v[4 k0 + j0] holds A[j0][k0]; twiddle by w16^(j0 k0)
     const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, c2 = 0.70710678118654752440;
     const double sg = sign > 0 ? 1.0 : -1.0;
-    const double2 w1 = make_double2(c1, sg * s1), w2 = make_double2(c2, sg * c2), w3 = make_double2(s1, sg * c1);
-    const double2 w6 = make_double2(-c2, sg * c2), w9 = make_double2(-c1, -sg * s1);
+    const V w1 = Cx<V>::make(c1, sg * s1), w2 = Cx<V>::make(c2, sg * c2), w3 = Cx<V>::make(s1, sg * c1);
+    const V w6 = Cx<V>::make(-c2, sg * c2), w9 = Cx<V>::make(-c1, -sg * s1);
     v[4 + 1] = cmul(v[4 + 1], w1);   // j0 = 1, k0 = 1
     v[8 + 1] = cmul(v[8 + 1], w2);   // j0 = 1, k0 = 2
     v[12 + 1] = cmul(v[12 + 1], w3); // j0 = 1, k0 = 3
@@ -575,7 +600,7 @@ __device__ __forceinline__ void dft16(double2 (&v)[16], int sign)
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) dft4_inplace(v[4 * k0], v[4 * k0 + 1], v[4 * k0 + 2], v[4 * k0 + 3], sign);
     // now v[4 k0 + k1] = out[k0 + 4 k1]; reorder to natural order
-    double2 o[16];
+    V o[16];
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++)
 #pragma unroll
@@ -820,29 +845,36 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
 // (u + i a g packed as in the cross product) instead of the packed pair
 // (N_theta + i N_phi, N_r).  A sequence's 16 lanes lie in one wave, so the
 // exchanges inside one 256-point sequence need only wave-local ordering.
+// V = float2: the row is transformed in FP32 (36 KB of LDS, three rows per CU)
+// for the BF16x3 form, whose Legendre error (6e-6) is two orders above what
+// the FP32 transforms add; loads, the pack scale and the stores stay FP64.
+// The FP32 form keeps FP64 rows: FP32 transforms raised its u x omega error at
+// lmax 511 from 4e-7 to 2e-6.
 constexpr int PHI6_BUF = 18 * 256;
 
 // exp(sign 2 pi i t / 1536) from the quarter table of exp(-2 pi i t / 1536),
 // t < 1536, using w(t + 384 q) = (-i)^q w(t).
-__device__ __forceinline__ double2 tw1536(const double2 *tw, int t, int sign)
+template <typename V>
+__device__ __forceinline__ V tw1536(const V *tw, int t, int sign)
 {
     const int q = t / 384;
-    const double2 w = tw[t - 384 * q];
-    double2 r;
+    const V w = tw[t - 384 * q];
+    V r;
     if (q == 0) r = w;
-    else if (q == 1) r = make_double2(w.y, -w.x);
-    else if (q == 2) r = make_double2(-w.x, -w.y);
-    else r = make_double2(-w.y, w.x);
-    return sign > 0 ? make_double2(r.x, -r.y) : r;
+    else if (q == 1) r = {w.y, -w.x};
+    else if (q == 2) r = {-w.x, -w.y};
+    else r = {-w.y, w.x};
+    return sign > 0 ? V{r.x, -r.y} : r;
 }
 
-template <bool DOT>
+template <typename V, bool DOT>
 __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                          const double2 *tw)
 {
+    using S = typename Cx<V>::S;
     constexpr int N = 1536, NQ = 384, NS = DOT ? 1 : 2;
-    __shared__ double2 twl[NQ];
-    __shared__ double2 B[PHI6_BUF];
+    __shared__ V twl[NQ];
+    __shared__ V B[PHI6_BUF];
     __shared__ double red[8][2];
     __shared__ double ia_s;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -850,7 +882,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = tw[tid];
+    if (tid < NQ) twl[tid] = Cx<V>::make(tw[tid].x, tw[tid].y);
     // the row's six coefficient vectors at B[f 512 + m], m = 0..511 (zero
     // above lmax), and the sums of squares of u and of the second factor
     double s[2] = {0.0, 0.0};
@@ -859,16 +891,17 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         double2 v = make_double2(0.0, 0.0);
         if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
         if (tid == 0) v.y = 0.0;
-        B[f * 512 + tid] = v;
+        B[f * 512 + tid] = Cx<V>::make(v.x, v.y);
         s[f / 3] += v.x * v.x + v.y * v.y;
     }
     block_sums<2, 8>(s, red);
     const double a = wave_uniform(pack_scale(s[0], s[1]));
+    const S as = (S)a;   // a power of two: exact in FP32 as well
     if (tid == 0) ia_s = 1.0 / a;
 
     // ---- R6 -> Y[6x+r][b] for the items (x, b) = tid and tid + 512
     {
-        double2 z[2][4];
+        V z[2][4];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int i = tid + 512 * h, x = i >> 8, b = i & 255;
@@ -879,10 +912,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
                 const int ms[4] = {b, 256 + b, 512 - b, 256 - b};
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    double2 zz = make_double2(0.0, 0.0);
+                    V zz = {S(0), S(0)};
                     if (q != 2 || b != 0) {
-                        const double2 u = B[x * 512 + ms[q]], w = cscale(B[(3 + x) * 512 + ms[q]], a);
-                        zz = q < 2 ? make_double2(u.x - w.y, u.y + w.x) : make_double2(u.x + w.y, -u.y + w.x);
+                        const V u = B[x * 512 + ms[q]], w = cscale(B[(3 + x) * 512 + ms[q]], as);
+                        zz = q < 2 ? V{u.x - w.y, u.y + w.x} : V{u.x + w.y, -u.y + w.x};
                     }
                     z[h][q] = zz;
                 }
@@ -895,9 +928,9 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
             if (i < 768) {
 #pragma unroll
                 for (int r = 0; r < 6; r++) {
-                    const double2 w1 = tw1536(twl, 256 * r, +1), w4 = tw1536(twl, (1024 * r) % N, +1);
-                    const double2 w5 = tw1536(twl, (1280 * r) % N, +1);
-                    const double2 y = cadd(cadd(z[h][0], cmul(z[h][1], w1)), cadd(cmul(z[h][2], w4), cmul(z[h][3], w5)));
+                    const V w1 = tw1536(twl, 256 * r, +1), w4 = tw1536(twl, (1024 * r) % N, +1);
+                    const V w5 = tw1536(twl, (1280 * r) % N, +1);
+                    const V y = cadd(cadd(z[h][0], cmul(z[h][1], w1)), cadd(cmul(z[h][2], w4), cmul(z[h][3], w5)));
                     B[(6 * x + r) * 256 + b] = r == 0 ? y : cmul(y, tw1536(twl, b * r, +1));
                 }
             }
@@ -908,7 +941,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     {
         const int xr = tid >> 4, b1 = tid & 15;
         const bool act = tid < 288;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
@@ -928,7 +961,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         asm volatile("" : "+v"(t1));
         const int xr = t1 >> 4, q1 = t1 & 15, x = xr / 6, r = xr - 6 * x;
         const bool act = t1 < 288;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
@@ -943,17 +976,17 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     __syncthreads();
     // ---- CR: each thread overwrites only the slots it reads
     {
-        const double ia = ia_s;
+        const S ia = (S)ia_s;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int j = tid + 512 * i;
-            const double2 zr = B[j], zt = B[N + j], zp = B[2 * N + j];
-            const double wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
+            const V zr = B[j], zt = B[N + j], zp = B[2 * N + j];
+            const S wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
             if (DOT) {
-                B[j] = make_double2(zr.x * wr + zt.x * wt + zp.x * wp, 0.0);
+                B[j] = V{zr.x * wr + zt.x * wt + zp.x * wp, S(0)};
             } else {
-                B[j] = make_double2(zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr);
-                B[N + j] = make_double2(zt.x * wp - zp.x * wt, 0.0);
+                B[j] = V{zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr};
+                B[N + j] = V{zt.x * wp - zp.x * wt, S(0)};
             }
         }
     }
@@ -962,7 +995,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     {
         const int s = tid / 96, rem = tid % 96, n2 = rem >> 4, a1 = rem & 15;
         const bool act = tid < 96 * NS;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 6 * (16 * a2 + a1) + n2];
@@ -982,7 +1015,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         asm volatile("" : "+v"(t2));
         const int s = t2 / 96, rem = t2 % 96, n2 = rem >> 4, k1 = rem & 15;
         const bool act = t2 < 96 * NS;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
@@ -999,12 +1032,13 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     const int m = tid;
     if (m < L) {
         const int mm = m & 255, mneg = (256 - mm) & 255;
-        double2 c[6];
+        V c[6];
 #pragma unroll
         for (int n2 = 0; n2 < 6; n2++) c[n2] = tw1536(twl, (n2 * m) % N, -1);
-        double2 xp = make_double2(0.0, 0.0);
+        V xpv = {S(0), S(0)};
 #pragma unroll
-        for (int n2 = 0; n2 < 6; n2++) xp = cadd(xp, cmul(B[n2 * 256 + mm], c[n2]));
+        for (int n2 = 0; n2 < 6; n2++) xpv = cadd(xpv, cmul(B[n2 * 256 + mm], c[n2]));
+        const double2 xp = make_double2(xpv.x, xpv.y);
         const long o = ((long)k * L + m) * nlat + th;
         if (DOT) {
             double2 x = cscale(xp, 2.0 * M_PI / N);
@@ -1012,12 +1046,14 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
             P.Fo[0][o] = x;
         } else {
             const double sc = 0.5 * (2.0 * M_PI / N);
-            double2 x1 = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+            V x1v = {S(0), S(0)}, xmv = {S(0), S(0)};
 #pragma unroll
             for (int n2 = 0; n2 < 6; n2++) {
-                x1 = cadd(x1, cmul(B[(6 + n2) * 256 + mm], c[n2]));
-                xm = cadd(xm, cmul(B[n2 * 256 + mneg], cconj(c[n2])));
+                x1v = cadd(x1v, cmul(B[(6 + n2) * 256 + mm], c[n2]));
+                xmv = cadd(xmv, cmul(B[n2 * 256 + mneg], cconj(c[n2])));
             }
+            const double2 x1 = make_double2(x1v.x, x1v.y);
+            double2 xm = make_double2(xmv.x, xmv.y);
             if (m == 0) xm = xp;
             double2 gt = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
             double2 gp = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
@@ -1035,14 +1071,18 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
 }
 
 // Fin: (u_r, u_t, u_p, g_r, g_t, g_p) Fourier fields -> Fout = u . g.
-int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout)
+int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout, int prec)
 {
     PhiPtrs P = {};
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
     P.Fo[0] = Fout;
     if (p->nphi == 1536 && p->lmax <= 511) {
-        hipLaunchKernelGGL(phi1536_kernel<true>, dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P, p->lmax,
-                           p->nlat, nplanes, p->tw_phi);
+        if (prec == RD_PREC_BF16X3)
+            hipLaunchKernelGGL((phi1536_kernel<float2, true>), dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P,
+                               p->lmax, p->nlat, nplanes, p->tw_phi);
+        else
+            hipLaunchKernelGGL((phi1536_kernel<double2, true>), dim3(nplanes * p->nlat), dim3(512), 0, p->stream,
+                               P, p->lmax, p->nlat, nplanes, p->tw_phi);
         RD_LAUNCH_CHECK();
         return RD_OK;
     }
@@ -1057,7 +1097,7 @@ int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fou
     return RD_OK;
 }
 
-int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout)
+int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout, int prec)
 {
     PhiPtrs P = {};
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
@@ -1069,8 +1109,12 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
         return RD_OK;
     }
     if (p->nphi == 1536 && p->lmax <= 511) {
-        hipLaunchKernelGGL(phi1536_kernel<false>, dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P, p->lmax,
-                           p->nlat, nplanes, p->tw_phi);
+        if (prec == RD_PREC_BF16X3)
+            hipLaunchKernelGGL((phi1536_kernel<float2, false>), dim3(nplanes * p->nlat), dim3(512), 0, p->stream, P,
+                               p->lmax, p->nlat, nplanes, p->tw_phi);
+        else
+            hipLaunchKernelGGL((phi1536_kernel<double2, false>), dim3(nplanes * p->nlat), dim3(512), 0, p->stream,
+                               P, p->lmax, p->nlat, nplanes, p->tw_phi);
         RD_LAUNCH_CHECK();
         return RD_OK;
     }
