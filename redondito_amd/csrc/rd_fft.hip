@@ -611,16 +611,17 @@ __device__ __forceinline__ void dft16(V (&v)[16], int sign)
 
 // exp(sign 2 pi i t / 768) from a quarter table of exp(-2 pi i t / 768),
 // t < 192, using w(t + 192 q) = (-i)^q w(t).
-__device__ __forceinline__ double2 twq(const double2 *tw, int t, int sign)
+template <typename V>
+__device__ __forceinline__ V twq(const V *tw, int t, int sign)
 {
     const int q = t / 192;
-    const double2 w = tw[t - 192 * q];
-    double2 r;
+    const V w = tw[t - 192 * q];
+    V r;
     if (q == 0) r = w;
-    else if (q == 1) r = make_double2(w.y, -w.x);
-    else if (q == 2) r = make_double2(-w.x, -w.y);
-    else r = make_double2(-w.y, w.x);
-    return sign > 0 ? make_double2(r.x, -r.y) : r;
+    else if (q == 1) r = {w.y, -w.x};
+    else if (q == 2) r = {-w.x, -w.y};
+    else r = {-w.y, w.x};
+    return sign > 0 ? V{r.x, -r.y} : r;
 }
 
 // One (plane, theta) row per workgroup, in place in a single LDS buffer so
@@ -651,12 +652,14 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <typename V>
 __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                               const double2 *tw)
 {
+    using S = typename Cx<V>::S;
     constexpr int N = 768, NQ = 192;
-    __shared__ double2 twl[NQ];
-    __shared__ double2 B[PHI_BUF];
+    __shared__ V twl[NQ];
+    __shared__ V B[PHI_BUF];
     __shared__ double red[4][2];
     __shared__ double ia_s;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -664,7 +667,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = tw[tid];
+    if (tid < NQ) twl[tid] = Cx<V>::make(tw[tid].x, tw[tid].y);
     // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax),
     // and the sums of squares of u and of w
     double s[2] = {0.0, 0.0};
@@ -673,29 +676,30 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         double2 v = make_double2(0.0, 0.0);
         if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
         if (tid == 0) v.y = 0.0;
-        B[f * 256 + tid] = v;
+        B[f * 256 + tid] = Cx<V>::make(v.x, v.y);
         s[f / 3] += v.x * v.x + v.y * v.y;
     }
     block_sums<2, 4>(s, red);
     // balanced packs z_x = u_x + i a w_x; 1/a kept in LDS for CR (read there
     // after several barriers)
     const double a = wave_uniform(pack_scale(s[0], s[1]));
+    const S as = (S)a;   // a power of two: exact in FP32 as well
     if (tid == 0) ia_s = 1.0 / a;
 
     // ---- R3 -> Y[3x+r][b]
     {
         const int b = tid, mm = (256 - b) & 255;
-        double2 zb[3], zc[3];
+        V zb[3], zc[3];
 #pragma unroll
         for (int x = 0; x < 3; x++) {
-            const double2 u = B[x * 256 + b], w = cscale(B[(3 + x) * 256 + b], a);
-            const double2 uc = B[x * 256 + mm], wc = cscale(B[(3 + x) * 256 + mm], a);
-            zb[x] = make_double2(u.x - w.y, u.y + w.x);
-            zc[x] = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
+            const V u = B[x * 256 + b], w = cscale(B[(3 + x) * 256 + b], as);
+            const V uc = B[x * 256 + mm], wc = cscale(B[(3 + x) * 256 + mm], as);
+            zb[x] = V{u.x - w.y, u.y + w.x};
+            zc[x] = b == 0 ? V{S(0), S(0)} : V{uc.x + wc.y, -uc.y + wc.x};
         }
         __syncthreads();
-        const double2 w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
-        const double2 t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
+        const V w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
+        const V t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
 #pragma unroll
         for (int x = 0; x < 3; x++) {
             B[(3 * x) * 256 + b] = cadd(zb[x], zc[x]);
@@ -712,7 +716,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     {
         const int x = tid >> 6, r = (tid & 63) >> 4, xr = 3 * x + r, b1 = tid & 15;
         const bool act = tid < 192 && r < 3;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
@@ -732,7 +736,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         asm volatile("" : "+v"(t1));
         const int x = t1 >> 6, r = (t1 & 63) >> 4, xr = 3 * x + r, q1 = t1 & 15;
         const bool act = t1 < 192 && r < 3;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
@@ -747,14 +751,14 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     __syncthreads();
     // ---- CR: (N_theta + i N_phi) at j, N_r at N + j
     {
-        double2 o0[3], o1[3];
+        V o0[3], o1[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int j = tid + 256 * i;
-            const double2 zr = B[j], zt = B[N + j], zp = B[2 * N + j];
-            const double ia = ia_s, wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
-            o0[i] = make_double2(zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr);
-            o1[i] = make_double2(zt.x * wp - zp.x * wt, 0.0);
+            const V zr = B[j], zt = B[N + j], zp = B[2 * N + j];
+            const S ia = (S)ia_s, wr = zr.y * ia, wt = zt.y * ia, wp = zp.y * ia;
+            o0[i] = V{zp.x * wr - zr.x * wp, zr.x * wt - zt.x * wr};
+            o1[i] = V{zt.x * wp - zp.x * wt, S(0)};
         }
         // each thread overwrites only the slots it has just read
 #pragma unroll
@@ -771,7 +775,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     {
         const int s = tid >> 6, rem = tid & 63, n2 = rem >> 4, a1 = rem & 15;
         const bool act = tid < 128 && rem < 48;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 3 * (16 * a2 + a1) + n2];
@@ -793,7 +797,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         asm volatile("" : "+v"(t2));
         const int s = t2 >> 6, rem = t2 & 63, n2 = rem >> 4, k1 = rem & 15;
         const bool act = t2 < 128 && rem < 48;
-        double2 v[16];
+        V v[16];
         if (act) {
 #pragma unroll
             for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
@@ -811,10 +815,12 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     const int m = tid;
     if (m < L) {
         const int mneg = (256 - m) & 255;
-        const double2 c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
-        const double2 xp = cadd(B[m], cadd(cmul(B[256 + m], c1), cmul(B[512 + m], c2)));
-        const double2 x1 = cadd(B[768 + m], cadd(cmul(B[1024 + m], c1), cmul(B[1280 + m], c2)));
-        double2 xm = cadd(B[mneg], cadd(cmul(B[256 + mneg], cconj(c1)), cmul(B[512 + mneg], cconj(c2))));
+        const V c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
+        const V xpv = cadd(B[m], cadd(cmul(B[256 + m], c1), cmul(B[512 + m], c2)));
+        const V x1v = cadd(B[768 + m], cadd(cmul(B[1024 + m], c1), cmul(B[1280 + m], c2)));
+        const V xmv = cadd(B[mneg], cadd(cmul(B[256 + mneg], cconj(c1)), cmul(B[512 + mneg], cconj(c2))));
+        const double2 xp = make_double2(xpv.x, xpv.y), x1 = make_double2(x1v.x, x1v.y);
+        double2 xm = make_double2(xmv.x, xmv.y);
         if (m == 0) xm = xp;
         double2 gt = make_double2(sc * (xp.x + xm.x), sc * (xp.y - xm.y));
         double2 gp = make_double2(sc * (xp.y + xm.y), -sc * (xp.x - xm.x));
@@ -1103,8 +1109,12 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
     for (int i = 0; i < 3; i++) P.Fo[i] = Fout[i];
     if (p->nphi == 768 && p->lmax <= 255) {
-        hipLaunchKernelGGL(phi_cross768_kernel, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P, p->lmax,
-                           p->nlat, nplanes, p->tw_phi);
+        if (prec == RD_PREC_BF16X3)
+            hipLaunchKernelGGL(phi_cross768_kernel<float2>, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P,
+                               p->lmax, p->nlat, nplanes, p->tw_phi);
+        else
+            hipLaunchKernelGGL(phi_cross768_kernel<double2>, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P,
+                               p->lmax, p->nlat, nplanes, p->tw_phi);
         RD_LAUNCH_CHECK();
         return RD_OK;
     }

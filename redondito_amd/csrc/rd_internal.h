@@ -134,7 +134,7 @@ int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *
 int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f, double2 *const *F);
 // Fused: six Fourier fields (u_r,u_t,u_p,w_r,w_t,w_p) -> u x w -> three Fourier fields.
 // prec RD_PREC_BF16X3 transforms the rows in FP32 where an FP32 kernel exists
-// (nphi = 1536); every other form in FP64.
+// (nphi = 768 and 1536); every other form in FP64.
 int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *const *Fout, int prec);
 // Fused: (u_r,u_t,u_p,g_r,g_t,g_p) -> u . g -> one Fourier field.
 int launch_phi_dot(Plan *p, int nplanes, const double2 *const *Fin, double2 *Fout, int prec);

@@ -142,8 +142,9 @@ def test_coupled_large_closed_forms(large):
 
 
 def test_coupled_large_precision_forms(large):
-    """Seeded coupled state at the large shape: the FP32 and BF16x3 Legendre
-    forms against the FP64 step, per output field and per degree band."""
+    """Seeded coupled state at the large shape: the FP32, BF16x3 (with FP32 phi
+    rows) and int8 digit Legendre forms against the FP64 step, per output field
+    and per degree band."""
     plan, lmax, nr, n = large["plan"], large["lmax"], large["nr"], large["n"]
     print(f"case large precision forms lmax={lmax} nr={nr}", flush=True)
     ins = H.coupled_fields(lmax, nr, n, RI, RO, seed=20261207)
