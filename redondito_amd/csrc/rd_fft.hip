@@ -640,6 +640,8 @@ __device__ __forceinline__ V twq(const V *tw, int t, int sign)
 //       real spectra, scaling and store.
 // The 16 x 16 transposes use an XOR swizzle (column q ^ row) instead of
 // padding; twiddles come from a quarter table of the 768 roots.
+// V = float2 transforms the row in FP32 (20 KB of LDS) for the BF16x3 form;
+// loads, the pack scale and the stores stay FP64.
 constexpr int PHI_BUF = 9 * 256;
 
 // Orders LDS accesses among the lanes of one wave: a wave's LDS instructions
