@@ -1516,7 +1516,9 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
     const int tid = threadIdx.x;
     for (int t = tid; t < N; t += 256) twl[t] = tw[t];
     const int nr = N - A.C;
-    const int c0 = blockIdx.x * CB;
+    // consecutive column blocks on one XCD: a 128-byte line shared by two
+    // blocks' row segments (CB x 16 bytes) is then fetched into one L2 only
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * CB;
     const bool inA = tid < CB * N2, inB = tid < CB * N1;
     const int ca = tid % CB, n2 = tid / CB;   // pass A
     const int cb = tid % CB, k1 = tid / CB;   // pass B
