@@ -513,43 +513,6 @@ __global__ __launch_bounds__(PHI_GEN_THREADS) void phi_dot_kernel(PhiPtrs P, int
     }
 }
 
-int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f)
-{
-    PhiPtrs P = {};
-    for (int i = 0; i < nfields; i++) {
-        P.F[i] = F[i];
-        P.fo[i] = f[i];
-    }
-    const int ntt = (p->nlat + 3) / 4;
-    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
-    {
-        int lrc = allow_lds(phi_inverse_kernel, shm);
-        if (lrc) return lrc;
-    }
-    hipLaunchKernelGGL(phi_inverse_kernel, dim3(nplanes * ntt, nfields), dim3(256), shm, p->stream, P,
-                       p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
-    RD_LAUNCH_CHECK();
-    return RD_OK;
-}
-
-int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f, double2 *const *F)
-{
-    PhiPtrs P = {};
-    for (int i = 0; i < nfields; i++) {
-        P.f[i] = f[i];
-        P.Fo[i] = F[i];
-    }
-    const int ntt = (p->nlat + 3) / 4;
-    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
-    {
-        int lrc = allow_lds(phi_forward_kernel, shm);
-        if (lrc) return lrc;
-    }
-    hipLaunchKernelGGL(phi_forward_kernel, dim3(nplanes * ntt, nfields), dim3(256), shm, p->stream, P,
-                       p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
-    RD_LAUNCH_CHECK();
-    return RD_OK;
-}
 
 // ------------------------------------------------ phi / cross, nphi = 768
 // The same step for nphi = 3 x 16 x 16 with at most 256 stored m:
@@ -839,6 +802,203 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     }
 }
 
+// --------------------------------------- scalar phi transforms, nphi = 768
+// Four theta rows of one plane per workgroup (the tiling of phi_inverse_kernel
+// and phi_forward_kernel), packed in pairs (row 2p + i row 2p+1) into two
+// complex 768-point transforms that follow the steps of phi_cross768_kernel:
+// the radix-3 step on the loads (block 256..511 of the padded spectrum is
+// zero) and 16 x 16 register passes.  Packed sequence p runs on the lanes of
+// wave p, so only the phases that read across packed sequences need
+// workgroup barriers.  Loads and stores cover four adjacent theta.
+// F[k][m][theta] -> f[k][phi][theta]; blockIdx.y selects the field.
+__global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                             const double2 *tw)
+{
+    constexpr int N = 768, NQ = 192;
+    __shared__ double2 twl[NQ];
+    __shared__ double2 B[6 * 256];
+    const int ntt = (nlat + 3) / 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / ntt, th0 = (t % ntt) * 4;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    const double2 *F = P.F[blockIdx.y];
+    double *f = P.fo[blockIdx.y];
+    const int tid = threadIdx.x;
+    if (tid < NQ) twl[tid] = tw[tid];
+    // coefficient m = tid of rows th0 .. th0+3 at B[row 256 + m]
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (tid <= lmax && th0 + r < nlat) v = F[((long)k * L + tid) * nlat + th0 + r];
+        if (tid == 0) v.y = 0.0;
+        B[r * 256 + tid] = v;
+    }
+    __syncthreads();
+    // ---- R3 -> Y[3p+r][b]
+    {
+        const int b = tid, mm = (256 - b) & 255;
+        double2 zb[2], zc[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const double2 u = B[(2 * p) * 256 + b], w = B[(2 * p + 1) * 256 + b];
+            const double2 uc = B[(2 * p) * 256 + mm], wc = B[(2 * p + 1) * 256 + mm];
+            zb[p] = make_double2(u.x - w.y, u.y + w.x);
+            zc[p] = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
+        }
+        __syncthreads();
+        const double2 w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
+        const double2 t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            B[(3 * p) * 256 + b] = cadd(zb[p], zc[p]);
+            B[(3 * p + 1) * 256 + b] = cmul(cadd(zb[p], cmul(zc[p], w3a)), t1);
+            B[(3 * p + 2) * 256 + b] = cmul(cadd(zb[p], cmul(zc[p], w3b)), t2);
+        }
+    }
+    __syncthreads();
+    // ---- P1 (packed sequence x on lanes 0..47 of wave x)
+    {
+        const int x = tid >> 6, r = (tid & 63) >> 4, xr = 3 * x + r, b1 = tid & 15;
+        const bool act = tid < 128 && r < 3;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int b2 = 0; b2 < 16; b2++) v[b2] = B[xr * 256 + 16 * b2 + b1];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q1 = 0; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
+        }
+    }
+    wave_lds_sync();
+    // ---- P2: z_x at j = 3 (q1 + 16 q2) + r, stored at x N + j
+    {
+        int t1 = tid;   // opaque copy: shared with P1 the indices stay live
+        asm volatile("" : "+v"(t1));
+        const int x = t1 >> 6, r = (t1 & 63) >> 4, xr = 3 * x + r, q1 = t1 & 15;
+        const bool act = t1 < 128 && r < 3;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int b1 = 0; b1 < 16; b1++) v[b1] = B[(xr * 16 + b1) * 16 + (q1 ^ b1)];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, +1);
+#pragma unroll
+            for (int q2 = 0; q2 < 16; q2++) B[x * N + 3 * (q1 + 16 * q2) + r] = v[q2];
+        }
+    }
+    __syncthreads();
+    // ---- rows th0 + 2p (real part) and th0 + 2p + 1 (imaginary part)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = tid + 256 * i;
+        const double2 z0 = B[j], z1 = B[N + j];
+        const long o = ((long)k * N + j) * nlat + th0;
+        if (th0 < nlat) f[o] = z0.x;
+        if (th0 + 1 < nlat) f[o + 1] = z0.y;
+        if (th0 + 2 < nlat) f[o + 2] = z1.x;
+        if (th0 + 3 < nlat) f[o + 3] = z1.y;
+    }
+}
+
+// f[k][phi][theta] -> F[k][m][theta] = (2 pi / 768) sum_j f e^{-i m phi_j}, m <= lmax,
+// four theta rows per workgroup as in phi_inverse768_kernel.
+__global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                             const double2 *tw)
+{
+    constexpr int N = 768, NQ = 192;
+    __shared__ double2 twl[NQ];
+    __shared__ double2 B[6 * 256];
+    const int ntt = (nlat + 3) / 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / ntt, th0 = (t % ntt) * 4;
+    if (k >= nplanes) return;
+    const int L = lmax + 1;
+    const double *f = P.f[blockIdx.y];
+    double2 *F = P.Fo[blockIdx.y];
+    const int tid = threadIdx.x;
+    if (tid < NQ) twl[tid] = tw[tid];
+    // packed rows z_p[j] = row th0+2p + i row th0+2p+1 at B[p N + j]
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = tid + 256 * i;
+        const long o = ((long)k * N + j) * nlat + th0;
+        double v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = th0 + r < nlat ? f[o + r] : 0.0;
+        B[j] = make_double2(v[0], v[1]);
+        B[N + j] = make_double2(v[2], v[3]);
+    }
+    __syncthreads();
+    // ---- F1 (packed sequence s on lanes 0..47 of wave s)
+    {
+        const int s = tid >> 6, rem = tid & 63, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = tid < 128 && rem < 48;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int a2 = 0; a2 < 16; a2++) v[a2] = B[s * N + 3 * (16 * a2 + a1) + n2];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++)
+                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
+        }
+    }
+    wave_lds_sync();
+    // ---- F2: Yf[3s+n2][k1 + 16 k2]
+    {
+        int t2 = tid;   // opaque copy, as in phi_cross768_kernel
+        asm volatile("" : "+v"(t2));
+        const int s = t2 >> 6, rem = t2 & 63, n2 = rem >> 4, k1 = rem & 15;
+        const bool act = t2 < 128 && rem < 48;
+        double2 v[16];
+        if (act) {
+#pragma unroll
+            for (int a1 = 0; a1 < 16; a1++) v[a1] = B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)];
+        }
+        wave_lds_sync();
+        if (act) {
+            dft16(v, -1);
+#pragma unroll
+            for (int k2 = 0; k2 < 16; k2++) B[(s * 3 + n2) * 256 + k1 + 16 * k2] = v[k2];
+        }
+    }
+    __syncthreads();
+    // ---- X[m] = sum_n2 e^{-2 pi i n2 m/768} Yf[n2][m], unpacked into the two rows
+    const int m = tid;
+    if (m < L) {
+        const int mneg = (256 - m) & 255;
+        const double2 c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
+        const double sc = 0.5 * (2.0 * M_PI / N);
+        const long base = ((long)k * L + m) * nlat + th0;
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const double2 *Y = B + 3 * p * 256;
+            const double2 zp = cadd(Y[m], cadd(cmul(Y[256 + m], c1), cmul(Y[512 + m], c2)));
+            double2 zm = cadd(Y[mneg], cadd(cmul(Y[256 + mneg], cconj(c1)), cmul(Y[512 + mneg], cconj(c2))));
+            if (m == 0) zm = zp;
+            // X1 = (Z[m] + conj Z[-m]) / 2, X2 = (Z[m] - conj Z[-m]) / (2i)
+            double2 x1 = make_double2(sc * (zp.x + zm.x), sc * (zp.y - zm.y));
+            double2 x2 = make_double2(sc * (zp.y + zm.y), -sc * (zp.x - zm.x));
+            if (m == 0) {
+                x1.y = 0.0;
+                x2.y = 0.0;
+            }
+            if (th0 + 2 * p < nlat) F[base + 2 * p] = x1;
+            if (th0 + 2 * p + 1 < nlat) F[base + 2 * p + 1] = x2;
+        }
+    }
+}
+
 // ----------------------------------------- phi / cross and dot, nphi = 1536
 // The same step for nphi = 6 x 16 x 16 with at most 512 stored m:
 //   inverse: m = 256 a + b, phi index j = 6 q + r.  Blocks a = 2, 3 of the
@@ -1076,6 +1236,56 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
             P.Fo[2][o] = gp;
         }
     }
+}
+
+int launch_phi_inverse(Plan *p, int nfields, int nplanes, const double2 *const *F, double *const *f)
+{
+    PhiPtrs P = {};
+    for (int i = 0; i < nfields; i++) {
+        P.F[i] = F[i];
+        P.fo[i] = f[i];
+    }
+    const int ntt = (p->nlat + 3) / 4;
+    if (p->nphi == 768 && p->lmax <= 255) {
+        hipLaunchKernelGGL(phi_inverse768_kernel, dim3(nplanes * ntt, nfields), dim3(256), 0, p->stream, P,
+                           p->lmax, p->nlat, nplanes, p->tw_phi);
+        RD_LAUNCH_CHECK();
+        return RD_OK;
+    }
+    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
+    {
+        int lrc = allow_lds(phi_inverse_kernel, shm);
+        if (lrc) return lrc;
+    }
+    hipLaunchKernelGGL(phi_inverse_kernel, dim3(nplanes * ntt, nfields), dim3(256), shm, p->stream, P,
+                       p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+int launch_phi_forward(Plan *p, int nfields, int nplanes, const double *const *f, double2 *const *F)
+{
+    PhiPtrs P = {};
+    for (int i = 0; i < nfields; i++) {
+        P.f[i] = f[i];
+        P.Fo[i] = F[i];
+    }
+    const int ntt = (p->nlat + 3) / 4;
+    if (p->nphi == 768 && p->lmax <= 255) {
+        hipLaunchKernelGGL(phi_forward768_kernel, dim3(nplanes * ntt, nfields), dim3(256), 0, p->stream, P,
+                           p->lmax, p->nlat, nplanes, p->tw_phi);
+        RD_LAUNCH_CHECK();
+        return RD_OK;
+    }
+    size_t shm = sizeof(double2) * (4 * p->nphi + p->nphi / 2);
+    {
+        int lrc = allow_lds(phi_forward_kernel, shm);
+        if (lrc) return lrc;
+    }
+    hipLaunchKernelGGL(phi_forward_kernel, dim3(nplanes * ntt, nfields), dim3(256), shm, p->stream, P,
+                       p->lmax, p->nlat, p->nphi, nplanes, p->seq_phi, p->tw_phi);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
 }
 
 // Fin: (u_r, u_t, u_p, g_r, g_t, g_p) Fourier fields -> Fout = u . g.

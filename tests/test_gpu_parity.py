@@ -83,7 +83,10 @@ def test_prueba_sphp_known_fields():
 
 
 @pytest.mark.parametrize("lmax,nlat,nphi,npl", [(15, 24, 48, 6), (31, 48, 96, 24), (127, 192, 384, 5),
-                                                (255, 384, 768, 3)])
+                                                (255, 384, 768, 3),
+                                                # nphi = 768 kernels with m > lmax zero and a last
+                                                # tile of two theta rows
+                                                (200, 386, 768, 3)])
 def test_scalar_sht_vs_oracle(lmax, nlat, nphi, npl):
     S = sphp()
     plan = S.FCPlan(lmax, nlat, nphi, npl)
@@ -125,7 +128,8 @@ def test_prueba_vec_known_fields():
         assert ea < 1e-12 and es < 5e-12
 
 
-@pytest.mark.parametrize("lmax,nlat,nphi,npl", [(15, 24, 48, 6), (31, 48, 96, 8), (127, 192, 384, 4)])
+@pytest.mark.parametrize("lmax,nlat,nphi,npl", [(15, 24, 48, 6), (31, 48, 96, 8), (127, 192, 384, 4),
+                                                (255, 384, 768, 2)])
 def test_vector_sht_vs_oracle(lmax, nlat, nphi, npl):
     S = sphp()
     plan = S.FCPlan(lmax, nlat, nphi, npl)
