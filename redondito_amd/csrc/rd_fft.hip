@@ -1485,6 +1485,8 @@ __global__ __launch_bounds__(512) void z_kernel(ZArgs A, int mode, ZFft Z, const
     z_store(R2, A.a, A.ld, c0, A.ncols, n, n, 1.0);
 }
 
+int z_fact_launch(Plan *p, const ZArgs &A, int mode);
+
 static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order, int ncols, long ld)
 {
     ZArgs A;
@@ -1498,6 +1500,10 @@ static int z_launch(Plan *p, int mode, double2 *a, const double2 *ain, int order
     A.C = p->C;
     A.d = p->d;
     A.order = order;
+    {
+        const int frc = z_fact_launch(p, A, mode);
+        if (frc != 1) return frc;
+    }
     size_t shm = sizeof(double2) * ((2 * ZTC + 1) * p->nz + kZScalars);
     const dim3 grid((ncols + ZTC - 1) / ZTC), block(512);
 #define RD_Z_CASE(RBV)                                                                         \
@@ -1894,6 +1900,133 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
 static int radial_factorised(int n)
 {
     return (n == 153 || n == 121 || n == 49) ? n : 0;
+}
+
+// Single-field radial transforms for n = N1 N2, with the register DFTs and
+// the two passes of uxw_radial_fact_kernel; the modes and stored rows are
+// those of z_kernel: 0 forward (continuation, FFT -1, / n), 1 inverse
+// (FFT +1), 2 derivative of order A.order (forward, (i kz)^p / n, inverse)
+// from A.ain into A.a.  Every one of the n rows is stored.  A workgroup
+// reads all rows of its CB columns before it writes any, so a and ain may be
+// the same array.
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const double2 *tw)
+{
+    constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
+    __shared__ double2 buf[CB * N];
+    __shared__ double2 twl[N];   // e^{-2 pi i t / n}
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N; t += 256) twl[t] = tw[t];
+    const int nr = N - A.C;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * CB;
+    const bool inA = tid < CB * N2, inB = tid < CB * N1;
+    const int ca = tid % CB, n2 = tid / CB;   // pass A
+    const int cb = tid % CB, k1 = tid / CB;   // pass B
+    const bool okA = inA && c0 + ca < A.ncols, okB = inB && c0 + cb < A.ncols;
+    const double2 *src = (mode == 2) ? A.ain : A.a;
+    __syncthreads();
+    double2 x[N1], y[N2];
+    if (mode == 1) {
+        // pass-B thread (column, k1) holds the spectrum rows k1 + N1 q
+        if (inB) {
+#pragma unroll
+            for (int q = 0; q < N2; q++)
+                y[q] = okB ? src[(long)(k1 + N1 * q) * A.ld + c0 + cb] : make_double2(0.0, 0.0);
+            dft_odd<N2>(y, +1);
+#pragma unroll
+            for (int q = 0; q < N2; q++) buf[cb * N + q * N1 + k1] = cmul(y[q], cconj(twl[(q * k1) % N]));
+        }
+    } else {
+        // pass A: rows n2 + N2 n1, continuation rows from the wall rows
+        if (inA) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) {
+                const int r = n2 + N2 * n1;
+                double2 v = make_double2(0.0, 0.0);
+                if (okA) {
+                    if (r < nr) {
+                        v = src[(long)r * A.ld + c0 + ca];
+                    } else {
+                        const int ii = r - nr + 1;
+                        for (int jj = 1; jj <= A.d; jj++) {
+                            const double a1 = A.dir[(ii - 1) + A.C * (jj - 1)];
+                            const double a2 = A.dir[(A.C - ii) + A.C * (jj - 1)];
+                            const double2 xr = src[(long)(nr - A.d + jj - 1) * A.ld + c0 + ca];
+                            const double2 xl = src[(long)(A.d - jj) * A.ld + c0 + ca];
+                            v.x += a1 * xr.x + a2 * xl.x;
+                            v.y += a1 * xr.y + a2 * xl.y;
+                        }
+                    }
+                }
+                x[n1] = v;
+            }
+            dft_odd<N1>(x, -1);
+#pragma unroll
+            for (int q = 0; q < N1; q++) buf[ca * N + q * N2 + n2] = cmul(x[q], twl[(n2 * q) % N]);
+        }
+        __syncthreads();
+        // pass B: y[q] = X[k1 + N1 q]
+        if (inB) {
+#pragma unroll
+            for (int q = 0; q < N2; q++) y[q] = buf[cb * N + k1 * N2 + q];
+            dft_odd<N2>(y, -1);
+        }
+        if (mode == 0) {
+            if (okB)
+#pragma unroll
+                for (int q = 0; q < N2; q++) A.a[(long)(k1 + N1 * q) * A.ld + c0 + cb] = cscale(y[q], 1.0 / N);
+            return;
+        }
+        __syncthreads();
+        // (i kz)^p / n, then the inverse pass B
+        if (inB) {
+            const int pm = A.order & 3;
+#pragma unroll
+            for (int q = 0; q < N2; q++) {
+                double kp = 1.0;
+                for (int e = 0; e < A.order; e++) kp *= A.kz[k1 + N1 * q];
+                double2 v = cscale(y[q], kp / N);
+                if (pm == 1) v = make_double2(-v.y, v.x);
+                else if (pm == 2) v = make_double2(-v.x, -v.y);
+                else if (pm == 3) v = make_double2(v.y, -v.x);
+                y[q] = v;
+            }
+            dft_odd<N2>(y, +1);
+#pragma unroll
+            for (int q = 0; q < N2; q++) buf[cb * N + q * N1 + k1] = cmul(y[q], cconj(twl[(q * k1) % N]));
+        }
+    }
+    __syncthreads();
+    // inverse pass A: rows n2 + N2 n1
+    if (inA) {
+#pragma unroll
+        for (int q = 0; q < N1; q++) x[q] = buf[ca * N + n2 * N1 + q];
+        dft_odd<N1>(x, +1);
+        if (okA)
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) A.a[(long)(n2 + N2 * n1) * A.ld + c0 + ca] = x[n1];
+    }
+}
+
+// The factorised single-field radial transform when n is one of the factor
+// pairs above; returns 1 (nothing launched) otherwise.
+int z_fact_launch(Plan *p, const ZArgs &A, int mode)
+{
+    switch (radial_factorised(p->nz)) {
+#define RD_ZF_CASE(NV, F1, F2)                                                                  \
+    case NV: {                                                                                  \
+        constexpr int CB = RadFact<F1, F2>::CB;                                                 \
+        hipLaunchKernelGGL((z_fact_kernel<F1, F2>), dim3((A.ncols + CB - 1) / CB), dim3(256), 0, p->stream, A, \
+                           mode, p->tw_z);                                                      \
+        RD_LAUNCH_CHECK();                                                                      \
+        return RD_OK;                                                                           \
+    }
+        RD_ZF_CASE(153, 17, 9)
+        RD_ZF_CASE(121, 11, 11)
+        RD_ZF_CASE(49, 7, 7)
+#undef RD_ZF_CASE
+    default: return 1;
+    }
 }
 
 int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst)
