@@ -3,7 +3,10 @@
 // (tables, twiddles, one Fourier and one spectral scratch array); the
 // Legendre stage of all problems runs as one persistent launch per direction
 // (work items ordered largest first, pulled from a counter), the longitude
-// and radial FFTs run per problem on the batch stream.  The whole round trip
+// and radial FFTs run per problem, spread over four lanes (the batch stream
+// and three side streams forked from it and joined back with events, so that
+// the small launches of different problems overlap; a captured graph keeps
+// them as parallel branches).  The whole round trip
 // can be captured once into a hipGraph and replayed, and a tuner times the
 // tile / grid choices of the persistent launches and keeps the fastest.
 //
@@ -21,10 +24,14 @@
 
 using namespace rd;
 
+constexpr int kLanes = 4;   // streams the per-problem stages are spread over
+
 struct rd_batch {
     int nprob, prec;
     std::vector<Plan *> plans;
     hipStream_t stream, own_stream;
+    hipStream_t side[kLanes - 1];   // default priority, like every stream here
+    hipEvent_t ev_fork, ev_join[kLanes - 1];
     BatchLeg leg;
     int jb, wg_per_cu, n_cu;
     std::vector<double2 *> ana_out;   // bound output arrays of the analysis
@@ -77,43 +84,61 @@ int bind(rd_batch *b, double *const *spec_out)
     return batch_leg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
 }
 
-// the inverse chain, enqueued on the batch stream (no host synchronisation)
+// Enqueues f(i) for every problem i with plan i bound to lane i % kLanes:
+// the side streams wait for the batch stream's work so far, and the batch
+// stream waits for all of them afterwards.
+template <typename Fn>
+int per_problem(rd_batch *b, Fn &&f)
+{
+    RD_HIP(hipEventRecord(b->ev_fork, b->stream));
+    for (int l = 0; l < kLanes - 1; l++) RD_HIP(hipStreamWaitEvent(b->side[l], b->ev_fork, 0));
+    int rc = RD_OK;
+    for (int i = 0; i < b->nprob && !rc; i++) {
+        const int lane = i % kLanes;
+        Plan *p = b->plans[i];
+        p->stream = lane == 0 ? b->stream : b->side[lane - 1];
+        rc = f(i);
+        p->stream = b->stream;
+    }
+    for (int l = 0; l < kLanes - 1; l++) {
+        RD_HIP(hipEventRecord(b->ev_join[l], b->side[l]));
+        RD_HIP(hipStreamWaitEvent(b->stream, b->ev_join[l], 0));
+    }
+    return rc;
+}
+
+// the inverse chain, enqueued on the batch stream and its lanes (no host
+// synchronisation)
 int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
 {
-    for (int i = 0; i < b->nprob; i++) {
+    int rc = per_problem(b, [&](int i) {
         Plan *p = b->plans[i];
-        RD_HIP(hipMemcpyAsync(p->S[0], spec[i], sizeof(double2) * p->ssize, hipMemcpyDeviceToDevice, b->stream));
-        int rc = launch_z_inverse(p, p->S[0]);
-        if (rc) return rc;
-    }
-    int rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
+        RD_HIP(hipMemcpyAsync(p->S[0], spec[i], sizeof(double2) * p->ssize, hipMemcpyDeviceToDevice, p->stream));
+        return launch_z_inverse(p, p->S[0]);
+    });
     if (rc) return rc;
-    for (int i = 0; i < b->nprob; i++) {
+    rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    return per_problem(b, [&](int i) {
         Plan *p = b->plans[i];
         const double2 *F[1] = {p->F[0]};
         double *f[1] = {grid[i]};
-        rc = launch_phi_inverse(p, 1, p->nz, F, f);
-        if (rc) return rc;
-    }
-    return RD_OK;
+        return launch_phi_inverse(p, 1, p->nz, F, f);
+    });
 }
 
 int enqueue_forward(rd_batch *b, const double *const *grid)
 {
-    for (int i = 0; i < b->nprob; i++) {
+    int rc = per_problem(b, [&](int i) {
         Plan *p = b->plans[i];
         const double *f[1] = {grid[i]};
         double2 *F[1] = {p->F[0]};
-        int rc = launch_phi_forward(p, 1, p->nz, f, F);
-        if (rc) return rc;
-    }
-    int rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
+        return launch_phi_forward(p, 1, p->nz, f, F);
+    });
     if (rc) return rc;
-    for (int i = 0; i < b->nprob; i++) {
-        rc = launch_z_forward(b->plans[i], b->ana_out[i]);
-        if (rc) return rc;
-    }
-    return RD_OK;
+    rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    return per_problem(b, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
 }
 
 int configure(rd_batch *b, int jb, int wg_per_cu)
@@ -140,6 +165,11 @@ void destroy(rd_batch *b)
         p->stream = p->own_stream;
         plan_destroy(p);
     }
+    for (int l = 0; l < kLanes - 1; l++)
+        if (b->side[l]) (void)hipStreamDestroy(b->side[l]);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (int l = 0; l < kLanes - 1; l++)
+        if (b->ev_join[l]) (void)hipEventDestroy(b->ev_join[l]);
     if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
     delete b;
 }
@@ -159,7 +189,12 @@ int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat,
     b->prec = prec;
     b->plans.assign(nprob, nullptr);
     b->ana_out.assign(nprob, nullptr);
-    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 0; l < kLanes - 1 && ok; l++)
+        ok = hipStreamCreateWithFlags(&b->side[l], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->ev_join[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         destroy(b);
         return RD_ERR_HIP;
     }
