@@ -1014,8 +1014,9 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
 // (N_theta + i N_phi, N_r).  A sequence's 16 lanes lie in one wave, so the
 // exchanges inside one 256-point sequence need only wave-local ordering.
 // V = float2: the row is transformed in FP32 (36 KB of LDS, three rows per CU)
-// for the BF16x3 form, whose Legendre error (6e-6) is two orders above what
-// the FP32 transforms add; loads, the pack scale and the stores stay FP64.
+// for the BF16x3 form, whose Legendre error dominates what the FP32
+// transforms add (u x omega 5.8e-6 -> 6.2e-6 at lmax 511); loads, the pack
+// scale and the stores stay FP64.
 // The FP32 form keeps FP64 rows: FP32 transforms raised its u x omega error at
 // lmax 511 from 4e-7 to 2e-6.
 constexpr int PHI6_BUF = 18 * 256;
