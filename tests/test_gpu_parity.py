@@ -199,8 +199,11 @@ def test_z_fcgram_vs_oracle(nr, lmax):
     S.derivr(plan, to_dev(a), d1, 1, "r")
     kz = O.kz(n, dz)
     e3 = H.rell2(to_host(d1)[:nr], fc.derivr_r(a.copy(), kz, 1)[:nr])
-    print(f"n={n}: forward {e1:.2e} inverse {e2:.2e} derivr {e3:.2e}")
-    assert e1 < TOL and e2 < TOL and e3 < TOL
+    d2 = torch.zeros_like(x)
+    S.derivr(plan, to_dev(a), d2, 2, "r")
+    e4 = H.rell2(to_host(d2)[:nr], fc.derivr_r(a.copy(), kz, 2)[:nr])
+    print(f"n={n}: forward {e1:.2e} inverse {e2:.2e} derivr {e3:.2e} derivr2 {e4:.2e}")
+    assert e1 < TOL and e2 < TOL and e3 < TOL and e4 < TOL
 
 
 @pytest.mark.parametrize("nz", [64, 128, 256])
