@@ -686,7 +686,7 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
 
 // One nonlinear step of the coupled state on the plan's six Fourier and six
 // spectral work arrays:
-//   B:      radial(Pb, Tb) -> S0..S2 (S3..S5 = curl B, unused), synthesis -> F3..F5
+//   B:      radial(Pb, Tb) -> S0..S2 (B only: one inverse radial transform), synthesis -> F3..F5
 //   u, w:   radial(W, Z) -> S0..S5, synthesis of u -> F0..F2
 //   u x B:  phi_cross(F0..F2, F3..F5) -> F3..F5, analysis -> MQ, MS, MT
 //   u x w:  synthesis of w -> F3..F5, phi_cross -> F3..F5, analysis -> NQ, NS, NT
@@ -705,7 +705,7 @@ int rd_nonlinear_coupled(rd_plan *plan, int prec, const double *W, const double 
     double2 **S = p->S, **F = p->F;
     const double2 *u_b[6] = {F[0], F[1], F[2], F[3], F[4], F[5]};
     double2 *o3[3] = {F[3], F[4], F[5]};
-    int rc = launch_uxw_radial(p, (const double2 *)Pb, (const double2 *)Tb, S);
+    int rc = launch_uxw_radial(p, (const double2 *)Pb, (const double2 *)Tb, S, 3);   // B only
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[3], F[4], F[5]);
     if (!rc) rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, S);
     if (!rc) rc = launch_synthesis(p, prec, true, nr, S[0], S[1], S[2], p->nlm, F[0], F[1], F[2]);

@@ -1553,6 +1553,7 @@ struct UxwRadArgs {
     const int *ell;
     long ld;
     int ncols, n, C, d;
+    int nout;   // 6: u and omega; 3: u only (Q, S, T of W, Z; no curl)
 };
 
 // lm columns per workgroup in the fused radial step.  One column keeps the
@@ -1583,18 +1584,22 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, ZFft Z, c
     double2 *P = sm, *Q = sm + 3 * UTC * n, *twl = sm + 6 * UTC * n, *xs = sm + (6 * UTC + 1) * n;
     const int nh = load_twiddles(twl, tw, Z.seq.n);
     const int rows = A.C > 0 ? nr : n;
+    // the spectrum of Z only feeds the curl outputs (nout = 6)
+    const int nfwd = A.nout == 6 ? 2 : 1;
     z_load_cols(P, A.W, A.ld, c0, A.ncols, n, rows, UTC);
-    z_load_cols(P + UTC * n, A.Z, A.ld, c0, A.ncols, n, rows, UTC);
+    if (nfwd == 2) z_load_cols(P + UTC * n, A.Z, A.ld, c0, A.ncols, n, rows, UTC);
     __syncthreads();
     if (A.C > 0) {
-        fc_continue(P, 2 * UTC, n, A.C, A.d, A.dir);
+        fc_continue(P, nfwd * UTC, n, A.C, A.d, A.dir);
         __syncthreads();
     }
-    double2 *R = z_fft<RB>(P, Q, 2 * UTC, Z, -1, twl, nh, xs);
+    double2 *R = z_fft<RB>(P, Q, nfwd * UTC, Z, -1, twl, nh, xs);
     double2 *I = (R == P) ? Q : P;
     // spectra / n times (i kz)^p: sequences [0,UTC) dW, [UTC,2UTC) d2W, [2UTC,3UTC) dZ
+    // (only dW when nout = 3)
     const double inv_n = 1.0 / n;
-    for (int e = threadIdx.x; e < 3 * UTC * n; e += blockDim.x) {
+    const int ninv = A.nout == 6 ? 3 : 1;
+    for (int e = threadIdx.x; e < ninv * UTC * n; e += blockDim.x) {
         const int sq = e / n, r = e - sq * n;
         const int kind = sq / UTC, c = sq - kind * UTC;
         const double k = A.kz[r];
@@ -1608,7 +1613,7 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, ZFft Z, c
         I[e] = v;
     }
     __syncthreads();
-    double2 *D = z_fft<RB>(I, R, 3 * UTC, Z, +1, twl, nh, xs);
+    double2 *D = z_fft<RB>(I, R, ninv * UTC, Z, +1, twl, nh, xs);
     // u:     Q = l(l+1) W / r^2, S = dW / r, T = Z / r
     // omega: Q = l(l+1) Z / r^2, S = dZ / r, T = -(d2W - l(l+1) W / r^2) / r
     for (int e = threadIdx.x; e < nr * UTC; e += blockDim.x) {
@@ -1619,13 +1624,16 @@ __global__ __launch_bounds__(256) void uxw_radial_kernel(UxwRadArgs A, ZFft Z, c
         const double ll = (double)A.ell[lm] * (A.ell[lm] + 1);
         const long o = (long)r * A.ld + lm;
         const double2 w = A.W[o], zz = A.Z[o];
-        const double2 dw = D[c * n + r], d2w = D[(UTC + c) * n + r], dz = D[(2 * UTC + c) * n + r];
+        const double2 dw = D[c * n + r];
         A.out[0][o] = cscale(w, ll * r2);
         A.out[1][o] = cscale(dw, r1);
         A.out[2][o] = cscale(zz, r1);
-        A.out[3][o] = cscale(zz, ll * r2);
-        A.out[4][o] = cscale(dz, r1);
-        A.out[5][o] = cscale(make_double2(d2w.x - ll * w.x * r2, d2w.y - ll * w.y * r2), -r1);
+        if (ninv == 3) {
+            const double2 d2w = D[(UTC + c) * n + r], dz = D[(2 * UTC + c) * n + r];
+            A.out[3][o] = cscale(zz, ll * r2);
+            A.out[4][o] = cscale(dz, r1);
+            A.out[5][o] = cscale(make_double2(d2w.x - ll * w.x * r2, d2w.y - ll * w.y * r2), -r1);
+        }
     }
 }
 
@@ -1823,14 +1831,16 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
                 if (r < nr) {
                     const double rr = A.z[r];
                     A.out[2][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / rr);
-                    A.out[3][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
+                    if (A.nout == 6) A.out[3][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
                 }
             }
-        fwd_a(x);
+        if (A.nout == 6) fwd_a(x);
     }
     __syncthreads();
-    if (inB) fwd_b(XZ);
-    __syncthreads();
+    if (A.nout == 6) {   // the spectrum of Z only feeds the curl outputs
+        if (inB) fwd_b(XZ);
+        __syncthreads();
+    }
     // ---- u_S = dW / r
     if (inB) {
 #pragma unroll
@@ -1851,49 +1861,51 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
             }
     }
     __syncthreads();
-    // ---- w_T = -(d2W - l(l+1) W / r^2) / r
-    if (inB) {
+    if (A.nout == 6) {
+        // ---- w_T = -(d2W - l(l+1) W / r^2) / r
+        if (inB) {
 #pragma unroll
-        for (int q = 0; q < N2; q++) {
-            const double k = A.kz[k1 + N1 * q];
-            y[q] = cscale(XW[q], -k * k * inv_n);
+            for (int q = 0; q < N2; q++) {
+                const double k = A.kz[k1 + N1 * q];
+                y[q] = cscale(XW[q], -k * k * inv_n);
+            }
+            inv_b(y);
         }
-        inv_b(y);
-    }
-    __syncthreads();
-    if (inA) {
-        inv_a(x);
-        if (okA)
+        __syncthreads();
+        if (inA) {
+            inv_a(x);
+            if (okA)
 #pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                if (r < nr) {
-                    const long o = (long)r * A.ld + lma;
-                    const double rr = A.z[r], r2 = 1.0 / (rr * rr);
-                    const double2 w = A.W[o];
-                    A.out[5][o] = cscale(make_double2(x[n1].x - ll * w.x * r2, x[n1].y - ll * w.y * r2), -1.0 / rr);
+                for (int n1 = 0; n1 < N1; n1++) {
+                    const int r = n2 + N2 * n1;
+                    if (r < nr) {
+                        const long o = (long)r * A.ld + lma;
+                        const double rr = A.z[r], r2 = 1.0 / (rr * rr);
+                        const double2 w = A.W[o];
+                        A.out[5][o] = cscale(make_double2(x[n1].x - ll * w.x * r2, x[n1].y - ll * w.y * r2), -1.0 / rr);
+                    }
                 }
-            }
-    }
-    __syncthreads();
-    // ---- w_S = dZ / r
-    if (inB) {
-#pragma unroll
-        for (int q = 0; q < N2; q++) {
-            const double k = A.kz[k1 + N1 * q] * inv_n;
-            y[q] = make_double2(-k * XZ[q].y, k * XZ[q].x);
         }
-        inv_b(y);
-    }
-    __syncthreads();
-    if (inA) {
-        inv_a(x);
-        if (okA)
+        __syncthreads();
+        // ---- w_S = dZ / r
+        if (inB) {
 #pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+            for (int q = 0; q < N2; q++) {
+                const double k = A.kz[k1 + N1 * q] * inv_n;
+                y[q] = make_double2(-k * XZ[q].y, k * XZ[q].x);
             }
+            inv_b(y);
+        }
+        __syncthreads();
+        if (inA) {
+            inv_a(x);
+            if (okA)
+#pragma unroll
+                for (int n1 = 0; n1 < N1; n1++) {
+                    const int r = n2 + N2 * n1;
+                    if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+                }
+        }
     }
 }
 
@@ -2030,12 +2042,14 @@ int z_fact_launch(Plan *p, const ZArgs &A, int mode)
     }
 }
 
-int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst)
+int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout)
 {
+    if (nout != 6 && nout != 3) return RD_ERR_ARG;
     UxwRadArgs A;
     A.W = W;
     A.Z = Z;
-    for (int i = 0; i < 6; i++) A.out[i] = qst[i];
+    for (int i = 0; i < 6; i++) A.out[i] = i < nout ? qst[i] : nullptr;
+    A.nout = nout;
     A.dir = p->d_dir;
     A.kz = p->d_kz;
     A.z = p->d_z;

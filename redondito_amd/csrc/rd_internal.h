@@ -144,8 +144,9 @@ int launch_z_forward(Plan *p, double2 *a);
 int launch_z_inverse(Plan *p, double2 *a);
 // Radial derivative of order p in (lm,r): b = derivr(a, p).
 int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b);
-// Fused radial step of the nonlinear term: W, Z -> the six (Q,S,T) arrays.
-int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst);
+// Fused radial step of the nonlinear term: W, Z -> the six (Q,S,T) arrays of
+// u and omega (nout = 6), or the three of u only (nout = 3).
+int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout = 6);
 // Radial derivative of order p of ncols columns with plane stride ld.
 int launch_derivr_cols(Plan *p, int order, const double2 *a, double2 *b, int ncols, long ld);
 
