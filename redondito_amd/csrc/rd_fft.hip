@@ -572,19 +572,16 @@ __device__ __forceinline__ void dft16(V (&v)[16], int sign)
     for (int i = 0; i < 16; i++) v[i] = o[i];
 }
 
-// exp(sign 2 pi i t / 768) from a quarter table of exp(-2 pi i t / 768),
-// t < 192, using w(t + 192 q) = (-i)^q w(t).
+// T16[(q-1) 16 + b] = e^{-2 pi i b q/256}, q = 1..15, b = 0..15, from the
+// global table exp(-2 pi i t/n) of the n = 256 st roots of unity: the
+// twiddles of a 16 x 16 pass, read at a constant offset per q.
 template <typename V>
-__device__ __forceinline__ V twq(const V *tw, int t, int sign)
+__device__ __forceinline__ void fill_t16(V *T16, const double2 *tw, int st, int tid)
 {
-    const int q = t / 192;
-    const V w = tw[t - 192 * q];
-    V r;
-    if (q == 0) r = w;
-    else if (q == 1) r = {w.y, -w.x};
-    else if (q == 2) r = {-w.x, -w.y};
-    else r = {-w.y, w.x};
-    return sign > 0 ? V{r.x, -r.y} : r;
+    if (tid < 15 * 16) {
+        const double2 w = tw[st * (tid & 15) * ((tid >> 4) + 1)];
+        T16[tid] = Cx<V>::make(w.x, w.y);
+    }
 }
 
 // One (plane, theta) row per workgroup, in place in a single LDS buffer so
@@ -602,7 +599,10 @@ __device__ __forceinline__ V twq(const V *tw, int t, int sign)
 //   FIN X[m] = sum_n2 e^{-2 pi i n2 m/768} Yf[n2][m mod 256], unpacking of the
 //       real spectra, scaling and store.
 // The 16 x 16 transposes use an XOR swizzle (column q ^ row) instead of
-// padding; twiddles come from a quarter table of the 768 roots.
+// padding.  The 16-point passes take their twiddles from a [q][b] table of
+// the 256th roots in LDS (one read at a constant offset per value, lanes of
+// consecutive b on consecutive slots); R3 and FIN read e^{-2 pi i b/768} and
+// e^{-2 pi i 2b/768} straight from the global table of the 768 roots.
 // V = float2 transforms the row in FP32 (20 KB of LDS) for the BF16x3 form;
 // loads, the pack scale and the stores stay FP64.
 constexpr int PHI_BUF = 9 * 256;
@@ -622,8 +622,8 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
                                                               const double2 *tw)
 {
     using S = typename Cx<V>::S;
-    constexpr int N = 768, NQ = 192;
-    __shared__ V twl[NQ];
+    constexpr int N = 768;
+    __shared__ V T16[15 * 16];   // T16[(q-1) 16 + b] = e^{-2 pi i b q/256}, q = 1..15
     __shared__ V B[PHI_BUF];
     __shared__ double red[4][2];
     __shared__ double ia_s;
@@ -632,7 +632,10 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = Cx<V>::make(tw[tid].x, tw[tid].y);
+    fill_t16(T16, tw, 3, tid);
+    // e^{-2 pi i b/768} and e^{-2 pi i 2b/768} for b = tid: R3 takes their
+    // conjugates, FIN (m = tid) the values themselves
+    const double2 g1 = tw[tid], g2 = tw[2 * tid];
     // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax),
     // and the sums of squares of u and of w
     double s[2] = {0.0, 0.0};
@@ -663,8 +666,10 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
             zc[x] = b == 0 ? V{S(0), S(0)} : V{uc.x + wc.y, -uc.y + wc.x};
         }
         __syncthreads();
-        const V w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
-        const V t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
+        // e^{2 pi i 512/768} and e^{2 pi i 256/768}
+        const S h = S(0.86602540378443864676);
+        const V w3a = V{S(-0.5), -h}, w3b = V{S(-0.5), h};
+        const V t1 = Cx<V>::make(g1.x, -g1.y), t2 = Cx<V>::make(g2.x, -g2.y);
 #pragma unroll
         for (int x = 0; x < 3; x++) {
             B[(3 * x) * 256 + b] = cadd(zb[x], zc[x]);
@@ -689,9 +694,10 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 0; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
+            for (int q1 = 1; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
         }
     }
     wave_lds_sync();
@@ -748,9 +754,10 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
         wave_lds_sync();
         if (act) {
             dft16(v, -1);
+            B[((s * 3 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 0; k1 < 16; k1++)
-                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
+            for (int k1 = 1; k1 < 16; k1++)
+                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
         }
     }
     wave_lds_sync();
@@ -780,7 +787,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     const int m = tid;
     if (m < L) {
         const int mneg = (256 - m) & 255;
-        const V c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
+        const V c1 = Cx<V>::make(g1.x, g1.y), c2 = Cx<V>::make(g2.x, g2.y);
         const V xpv = cadd(B[m], cadd(cmul(B[256 + m], c1), cmul(B[512 + m], c2)));
         const V x1v = cadd(B[768 + m], cadd(cmul(B[1024 + m], c1), cmul(B[1280 + m], c2)));
         const V xmv = cadd(B[mneg], cadd(cmul(B[256 + mneg], cconj(c1)), cmul(B[512 + mneg], cconj(c2))));
@@ -814,8 +821,8 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
 __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                              const double2 *tw)
 {
-    constexpr int N = 768, NQ = 192;
-    __shared__ double2 twl[NQ];
+    constexpr int N = 768;
+    __shared__ double2 T16[15 * 16];
     __shared__ double2 B[6 * 256];
     const int ntt = (nlat + 3) / 4;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -825,7 +832,8 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
     const double2 *F = P.F[blockIdx.y];
     double *f = P.fo[blockIdx.y];
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = tw[tid];
+    fill_t16(T16, tw, 3, tid);
+    const double2 g1 = tw[tid], g2 = tw[2 * tid];
     // coefficient m = tid of rows th0 .. th0+3 at B[row 256 + m]
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -847,8 +855,9 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
             zc[p] = b == 0 ? make_double2(0.0, 0.0) : make_double2(uc.x + wc.y, -uc.y + wc.x);
         }
         __syncthreads();
-        const double2 w3a = twq(twl, 512, +1), w3b = twq(twl, 256, +1);
-        const double2 t1 = twq(twl, b, +1), t2 = twq(twl, 2 * b, +1);
+        const double h = 0.86602540378443864676;
+        const double2 w3a = make_double2(-0.5, -h), w3b = make_double2(-0.5, h);
+        const double2 t1 = cconj(g1), t2 = cconj(g2);
 #pragma unroll
         for (int p = 0; p < 2; p++) {
             B[(3 * p) * 256 + b] = cadd(zb[p], zc[p]);
@@ -869,9 +878,10 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 0; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], twq(twl, 3 * b1 * q1, +1));
+            for (int q1 = 1; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
         }
     }
     wave_lds_sync();
@@ -912,8 +922,8 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
 __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
                                                              const double2 *tw)
 {
-    constexpr int N = 768, NQ = 192;
-    __shared__ double2 twl[NQ];
+    constexpr int N = 768;
+    __shared__ double2 T16[15 * 16];
     __shared__ double2 B[6 * 256];
     const int ntt = (nlat + 3) / 4;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -923,7 +933,8 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     const double *f = P.f[blockIdx.y];
     double2 *F = P.Fo[blockIdx.y];
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = tw[tid];
+    fill_t16(T16, tw, 3, tid);
+    const double2 c1 = tw[tid], c2 = tw[2 * tid];   // e^{-2 pi i m/768}, e^{-2 pi i 2m/768}, m = tid
     // packed rows z_p[j] = row th0+2p + i row th0+2p+1 at B[p N + j]
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -948,9 +959,10 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
         wave_lds_sync();
         if (act) {
             dft16(v, -1);
+            B[((s * 3 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 0; k1 < 16; k1++)
-                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], twq(twl, 3 * a1 * k1, -1));
+            for (int k1 = 1; k1 < 16; k1++)
+                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
         }
     }
     wave_lds_sync();
@@ -977,7 +989,6 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     const int m = tid;
     if (m < L) {
         const int mneg = (256 - m) & 255;
-        const double2 c1 = twq(twl, m, -1), c2 = twq(twl, 2 * m, -1);
         const double sc = 0.5 * (2.0 * M_PI / N);
         const long base = ((long)k * L + m) * nlat + th0;
 #pragma unroll
@@ -1021,19 +1032,30 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
 // lmax 511 from 4e-7 to 2e-6.
 constexpr int PHI6_BUF = 18 * 256;
 
-// exp(sign 2 pi i t / 1536) from the quarter table of exp(-2 pi i t / 1536),
-// t < 1536, using w(t + 384 q) = (-i)^q w(t).
-template <typename V>
-__device__ __forceinline__ V tw1536(const V *tw, int t, int sign)
+// p[r-1] = w^r, r = 1..5, in FP64 (a few ulps from the correctly rounded roots)
+__device__ __forceinline__ void pow5(double2 w, double2 (&p)[5])
 {
-    const int q = t / 384;
-    const V w = tw[t - 384 * q];
-    V r;
-    if (q == 0) r = w;
-    else if (q == 1) r = {w.y, -w.x};
-    else if (q == 2) r = {-w.x, -w.y};
-    else r = {-w.y, w.x};
-    return sign > 0 ? V{r.x, -r.y} : r;
+    p[0] = w;
+    p[1] = cmul(w, w);
+    p[2] = cmul(p[1], w);
+    p[3] = cmul(p[1], p[1]);
+    p[4] = cmul(p[3], w);
+}
+
+// e^{2 pi i k/6} for a compile-time k (any integer)
+template <typename V>
+__device__ __forceinline__ V root6(int k)
+{
+    using S = typename Cx<V>::S;
+    const S h = S(0.86602540378443864676);
+    switch (((k % 6) + 6) % 6) {
+    case 0: return V{S(1), S(0)};
+    case 1: return V{S(0.5), h};
+    case 2: return V{S(-0.5), h};
+    case 3: return V{S(-1), S(0)};
+    case 4: return V{S(-0.5), -h};
+    default: return V{S(0.5), -h};
+    }
 }
 
 template <typename V, bool DOT>
@@ -1041,8 +1063,8 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
                                                          const double2 *tw)
 {
     using S = typename Cx<V>::S;
-    constexpr int N = 1536, NQ = 384, NS = DOT ? 1 : 2;
-    __shared__ V twl[NQ];
+    constexpr int N = 1536, NS = DOT ? 1 : 2;
+    __shared__ V T16[15 * 16];
     __shared__ V B[PHI6_BUF];
     __shared__ double red[8][2];
     __shared__ double ia_s;
@@ -1051,7 +1073,8 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
-    if (tid < NQ) twl[tid] = Cx<V>::make(tw[tid].x, tw[tid].y);
+    fill_t16(T16, tw, 6, tid);
+    const double2 g1 = tw[tid & 255];   // e^{-2 pi i b/1536}, b = tid mod 256
     // the row's six coefficient vectors at B[f 512 + m], m = 0..511 (zero
     // above lmax), and the sums of squares of u and of the second factor
     double s[2] = {0.0, 0.0};
@@ -1090,6 +1113,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
                 }
             }
         }
+        // e^{-2 pi i b r/1536}, r = 1..5, as FP64 powers of the table value
+        // (b = tid mod 256 for both items)
+        double2 g[5];
+        pow5(g1, g);
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -1097,10 +1124,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
             if (i < 768) {
 #pragma unroll
                 for (int r = 0; r < 6; r++) {
-                    const V w1 = tw1536(twl, 256 * r, +1), w4 = tw1536(twl, (1024 * r) % N, +1);
-                    const V w5 = tw1536(twl, (1280 * r) % N, +1);
+                    // e^{2 pi i 256 r/1536}, e^{2 pi i 1024 r/1536}, e^{2 pi i 1280 r/1536}: sixth roots
+                    const V w1 = root6<V>(r), w4 = root6<V>(4 * r), w5 = root6<V>(5 * r);
                     const V y = cadd(cadd(z[h][0], cmul(z[h][1], w1)), cadd(cmul(z[h][2], w4), cmul(z[h][3], w5)));
-                    B[(6 * x + r) * 256 + b] = r == 0 ? y : cmul(y, tw1536(twl, b * r, +1));
+                    B[(6 * x + r) * 256 + b] = r == 0 ? y : cmul(y, Cx<V>::make(g[r - 1].x, -g[r - 1].y));
                 }
             }
         }
@@ -1118,9 +1145,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 0; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], tw1536(twl, 6 * b1 * q1, +1));
+            for (int q1 = 1; q1 < 16; q1++)
+                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
         }
     }
     wave_lds_sync();
@@ -1172,9 +1200,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         __syncthreads();
         if (act) {
             dft16(v, -1);
+            B[((s * 6 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 0; k1 < 16; k1++)
-                B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], tw1536(twl, 6 * a1 * k1, -1));
+            for (int k1 = 1; k1 < 16; k1++)
+                B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
         }
     }
     wave_lds_sync();
@@ -1196,14 +1225,20 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
             for (int k2 = 0; k2 < 16; k2++) B[(s * 6 + n2) * 256 + k1 + 16 * k2] = v[k2];
         }
     }
+    // FIN's twiddles e^{-2 pi i n2 m/1536} (m = tid): FP64 powers of the
+    // table value, loaded before the barrier
+    const double2 f1 = tw[tid];
     __syncthreads();
     // ---- FIN: X[m] = sum_n2 e^{-2 pi i n2 m/1536} Yf[n2][m mod 256]
     const int m = tid;
     if (m < L) {
         const int mm = m & 255, mneg = (256 - mm) & 255;
+        double2 cg[5];
+        pow5(f1, cg);
         V c[6];
+        c[0] = V{S(1), S(0)};
 #pragma unroll
-        for (int n2 = 0; n2 < 6; n2++) c[n2] = tw1536(twl, (n2 * m) % N, -1);
+        for (int n2 = 1; n2 < 6; n2++) c[n2] = Cx<V>::make(cg[n2 - 1].x, cg[n2 - 1].y);
         V xpv = {S(0), S(0)};
 #pragma unroll
         for (int n2 = 0; n2 < 6; n2++) xpv = cadd(xpv, cmul(B[n2 * 256 + mm], c[n2]));
