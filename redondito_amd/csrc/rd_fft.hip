@@ -607,6 +607,16 @@ __device__ __forceinline__ void fill_t16(V *T16, const double2 *tw, int st, int 
 // loads, the pack scale and the stores stay FP64.
 constexpr int PHI_BUF = 9 * 256;
 
+// Thread index with the wave index permuted per workgroup (XOR with a hash of
+// blockIdx.x, LW = log2 of the waves per workgroup).  The 16-point passes give
+// their work to the first few waves; rotating the roles from row to row keeps
+// any one SIMD of the CU from always receiving the busiest wave.
+template <int LW>
+__device__ __forceinline__ int rotated_tid(int tid)
+{
+    return tid ^ (int)((((unsigned)blockIdx.x * 2654435761u) >> (32 - LW)) << 6);
+}
+
 // Orders LDS accesses among the lanes of one wave: a wave's LDS instructions
 // complete in issue order, so a compiler fence at wavefront scope is enough
 // when every lane that reads a slot is in the wave that wrote it.
@@ -632,6 +642,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
+    const int rtid = rotated_tid<2>(tid);   // wave 0 works all four passes, wave 3 none
     fill_t16(T16, tw, 3, tid);
     // e^{-2 pi i b/768} and e^{-2 pi i 2b/768} for b = tid: R3 takes their
     // conjugates, FIN (m = tid) the values themselves
@@ -684,8 +695,8 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     // (P2 writes the grid order j = 3 (q1 + 16 q2) + r): wave-local ordering
     // suffices until CR.
     {
-        const int x = tid >> 6, r = (tid & 63) >> 4, xr = 3 * x + r, b1 = tid & 15;
-        const bool act = tid < 192 && r < 3;
+        const int x = rtid >> 6, r = (rtid & 63) >> 4, xr = 3 * x + r, b1 = rtid & 15;
+        const bool act = rtid < 192 && r < 3;
         V v[16];
         if (act) {
 #pragma unroll
@@ -703,7 +714,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     wave_lds_sync();
     // ---- P2: grid values z_x at j = 3 (q1 + 16 q2) + r, stored at x N + j
     {
-        int t1 = tid;   // opaque copy, as in F2
+        int t1 = rtid;   // opaque copy, as in F2
         asm volatile("" : "+v"(t1));
         const int x = t1 >> 6, r = (t1 & 63) >> 4, xr = 3 * x + r, q1 = t1 & 15;
         const bool act = t1 < 192 && r < 3;
@@ -744,8 +755,8 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     // it touch only slots s N .. s N + 767: wave-local ordering suffices
     // until FIN.
     {
-        const int s = tid >> 6, rem = tid & 63, n2 = rem >> 4, a1 = rem & 15;
-        const bool act = tid < 128 && rem < 48;
+        const int s = rtid >> 6, rem = rtid & 63, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = rtid < 128 && rem < 48;
         V v[16];
         if (act) {
 #pragma unroll
@@ -765,7 +776,7 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     {
         // indices recomputed from an opaque copy of tid: shared with F1 they
         // stay live across the barrier and cost a spill
-        int t2 = tid;
+        int t2 = rtid;
         asm volatile("" : "+v"(t2));
         const int s = t2 >> 6, rem = t2 & 63, n2 = rem >> 4, k1 = rem & 15;
         const bool act = t2 < 128 && rem < 48;
@@ -832,6 +843,7 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
     const double2 *F = P.F[blockIdx.y];
     double *f = P.fo[blockIdx.y];
     const int tid = threadIdx.x;
+    const int rtid = rotated_tid<2>(tid);
     fill_t16(T16, tw, 3, tid);
     const double2 g1 = tw[tid], g2 = tw[2 * tid];
     // coefficient m = tid of rows th0 .. th0+3 at B[row 256 + m]
@@ -868,8 +880,8 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
     __syncthreads();
     // ---- P1 (packed sequence x on lanes 0..47 of wave x)
     {
-        const int x = tid >> 6, r = (tid & 63) >> 4, xr = 3 * x + r, b1 = tid & 15;
-        const bool act = tid < 128 && r < 3;
+        const int x = rtid >> 6, r = (rtid & 63) >> 4, xr = 3 * x + r, b1 = rtid & 15;
+        const bool act = rtid < 128 && r < 3;
         double2 v[16];
         if (act) {
 #pragma unroll
@@ -887,7 +899,7 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
     wave_lds_sync();
     // ---- P2: z_x at j = 3 (q1 + 16 q2) + r, stored at x N + j
     {
-        int t1 = tid;   // opaque copy: shared with P1 the indices stay live
+        int t1 = rtid;   // opaque copy: shared with P1 the indices stay live
         asm volatile("" : "+v"(t1));
         const int x = t1 >> 6, r = (t1 & 63) >> 4, xr = 3 * x + r, q1 = t1 & 15;
         const bool act = t1 < 128 && r < 3;
@@ -933,6 +945,7 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     const double *f = P.f[blockIdx.y];
     double2 *F = P.Fo[blockIdx.y];
     const int tid = threadIdx.x;
+    const int rtid = rotated_tid<2>(tid);
     fill_t16(T16, tw, 3, tid);
     const double2 c1 = tw[tid], c2 = tw[2 * tid];   // e^{-2 pi i m/768}, e^{-2 pi i 2m/768}, m = tid
     // packed rows z_p[j] = row th0+2p + i row th0+2p+1 at B[p N + j]
@@ -949,8 +962,8 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     __syncthreads();
     // ---- F1 (packed sequence s on lanes 0..47 of wave s)
     {
-        const int s = tid >> 6, rem = tid & 63, n2 = rem >> 4, a1 = rem & 15;
-        const bool act = tid < 128 && rem < 48;
+        const int s = rtid >> 6, rem = rtid & 63, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = rtid < 128 && rem < 48;
         double2 v[16];
         if (act) {
 #pragma unroll
@@ -968,7 +981,7 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     wave_lds_sync();
     // ---- F2: Yf[3s+n2][k1 + 16 k2]
     {
-        int t2 = tid;   // opaque copy, as in phi_cross768_kernel
+        int t2 = rtid;   // opaque copy, as in phi_cross768_kernel
         asm volatile("" : "+v"(t2));
         const int s = t2 >> 6, rem = t2 & 63, n2 = rem >> 4, k1 = rem & 15;
         const bool act = t2 < 128 && rem < 48;
@@ -1073,6 +1086,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     if (k >= nplanes) return;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
+    const int rtid = rotated_tid<3>(tid);
     fill_t16(T16, tw, 6, tid);
     const double2 g1 = tw[tid & 255];   // e^{-2 pi i b/1536}, b = tid mod 256
     // the row's six coefficient vectors at B[f 512 + m], m = 0..511 (zero
@@ -1135,8 +1149,8 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     __syncthreads();
     // ---- P1: T[xr][b1][q1] = e^{2 pi i b1 q1/256} sum_b2 Y[xr][16 b2 + b1] e^{2 pi i b2 q1/16}
     {
-        const int xr = tid >> 4, b1 = tid & 15;
-        const bool act = tid < 288;
+        const int xr = rtid >> 4, b1 = rtid & 15;
+        const bool act = rtid < 288;
         V v[16];
         if (act) {
 #pragma unroll
@@ -1154,7 +1168,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     wave_lds_sync();
     // ---- P2: grid values z_x at j = 6 (q1 + 16 q2) + r, stored at x N + j
     {
-        int t1 = tid;   // opaque copy: shared with P1 the indices stay live and spill
+        int t1 = rtid;   // opaque copy: shared with P1 the indices stay live and spill
         asm volatile("" : "+v"(t1));
         const int xr = t1 >> 4, q1 = t1 & 15, x = xr / 6, r = xr - 6 * x;
         const bool act = t1 < 288;
@@ -1190,8 +1204,8 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     __syncthreads();
     // ---- F1: F[6s+n2][a1][k1] = e^{-2 pi i a1 k1/256} sum_a2 P_s[6 (16 a2 + a1) + n2] e^{-2 pi i a2 k1/16}
     {
-        const int s = tid / 96, rem = tid % 96, n2 = rem >> 4, a1 = rem & 15;
-        const bool act = tid < 96 * NS;
+        const int s = rtid / 96, rem = rtid % 96, n2 = rem >> 4, a1 = rem & 15;
+        const bool act = rtid < 96 * NS;
         V v[16];
         if (act) {
 #pragma unroll
@@ -1209,7 +1223,7 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     wave_lds_sync();
     // ---- F2: Yf[6s+n2][k1 + 16 k2]
     {
-        int t2 = tid;
+        int t2 = rtid;
         asm volatile("" : "+v"(t2));
         const int s = t2 / 96, rem = t2 % 96, n2 = rem >> 4, k1 = rem & 15;
         const bool act = t2 < 96 * NS;
