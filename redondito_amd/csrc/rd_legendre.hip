@@ -17,6 +17,7 @@
 // HBM/L2 in the layout the MFMA A-operand wants (16 consecutive doubles per
 // lane group).
 #include "rd_internal.h"
+#include <type_traits>
 
 #include <algorithm>
 #include <cmath>
@@ -753,19 +754,155 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
     }
 }
 
+// FP64 vector analysis tile (JB = 1: 16 planes per workgroup, 8 per wave
+// pair) whose grid chunks (8 northern rows and their southern mirrors) move
+// from global memory straight into LDS (global_load_lds_dwordx4), double-
+// buffered: the moves of chunk c + 1 are issued when chunk c starts, so the
+// staging latency overlaps the MFMAs and one barrier per chunk suffices.
+// One move writes 64 consecutive 16-byte LDS slots (8 planes x 8 rows) from a
+// wave-uniform base; lane 8 p + s loads row (s - p) mod 8 of plane p, so that
+// each plane's eight lanes read one 128-byte segment and the MFMA operand
+// reads (one row, eight planes) fall on eight different bank groups.  Slot
+// of (field f, north/south h, plane pl, row t): 128 (2 f + h) + 8 pl + (t + pl) mod 8.
+__device__ double2 ana_zero16 = {0.0, 0.0};
+
+constexpr int ADC = 8;   // rows per LDS-move chunk
+
+__device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<double> &A, int m, int lt, int jt,
+                                                 double2 (&G)[2][6 * 128])
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int L = A.lmax + 1;
+    const int wl = wave & 1, wj = wave >> 1;
+    const int k0 = jt * 16;
+    const int col = lane & 15, kk = lane >> 4;
+    const double sgn_mi = (col & 1) ? -1.0 : 1.0;   // (-i X)_j = sgn_mi * X_{j^1}
+    const int ncol = A.ncol[m];
+    const int lp0 = lt * 32 + wl * 16;
+    const bool live = lp0 < (L - m + 1) / 2;
+
+    v4d acc[3][2];
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) acc[f][s] = (v4d){0, 0, 0, 0};
+
+    // wave w moves blocks 3 w .. 3 w + 2 of a chunk; block ib = 4 f + 2 h + half
+    // holds planes 8 half .. 8 half + 7 of field f, rows north (h = 0) or south
+    auto issue = [&](int tc, int buf) {
+        const int p = lane >> 3, t = ((lane & 7) - p) & 7;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ib = 3 * wave + i;   // wave-uniform
+            const int f = ib >> 2, h = (ib >> 1) & 1, half = ib & 1;
+            const int th = tc + t, k = k0 + 8 * half + p;
+            const double2 *src = &ana_zero16;
+            if (th < A.nth && k < A.nplanes)
+                src = A.in[f] + (((long)k * L + m) * A.nlat + (h ? A.nlat - 1 - th : th));
+            __builtin_amdgcn_global_load_lds((const void *)src,
+                                             (__attribute__((address_space(3))) void *)&G[buf][64 * ib], 16, 0, 0);
+        }
+    };
+    const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
+    struct Tab {
+        double Pe, Po, De, Do, Me, Mo;
+    };
+    auto ltab = [&](int t) {
+        Tab v;
+        const unsigned rowoff = (unsigned)(t * ncol + lp0 + col);
+        const unsigned ie = (unsigned)offe + rowoff, io = (unsigned)offo + rowoff;
+        v.Pe = A.tP[ie];
+        v.Po = A.tP[io];
+        v.De = A.tD[ie];
+        v.Do = A.tD[io];
+        v.Me = A.tM[ie];
+        v.Mo = A.tM[io];
+        return v;
+    };
+    const int nsteps = ((A.nth + ADC - 1) / ADC) * (ADC / 4);
+    int g = 0, buf = 0;
+    Tab tb[2];
+    tb[0] = ltab(kk);
+    issue(0, 0);
+    // this lane's column j = 16 wj + col: plane pl = j / 2, part j & 1
+    const int pl = (16 * wj + col) >> 1, c = col & 1;
+    for (int tc = 0; tc < A.nth; tc += ADC, buf ^= 1) {
+        __syncthreads();
+        if (tc + ADC < A.nth) issue(tc + ADC, buf ^ 1);
+        const double *Gd = (const double *)G[buf];
+#pragma unroll
+        for (int st = 0; st < ADC / 4; st++, g++) {
+            const int tl = 4 * st + kk;
+            const int gn = (g + 1 < nsteps) ? g + 1 : 0;   // the last step reloads row kk
+            tb[(st + 1) & 1] = ltab(4 * gn + kk);
+            const Tab cur = tb[st & 1];
+            if (live) {
+                const int sl = 8 * pl + ((tl + pl) & 7);
+                auto at = [&](int f, int h, int part) { return Gd[2 * (128 * (2 * f + h) + sl) + part]; };
+                const double rn = at(0, 0, c), rs = at(0, 1, c);
+                acc[0][0] = Mf<double>::mma(cur.Pe, rn + rs, acc[0][0]);
+                acc[0][1] = Mf<double>::mma(cur.Po, rn - rs, acc[0][1]);
+                const double tn = at(1, 0, c), ts = at(1, 1, c);
+                const double pn = at(2, 0, c), ps = at(2, 1, c);
+                const double mtn = sgn_mi * at(1, 0, c ^ 1), mts = sgn_mi * at(1, 1, c ^ 1);
+                const double mpn = sgn_mi * at(2, 0, c ^ 1), mps = sgn_mi * at(2, 1, c ^ 1);
+                // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
+                acc[1][0] = Mf<double>::mma(cur.De, tn - ts, acc[1][0]);
+                acc[1][0] = Mf<double>::mma(cur.Me, mpn + mps, acc[1][0]);
+                // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
+                acc[1][1] = Mf<double>::mma(cur.Do, tn + ts, acc[1][1]);
+                acc[1][1] = Mf<double>::mma(cur.Mo, mpn - mps, acc[1][1]);
+                // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
+                acc[2][0] = Mf<double>::mma(cur.Me, mtn + mts, acc[2][0]);
+                acc[2][0] = Mf<double>::mma(cur.De, ps - pn, acc[2][0]);
+                // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
+                acc[2][1] = Mf<double>::mma(cur.Mo, mtn - mts, acc[2][1]);
+                acc[2][1] = Mf<double>::mma(cur.Do, -(pn + ps), acc[2][1]);
+            }
+        }
+    }
+
+    const long lm0 = lm_index(A.lmax, m, m);
+    const int k = k0 + pl;
+    if (k < A.nplanes) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            double *out = (double *)A.out[f];
+#pragma unroll
+            for (int par = 0; par < 2; par++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int lp = lp0 + Mf<double>::row(kk, v);
+                    const int l = m + par + 2 * lp;
+                    if (l > A.lmax) continue;
+                    double val = acc[f][par][v];
+                    if (m == 0 && c == 1) val = 0.0;
+                    out[2 * (lm0 + (l - m) + k * A.ld) + c] = val;
+                }
+        }
+    }
+}
+
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256, 4) void analysis_kernel(AnaArgs<T> A)
 {
-    using Cfg = AnaCfg<VEC, ana_jb<T, VEC>()>;
-    __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
-    __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
     // per-XCD queue entry (m, l tile); the plane tiles of one entry run back
     // to back on one XCD
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
     const int jt = i % A.n_jt;
     const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
     if (ent < 0) return;
-    ana_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Gn, Gs);
+    if constexpr (VEC && std::is_same<T, double>::value) {
+        static_assert(ana_jb<T, VEC>() == 1, "the LDS-move tile covers one 16-column block per wave");
+        __shared__ double2 G[2][6 * 128];
+        ana_tile_vec_dma(A, ent & 0xffff, ent >> 16, jt, G);
+    } else {
+        using Cfg = AnaCfg<VEC, ana_jb<T, VEC>()>;
+        __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
+        __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
+        ana_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Gn, Gs);
+    }
 }
 
 template <typename T>
