@@ -754,7 +754,7 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
     }
 }
 
-// FP64 vector analysis tile (JB = 1: 16 planes per workgroup, 8 per wave
+// Vector analysis tile, FP64 and FP32 forms (JB = 1: 16 planes per workgroup, 8 per wave
 // pair) whose grid chunks (8 northern rows and their southern mirrors) move
 // from global memory straight into LDS (global_load_lds_dwordx4), double-
 // buffered: the moves of chunk c + 1 are issued when chunk c starts, so the
@@ -768,25 +768,27 @@ __device__ double2 ana_zero16 = {0.0, 0.0};
 
 constexpr int ADC = 8;   // rows per LDS-move chunk
 
-__device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<double> &A, int m, int lt, int jt,
+template <typename T>
+__device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<T> &A, int m, int lt, int jt,
                                                  double2 (&G)[2][6 * 128])
 {
+    using V = typename Mf<T>::V;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int L = A.lmax + 1;
     const int wl = wave & 1, wj = wave >> 1;
     const int k0 = jt * 16;
     const int col = lane & 15, kk = lane >> 4;
-    const double sgn_mi = (col & 1) ? -1.0 : 1.0;   // (-i X)_j = sgn_mi * X_{j^1}
+    const T sgn_mi = (col & 1) ? T(-1) : T(1);   // (-i X)_j = sgn_mi * X_{j^1}
     const int ncol = A.ncol[m];
     const int lp0 = lt * 32 + wl * 16;
     const bool live = lp0 < (L - m + 1) / 2;
 
-    v4d acc[3][2];
+    V acc[3][2];
 #pragma unroll
     for (int f = 0; f < 3; f++)
 #pragma unroll
-        for (int s = 0; s < 2; s++) acc[f][s] = (v4d){0, 0, 0, 0};
+        for (int s = 0; s < 2; s++) acc[f][s] = (V){0, 0, 0, 0};
 
     // wave w moves blocks 3 w .. 3 w + 2 of a chunk; block ib = 4 f + 2 h + half
     // holds planes 8 half .. 8 half + 7 of field f, rows north (h = 0) or south
@@ -806,7 +808,7 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<double> &A, int m
     };
     const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
     struct Tab {
-        double Pe, Po, De, Do, Me, Mo;
+        T Pe, Po, De, Do, Me, Mo;
     };
     auto ltab = [&](int t) {
         Tab v;
@@ -839,26 +841,27 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<double> &A, int m
             const Tab cur = tb[st & 1];
             if (live) {
                 const int sl = 8 * pl + ((tl + pl) & 7);
-                auto at = [&](int f, int h, int part) { return Gd[2 * (128 * (2 * f + h) + sl) + part]; };
-                const double rn = at(0, 0, c), rs = at(0, 1, c);
-                acc[0][0] = Mf<double>::mma(cur.Pe, rn + rs, acc[0][0]);
-                acc[0][1] = Mf<double>::mma(cur.Po, rn - rs, acc[0][1]);
-                const double tn = at(1, 0, c), ts = at(1, 1, c);
-                const double pn = at(2, 0, c), ps = at(2, 1, c);
-                const double mtn = sgn_mi * at(1, 0, c ^ 1), mts = sgn_mi * at(1, 1, c ^ 1);
-                const double mpn = sgn_mi * at(2, 0, c ^ 1), mps = sgn_mi * at(2, 1, c ^ 1);
+                // FP64 in LDS; the FP32 form rounds on the operand read
+                auto at = [&](int f, int h, int part) { return (T)Gd[2 * (128 * (2 * f + h) + sl) + part]; };
+                const T rn = at(0, 0, c), rs = at(0, 1, c);
+                acc[0][0] = Mf<T>::mma(cur.Pe, rn + rs, acc[0][0]);
+                acc[0][1] = Mf<T>::mma(cur.Po, rn - rs, acc[0][1]);
+                const T tn = at(1, 0, c), ts = at(1, 1, c);
+                const T pn = at(2, 0, c), ps = at(2, 1, c);
+                const T mtn = sgn_mi * at(1, 0, c ^ 1), mts = sgn_mi * at(1, 1, c ^ 1);
+                const T mpn = sgn_mi * at(2, 0, c ^ 1), mps = sgn_mi * at(2, 1, c ^ 1);
                 // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
-                acc[1][0] = Mf<double>::mma(cur.De, tn - ts, acc[1][0]);
-                acc[1][0] = Mf<double>::mma(cur.Me, mpn + mps, acc[1][0]);
+                acc[1][0] = Mf<T>::mma(cur.De, tn - ts, acc[1][0]);
+                acc[1][0] = Mf<T>::mma(cur.Me, mpn + mps, acc[1][0]);
                 // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
-                acc[1][1] = Mf<double>::mma(cur.Do, tn + ts, acc[1][1]);
-                acc[1][1] = Mf<double>::mma(cur.Mo, mpn - mps, acc[1][1]);
+                acc[1][1] = Mf<T>::mma(cur.Do, tn + ts, acc[1][1]);
+                acc[1][1] = Mf<T>::mma(cur.Mo, mpn - mps, acc[1][1]);
                 // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
-                acc[2][0] = Mf<double>::mma(cur.Me, mtn + mts, acc[2][0]);
-                acc[2][0] = Mf<double>::mma(cur.De, ps - pn, acc[2][0]);
+                acc[2][0] = Mf<T>::mma(cur.Me, mtn + mts, acc[2][0]);
+                acc[2][0] = Mf<T>::mma(cur.De, ps - pn, acc[2][0]);
                 // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
-                acc[2][1] = Mf<double>::mma(cur.Mo, mtn - mts, acc[2][1]);
-                acc[2][1] = Mf<double>::mma(cur.Do, -(pn + ps), acc[2][1]);
+                acc[2][1] = Mf<T>::mma(cur.Mo, mtn - mts, acc[2][1]);
+                acc[2][1] = Mf<T>::mma(cur.Do, -(pn + ps), acc[2][1]);
             }
         }
     }
@@ -873,7 +876,7 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<double> &A, int m
             for (int par = 0; par < 2; par++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int lp = lp0 + Mf<double>::row(kk, v);
+                    const int lp = lp0 + Mf<T>::row(kk, v);
                     const int l = m + par + 2 * lp;
                     if (l > A.lmax) continue;
                     double val = acc[f][par][v];
@@ -893,7 +896,7 @@ __global__ __launch_bounds__(256, 4) void analysis_kernel(AnaArgs<T> A)
     const int jt = i % A.n_jt;
     const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
     if (ent < 0) return;
-    if constexpr (VEC && std::is_same<T, double>::value) {
+    if constexpr (VEC) {
         static_assert(ana_jb<T, VEC>() == 1, "the LDS-move tile covers one 16-column block per wave");
         __shared__ double2 G[2][6 * 128];
         ana_tile_vec_dma(A, ent & 0xffff, ent >> 16, jt, G);
