@@ -754,51 +754,69 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
     }
 }
 
-// Vector analysis tile, FP64 and FP32 forms (JB = 1: 16 planes per workgroup, 8 per wave
-// pair) whose grid chunks (8 northern rows and their southern mirrors) move
-// from global memory straight into LDS (global_load_lds_dwordx4), double-
-// buffered: the moves of chunk c + 1 are issued when chunk c starts, so the
-// staging latency overlaps the MFMAs and one barrier per chunk suffices.
-// One move writes 64 consecutive 16-byte LDS slots (8 planes x 8 rows) from a
-// wave-uniform base; lane 8 p + s loads row (s - p) mod 8 of plane p, so that
-// each plane's eight lanes read one 128-byte segment and the MFMA operand
-// reads (one row, eight planes) fall on eight different bank groups.  Slot
-// of (field f, north/south h, plane pl, row t): 128 (2 f + h) + 8 pl + (t + pl) mod 8.
+// Analysis tile of the one-problem kernels, FP64 and FP32 forms, whose grid
+// chunks (8 northern rows and their southern mirrors of NF fields and P
+// planes, P = 16 for vector tiles, 64 for scalar ones) move from global
+// memory straight into LDS (global_load_lds_dwordx4), double-buffered: the
+// moves of chunk c + 1 are issued when chunk c starts, so the staging latency
+// overlaps the MFMAs and one barrier per chunk suffices.  One move writes 64
+// consecutive 16-byte LDS slots (8 planes x 8 rows) from a wave-uniform base;
+// lane 8 p + s loads row (s - p) mod 8 of plane p, so that each plane's eight
+// lanes read one 128-byte segment and the MFMA operand reads (one row, eight
+// planes) fall on eight different bank groups.  Slot of (field f, north/south
+// h, plane pl, row t): 8 P (2 f + h) + 8 pl + (t + pl) mod 8.  LDS holds FP64;
+// the FP32 form rounds on the operand read.
 __device__ double2 ana_zero16 = {0.0, 0.0};
 
 constexpr int ADC = 8;   // rows per LDS-move chunk
 
-template <typename T>
-__device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<T> &A, int m, int lt, int jt,
-                                                 double2 (&G)[2][6 * 128])
+template <bool VEC>
+struct AnaDma {
+    static constexpr int NF = VEC ? 3 : 1;
+    static constexpr int JB = VEC ? 1 : 4;    // 16-column blocks per wave
+    static constexpr int P = 16 * JB;         // planes per workgroup
+    static constexpr int SLOTS = NF * 16 * P; // double2 per chunk buffer
+    static constexpr int NB = SLOTS / 64;     // moves per chunk
+    static_assert(NB % 4 == 0, "moves must split evenly over the four waves");
+};
+
+template <typename T, bool VEC>
+__device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt, int jt,
+                                             double2 (&G)[2][AnaDma<VEC>::SLOTS])
 {
     using V = typename Mf<T>::V;
+    using D = AnaDma<VEC>;
+    constexpr int NF = D::NF, JB = D::JB, P = D::P;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int L = A.lmax + 1;
     const int wl = wave & 1, wj = wave >> 1;
-    const int k0 = jt * 16;
+    const int k0 = jt * P;
     const int col = lane & 15, kk = lane >> 4;
     const T sgn_mi = (col & 1) ? T(-1) : T(1);   // (-i X)_j = sgn_mi * X_{j^1}
     const int ncol = A.ncol[m];
     const int lp0 = lt * 32 + wl * 16;
     const bool live = lp0 < (L - m + 1) / 2;
 
-    V acc[3][2];
+    V acc[NF][2][JB];
 #pragma unroll
-    for (int f = 0; f < 3; f++)
+    for (int f = 0; f < NF; f++)
 #pragma unroll
-        for (int s = 0; s < 2; s++) acc[f][s] = (V){0, 0, 0, 0};
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int b = 0; b < JB; b++) acc[f][s][b] = (V){0, 0, 0, 0};
 
-    // wave w moves blocks 3 w .. 3 w + 2 of a chunk; block ib = 4 f + 2 h + half
-    // holds planes 8 half .. 8 half + 7 of field f, rows north (h = 0) or south
+    // wave w issues moves NB/4 w .. NB/4 (w + 1) - 1 of a chunk; move
+    // ib = (2 f + h) P/8 + pb holds planes 8 pb .. 8 pb + 7 of field f,
+    // rows north (h = 0) or south
     auto issue = [&](int tc, int buf) {
         const int p = lane >> 3, t = ((lane & 7) - p) & 7;
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int ib = 3 * wave + i;   // wave-uniform
-            const int f = ib >> 2, h = (ib >> 1) & 1, half = ib & 1;
-            const int th = tc + t, k = k0 + 8 * half + p;
+        for (int i = 0; i < D::NB / 4; i++) {
+            const int ib = (D::NB / 4) * wave + i;   // wave-uniform
+            const int fh = ib / (P / 8), pb = ib % (P / 8);
+            const int f = fh >> 1, h = fh & 1;
+            const int th = tc + t, k = k0 + 8 * pb + p;
             const double2 *src = &ana_zero16;
             if (th < A.nth && k < A.nplanes)
                 src = A.in[f] + (((long)k * L + m) * A.nlat + (h ? A.nlat - 1 - th : th));
@@ -816,10 +834,13 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<T> &A, int m, int
         const unsigned ie = (unsigned)offe + rowoff, io = (unsigned)offo + rowoff;
         v.Pe = A.tP[ie];
         v.Po = A.tP[io];
-        v.De = A.tD[ie];
-        v.Do = A.tD[io];
-        v.Me = A.tM[ie];
-        v.Mo = A.tM[io];
+        v.De = v.Do = v.Me = v.Mo = T(0);
+        if (VEC) {
+            v.De = A.tD[ie];
+            v.Do = A.tD[io];
+            v.Me = A.tM[ie];
+            v.Mo = A.tM[io];
+        }
         return v;
     };
     const int nsteps = ((A.nth + ADC - 1) / ADC) * (ADC / 4);
@@ -827,8 +848,6 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<T> &A, int m, int
     Tab tb[2];
     tb[0] = ltab(kk);
     issue(0, 0);
-    // this lane's column j = 16 wj + col: plane pl = j / 2, part j & 1
-    const int pl = (16 * wj + col) >> 1, c = col & 1;
     for (int tc = 0; tc < A.nth; tc += ADC, buf ^= 1) {
         __syncthreads();
         if (tc + ADC < A.nth) issue(tc + ADC, buf ^ 1);
@@ -840,50 +859,61 @@ __device__ __forceinline__ void ana_tile_vec_dma(const AnaArgs<T> &A, int m, int
             tb[(st + 1) & 1] = ltab(4 * gn + kk);
             const Tab cur = tb[st & 1];
             if (live) {
-                const int sl = 8 * pl + ((tl + pl) & 7);
-                // FP64 in LDS; the FP32 form rounds on the operand read
-                auto at = [&](int f, int h, int part) { return (T)Gd[2 * (128 * (2 * f + h) + sl) + part]; };
-                const T rn = at(0, 0, c), rs = at(0, 1, c);
-                acc[0][0] = Mf<T>::mma(cur.Pe, rn + rs, acc[0][0]);
-                acc[0][1] = Mf<T>::mma(cur.Po, rn - rs, acc[0][1]);
-                const T tn = at(1, 0, c), ts = at(1, 1, c);
-                const T pn = at(2, 0, c), ps = at(2, 1, c);
-                const T mtn = sgn_mi * at(1, 0, c ^ 1), mts = sgn_mi * at(1, 1, c ^ 1);
-                const T mpn = sgn_mi * at(2, 0, c ^ 1), mps = sgn_mi * at(2, 1, c ^ 1);
-                // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
-                acc[1][0] = Mf<T>::mma(cur.De, tn - ts, acc[1][0]);
-                acc[1][0] = Mf<T>::mma(cur.Me, mpn + mps, acc[1][0]);
-                // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
-                acc[1][1] = Mf<T>::mma(cur.Do, tn + ts, acc[1][1]);
-                acc[1][1] = Mf<T>::mma(cur.Mo, mpn - mps, acc[1][1]);
-                // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
-                acc[2][0] = Mf<T>::mma(cur.Me, mtn + mts, acc[2][0]);
-                acc[2][0] = Mf<T>::mma(cur.De, ps - pn, acc[2][0]);
-                // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
-                acc[2][1] = Mf<T>::mma(cur.Mo, mtn - mts, acc[2][1]);
-                acc[2][1] = Mf<T>::mma(cur.Do, -(pn + ps), acc[2][1]);
+#pragma unroll
+                for (int b = 0; b < JB; b++) {
+                    // column j = 16 (wj JB + b) + col: plane j / 2, part j & 1
+                    const int pl = (16 * (wj * JB + b) + col) >> 1, c = col & 1;
+                    const int sl = 8 * pl + ((tl + pl) & 7);
+                    auto at = [&](int f, int h, int part) {
+                        return (T)Gd[2 * (8 * P * (2 * f + h) + sl) + part];
+                    };
+                    const T rn = at(0, 0, c), rs = at(0, 1, c);
+                    acc[0][0][b] = Mf<T>::mma(cur.Pe, rn + rs, acc[0][0][b]);
+                    acc[0][1][b] = Mf<T>::mma(cur.Po, rn - rs, acc[0][1][b]);
+                    if (VEC) {
+                        const T tn = at(1, 0, c), ts = at(1, 1, c);
+                        const T pn = at(2, 0, c), ps = at(2, 1, c);
+                        const T mtn = sgn_mi * at(1, 0, c ^ 1), mts = sgn_mi * at(1, 1, c ^ 1);
+                        const T mpn = sgn_mi * at(2, 0, c ^ 1), mps = sgn_mi * at(2, 1, c ^ 1);
+                        // S_e = dP_e (t_n - t_s) + mP_e (-i)(p_n + p_s)
+                        acc[1][0][b] = Mf<T>::mma(cur.De, tn - ts, acc[1][0][b]);
+                        acc[1][0][b] = Mf<T>::mma(cur.Me, mpn + mps, acc[1][0][b]);
+                        // S_o = dP_o (t_n + t_s) + mP_o (-i)(p_n - p_s)
+                        acc[1][1][b] = Mf<T>::mma(cur.Do, tn + ts, acc[1][1][b]);
+                        acc[1][1][b] = Mf<T>::mma(cur.Mo, mpn - mps, acc[1][1][b]);
+                        // T_e = mP_e (-i)(t_n + t_s) - dP_e (p_n - p_s)
+                        acc[2][0][b] = Mf<T>::mma(cur.Me, mtn + mts, acc[2][0][b]);
+                        acc[2][0][b] = Mf<T>::mma(cur.De, ps - pn, acc[2][0][b]);
+                        // T_o = mP_o (-i)(t_n - t_s) - dP_o (p_n + p_s)
+                        acc[2][1][b] = Mf<T>::mma(cur.Mo, mtn - mts, acc[2][1][b]);
+                        acc[2][1][b] = Mf<T>::mma(cur.Do, -(pn + ps), acc[2][1][b]);
+                    }
+                }
             }
         }
     }
 
     const long lm0 = lm_index(A.lmax, m, m);
-    const int k = k0 + pl;
-    if (k < A.nplanes) {
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-            double *out = (double *)A.out[f];
+    for (int f = 0; f < NF; f++) {
+        double *out = (double *)A.out[f];
 #pragma unroll
-            for (int par = 0; par < 2; par++)
+        for (int par = 0; par < 2; par++)
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const int j = (wj * JB + b) * 16 + col;
+                const int k = k0 + (j >> 1), c = j & 1;
+                if (k >= A.nplanes) continue;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int lp = lp0 + Mf<T>::row(kk, v);
                     const int l = m + par + 2 * lp;
                     if (l > A.lmax) continue;
-                    double val = acc[f][par][v];
+                    double val = acc[f][par][b][v];
                     if (m == 0 && c == 1) val = 0.0;
                     out[2 * (lm0 + (l - m) + k * A.ld) + c] = val;
                 }
-        }
+            }
     }
 }
 
@@ -896,16 +926,9 @@ __global__ __launch_bounds__(256, 4) void analysis_kernel(AnaArgs<T> A)
     const int jt = i % A.n_jt;
     const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
     if (ent < 0) return;
-    if constexpr (VEC) {
-        static_assert(ana_jb<T, VEC>() == 1, "the LDS-move tile covers one 16-column block per wave");
-        __shared__ double2 G[2][6 * 128];
-        ana_tile_vec_dma(A, ent & 0xffff, ent >> 16, jt, G);
-    } else {
-        using Cfg = AnaCfg<VEC, ana_jb<T, VEC>()>;
-        __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
-        __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
-        ana_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Gn, Gs);
-    }
+    static_assert(AnaDma<VEC>::JB == ana_jb<T, VEC>(), "the LDS-move tile and the launch agree on the tiling");
+    __shared__ double2 G[2][AnaDma<VEC>::SLOTS];
+    ana_tile_dma<T, VEC>(A, ent & 0xffff, ent >> 16, jt, G);
 }
 
 template <typename T>
