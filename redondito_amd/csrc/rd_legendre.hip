@@ -609,11 +609,11 @@ struct AnaArgs {
     int lmax, nlat, nth, nthp16, nplanes, n_jt;
 };
 
-// One analysis tile (m, l tile lt, plane tile jt) of one problem; T as in
+// One analysis tile (m, l tile lt, plane tile jt) of one problem, staged
+// through registers with in-step table loads: the tile of the persistent
+// batch kernels (the one-problem kernels use ana_tile_dma below); T as in
 // syn_tile.
-// PF: load the next step's table values one step ahead into the second of
-// two register sets (the one-problem kernels; off for the batch kernels).
-template <typename T, bool VEC, int JBV, bool PF = true>
+template <typename T, bool VEC, int JBV>
 __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int jt,
                                          T (&Gn)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP],
                                          T (&Gs)[AnaCfg<VEC, JBV>::NF][ATC][AnaCfg<VEC, JBV>::JTP])
@@ -664,11 +664,6 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
         }
         return v;
     };
-    const int nsteps = ((A.nth + ATC - 1) / ATC) * (ATC / 4);
-    int g = 0;
-    Tab tb[2] = {};
-    if constexpr (PF) tb[0] = ltab(kk);
-
     for (int tc = 0; tc < A.nth; tc += ATC) {
         constexpr int PER = (JT / 2) * ATC;
         for (int e = tid; e < NF * PER; e += 256) {
@@ -688,16 +683,9 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
         }
         __syncthreads();
 #pragma unroll
-        for (int st = 0; st < ATC / 4; st++, g++) {
+        for (int st = 0; st < ATC / 4; st++) {
             const int tl = 4 * st + kk;
-            Tab cur;
-            if constexpr (PF) {
-                const int gn = (g + 1 < nsteps) ? g + 1 : 0;   // the last step reloads row kk
-                tb[(st + 1) & 1] = ltab(4 * gn + kk);
-                cur = tb[st & 1];
-            } else {
-                cur = ltab(tc + tl);
-            }
+            const Tab cur = ltab(tc + tl);
             const T Pe = cur.Pe, Po = cur.Po, De = cur.De, Do = cur.Do, Me = cur.Me, Mo = cur.Mo;
             if (live) {
 #pragma unroll
@@ -1026,7 +1014,7 @@ __global__ __launch_bounds__(256) void batch_analysis_kernel(const AnaArgs<T> *p
         if (it >= nitems) return;
         const int4 w = items[it];
         const AnaArgs<T> A = probs[w.x];
-        ana_tile<T, false, JBV, false>(A, w.y, w.z, w.w, Gn, Gs);
+        ana_tile<T, false, JBV>(A, w.y, w.z, w.w, Gn, Gs);
     }
 }
 
