@@ -747,7 +747,12 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
 // planes, P = 16 for vector tiles, 64 for scalar ones) move from global
 // memory straight into LDS (global_load_lds_dwordx4), double-buffered: the
 // moves of chunk c + 1 are issued when chunk c starts, so the staging latency
-// overlaps the MFMAs and one barrier per chunk suffices.  One move writes 64
+// overlaps the MFMAs and one barrier per chunk suffices.  The moves are
+// inline asm (lds_move16) with explicit waits: each step issues exactly NT
+// table loads (six for vector tiles, two for scalar ones), so at the end of a
+// chunk a wait for all but the 2 NT newest vector-memory loads covers this
+// wave's moves of the next chunk, and a plain s_barrier publishes them; the
+// fence of __syncthreads would also drain the table prefetch in flight.  One move writes 64
 // consecutive 16-byte LDS slots (8 planes x 8 rows) from a wave-uniform base;
 // lane 8 p + s loads row (s - p) mod 8 of plane p, so that each plane's eight
 // lanes read one 128-byte segment and the MFMA operand reads (one row, eight
@@ -755,6 +760,23 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
 // h, plane pl, row t): 8 P (2 f + h) + 8 pl + (t + pl) mod 8.  LDS holds FP64;
 // the FP32 form rounds on the operand read.
 __device__ double2 ana_zero16 = {0.0, 0.0};
+
+// One 16-byte LDS move per lane: lane i's global element lands at slot i
+// after the wave-uniform LDS base.  Inline asm, because the compiler, seeing
+// an LDS move in flight, waits for it before the next barrier with a fence
+// (vmcnt(0)); the caller orders its moves with explicit vmcnt waits instead.
+// The instruction has no register result: its address operand is read at
+// issue.  Waits the compiler places for its own loads stay valid with these
+// moves in the count: vmcnt retires in order, so extra operations only make
+// them stricter.
+__device__ __forceinline__ void lds_move16(const double2 *src, double2 *lds)
+{
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) void *)lds;
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(base), "v"(src) : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(n) with expcnt and lgkmcnt left free (n < 64)
+#define RD_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
 
 constexpr int ADC = 8;   // rows per LDS-move chunk
 
@@ -808,8 +830,7 @@ __device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt,
             const double2 *src = &ana_zero16;
             if (th < A.nth && k < A.nplanes)
                 src = A.in[f] + (((long)k * L + m) * A.nlat + (h ? A.nlat - 1 - th : th));
-            __builtin_amdgcn_global_load_lds((const void *)src,
-                                             (__attribute__((address_space(3))) void *)&G[buf][64 * ib], 16, 0, 0);
+            lds_move16(src, &G[buf][64 * ib]);
         }
     };
     const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
@@ -833,12 +854,16 @@ __device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt,
     };
     const int nsteps = ((A.nth + ADC - 1) / ADC) * (ADC / 4);
     int g = 0, buf = 0;
+    constexpr int NT = VEC ? 6 : 2;   // table loads per step
     Tab tb[2];
     tb[0] = ltab(kk);
     issue(0, 0);
+    RD_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
     for (int tc = 0; tc < A.nth; tc += ADC, buf ^= 1) {
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
         if (tc + ADC < A.nth) issue(tc + ADC, buf ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
         const double *Gd = (const double *)G[buf];
 #pragma unroll
         for (int st = 0; st < ADC / 4; st++, g++) {
@@ -879,6 +904,9 @@ __device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt,
                 }
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
+        RD_WAIT_VMCNT(NT * (ADC / 4));   // this wave's moves of the next chunk have landed
+        __builtin_amdgcn_s_barrier();
     }
 
     const long lm0 = lm_index(A.lmax, m, m);
