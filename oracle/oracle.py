@@ -360,3 +360,69 @@ def v_imposebc_fast(fc, kz, z, ell, binv, d2b, lmax, tor, phi):
         phi[i] = d2b[k] @ pol[:N] - ll * pol[i] / z[i] ** 2
     phi[:, ell == 0] = 0
     return tor, phi, pol
+
+
+# ---------------------------------------------------------------------------
+# Extended-precision build of the same source (liboracle_ld.so): every step in
+# x87 long double (64-bit significand) on the same FP64 inputs and FC-Gram
+# tables.  Used as a higher-precision truth when two FP64 implementations
+# (this oracle and the GPU path) disagree by more than their rounding.
+_LIB_LD = os.path.join(_HERE, "_build", "liboracle_ld.so")
+_lib_ld = None
+_DL = ctypes.POINTER(ctypes.c_longdouble)
+_LDT = ctypes.c_longdouble
+
+
+def lib_ld():
+    global _lib_ld
+    if _lib_ld is None:
+        if not os.path.exists(_LIB_LD):
+            build()
+        L = ctypes.CDLL(_LIB_LD)
+        L.orc_sht_create.restype = _P
+        L.orc_sht_create.argtypes = [_I, _I, _I]
+        L.orc_sht_destroy.argtypes = [_P]
+        L.orc_sht_grid.argtypes = [_P, _DL, _DL, _DL]
+        L.orc_fc_dir.argtypes = [_I, _I, _DL, _DL, _DL]
+        L.orc_fc_create.restype = _P
+        L.orc_fc_create.argtypes = [_I, _I, _I, _DL]
+        L.orc_fc_destroy.argtypes = [_P]
+        L.orc_nonlinear_uxw.argtypes = [_P, _P, _LDT, _LDT, _DL, _DL, _DL, _DL, _DL, _I, _I]
+        _lib_ld = L
+    return _lib_ld
+
+
+def _dl(a):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.longdouble, np.clongdouble)
+    return a.ctypes.data_as(_DL)
+
+
+def nonlinear_uxw_ld(lmax, nlat, nphi, A, Q, rint, dz, W, Z, plane0=0, nplanes=None):
+    """u x omega as in nonlinear_uxw, computed in extended precision from the
+    FP64 inputs W, Z (n, nlm) and the FP64 tables A (C x d), Q (d x d).
+    Returns (NQ, NS, NT) as clongdouble arrays; rows outside
+    plane0 .. plane0 + nplanes - 1 are zero."""
+    L = lib_ld()
+    C, d = A.shape
+    n = W.shape[0]
+    if nplanes is None:
+        nplanes = n - C
+    Af = np.ascontiguousarray(np.asfortranarray(A).ravel(order="F"), np.longdouble)
+    Qf = np.ascontiguousarray(np.asfortranarray(Q).ravel(order="F"), np.longdouble)
+    dr = np.empty(C * d, np.longdouble)
+    L.orc_fc_dir(C, d, _dl(Af), _dl(Qf), _dl(dr))
+    # orc_fc_create takes dir column major, as orc_fc_dir writes it
+    fc = L.orc_fc_create(n, C, d, _dl(dr))
+    sht = L.orc_sht_create(lmax, nlat, nphi)
+    if not fc or not sht:
+        raise ValueError("bad sizes")
+    Wl = np.ascontiguousarray(W, np.clongdouble)
+    Zl = np.ascontiguousarray(Z, np.clongdouble)
+    out = [np.empty_like(Wl) for _ in range(3)]
+    try:
+        L.orc_nonlinear_uxw(sht, fc, _LDT(rint), _LDT(dz), _dl(Wl), _dl(Zl), *[_dl(o) for o in out],
+                            plane0, nplanes)
+    finally:
+        L.orc_sht_destroy(sht)
+        L.orc_fc_destroy(fc)
+    return out

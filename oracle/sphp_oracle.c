@@ -27,12 +27,24 @@
  * Layouts follow the reference: spectral a(lm, k) complex, lm fastest;
  * physical f(theta, phi, k) real, theta fastest.
  */
-#include <complex.h>
-#include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <tgmath.h>
 
+/* The arithmetic type.  The default build is FP64 (liboracle.so).  The same
+ * source built with -DORC_LONG_DOUBLE (liboracle_ld.so) runs every step in
+ * x87 extended precision (64-bit significand) on the same FP64 inputs and
+ * tables: that build serves as a higher-precision truth against which the
+ * rounding of two FP64 implementations (this one and the GPU's) is measured.
+ * Through <tgmath.h> each math call takes the precision of its argument. */
+#ifdef ORC_LONG_DOUBLE
+typedef long double real;
+typedef long double _Complex cplx;
+#else
+typedef double real;
 typedef double _Complex cplx;
+#endif
+#define ORC_PI ((real)3.14159265358979323846264338327950288L)
 
 /* ------------------------------------------------------------------ */
 /* Complex FFT: recursive mixed radix, generic radix-p butterflies.     */
@@ -58,7 +70,7 @@ static orc_fft *fft_create(int n)
     p->twf = malloc(sizeof(cplx) * n);
     p->twb = malloc(sizeof(cplx) * n);
     for (int t = 0; t < n; t++) {
-        double a = 2.0 * M_PI * (double)t / (double)n;
+        real a = 2.0 * ORC_PI * (real)t / (real)n;
         p->twf[t] = cos(a) - I * sin(a);
         p->twb[t] = cos(a) + I * sin(a);
     }
@@ -105,9 +117,9 @@ static void fft_exec(const orc_fft *p, int sign, const cplx *in, long is, cplx *
 
 /* Coefficients of P_n(cos t) = sum_{k=0..n} h_k cos((n - 2k) t),
  * h_k = g_k g_{n-k}, g_k = prod_{j<=k} (2j-1)/(2j). */
-void orc_gauss_coefs(int n, double *h)
+void orc_gauss_coefs(int n, real *h)
 {
-    double *g = malloc(sizeof(double) * (n + 1));
+    real *g = malloc(sizeof(real) * (n + 1));
     g[0] = 1.0;
     for (int k = 1; k <= n; k++) g[k] = g[k - 1] * (2.0 * k - 1.0) / (2.0 * k);
     for (int k = 0; k <= n; k++) h[k] = g[k] * g[n - k];
@@ -116,15 +128,15 @@ void orc_gauss_coefs(int n, double *h)
 
 /* P_n(cos t) and dP_n/dt from the cosine series, the derivative summed with
  * Neumaier compensation. */
-static void legendre_cos_series(int n, const double *h, double t, double *P, double *dP)
+static void legendre_cos_series(int n, const real *h, real t, real *P, real *dP)
 {
-    double p = (n % 2 == 0) ? h[n / 2] : 0.0;
-    double s = 0.0, c = 0.0;
+    real p = (n % 2 == 0) ? h[n / 2] : 0.0;
+    real s = 0.0, c = 0.0;
     for (int k = 0; k < (n + 1) / 2; k++) {
         int m = n - 2 * k;
         p += 2.0 * h[k] * cos(m * t);
-        double term = -2.0 * h[k] * m * sin(m * t);
-        double u = s + term;
+        real term = -2.0 * h[k] * m * sin(m * t);
+        real u = s + term;
         c += (fabs(s) >= fabs(term)) ? (s - u) + term : (term - u) + s;
         s = u;
     }
@@ -136,22 +148,22 @@ static void legendre_cos_series(int n, const double *h, double t, double *P, dou
  * P_n(cos theta) evaluated from its cosine series, which keeps theta (and so
  * sin theta) accurate to an ulp at the poles, where cos theta alone cannot;
  * w = 2 / (dP_n/dtheta)^2. */
-void orc_gauss(int nlat, double *ct, double *st, double *wt)
+void orc_gauss(int nlat, real *ct, real *st, real *wt)
 {
-    double *h = malloc(sizeof(double) * (nlat + 1));
+    real *h = malloc(sizeof(real) * (nlat + 1));
     orc_gauss_coefs(nlat, h);
     for (int i = 0; i < (nlat + 1) / 2; i++) {
-        double th = M_PI * (i + 0.75) / (nlat + 0.5);
-        double P, dP;
+        real th = ORC_PI * (i + 0.75) / (nlat + 0.5);
+        real P, dP;
         int extra = 0;
         for (int it = 0; it < 60; it++) {
             legendre_cos_series(nlat, h, th, &P, &dP);
-            double dth = P / dP;
+            real dth = P / dP;
             th -= dth;
             if (fabs(dth) < 1e-15 * th && ++extra > 2) break;
         }
         legendre_cos_series(nlat, h, th, &P, &dP);
-        double w = 2.0 / (dP * dP);
+        real w = 2.0 / (dP * dP);
         ct[i] = cos(th);
         st[i] = sin(th);
         wt[i] = w;
@@ -172,37 +184,37 @@ static inline long lm_index(int lmax, int l, int m)
  * each stored [lm][i] (m-major lm).  P~_lm = s^m Q_lm(x), the recurrence is
  * run on P~ and on D = s^m dQ/dx with a power-of-two scale that keeps the
  * sectoral start s^m representable. */
-void orc_legendre(int lmax, int nlat, const double *ct, const double *st,
-                  double *P, double *dP, double *mPs)
+void orc_legendre(int lmax, int nlat, const real *ct, const real *st,
+                  real *P, real *dP, real *mPs)
 {
     #pragma omp parallel for schedule(dynamic)
     for (int m = 0; m <= lmax; m++) {
         /* c_m = sqrt((2m+1)/(4 pi)) prod_{k=1..m} sqrt((2k-1)/(2k)) */
-        double cm = sqrt((2.0 * m + 1.0) / (4.0 * M_PI));
-        for (int k = 1; k <= m; k++) cm *= sqrt((2.0 * k - 1.0) / (2.0 * k));
+        real cm = sqrt((2.0 * m + 1.0) / (4.0 * ORC_PI));
+        for (int k = 1; k <= m; k++) cm *= sqrt((real)(2 * k - 1) / (2 * k));
         for (int i = 0; i < nlat; i++) {
-            double x = ct[i], s = st[i];
-            double v = cm;
+            real x = ct[i], s = st[i];
+            real v = cm;
             int e2 = 0;
             for (int k = 0; k < m; k++) {
                 v *= s;
                 if (v < 0x1p-400) { v *= 0x1p400; e2 -= 400; }
             }
             /* l = m */
-            double p_lm2 = 0.0, p_lm1 = v;     /* scaled P~ */
-            double d_lm2 = 0.0, d_lm1 = 0.0;   /* scaled s^m dQ/dx */
+            real p_lm2 = 0.0, p_lm1 = v;     /* scaled P~ */
+            real d_lm2 = 0.0, d_lm1 = 0.0;   /* scaled s^m dQ/dx */
             for (int l = m; l <= lmax; l++) {
-                double p, d;
+                real p, d;
                 if (l == m) {
                     p = p_lm1;
                     d = 0.0;
                 } else if (l == m + 1) {
-                    double a = sqrt(2.0 * m + 3.0);
+                    real a = sqrt((real)(2 * m + 3));
                     p = a * x * p_lm1;
                     d = a * p_lm1;
                 } else {
-                    double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-                    double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
+                    real a = sqrt((4.0 * l * l - 1.0) / ((real)l * l - (real)m * m));
+                    real b = sqrt(((l - 1.0) * (l - 1.0) - (real)m * m) /
                                     (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
                     p = a * (x * p_lm1 - b * p_lm2);
                     d = a * (p_lm1 + x * d_lm1 - b * d_lm2);
@@ -214,7 +226,7 @@ void orc_legendre(int lmax, int nlat, const double *ct, const double *st,
                     d_lm1 = 0.0;
                 }
                 long idx = lm_index(lmax, l, m) * nlat + i;
-                double pv = ldexp(p, e2), dv = ldexp(d, e2);
+                real pv = ldexp(p, e2), dv = ldexp(d, e2);
                 P[idx] = pv;
                 dP[idx] = (m * x * pv - s * s * dv) / s;
                 mPs[idx] = m * pv / s;
@@ -229,8 +241,8 @@ void orc_legendre(int lmax, int nlat, const double *ct, const double *st,
 typedef struct {
     int lmax, nlat, nphi;
     long nlm;
-    double *ct, *st, *wt;
-    double *P, *dP, *mPs;
+    real *ct, *st, *wt;
+    real *P, *dP, *mPs;
     orc_fft *fphi;
 } orc_sht;
 
@@ -242,13 +254,13 @@ void *orc_sht_create(int lmax, int nlat, int nphi)
     h->nlat = nlat;
     h->nphi = nphi;
     h->nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    h->ct = malloc(sizeof(double) * nlat);
-    h->st = malloc(sizeof(double) * nlat);
-    h->wt = malloc(sizeof(double) * nlat);
+    h->ct = malloc(sizeof(real) * nlat);
+    h->st = malloc(sizeof(real) * nlat);
+    h->wt = malloc(sizeof(real) * nlat);
     orc_gauss(nlat, h->ct, h->st, h->wt);
-    h->P = malloc(sizeof(double) * h->nlm * nlat);
-    h->dP = malloc(sizeof(double) * h->nlm * nlat);
-    h->mPs = malloc(sizeof(double) * h->nlm * nlat);
+    h->P = malloc(sizeof(real) * h->nlm * nlat);
+    h->dP = malloc(sizeof(real) * h->nlm * nlat);
+    h->mPs = malloc(sizeof(real) * h->nlm * nlat);
     orc_legendre(lmax, nlat, h->ct, h->st, h->P, h->dP, h->mPs);
     h->fphi = fft_create(nphi);
     return h;
@@ -264,25 +276,25 @@ void orc_sht_destroy(void *vh)
     free(h);
 }
 
-void orc_sht_grid(void *vh, double *ct, double *st, double *wt)
+void orc_sht_grid(void *vh, real *ct, real *st, real *wt)
 {
     orc_sht *h = vh;
-    memcpy(ct, h->ct, sizeof(double) * h->nlat);
-    memcpy(st, h->st, sizeof(double) * h->nlat);
-    memcpy(wt, h->wt, sizeof(double) * h->nlat);
+    memcpy(ct, h->ct, sizeof(real) * h->nlat);
+    memcpy(st, h->st, sizeof(real) * h->nlat);
+    memcpy(wt, h->wt, sizeof(real) * h->nlat);
 }
 
-void orc_sht_tables(void *vh, double *P, double *dP, double *mPs)
+void orc_sht_tables(void *vh, real *P, real *dP, real *mPs)
 {
     orc_sht *h = vh;
-    size_t nb = sizeof(double) * h->nlm * h->nlat;
+    size_t nb = sizeof(real) * h->nlm * h->nlat;
     memcpy(P, h->P, nb);
     memcpy(dP, h->dP, nb);
     memcpy(mPs, h->mPs, nb);
 }
 
 /* F[m*nlat + i] -> f[i + nlat*j]: f = Re F_0 + 2 Re sum_{m>0} F_m e^{i m phi_j} */
-static void phi_synth(const orc_sht *h, const cplx *F, double *f, cplx *X, cplx *Y, cplx *tmp)
+static void phi_synth(const orc_sht *h, const cplx *F, real *f, cplx *X, cplx *Y, cplx *tmp)
 {
     int nlat = h->nlat, nphi = h->nphi, lmax = h->lmax;
     for (int i = 0; i < nlat; i++) {
@@ -298,10 +310,10 @@ static void phi_synth(const orc_sht *h, const cplx *F, double *f, cplx *X, cplx 
 }
 
 /* f -> G[m*nlat + i] = (2 pi / nphi) sum_j f(i,j) e^{-i m phi_j} */
-static void phi_anal(const orc_sht *h, const double *f, cplx *G, cplx *X, cplx *Y, cplx *tmp)
+static void phi_anal(const orc_sht *h, const real *f, cplx *G, cplx *X, cplx *Y, cplx *tmp)
 {
     int nlat = h->nlat, nphi = h->nphi, lmax = h->lmax;
-    double sc = 2.0 * M_PI / nphi;
+    real sc = 2.0 * ORC_PI / nphi;
     for (int i = 0; i < nlat; i++) {
         for (int j = 0; j < nphi; j++) X[j] = f[i + (long)nlat * j];
         fft_exec(h->fphi, -1, X, 1, Y, tmp);
@@ -333,7 +345,7 @@ static void work_free(sht_work *w)
 }
 
 /* One plane of SH_to_spat: a(nlm) -> f(nlat, nphi). */
-static void sh_to_spat_plane(const orc_sht *h, const cplx *a, double *f, sht_work *w)
+static void sh_to_spat_plane(const orc_sht *h, const cplx *a, real *f, sht_work *w)
 {
     int nlat = h->nlat, lmax = h->lmax;
     for (int m = 0; m <= lmax; m++) {
@@ -350,7 +362,7 @@ static void sh_to_spat_plane(const orc_sht *h, const cplx *a, double *f, sht_wor
 }
 
 /* One plane of spat_to_SH: f -> a, a_lm = sum_i w_i P~_lm G_m(i). */
-static void spat_to_sh_plane(const orc_sht *h, const double *f, cplx *a, sht_work *w)
+static void spat_to_sh_plane(const orc_sht *h, const real *f, cplx *a, sht_work *w)
 {
     int nlat = h->nlat, lmax = h->lmax;
     phi_anal(h, f, w->F0, w->X, w->Y, w->tmp);
@@ -368,7 +380,7 @@ static void spat_to_sh_plane(const orc_sht *h, const double *f, cplx *a, sht_wor
 /* One plane of SHqst_to_spat:
  *   v_r = sum Q Y;  v_th = dS/dth + (1/s) dT/dph;  v_ph = (1/s) dS/dph - dT/dth */
 static void shqst_to_spat_plane(const orc_sht *h, const cplx *q, const cplx *s_, const cplx *t,
-                                double *vr, double *vt, double *vp, sht_work *w)
+                                real *vr, real *vt, real *vp, sht_work *w)
 {
     int nlat = h->nlat, lmax = h->lmax;
     for (int m = 0; m <= lmax; m++) {
@@ -376,8 +388,8 @@ static void shqst_to_spat_plane(const orc_sht *h, const cplx *q, const cplx *s_,
             cplx r = 0, th = 0, ph = 0;
             for (int l = m; l <= lmax; l++) {
                 long lm = lm_index(lmax, l, m);
-                double P = h->P[lm * nlat + i], dP = h->dP[lm * nlat + i];
-                double mP = h->mPs[lm * nlat + i];
+                real P = h->P[lm * nlat + i], dP = h->dP[lm * nlat + i];
+                real mP = h->mPs[lm * nlat + i];
                 r += P * q[lm];
                 th += dP * s_[lm] + I * mP * t[lm];
                 ph += I * mP * s_[lm] - dP * t[lm];
@@ -393,8 +405,8 @@ static void shqst_to_spat_plane(const orc_sht *h, const cplx *q, const cplx *s_,
 }
 
 /* One plane of spat_to_SHqst (S, T of l = 0 are zero). */
-static void spat_to_shqst_plane(const orc_sht *h, const double *vr, const double *vt,
-                                const double *vp, cplx *q, cplx *s_, cplx *t, sht_work *w)
+static void spat_to_shqst_plane(const orc_sht *h, const real *vr, const real *vt,
+                                const real *vp, cplx *q, cplx *s_, cplx *t, sht_work *w)
 {
     int nlat = h->nlat, lmax = h->lmax;
     phi_anal(h, vr, w->F0, w->X, w->Y, w->tmp);
@@ -406,14 +418,14 @@ static void spat_to_shqst_plane(const orc_sht *h, const double *vr, const double
             cplx aq = 0, as = 0, at = 0;
             for (int i = 0; i < nlat; i++) {
                 long k = (long)m * nlat + i;
-                double wi = h->wt[i];
-                double P = h->P[lm * nlat + i], dP = h->dP[lm * nlat + i];
-                double mP = h->mPs[lm * nlat + i];
+                real wi = h->wt[i];
+                real P = h->P[lm * nlat + i], dP = h->dP[lm * nlat + i];
+                real mP = h->mPs[lm * nlat + i];
                 aq += wi * P * w->F0[k];
                 as += wi * (dP * w->F1[k] - I * mP * w->F2[k]);
                 at += wi * (-I * mP * w->F1[k] - dP * w->F2[k]);
             }
-            double ll = (double)l * (l + 1);
+            real ll = (real)l * (l + 1);
             q[lm] = (m == 0) ? creal(aq) : aq;
             if (l == 0) {
                 s_[lm] = 0;
@@ -427,7 +439,7 @@ static void spat_to_shqst_plane(const orc_sht *h, const double *vr, const double
 }
 
 /* Plane loops.  Spectral a + k*nlm, physical f + k*nlat*nphi. */
-void orc_sh_to_spat(void *vh, int nplanes, const double *a_, double *f)
+void orc_sh_to_spat(void *vh, int nplanes, const real *a_, real *f)
 {
     orc_sht *h = vh;
     const cplx *a = (const cplx *)a_;
@@ -441,7 +453,7 @@ void orc_sh_to_spat(void *vh, int nplanes, const double *a_, double *f)
     }
 }
 
-void orc_spat_to_sh(void *vh, int nplanes, const double *f, double *a_)
+void orc_spat_to_sh(void *vh, int nplanes, const real *f, real *a_)
 {
     orc_sht *h = vh;
     cplx *a = (cplx *)a_;
@@ -455,8 +467,8 @@ void orc_spat_to_sh(void *vh, int nplanes, const double *f, double *a_)
     }
 }
 
-void orc_shqst_to_spat(void *vh, int nplanes, const double *q, const double *s, const double *t,
-                       double *vr, double *vt, double *vp)
+void orc_shqst_to_spat(void *vh, int nplanes, const real *q, const real *s, const real *t,
+                       real *vr, real *vt, real *vp)
 {
     orc_sht *h = vh;
     long np = (long)h->nlat * h->nphi;
@@ -472,8 +484,8 @@ void orc_shqst_to_spat(void *vh, int nplanes, const double *q, const double *s, 
     }
 }
 
-void orc_spat_to_shqst(void *vh, int nplanes, const double *vr, const double *vt, const double *vp,
-                       double *q, double *s, double *t)
+void orc_spat_to_shqst(void *vh, int nplanes, const real *vr, const real *vt, const real *vp,
+                       real *q, real *s, real *t)
 {
     orc_sht *h = vh;
     long np = (long)h->nlat * h->nphi;
@@ -494,22 +506,22 @@ void orc_spat_to_shqst(void *vh, int nplanes, const double *vr, const double *vt
 /* ------------------------------------------------------------------ */
 typedef struct {
     int n, C, d;
-    double *dir; /* C x d, column major: dir[(ii-1) + C*(jj-1)] */
+    real *dir; /* C x d, column major: dir[(ii-1) + C*(jj-1)] */
     orc_fft *fz;
 } orc_fc;
 
 /* dir = A . Q^T with A (C x d) and Q (d x d) column major (fcgram_mod.f90:257-258). */
-void orc_fc_dir(int C, int d, const double *A, const double *Q, double *dir)
+void orc_fc_dir(int C, int d, const real *A, const real *Q, real *dir)
 {
     for (int i = 0; i < C; i++)
         for (int j = 0; j < d; j++) {
-            double s = 0.0;
+            real s = 0.0;
             for (int k = 0; k < d; k++) s += A[i + (long)C * k] * Q[j + (long)d * k];
             dir[i + (long)C * j] = s;
         }
 }
 
-void *orc_fc_create(int n, int C, int d, const double *dir)
+void *orc_fc_create(int n, int C, int d, const real *dir)
 {
     if (n < 1 || C < 0 || (C > 0 && (d < 1 || n < C + 2 * d))) return NULL;
     orc_fc *p = calloc(1, sizeof(*p));
@@ -517,8 +529,8 @@ void *orc_fc_create(int n, int C, int d, const double *dir)
     p->C = C;
     p->d = d;
     if (C > 0) {
-        p->dir = malloc(sizeof(double) * C * d);
-        memcpy(p->dir, dir, sizeof(double) * C * d);
+        p->dir = malloc(sizeof(real) * C * d);
+        memcpy(p->dir, dir, sizeof(real) * C * d);
     }
     p->fz = fft_create(n);
     return p;
@@ -548,7 +560,7 @@ static void fc_continue_col(const orc_fc *p, cplx *col, long ld)
 }
 
 /* fftp1d_real_to_complex_z: continuation, forward c2c (sign -1), /n; in place. */
-void orc_z_forward(void *vp, int ncols, long ld, double *a_)
+void orc_z_forward(void *vp, int ncols, long ld, real *a_)
 {
     orc_fc *p = vp;
     cplx *a = (cplx *)a_;
@@ -562,14 +574,14 @@ void orc_z_forward(void *vp, int ncols, long ld, double *a_)
             if (p->C > 0) fc_continue_col(p, a + c, ld);
             for (int k = 0; k < n; k++) x[k] = a[c + k * ld];
             fft_exec(p->fz, -1, x, 1, y, tmp);
-            for (int k = 0; k < n; k++) a[c + k * ld] = y[k] / (double)n;
+            for (int k = 0; k < n; k++) a[c + k * ld] = y[k] / (real)n;
         }
         free(x); free(y); free(tmp);
     }
 }
 
 /* fftp1d_complex_to_real_z: backward c2c (sign +1), unnormalized, in place. */
-void orc_z_inverse(void *vp, int ncols, long ld, double *a_)
+void orc_z_inverse(void *vp, int ncols, long ld, real *a_)
 {
     orc_fc *p = vp;
     cplx *a = (cplx *)a_;
@@ -592,9 +604,9 @@ void orc_z_inverse(void *vp, int ncols, long ld, double *a_)
  * Even n keeps the Nyquist bin at -n/2 as the reference does; for odd n the
  * reference loop leaves index n unset, here the correct ordering
  * 0..(n-1)/2, -(n-1)/2..-1 is used. */
-void orc_kz(int n, double dz, double *kz)
+void orc_kz(int n, real dz, real *kz)
 {
-    double dk = 2.0 * M_PI / (dz * n);
+    real dk = 2.0 * ORC_PI / (dz * n);
     if (n % 2 == 0) {
         for (int k = 0; k < n / 2; k++) {
             kz[k] = k * dk;
@@ -606,7 +618,7 @@ void orc_kz(int n, double dz, double *kz)
 }
 
 /* derivr_k: b = i^p kz^p a (pseudospec_hd.f90:60-98). */
-void orc_derivr_k(int n, int ncols, long ld, const double *kz, int p, const double *a_, double *b_)
+void orc_derivr_k(int n, int ncols, long ld, const real *kz, int p, const real *a_, real *b_)
 {
     const cplx *a = (const cplx *)a_;
     cplx *b = (cplx *)b_;
@@ -615,19 +627,19 @@ void orc_derivr_k(int n, int ncols, long ld, const double *kz, int p, const doub
     else if (p == 4) ip = 1;
     #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
-        double kp = 1.0;
+        real kp = 1.0;
         for (int e = 0; e < p; e++) kp *= kz[k];
         for (int c = 0; c < ncols; c++) b[c + k * ld] = ip * kp * a[c + k * ld];
     }
 }
 
 /* derivr_r: copy, forward z, derivr_k, inverse z (pseudospec_hd.f90:101-135). */
-void orc_derivr_r(void *vp, const double *kz, int p, int ncols, long ld, const double *a, double *b)
+void orc_derivr_r(void *vp, const real *kz, int p, int ncols, long ld, const real *a, real *b)
 {
     orc_fc *fp = vp;
     int n = fp->n;
     size_t nb = sizeof(cplx) * (size_t)ld * n;
-    double *c1 = malloc(nb);
+    real *c1 = malloc(nb);
     memcpy(c1, a, nb);
     orc_z_forward(vp, ncols, ld, c1);
     orc_derivr_k(n, ncols, ld, kz, p, c1, b);
@@ -636,22 +648,22 @@ void orc_derivr_r(void *vp, const double *kz, int p, int ncols, long ld, const d
 }
 
 /* laplang: b = -l(l+1) a (pseudospec_hd.f90:138-172). */
-void orc_laplang(int n, int ncols, long ld, const int *ell, const double *a_, double *b_)
+void orc_laplang(int n, int ncols, long ld, const int *ell, const real *a_, real *b_)
 {
     const cplx *a = (const cplx *)a_;
     cplx *b = (cplx *)b_;
     for (int k = 0; k < n; k++)
         for (int c = 0; c < ncols; c++)
-            b[c + k * ld] = -(double)ell[c] * (ell[c] + 1) * a[c + k * ld];
+            b[c + k * ld] = -(real)ell[c] * (ell[c] + 1) * a[c + k * ld];
 }
 
 /* multr: b = z_k^p a (pseudospec_hd.f90:175-208). */
-void orc_multr(int n, int ncols, long ld, const double *z, int p, const double *a_, double *b_)
+void orc_multr(int n, int ncols, long ld, const real *z, int p, const real *a_, real *b_)
 {
     const cplx *a = (const cplx *)a_;
     cplx *b = (cplx *)b_;
     for (int k = 0; k < n; k++) {
-        double zp = 1.0;
+        real zp = 1.0;
         for (int e = 0; e < abs(p); e++) zp *= z[k];
         if (p < 0) zp = 1.0 / zp;
         for (int c = 0; c < ncols; c++) b[c + k * ld] = zp * a[c + k * ld];
@@ -659,19 +671,19 @@ void orc_multr(int n, int ncols, long ld, const double *z, int p, const double *
 }
 
 /* saxpby_c: zz = a x + b y, complex (pseudospec_hd.f90:291-326). */
-void orc_saxpby_c(long npts, double *zz, const double *x, double a, const double *y, double b)
+void orc_saxpby_c(long npts, real *zz, const real *x, real a, const real *y, real b)
 {
     for (long i = 0; i < 2 * npts; i++) zz[i] = a * x[i] + b * y[i];
 }
 
 /* dlr: b = d2a/dr2 - l(l+1) a / r^2 (pseudospec_hd.f90:254-288). */
-void orc_dlr(void *vp, const double *kz, const double *z, const int *ell, int ncols, long ld,
-             const double *a, double *b)
+void orc_dlr(void *vp, const real *kz, const real *z, const int *ell, int ncols, long ld,
+             const real *a, real *b)
 {
     orc_fc *fp = vp;
     int n = fp->n;
     size_t nb = sizeof(cplx) * (size_t)ld * n;
-    double *c1 = malloc(nb), *c2 = malloc(nb);
+    real *c1 = malloc(nb), *c2 = malloc(nb);
     memset(c1, 0, nb);
     memset(c2, 0, nb);
     orc_derivr_r(vp, kz, 2, ncols, ld, a, b);
@@ -683,15 +695,15 @@ void orc_dlr(void *vp, const double *kz, const double *z, const int *ell, int nc
 }
 
 /* prodvec: C = A x B in (r,theta,phi) on npts points (pseudospec_hd.f90:211-251). */
-void orc_prodvec(long npts, const double *ar, const double *at, const double *ap,
-                 const double *br, const double *bt, const double *bp,
-                 double *cr, double *ct, double *cp)
+void orc_prodvec(long npts, const real *ar, const real *at, const real *ap,
+                 const real *br, const real *bt, const real *bp,
+                 real *cr, real *ct, real *cp)
 {
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < npts; i++) {
-        double r = at[i] * bp[i] - ap[i] * bt[i];
-        double t = ap[i] * br[i] - ar[i] * bp[i];
-        double p = ar[i] * bt[i] - at[i] * br[i];
+        real r = at[i] * bp[i] - ap[i] * bt[i];
+        real t = ap[i] * br[i] - ar[i] * bp[i];
+        real p = ar[i] * bt[i] - at[i] * br[i];
         cr[i] = r;
         ct[i] = t;
         cp[i] = p;
@@ -700,13 +712,13 @@ void orc_prodvec(long npts, const double *ar, const double *at, const double *ap
 
 /* fc_filter_k: a *= exp(-alpha (2 kz/(n Dkz))^(2p)), alpha = 16 ln 10, p = 50
  * (pseudospec_hd.f90:405-437). */
-void orc_fc_filter_k(int n, int ncols, long ld, const double *kz, double dkz, double *a_)
+void orc_fc_filter_k(int n, int ncols, long ld, const real *kz, real dkz, real *a_)
 {
     cplx *a = (cplx *)a_;
-    double alpha = 16.0 * log(10.0);
+    real alpha = 16.0 * log(10.0);
     for (int k = 0; k < n; k++) {
-        double x = 2.0 * kz[k] / n / dkz;
-        double f = exp(-alpha * pow(x, 100.0));
+        real x = 2.0 * kz[k] / n / dkz;
+        real f = exp(-alpha * pow(x, 100.0));
         for (int c = 0; c < ncols; c++) a[c + k * ld] *= f;
     }
 }
@@ -726,8 +738,8 @@ void orc_fc_filter_k(int n, int ncols, long ld, const double *kz, double dkz, do
  * physical shells.  W and Z are not modified (the radial transforms work on
  * copies).  Rows nr..n-1 of the outputs are set to zero.
  */
-void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
-                       const double *W, const double *Z, double *NQ, double *NS, double *NT,
+void orc_nonlinear_uxw(void *vsht, void *vfc, real rint, real dz,
+                       const real *W, const real *Z, real *NQ, real *NS, real *NT,
                        int plane0, int nplanes_eval)
 {
     orc_sht *h = vsht;
@@ -735,14 +747,14 @@ void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
     int n = fp->n, nr = n - fp->C, lmax = h->lmax;
     long nlm = h->nlm;
     size_t nb = sizeof(cplx) * (size_t)nlm * n;
-    double *kz = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+    real *kz = malloc(sizeof(real) * n), *z = malloc(sizeof(real) * n);
     int *ell = malloc(sizeof(int) * nlm);
     orc_kz(n, dz, kz);
     for (int k = 0; k < n; k++) z[k] = rint + dz * k;
     for (int m = 0; m <= lmax; m++)
         for (int l = m; l <= lmax; l++) ell[lm_index(lmax, l, m)] = l;
 
-    double *dW = malloc(nb), *dlW = malloc(nb), *dZ = malloc(nb);
+    real *dW = malloc(nb), *dlW = malloc(nb), *dZ = malloc(nb);
     orc_derivr_r(fp, kz, 1, (int)nlm, nlm, W, dW);
     orc_dlr(fp, kz, z, ell, (int)nlm, nlm, W, dlW);
     orc_derivr_r(fp, kz, 1, (int)nlm, nlm, Z, dZ);
@@ -751,10 +763,10 @@ void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
     cplx *wq = malloc(nb), *ws = malloc(nb), *wt = malloc(nb);
     const cplx *cW = (const cplx *)W, *cZ = (const cplx *)Z;
     for (int k = 0; k < nr; k++) {
-        double r1 = 1.0 / z[k], r2 = 1.0 / (z[k] * z[k]);
+        real r1 = 1.0 / z[k], r2 = 1.0 / (z[k] * z[k]);
         for (long c = 0; c < nlm; c++) {
             long i = c + k * nlm;
-            double ll = (double)ell[c] * (ell[c] + 1);
+            real ll = (real)ell[c] * (ell[c] + 1);
             uq[i] = ll * cW[i] * r2;
             us[i] = ((cplx *)dW)[i] * r1;
             ut[i] = cZ[i] * r1;
@@ -772,7 +784,7 @@ void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
     #pragma omp parallel
     {
         sht_work w = work_alloc(h);
-        double *g = malloc(sizeof(double) * np * 9);
+        real *g = malloc(sizeof(real) * np * 9);
         #pragma omp for schedule(dynamic)
         for (int k = k0; k < k1; k++) {
             long o = k * nlm;
@@ -794,22 +806,22 @@ void orc_nonlinear_uxw(void *vsht, void *vfc, double rint, double dz,
 /* (Q,S,T) of v = curl curl (P r) + curl (Q r) and of curl v, as in
  * orc_nonlinear_uxw, on rows 0..nr-1 of freshly allocated (nlm, n) arrays
  * (rows nr..n-1 zero): vq, vs, vt, cq, cs, ct. */
-static void curl_qst(orc_fc *fp, const double *kz, const double *z, const int *ell, long nlm,
-                     const double *P, const double *Q, cplx *out[6])
+static void curl_qst(orc_fc *fp, const real *kz, const real *z, const int *ell, long nlm,
+                     const real *P, const real *Q, cplx *out[6])
 {
     int n = fp->n, nr = n - fp->C;
     size_t nb = sizeof(cplx) * (size_t)nlm * n;
-    double *dP = malloc(nb), *dlP = malloc(nb), *dQ = malloc(nb);
+    real *dP = malloc(nb), *dlP = malloc(nb), *dQ = malloc(nb);
     orc_derivr_r(fp, kz, 1, (int)nlm, nlm, P, dP);
     orc_dlr(fp, kz, z, ell, (int)nlm, nlm, P, dlP);
     orc_derivr_r(fp, kz, 1, (int)nlm, nlm, Q, dQ);
     for (int i = 0; i < 6; i++) out[i] = calloc(1, nb);
     const cplx *cP = (const cplx *)P, *cQ = (const cplx *)Q;
     for (int k = 0; k < nr; k++) {
-        double r1 = 1.0 / z[k], r2 = 1.0 / (z[k] * z[k]);
+        real r1 = 1.0 / z[k], r2 = 1.0 / (z[k] * z[k]);
         for (long c = 0; c < nlm; c++) {
             long i = c + k * nlm;
-            double ll = (double)ell[c] * (ell[c] + 1);
+            real ll = (real)ell[c] * (ell[c] + 1);
             out[0][i] = ll * cP[i] * r2;
             out[1][i] = ((cplx *)dP)[i] * r1;
             out[2][i] = cQ[i] * r1;
@@ -829,17 +841,17 @@ static void curl_qst(orc_fc *fp, const double *kz, const double *z, const int *e
  * (Q,S,T) vector (derivr(T), T/r, 0).  Same composition and layouts as
  * orc_nonlinear_uxw; the scalar analysis is fftp2d_real_to_complex_xy.
  */
-void orc_nonlinear_coupled(void *vsht, void *vfc, double rint, double dz,
-                           const double *W, const double *Z, const double *Pb, const double *Tb,
-                           const double *T, double *NQ, double *NS, double *NT, double *MQ, double *MS,
-                           double *MT, double *NH, int plane0, int nplanes_eval)
+void orc_nonlinear_coupled(void *vsht, void *vfc, real rint, real dz,
+                           const real *W, const real *Z, const real *Pb, const real *Tb,
+                           const real *T, real *NQ, real *NS, real *NT, real *MQ, real *MS,
+                           real *MT, real *NH, int plane0, int nplanes_eval)
 {
     orc_sht *h = vsht;
     orc_fc *fp = vfc;
     int n = fp->n, nr = n - fp->C, lmax = h->lmax;
     long nlm = h->nlm;
     size_t nb = sizeof(cplx) * (size_t)nlm * n;
-    double *kz = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+    real *kz = malloc(sizeof(real) * n), *z = malloc(sizeof(real) * n);
     int *ell = malloc(sizeof(int) * nlm);
     orc_kz(n, dz, kz);
     for (int k = 0; k < n; k++) z[k] = rint + dz * k;
@@ -850,11 +862,11 @@ void orc_nonlinear_coupled(void *vsht, void *vfc, double rint, double dz,
     curl_qst(fp, kz, z, ell, nlm, W, Z, uw);
     curl_qst(fp, kz, z, ell, nlm, Pb, Tb, bj);
     cplx *gq = malloc(nb), *gs = calloc(1, nb), *gt = calloc(1, nb);
-    orc_derivr_r(fp, kz, 1, (int)nlm, nlm, T, (double *)gq);
+    orc_derivr_r(fp, kz, 1, (int)nlm, nlm, T, (real *)gq);
     for (int k = 0; k < nr; k++)
         for (long c = 0; c < nlm; c++) gs[c + k * nlm] = ((const cplx *)T)[c + k * nlm] / z[k];
 
-    double *outs[7] = {NQ, NS, NT, MQ, MS, MT, NH};
+    real *outs[7] = {NQ, NS, NT, MQ, MS, MT, NH};
     for (int i = 0; i < 7; i++) memset(outs[i], 0, nb);
     int k0 = plane0, k1 = plane0 + nplanes_eval;
     if (k1 > nr) k1 = nr;
@@ -862,8 +874,8 @@ void orc_nonlinear_coupled(void *vsht, void *vfc, double rint, double dz,
     #pragma omp parallel
     {
         sht_work w = work_alloc(h);
-        double *g = malloc(sizeof(double) * np * 16);
-        double *u = g, *om = g + 3 * np, *b = g + 6 * np, *gr = g + 9 * np, *c = g + 12 * np,
+        real *g = malloc(sizeof(real) * np * 16);
+        real *u = g, *om = g + 3 * np, *b = g + 6 * np, *gr = g + 9 * np, *c = g + 12 * np,
                *d = g + 15 * np;
         #pragma omp for schedule(dynamic)
         for (int k = k0; k < k1; k++) {
@@ -894,12 +906,12 @@ void orc_nonlinear_coupled(void *vsht, void *vfc, double rint, double dz,
 
 /* derivr_mat (boundary_mod.f90:96-137): d[i + N*j] = d^p/dr^p of e_j at r_i,
  * on the N = n - C physical points. */
-void orc_derivr_mat(void *vp, const double *kz, int p, double *dmat)
+void orc_derivr_mat(void *vp, const real *kz, int p, real *dmat)
 {
     orc_fc *fp = vp;
     int n = fp->n, N = n - fp->C;
     size_t nb = sizeof(cplx) * (size_t)N * n;
-    double *a = calloc(1, nb), *b = calloc(1, nb);
+    real *a = calloc(1, nb), *b = calloc(1, nb);
     for (int j = 0; j < N; j++) a[2 * (j + (long)N * j)] = 1.0;
     orc_derivr_r(vp, kz, p, N, N, a, b);
     for (int j = 0; j < N; j++)
@@ -909,25 +921,25 @@ void orc_derivr_mat(void *vp, const double *kz, int p, double *dmat)
 }
 
 /* Dense LU with partial pivoting, column major; returns 0 on success. */
-static int lu_factor(int n, double *A, int *piv)
+static int lu_factor(int n, real *A, int *piv)
 {
     for (int k = 0; k < n; k++) {
         int pk = k;
-        double mx = fabs(A[k + (long)n * k]);
+        real mx = fabs(A[k + (long)n * k]);
         for (int i = k + 1; i < n; i++)
             if (fabs(A[i + (long)n * k]) > mx) { mx = fabs(A[i + (long)n * k]); pk = i; }
         piv[k] = pk;
         if (mx == 0.0) return k + 1;
         if (pk != k)
             for (int j = 0; j < n; j++) {
-                double t = A[k + (long)n * j];
+                real t = A[k + (long)n * j];
                 A[k + (long)n * j] = A[pk + (long)n * j];
                 A[pk + (long)n * j] = t;
             }
-        double inv = 1.0 / A[k + (long)n * k];
+        real inv = 1.0 / A[k + (long)n * k];
         for (int i = k + 1; i < n; i++) A[i + (long)n * k] *= inv;
         for (int j = k + 1; j < n; j++) {
-            double akj = A[k + (long)n * j];
+            real akj = A[k + (long)n * j];
             if (akj != 0.0)
                 for (int i = k + 1; i < n; i++) A[i + (long)n * j] -= A[i + (long)n * k] * akj;
         }
@@ -935,11 +947,11 @@ static int lu_factor(int n, double *A, int *piv)
     return 0;
 }
 
-static void lu_solve(int n, const double *LU, const int *piv, double *b)
+static void lu_solve(int n, const real *LU, const int *piv, real *b)
 {
     for (int k = 0; k < n; k++) {
         int pk = piv[k];
-        if (pk != k) { double t = b[k]; b[k] = b[pk]; b[pk] = t; }
+        if (pk != k) { real t = b[k]; b[k] = b[pk]; b[pk] = t; }
     }
     for (int j = 0; j < n; j++)
         for (int i = j + 1; i < n; i++) b[i] -= LU[i + (long)n * j] * b[j];
@@ -951,35 +963,35 @@ static void lu_solve(int n, const double *LU, const int *piv, double *b)
 
 /* invDl_setup (boundary_mod.f90:140-241): binv[(i) + N*(j) + N*(N-4)*(l-1)],
  * M_l = columns 3..N-2 of B_l^{-1}; returns 0 or the failing l. */
-int orc_invdl_setup(int N, int lmax, const double *d1, const double *d2, const double *z,
-                    double *binv)
+int orc_invdl_setup(int N, int lmax, const real *d1, const real *d2, const real *z,
+                    real *binv)
 {
-    double *B = malloc(sizeof(double) * N * N), *esc = malloc(sizeof(double) * N);
-    double *col = malloc(sizeof(double) * N);
+    real *B = malloc(sizeof(real) * N * N), *esc = malloc(sizeof(real) * N);
+    real *col = malloc(sizeof(real) * N);
     int *piv = malloc(sizeof(int) * N);
     int fail = 0;
     for (int l = 1; l <= lmax && !fail; l++) {
-        memset(B, 0, sizeof(double) * N * N);
+        memset(B, 0, sizeof(real) * N * N);
         B[0] = 1.0;
         for (int j = 0; j < N; j++) B[1 + (long)N * j] = d1[0 + (long)N * j];
         for (int i = 2; i < N - 2; i++) {
             for (int j = 0; j < N; j++) B[i + (long)N * j] = d2[i + (long)N * j];
-            B[i + (long)N * i] -= (double)l * (l + 1) / (z[i] * z[i]);
+            B[i + (long)N * i] -= (real)l * (l + 1) / (z[i] * z[i]);
         }
         for (int j = 0; j < N; j++) B[N - 2 + (long)N * j] = d1[N - 1 + (long)N * j];
         B[N - 1 + (long)N * (N - 1)] = 1.0;
         for (int i = 0; i < N; i++) {
-            double mx = 0.0;
+            real mx = 0.0;
             for (int j = 0; j < N; j++) mx = fmax(mx, fabs(B[i + (long)N * j]));
             esc[i] = mx;
             for (int j = 0; j < N; j++) B[i + (long)N * j] /= mx;
         }
         if (lu_factor(N, B, piv)) { fail = l; break; }
         for (int j = 0; j < N - 4; j++) {
-            memset(col, 0, sizeof(double) * N);
+            memset(col, 0, sizeof(real) * N);
             col[j + 2] = 1.0 / esc[j + 2];
             lu_solve(N, B, piv, col);
-            memcpy(binv + (long)N * j + (long)N * (N - 4) * (l - 1), col, sizeof(double) * N);
+            memcpy(binv + (long)N * j + (long)N * (N - 4) * (l - 1), col, sizeof(real) * N);
         }
     }
     free(B); free(esc); free(col); free(piv);
@@ -988,7 +1000,7 @@ int orc_invdl_setup(int N, int lmax, const double *d1, const double *d2, const d
 
 /* invDl (boundary_mod.f90:244-292): W = M_l phi(3:N-2) per lm, l = 0 -> 0;
  * rows N..n-1 of b are zero. */
-void orc_invdl(int N, int n, int lmax, const double *binv, const double *a_, double *b_)
+void orc_invdl(int N, int n, int lmax, const real *binv, const real *a_, real *b_)
 {
     const cplx *a = (const cplx *)a_;
     cplx *b = (cplx *)b_;
@@ -998,7 +1010,7 @@ void orc_invdl(int N, int n, int lmax, const double *binv, const double *a_, dou
     for (int m = 0; m <= lmax; m++)
         for (int l = (m > 1 ? m : 1); l <= lmax; l++) {
             long lm = lm_index(lmax, l, m);
-            const double *M = binv + (long)N * (N - 4) * (l - 1);
+            const real *M = binv + (long)N * (N - 4) * (l - 1);
             for (int i = 0; i < N; i++) {
                 cplx s = 0;
                 for (int j = 0; j < N - 4; j++) s += M[i + (long)N * j] * a[lm + nlm * (j + 2)];

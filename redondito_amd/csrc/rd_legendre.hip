@@ -748,11 +748,13 @@ __device__ __forceinline__ void ana_tile(const AnaArgs<T> &A, int m, int lt, int
 // memory straight into LDS (global_load_lds_dwordx4), double-buffered: the
 // moves of chunk c + 1 are issued when chunk c starts, so the staging latency
 // overlaps the MFMAs and one barrier per chunk suffices.  The moves are
-// inline asm (lds_move16) with explicit waits: each step issues exactly NT
-// table loads (six for vector tiles, two for scalar ones), so at the end of a
-// chunk a wait for all but the 2 NT newest vector-memory loads covers this
-// wave's moves of the next chunk, and a plain s_barrier publishes them; the
-// fence of __syncthreads would also drain the table prefetch in flight.  One move writes 64
+// inline asm (lds_move16) with explicit waits.  Every table load a chunk
+// needs (its second step's values and the next chunk's first) is issued
+// before that chunk's moves, and the moves are the last vector-memory
+// operations of the chunk, so vmcnt(0) at its end waits for exactly what was
+// issued before the moves plus the moves themselves: no count of the loads
+// the compiler emits enters the wait.  A plain s_barrier then publishes the
+// moves (the fence of __syncthreads is not needed).  One move writes 64
 // consecutive 16-byte LDS slots (8 planes x 8 rows) from a wave-uniform base;
 // lane 8 p + s loads row (s - p) mod 8 of plane p, so that each plane's eight
 // lanes read one 128-byte segment and the MFMA operand reads (one row, eight
@@ -768,11 +770,15 @@ __device__ double2 ana_zero16 = {0.0, 0.0};
 // The instruction has no register result: its address operand is read at
 // issue.  Waits the compiler places for its own loads stay valid with these
 // moves in the count: vmcnt retires in order, so extra operations only make
-// them stricter.
+// them stricter.  An LDS move reads M0 one wait state after an SALU write of
+// M0 at the earliest (a gfx9 hazard), hence the s_nop between the two; the
+// compiler itself uses M0 for nothing else in these kernels (the CPU tests
+// check the built assembly for both).
 __device__ __forceinline__ void lds_move16(const double2 *src, double2 *lds)
 {
     const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) void *)lds;
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(base), "v"(src) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(base), "v"(src)
+                 : "memory", "m0");
 }
 
 // s_waitcnt vmcnt(n) with expcnt and lgkmcnt left free (n < 64)
@@ -852,25 +858,27 @@ __device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt,
         }
         return v;
     };
+    static_assert(ADC == 8, "two MFMA steps of four rows per chunk");
     const int nsteps = ((A.nth + ADC - 1) / ADC) * (ADC / 4);
-    int g = 0, buf = 0;
-    constexpr int NT = VEC ? 6 : 2;   // table loads per step
-    Tab tb[2];
+    int buf = 0;
+    // tb[0]: this chunk's first step (already landed), tb[1]: its second,
+    // tb[2]: the next chunk's first (the last chunk reloads row kk)
+    Tab tb[3];
     tb[0] = ltab(kk);
     issue(0, 0);
     RD_WAIT_VMCNT(0);
     __builtin_amdgcn_s_barrier();
-    for (int tc = 0; tc < A.nth; tc += ADC, buf ^= 1) {
+    for (int tc = 0, g = 0; tc < A.nth; tc += ADC, g += ADC / 4, buf ^= 1) {
+        tb[1] = ltab(4 * (g + 1) + kk);
+        tb[2] = ltab(4 * ((g + 2 < nsteps) ? g + 2 : 0) + kk);
         __builtin_amdgcn_sched_barrier(0);
         if (tc + ADC < A.nth) issue(tc + ADC, buf ^ 1);
         __builtin_amdgcn_sched_barrier(0);
         const double *Gd = (const double *)G[buf];
 #pragma unroll
-        for (int st = 0; st < ADC / 4; st++, g++) {
+        for (int st = 0; st < ADC / 4; st++) {
             const int tl = 4 * st + kk;
-            const int gn = (g + 1 < nsteps) ? g + 1 : 0;   // the last step reloads row kk
-            tb[(st + 1) & 1] = ltab(4 * gn + kk);
-            const Tab cur = tb[st & 1];
+            const Tab cur = tb[st];
             if (live) {
 #pragma unroll
                 for (int b = 0; b < JB; b++) {
@@ -905,8 +913,9 @@ __device__ __forceinline__ void ana_tile_dma(const AnaArgs<T> &A, int m, int lt,
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        RD_WAIT_VMCNT(NT * (ADC / 4));   // this wave's moves of the next chunk have landed
+        RD_WAIT_VMCNT(0);   // this wave's moves of the next chunk (its last memory operations) have landed
         __builtin_amdgcn_s_barrier();
+        tb[0] = tb[2];
     }
 
     const long lm0 = lm_index(A.lmax, m, m);

@@ -277,3 +277,35 @@ def band_errors(out, ref, lmax, nr, names=("NQ", "NS", "NT")):
         rep[name] = d
     rep["joint"] = rell2_joint([o[:nr] for o in out], [r[:nr] for r in ref])
     return rep
+
+
+# ---------------------------------------------------------- fc_filter
+FILTER_ALPHA = 16.0 * np.log(10.0)
+
+
+def filter_multiplier_mp(k, n, p=50):
+    """exp(-alpha (2 k / n)^(2p)) in 40-digit arithmetic, alpha = 16 ln 10
+    (pseudospec_hd.f90:425-437, with kz / Dkz = k, the FFT-ordered bin)."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    x = mp.mpf(2 * int(k)) / n
+    return float(mp.exp(-16 * mp.log(10) * x ** (2 * p)))
+
+
+def filter_check(mult, n, dz):
+    """True when `mult` (the per-bin factor of a filter) agrees with the
+    closed form at every bin and is 1 exactly at kz = 0.  The bound follows
+    the formula's conditioning: x = 2 kz/(n Dkz) carries a few ulps (kz/Dkz is
+    not exactly k in FP64), and d ln f / d ln x = -100 alpha x^100, so four
+    ulps of x give 4 eps 100 alpha x^100 relative in f (up to 3e-13 at the
+    Nyquist bin), plus 1e-14."""
+    k = np.rint(_kz(n, dz) / (2 * np.pi / (dz * n))).astype(int)
+    ref = np.array([filter_multiplier_mp(kk, n) for kk in k])
+    x = 2.0 * k / n
+    tol = (1e-14 + 4 * 2.2e-16 * 100 * FILTER_ALPHA * x ** 100) * ref + 1e-30
+    return mult[0] == 1.0 and bool(np.all(np.abs(mult - ref) <= tol))
+
+
+def _kz(n, dz):
+    from oracle import oracle as O
+    return O.kz(n, dz)

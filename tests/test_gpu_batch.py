@@ -167,3 +167,70 @@ def test_batch_bad_arguments():
     with pytest.raises(S.RdError):
         b.config(3, 2)
     b.close()
+
+
+def test_batch_benchmark_draw_48_problems():
+    """The batching workload itself: the 48-problem draw of the benchmark
+    (lmax from {31, 63, 95, 127, 191, 255}, nlat = 3(lmax+1)/2, nphi = 2 nlat,
+    nr from {32, 64, 128}).  This reaches the batch's nphi = 576 and 768 phi
+    paths and the lmax 191 / 255 persistent work items.
+
+    * FP64 batch == single plans, bit for bit, for every problem (grid and
+      spectra, sphp_sca.F90:237-270, 400-433 per problem).
+    * The single plan of each distinct (lmax, nr) against the oracle: the 3D
+      inverse (z inverse, then synthesis per plane) and forward (analysis per
+      plane, then continuation and z forward) to 1e-12.
+    * The FP32-Legendre batch within 1e-5 of FP64, and its round trip on
+      band-limited, FC-Gram-consistent spectra within 1e-5."""
+    from bench import batch_problems
+    from oracle import oracle as O
+    from redondito_amd import sphp as S
+    probs = batch_problems()
+    assert len(probs) == 48 and max(p[0] for p in probs) == 255
+    assert {p[2] for p in probs} >= {576, 768}
+    spec = seeded_spectra(probs, 20261204)
+    b = S.Batch(probs, prec=S.RD_PREC_FP64)
+    g, out = grids(probs), specs(probs)
+    b.inverse(spec, g)
+    b.forward(g, out)
+    torch.cuda.synchronize()
+    A, Q = O.load_tables(S.TABLE_DIR, 25, 5)
+    dirc = O.fc_dir(A, Q)
+    seen = set()
+    for i, p in enumerate(probs):
+        plan = S.FCPlan(p[0], p[1], p[2], p[3], 25, 5, p[4], p[5])
+        gr = torch.zeros_like(g[i])
+        S.fftp3d_complex_to_real(plan, spec[i].clone(), gr)
+        so = torch.zeros_like(out[i])
+        S.fftp3d_real_to_complex(plan, g[i].clone(), so)
+        torch.cuda.synchronize()
+        assert torch.equal(g[i], gr), f"problem {i} {p[:4]}: grid"
+        assert torch.equal(out[i], so), f"problem {i} {p[:4]}: spectra"
+        key = (p[0], p[3])
+        if key not in seen:
+            seen.add(key)
+            sht = O.SHT(p[0], p[1], p[2])
+            fc = O.FC(p[3], 25, 5, dirc)
+            ref_g = sht.synth(fc.inverse(spec[i].cpu().numpy().copy()))
+            gh = g[i].cpu().numpy()
+            ref_s = fc.forward(sht.anal(gh))
+            eg, es = H.rell2(gh, ref_g), H.rell2(out[i].cpu().numpy(), ref_s)
+            print(f"lmax={p[0]} nphi={p[2]} nz={p[3]}: inverse {eg:.2e} forward {es:.2e} vs oracle", flush=True)
+            assert eg < 1e-12 and es < 1e-12
+        plan.close()
+    b.close()
+    del g
+    b32 = S.Batch(probs, prec=S.RD_PREC_FP32)
+    g32, o32 = grids(probs), specs(probs)
+    b32.inverse(spec, g32)
+    b32.forward(g32, o32)
+    torch.cuda.synchronize()
+    e32 = max(H.rell2(a.cpu().numpy(), r.cpu().numpy()) for a, r in zip(o32, out))
+    # round trip of band-limited, FC-Gram-consistent spectra (the FP64 forward output)
+    b32.inverse(out, g32)
+    b32.forward(g32, o32)
+    torch.cuda.synchronize()
+    rt = max(H.rell2(a.cpu().numpy(), r.cpu().numpy()) for a, r in zip(o32, out))
+    print(f"fp32 batch vs fp64 {e32:.2e}; fp32 round trip {rt:.2e}")
+    assert 0 < e32 < 1e-5 and 0 < rt < 1e-5
+    b32.close()

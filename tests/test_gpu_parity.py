@@ -4,6 +4,8 @@ Tolerances: the FP64 path must match the oracle to 1e-12 relative L2 (the
 bound set for the plain form); known-answer fields of the reference's tests
 are held to the levels LEEME.md records for its CPU path.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -303,7 +305,92 @@ def test_prueba_pseudo_operators_vs_oracle():
     assert out["derivr1_vs_oracle"] < tol[0] and out["derivr2_vs_oracle"] < tol[1] and out["dlr_vs_oracle"] < tol[2]
     assert out["derivr1_exact"] < 2 * 2.3e-7 and out["derivr2_exact"] < 2 * 3.6e-5 and out["dlr_exact"] < 2 * 2.8e-6
     assert out["derivr_k_vs_r"] < 1e-13 and out["laplang"] < 1e-15 and out["multr"] < 1e-15
-    assert out["saxpby"] < 1e-15 and out["copy"] == 0.0 and out["fc_filter_change"] < 1e-9
+    assert out["saxpby"] < 1e-15 and out["copy"] == 0.0
+    # LEEME.md:45 publishes a change of 1.7e-11 for this field; a no-op filter
+    # (a round trip, ~5e-16) fails the lower bound
+    assert 0.3 * 1.7e-11 < out["fc_filter_change"] < 3 * 1.7e-11
+
+
+@pytest.mark.parametrize("nz", [128, 153])
+def test_fc_filter_vs_oracle(nz):
+    """fc_filter (pseudospec_hd.f90:369-467) in both spaces against the oracle,
+    and its multiplier against the 40-digit closed form: exactly 1 at kz = 0,
+    1e-16 at the even-n Nyquist bin."""
+    S, plan, dz = _pseudo_setup(nz=nz)
+    nlm = plan.nlm
+    rng = np.random.default_rng(nz)
+    a = rng.standard_normal((nz, nlm)) + 1j * rng.standard_normal((nz, nlm))
+    kzv = O.kz(nz, dz)
+    assert np.max(np.abs(plan.grid()[4] - kzv)) <= 1e-15 * np.max(np.abs(kzv))
+    dkz = 2 * np.pi / (dz * nz)
+    A = to_dev(a)
+    S.fc_filter(plan, A, "k")
+    ref = O.fc_filter_k(a.copy(), kzv, dkz)
+    e_k = H.rell2(to_host(A), ref)
+    ones = to_dev(np.ones((nz, nlm), dtype=np.complex128))
+    S.fc_filter(plan, ones, "k")
+    mult = to_host(ones)
+    assert np.all(mult == mult[:, :1]) and np.all(mult.imag == 0)
+    assert H.filter_check(mult[:, 0].real, nz, dz)
+    if nz % 2 == 0:
+        assert abs(mult[nz // 2, 0].real - 1e-16) < 1e-29
+    # physical space: forward FC-Gram transform, filter, inverse
+    n_phys = nz - plan.C
+    fc = O.FC(nz, 25, 5, fc_dir())
+    B = to_dev(a)
+    S.fc_filter(plan, B, "r")
+    filt_r = lambda x: fc.inverse(O.fc_filter_k(fc.forward(x.copy()), kzv, dkz))[:n_phys]
+    ref_r = filt_r(a)
+    e_r = H.rell2(to_host(B)[:n_phys], ref_r)
+    # random columns through the FC-Gram continuation (entries ~4e3): the
+    # r-space result is held to its FP64 conditioning, as derivr is
+    floor = H.noise_floor(filt_r, [a])
+    print(f"nz={nz}: k-space {e_k:.2e}, r-space {e_r:.2e} (noise floor {floor:.2e})")
+    assert e_k < 1e-14 and e_r < max(TOL, 30 * floor)
+
+
+def test_redist_is_the_one_rank_copy():
+    """fftp3d_redist_k_to_lm / _lm_to_k (src/sphp/sphp_sca.F90:629-760): with one
+    task the exchange is a copy of the (nlm, planes) block, both directions,
+    for a full and a partial plane count; out == in is a no-op; a plane count
+    beyond nz is an error."""
+    S, plan, dz = _pseudo_setup(lmax=15, nz=40)
+    nz, nlm = plan.nz, plan.nlm
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((nz, nlm)) + 1j * rng.standard_normal((nz, nlm))
+    A = to_dev(a)
+    for fn in (S.fftp3d_redist_k_to_lm, S.fftp3d_redist_lm_to_k):
+        B = torch.full_like(A, 5.0)
+        fn(plan, A, B)
+        assert np.array_equal(to_host(B), a)
+        part = A[:7].clone()
+        Bp = torch.full_like(part, 5.0)
+        fn(plan, part, Bp)
+        assert np.array_equal(to_host(Bp), a[:7])
+        fn(plan, A, A)
+        assert np.array_equal(to_host(A), a)
+    L = S.lib()
+    assert L.rd_redist_k_to_lm(plan.h, nz + 1, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(A.data_ptr())) != 0
+
+
+def test_saxpby_r_vs_numpy():
+    """saxpby_r (pseudospec_hd.f90:329-366): z = a x + b y on the grid, z may
+    alias x or y (the update is pointwise)."""
+    S, plan, dz = _pseudo_setup(lmax=15, nz=40)
+    shp = (plan.nz, plan.nphi, plan.nlat)
+    rng = np.random.default_rng(6)
+    x, y = rng.standard_normal(shp), rng.standard_normal(shp)
+    X, Y = to_dev(x), to_dev(y)
+    Z = torch.zeros_like(X)
+    S.saxpby_r(plan, Z, X, 1.25, Y, -0.5)
+    ref = 1.25 * x - 0.5 * y
+    assert np.max(np.abs(to_host(Z) - ref)) <= 2.3e-16 * np.max(np.abs(ref))
+    S.saxpby_r(plan, X, X, 2.0, Y, 3.0)          # z = x
+    assert np.max(np.abs(to_host(X) - (2 * x + 3 * y))) <= 4.5e-16 * np.max(np.abs(2 * x + 3 * y))
+    Xp, Yp = to_dev(x[:5]), to_dev(y[:5])
+    Zp = torch.zeros_like(Xp)
+    S.saxpby_r(plan, Zp, Xp, 1.0, Yp, 1.0)       # a partial plane range
+    assert np.max(np.abs(to_host(Zp) - (x[:5] + y[:5]))) <= 4.5e-16 * np.max(np.abs(x[:5] + y[:5]))
 
 
 def test_prodvec_vs_hand_cross():
@@ -424,3 +511,43 @@ def test_nonlinear_solid_body_rotation():
          float(np.max(np.abs(to_host(NT)[:nr])))]
     print(f"solid body: NQ {e[0]:.2e} NS {e[1]:.2e} |NT| {e[2]:.2e}")
     assert e[0] < 1e-9 and e[1] < 1e-9 and e[2] < 1e-10
+
+
+def test_nonlinear_uxw_headline_shells_vs_extended_truth():
+    """u x omega at the benchmark size (lmax 255, nlat 384, nphi 768, nr 128,
+    the benchmark's seeded input) on shells 62..65, against the same
+    composition computed in x87 extended precision (liboracle_ld.so, 64-bit
+    significand, same FP64 inputs and tables).  NT is ~1700x smaller than NQ
+    here, so FP64 rounding of the large fields dominates its relative error;
+    the truth shows how much of it each FP64 implementation carries.  The GPU
+    must be at least as close to the truth as the FP64 oracle, field by
+    field (within 1.5x, plus 1e-13), and within 1e-12 jointly."""
+    S = sphp()
+    lmax, nr, C, d, rint, rext = 255, 128, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (rext - rint) / (nr - 1)
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    nphi = 2 * nlat
+    k0, nk = 62, 4
+    plan = S.FCPlan(lmax, nlat, nphi, n, C, d, dz, rint)
+    W, Z = H.shell_fields(lmax, nr, n, rint, rext, seed=20261203, slope=2.0)
+    Wd, Zd = to_dev(W), to_dev(Z)
+    out = [torch.zeros_like(Wd) for _ in range(3)]
+    S.nonlinear_uxw(plan, Wd, Zd, *out)
+    sl = slice(k0, k0 + nk)
+    got = [to_host(o)[sl] for o in out]
+    A, Q = O.load_tables(tables_dir(), C, d)
+    ref = [r[sl] for r in O.nonlinear_uxw(O.SHT(lmax, nlat, nphi), O.FC(n, C, d, O.fc_dir(A, Q)),
+                                          rint, dz, W, Z, k0, nk)]
+    tru = [t[sl].astype(np.complex128) for t in O.nonlinear_uxw_ld(lmax, nlat, nphi, A, Q, rint, dz, W, Z, k0, nk)]
+    e_gpu = [H.rell2(g, t) for g, t in zip(got, tru)]
+    e_orc = [H.rell2(r, t) for r, t in zip(ref, tru)]
+    j_gpu = H.rell2_joint(got, tru)
+    j_orc = H.rell2_joint(ref, tru)
+    j_go = H.rell2_joint(got, ref)
+    for nm, a, b_ in zip(("NQ", "NS", "NT"), e_gpu, e_orc):
+        print(f"{nm}: gpu vs truth {a:.3e}, oracle vs truth {b_:.3e}")
+    print(f"joint: gpu vs truth {j_gpu:.3e}, oracle vs truth {j_orc:.3e}, gpu vs oracle {j_go:.3e}")
+    assert j_gpu < TOL and j_go < TOL
+    assert all(a <= 1.5 * b_ + 1e-13 for a, b_ in zip(e_gpu, e_orc)), (e_gpu, e_orc)
+    plan.close()

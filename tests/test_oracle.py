@@ -275,6 +275,47 @@ def test_kz_conventions():
     assert np.allclose(k7, [0, 1, 2, 3, -3, -2, -1])
 
 
+# ------------------------------------------- extended-precision build
+def test_extended_build_nodes_vs_mpmath():
+    """liboracle_ld.so (the same source in x87 long double) really computes in
+    extended precision: its Gauss nodes and weights match 40-digit mpmath to
+    1e-18 relative, below what FP64 can hold."""
+    import ctypes
+    import mpmath as mp
+    L = O.lib_ld()
+    nlat = 48
+    h = L.orc_sht_create(15, nlat, 96)
+    ct, st, wt = (np.zeros(nlat, np.longdouble) for _ in range(3))
+    L.orc_sht_grid(h, O._dl(ct), O._dl(st), O._dl(wt))
+    L.orc_sht_destroy(h)
+    mp.mp.dps = 40
+    for i in range(nlat // 2):
+        xe = mp.mpf(np.format_float_positional(ct[i], unique=True))
+        we = mp.mpf(np.format_float_positional(wt[i], unique=True))
+        # 40-digit Newton from the node itself converges to the root it approximates
+        x = mp.findroot(lambda t: mp.legendre(nlat, t), xe)
+        dp = mp.diff(lambda t: mp.legendre(nlat, t), x)
+        w = 2 / ((1 - x ** 2) * dp ** 2)
+        assert abs(xe - x) < 1e-18 * abs(x) + 1e-19
+        assert abs(we - w) < 1e-17 * w
+
+
+def test_extended_build_matches_fp64_oracle():
+    """On a small shell the extended-precision u x omega agrees with the FP64
+    oracle to FP64 rounding (the same composition, finer arithmetic)."""
+    lmax, nr, C, d, ri, ro = 15, 24, 25, 5, 0.35, 1.0
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    nlat = 24
+    W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=3, slope=2.0)
+    A, Q = O.load_tables(tables(), C, d)
+    ref = O.nonlinear_uxw(O.SHT(lmax, nlat, 2 * nlat), O.FC(n, C, d, O.fc_dir(A, Q)), ri, dz, W, Z)
+    tru = O.nonlinear_uxw_ld(lmax, nlat, 2 * nlat, A, Q, ri, dz, W, Z)
+    assert all(t.dtype == np.clongdouble for t in tru)
+    e = H.rell2_joint([r[:nr] for r in ref], [t[:nr].astype(np.complex128) for t in tru])
+    assert 0 < e < 1e-12
+
+
 # ------------------------------------------------------------ operators
 def _pseudo(lmax=7, nz=128):
     C, d, rint, rext = 25, 5, 0.5, 1.5
@@ -320,7 +361,48 @@ def test_prueba_pseudo_levels():
     assert 0.3 * 3.6e-5 < e2 < 3 * 3.6e-5
     assert e3 < 1e-13 and e4 < 1e-15 and e5 < 1e-15 and e7 < 1e-15
     assert 0.3 * 2.8e-6 < e6 < 3 * 2.8e-6
-    assert e8 < 1e-9
+    # the published change of the smooth field, from both sides: a no-op
+    # filter (5e-16 here) fails the lower bound
+    assert 0.3 * 1.7e-11 < e8 < 3 * 1.7e-11
+
+
+@pytest.mark.parametrize("n", [128, 153, 49, 64])
+def test_fc_filter_multiplier_closed_form(n):
+    """fc_filter_k on a field of ones gives the multiplier per kz bin: exactly 1
+    at kz = 0, exp(-16 ln 10) = 1e-16 at the even-n Nyquist bin, and the
+    40-digit closed form everywhere.  A no-op filter and a p = 25 filter fail
+    the same check."""
+    dz = 0.7 / (n - 26)
+    dkz = 2 * np.pi / (dz * n)
+    kzv = O.kz(n, dz)
+    ones = np.ones((n, 3), dtype=np.complex128)
+    f = O.fc_filter_k(ones.copy(), kzv, dkz)
+    mult = f[:, 0].real
+    assert np.all(f.imag == 0) and np.all(f == f[:, :1])
+    assert H.filter_check(mult, n, dz)
+    if n % 2 == 0:
+        assert abs(mult[n // 2] - 1e-16) < 1e-29
+    # the check discriminates: a no-op and the p = 25 filter are rejected
+    k = np.rint(kzv / dkz)
+    assert not H.filter_check(np.ones(n), n, dz)
+    assert not H.filter_check(np.exp(-H.FILTER_ALPHA * (2 * k / n) ** 50), n, dz)
+
+
+def test_fc_filter_single_mode():
+    """A single radial mode at bin k comes back scaled by exp(-alpha (2k/n)^100),
+    every other bin stays zero."""
+    n, dz = 128, 1.0 / 102
+    dkz = 2 * np.pi / (dz * n)
+    kzv = O.kz(n, dz)
+    for k in (0, 5, 40, 51, 60, 63, 64, 100, 127):
+        a = np.zeros((n, 4), dtype=np.complex128)
+        a[k] = [1.0 + 2.0j, -0.5j, 3.0, 1e-3 - 1e-3j]
+        out = O.fc_filter_k(a.copy(), kzv, dkz)
+        fk = H.filter_multiplier_mp(round(kzv[k] / dkz), n)
+        assert np.all(out[np.arange(n) != k] == 0)
+        x = 2.0 * round(kzv[k] / dkz) / n
+        tol = (1e-14 + 4 * 2.2e-16 * 100 * H.FILTER_ALPHA * x ** 100) * fk
+        assert np.max(np.abs(out[k] - fk * a[k])) <= tol * np.max(np.abs(a[k])) + 1e-300
 
 
 def test_prodvec_hand_cross():
