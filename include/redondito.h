@@ -100,6 +100,20 @@ int rd_sht_analysis(rd_plan *plan, int nplanes, const double *in, double *out);
  * Replaces fftp2d_complex_to_real_xy, src/sphp/sphp_sca.F90:436-492. */
 int rd_sht_synthesis(rd_plan *plan, int nplanes, const double *in, double *out);
 
+/* Bytes of the fixed device scratch of rd_sht_synthesis_noalloc (64 MiB). */
+#define RD_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* rd_sht_synthesis with no allocation of any kind: the Fourier scratch is a
+ * fixed device array of RD_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends) instead of the plan's workspace; same kernels and results.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The plan must exist (created beforehand).
+ * Returns RD_ERR_ARG when 16 * nplanes * (lmax + 1) * nlat bytes exceed the
+ * scratch.  All calls on one GPU share its scratch: calls that overlap on one
+ * GPU (different streams, host threads) race. */
+int rd_sht_synthesis_noalloc(rd_plan *plan, int nplanes, const double *in, double *out);
+
 /* (v_r, v_theta, v_phi) -> (Q, S, T); S, T of l = 0 are 0.
  * Replaces fftp2d_vec_to_qst, src/sphp/sphp_vec.F90:49-106. */
 int rd_vsht_analysis(rd_plan *plan, int nplanes, const double *vr, const double *vt,

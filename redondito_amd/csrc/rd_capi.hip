@@ -275,6 +275,7 @@ static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, in
         }                            \
     } while (0)
 
+    TRYH(hipGetDevice(&p->device));
     TRYH(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
     p->stream = p->own_stream;
 

@@ -51,6 +51,7 @@ struct Plan {
     int lmax, L, nlat, nphi, nlm, nth, nthp, nthp16;
     int nz, C, d, nr;
     double dz, rint;
+    int device;                    // the device current at creation (the plan's tables live there)
     hipStream_t stream;
     hipStream_t own_stream;
     // grid (host copies)

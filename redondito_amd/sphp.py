@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "librdsphp.so")
 TABLE_DIR = os.path.join(_HERE, "tables")
 
 RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3, RD_PREC_OZAKI = 0, 1, 2, 3
+RD_NOALLOC_SCRATCH_BYTES = 64 << 20   # fixed scratch of rd_sht_synthesis_noalloc (include/redondito.h)
 
 _lib = None
 
@@ -47,7 +48,8 @@ def lib():
         L.rd_plan_sync.argtypes = [vp]
         L.rd_plan_info.argtypes = [vp, ctypes.POINTER(i)]
         L.rd_plan_grid.argtypes = [vp] + [ctypes.POINTER(d)] * 5 + [ctypes.POINTER(i)] * 2
-        for nm in ("rd_sht_analysis", "rd_sht_synthesis", "rd_redist_k_to_lm", "rd_redist_lm_to_k"):
+        for nm in ("rd_sht_analysis", "rd_sht_synthesis", "rd_sht_synthesis_noalloc", "rd_redist_k_to_lm",
+                   "rd_redist_lm_to_k"):
             getattr(L, nm).argtypes = [vp, i, vp, vp]
         for nm in ("rd_vsht_analysis", "rd_vsht_synthesis"):
             getattr(L, nm).argtypes = [vp, i] + [vp] * 6
