@@ -48,6 +48,14 @@ const char *rd_error_string(int code);
  * Replaces load_dirichlet_tables, src/sphp/fcgram_mod.f90:186-263. */
 int rd_load_dirichlet_tables(const char *tdir, int C, int d, double *dir);
 
+/* Neumann reconstructor of order ord (1: first, 2: second normal derivative)
+ * for matching points at spacing dx: reads Q{d}.dat and Q{ord}n{d}.dat (a
+ * leading FP64 dxp, then Qn, d x d column major) from tdir and forms
+ * neu = Q(d,:) Qn^T with neu(d) scaled by (dx/dxp)^ord, d values, on the host.
+ * An undefined order prints a warning and returns RD_OK without writing.
+ * Replaces load_neumann_tables, src/sphp/fcgram_mod.f90:267-371. */
+int rd_load_neumann_tables(const char *tdir, int d, int ord, double dx, double *neu);
+
 /* Creates the plan: Gauss-Legendre grid, associated-Legendre tables (FP64,
  * built on the device), phi and radial FFT twiddles, the FC-Gram operator
  * dir (C x d column major, may be NULL when C = 0), the radial grid
@@ -182,6 +190,22 @@ int rd_prodvec(rd_plan *plan, int nplanes, const double *ar, const double *at, c
  * fc_filter_k, fc_filter_r, pseudospec_hd.f90:369-467. */
 int rd_fc_filter(rd_plan *plan, char esp, double *a);
 
+/* Wall value of every lm column of the spectral array a (lm, nz) (device)
+ * from its normal derivative: row 0 (boun 5, the inner radius z(1)) or row
+ * nz - C - 1 (boun 6, z(nz - C)) holds the ord-th outward normal derivative
+ * on entry and the reconstructed value on exit, from the d - 1 matching rows
+ * next to the wall and the weights neu of rd_load_neumann_tables (host, d
+ * values).  Other boundaries or orders print the reference's message and
+ * return RD_ERR_ARG.  Replaces neumann_reconstruct along z,
+ * src/sphp/fcgram_mod.f90:374-517 (z is the radial row index here). */
+int rd_neumann_reconstruct(rd_plan *plan, double *a, int boun, int ord, const double *neu);
+
+/* As rd_neumann_reconstruct for the Robin condition f' + coef f = g with one
+ * real coef per lm column (device array of nlm): the value is divided by
+ * coef neu(d) + 1.  neu is the first-order table.  Replaces
+ * robin_reconstruct along z, src/sphp/fcgram_mod.f90:520-650. */
+int rd_robin_reconstruct(rd_plan *plan, double *a, int boun, const double *neu, const double *coef);
+
 /* b = a on (nlm, nz).  Replaces copy, pseudospec_hd.f90:470-497. */
 int rd_copy(rd_plan *plan, const double *a, double *b);
 
@@ -261,9 +285,14 @@ int rd_device_sync(void);
 int rd_dev_alloc(const void *host, size_t nbytes);
 /* gdev_free, gdevice_mod.f90:196-217. */
 int rd_dev_free(const void *host);
-/* Device address of a registered host address, NULL if none
- * (gdev_lookup, gdevice_mod.f90:220-235). */
+/* Device address of a registered host address (an interior address resolves
+ * to the same offset in the device copy), NULL if none. */
 void *rd_dev_lookup(const void *host);
+/* 1-based position of the array registered at exactly this host address in
+ * the registry, 0 if none (gdev_lookup, gdevice_mod.f90:220-235; positions
+ * follow the host addresses and change as arrays come and go, as the
+ * reference's registry slots do on gdev_free). */
+int rd_dev_index(const void *host);
 /* gdev_present, gdevice_mod.f90:305-317. */
 int rd_dev_present(const void *host);
 /* gdev_update_to / gdev_update_from, gdevice_mod.f90:238-279. */
@@ -306,6 +335,31 @@ int rd_batch_config(rd_batch *b, int jb, int wg_per_cu);
  * reports the fixed default (4, 2) and the chosen times and the choice. */
 int rd_batch_tune(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out,
                   int reps, double *ms_fixed, double *ms_tuned, int *choice);
+
+/* The nonlinear-term form of the batch: rd_nonlinear_uxw of every problem,
+ * with the radial step, the fused phi transforms and the cross product per
+ * problem (spread over the lanes) and the Legendre stage of all problems
+ * (u and omega syntheses, then the analysis) as one persistent launch per
+ * direction.  prec RD_PREC_FP64 or RD_PREC_FP32.  Every problem needs nz > C.
+ * The round-trip calls (rd_batch_3d_*, rd_batch_capture, rd_batch_tune)
+ * return RD_ERR_STATE on such a batch; rd_batch_replay, rd_batch_config
+ * (plane-tile blocks 1 or 2) and rd_batch_info apply.  No reference routine:
+ * the composition of rd_nonlinear_uxw, applied to many problems. */
+int rd_batch_create_uxw(rd_batch **b, int nprob, const int *lmax, const int *nlat, const int *nphi,
+                        const int *nz, int C, int d, const double *dz, const double *rint, const double *dir,
+                        int prec);
+/* W, Z -> NQ, NS, NT per problem (arrays of nprob device pointers, each
+ * (nlm, nz) complex); W and Z are not modified. */
+int rd_batch_nonlinear_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                           double *const *NS, double *const *NT);
+/* Captures one evaluation of every problem into one hipGraph
+ * (rd_batch_replay launches it on the bound stream). */
+int rd_batch_capture_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                         double *const *NS, double *const *NT);
+/* As rd_batch_tune, timing evaluations; the fixed default is (1, 2). */
+int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                      double *const *NS, double *const *NT, int reps, double *ms_fixed, double *ms_tuned,
+                      int *choice);
 
 #ifdef __cplusplus
 }

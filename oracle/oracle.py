@@ -58,6 +58,9 @@ def lib():
         L.orc_dlr.argtypes = [_P, _D, _D, ctypes.POINTER(ctypes.c_int), _I, _L, _D, _D]
         L.orc_prodvec.argtypes = [_L] + [_D] * 9
         L.orc_fc_filter_k.argtypes = [_I, _I, _L, _D, ctypes.c_double, _D]
+        L.orc_neumann_table.argtypes = [_I, _I, ctypes.c_double, _D, _D, ctypes.c_double, _D]
+        L.orc_fc_reconstruct.restype = _I
+        L.orc_fc_reconstruct.argtypes = [_I, _I, _I, _I, _L, _I, _D, _D, _D]
         L.orc_nonlinear_uxw.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double,
                                         _D, _D, _D, _D, _D, _I, _I]
         L.orc_nonlinear_coupled.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double] + [_D] * 12 + [_I, _I]
@@ -276,6 +279,38 @@ def prodvec(a, b):
 def fc_filter_k(a, kzv, dkz):
     a = np.ascontiguousarray(a)
     lib().orc_fc_filter_k(a.shape[0], a.shape[1], a.shape[1], _d(kzv), dkz, _d(a))
+    return a
+
+
+def load_neumann_files(tdir, d, ord_):
+    """Q (d x d) and (dxp, Qn) as load_neumann_tables reads them
+    (fcgram_mod.f90:325-353): Q{d}.dat, then Q1n{d}.dat or Q2n{d}.dat with a
+    leading FP64 dxp; stream FP64, column major."""
+    Q = np.fromfile(os.path.join(tdir, f"Q{d}.dat"), "<f8")
+    b = np.fromfile(os.path.join(tdir, f"Q{ord_}n{d}.dat"), "<f8")
+    if Q.size != d * d or b.size != 1 + d * d:
+        raise ValueError("Neumann table size mismatch")
+    return Q.reshape(d, d).T.copy(), float(b[0]), b[1:].reshape(d, d).T.copy()
+
+
+def neumann_table(d, ord_, dx, Q, Qn, dxp):
+    """neu of load_neumann_tables (fcgram_mod.f90:355-363)."""
+    Qf = np.ascontiguousarray(np.asfortranarray(Q).ravel(order="F"))
+    Qnf = np.ascontiguousarray(np.asfortranarray(Qn).ravel(order="F"))
+    neu = np.empty(d)
+    lib().orc_neumann_table(d, ord_, dx, _d(Qf), _d(Qnf), dxp, _d(neu))
+    return neu
+
+
+def fc_reconstruct(a, C, d, boun, neu, coef=None):
+    """neumann_reconstruct (coef None) or robin_reconstruct (coef: one real
+    per column) along z on a (n, ncols) complex array; returns a copy."""
+    a = np.ascontiguousarray(a, np.complex128).copy()
+    neu = np.ascontiguousarray(neu, np.float64)
+    cp = None if coef is None else _d(np.ascontiguousarray(coef, np.float64))
+    rc = lib().orc_fc_reconstruct(a.shape[0], C, d, a.shape[1], a.shape[1], boun, _d(neu), cp, _d(a))
+    if rc:
+        raise ValueError("unsupported boundary")
     return a
 
 

@@ -723,6 +723,47 @@ void orc_fc_filter_k(int n, int ncols, long ld, const real *kz, real dkz, real *
     }
 }
 
+/* Neumann reconstructor of order ord (1 or 2) from Q (d x d) and Qn (d x d),
+ * both column major, read with the table spacing dxp:
+ * neu = Q(d,:) Qn^T, neu(d) *= (dx / dxp)^ord (fcgram_mod.f90:355-363). */
+void orc_neumann_table(int d, int ord, real dx, const real *Q, const real *Qn, real dxp, real *neu)
+{
+    for (int j = 0; j < d; j++) {
+        real s = 0.0;
+        for (int k = 0; k < d; k++) s += Q[(d - 1) + (long)d * k] * Qn[j + (long)d * k];
+        neu[j] = s;
+    }
+    const real f = dx / dxp;
+    neu[d - 1] *= (ord == 2) ? f * f : f;
+}
+
+/* neumann_reconstruct / robin_reconstruct along z (boundaries 5, z(1), and
+ * 6, z(nz - Cz); fcgram_mod.f90:466-517, 612-650) on (ncols, n) complex
+ * arrays with row stride ld, z being the row index as in the spectral (lm, nz)
+ * arrays of the path.  The boundary row holds the normal derivative g on
+ * entry and the reconstructed value on exit:
+ *   f_b = neu(d) g + sum_{k=1}^{d-1} neu(k) f(row_k),
+ * rows d+1-k (boundary 5) or nz-Cz-d+k (boundary 6), 1-based; with coef
+ * (one real per column, Robin f' + a f = g) f_b is further divided by
+ * a neu(d) + 1.  Returns 0, or -1 for another boundary. */
+int orc_fc_reconstruct(int n, int C, int d, int ncols, long ld, int boun, const real *neu, const real *coef,
+                       real *a_)
+{
+    cplx *a = (cplx *)a_;
+    if (boun != 5 && boun != 6) return -1;
+    const int rb = (boun == 5) ? 0 : n - C - 1;
+    for (int c = 0; c < ncols; c++) {
+        cplx v = neu[d - 1] * a[c + (long)rb * ld];
+        for (int k = 1; k <= d - 1; k++) {
+            const int row = (boun == 5) ? d - k : n - C - 1 - d + k;
+            v += neu[k - 1] * a[c + (long)row * ld];
+        }
+        if (coef) v = v / (coef[c] * neu[d - 1] + 1);
+        a[c + (long)rb * ld] = v;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Nonlinear term u x omega, composed from the reference operators      */
 /* ------------------------------------------------------------------ */

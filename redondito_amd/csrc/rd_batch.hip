@@ -28,6 +28,7 @@ constexpr int kLanes = 4;   // streams the per-problem stages are spread over
 
 struct rd_batch {
     int nprob, prec;
+    int uxw;                          // the nonlinear-term form (rd_batch_create_uxw)
     std::vector<Plan *> plans;
     hipStream_t stream, own_stream;
     hipStream_t side[kLanes - 1];   // default priority, like every stream here
@@ -141,13 +142,74 @@ int enqueue_forward(rd_batch *b, const double *const *grid)
     return per_problem(b, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
 }
 
+// The nonlinear term of every problem (rd_nonlinear_uxw per problem): the
+// radial steps, then one persistent vector synthesis over the u and omega
+// sets of all problems, the fused phi transforms and cross products, one
+// persistent vector analysis, and the zeroed continuation rows.
+int bind_uxw(rd_batch *b, double *const *NQ, double *const *NS, double *const *NT)
+{
+    const int np = b->nprob;
+    std::vector<const double2 *> sin(np * 6), ain(np * 3);
+    std::vector<double2 *> sout(np * 6), aout(np * 3);
+    for (int p = 0; p < np; p++) {
+        Plan *pl = b->plans[p];
+        for (int f = 0; f < 6; f++) {
+            sin[p * 6 + f] = pl->S[f];
+            sout[p * 6 + f] = pl->F[f];
+        }
+        for (int f = 0; f < 3; f++) ain[p * 3 + f] = pl->F[f];
+        aout[p * 3 + 0] = (double2 *)NQ[p];
+        aout[p * 3 + 1] = (double2 *)NS[p];
+        aout[p * 3 + 2] = (double2 *)NT[p];
+    }
+    return batch_vleg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
+}
+
+int enqueue_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                double *const *NS, double *const *NT)
+{
+    int rc = per_problem(b, [&](int i) {
+        Plan *p = b->plans[i];
+        return launch_uxw_radial(p, (const double2 *)W[i], (const double2 *)Z[i], p->S);
+    });
+    if (rc) return rc;
+    rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    rc = per_problem(b, [&](int i) {
+        Plan *p = b->plans[i];
+        const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
+        double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
+        return launch_phi_cross(p, p->nr, Fin, Fout, b->prec);
+    });
+    if (rc) return rc;
+    rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    return per_problem(b, [&](int i) {
+        Plan *p = b->plans[i];
+        int r = launch_zero_rows(p, (double2 *)NQ[i], p->nr, p->nz - p->nr);
+        if (!r) r = launch_zero_rows(p, (double2 *)NS[i], p->nr, p->nz - p->nr);
+        if (!r) r = launch_zero_rows(p, (double2 *)NT[i], p->nr, p->nz - p->nr);
+        return r;
+    });
+}
+
+bool uxw_args_ok(const rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                 double *const *NS, double *const *NT)
+{
+    if (!b || !b->uxw || !W || !Z || !NQ || !NS || !NT) return false;
+    for (int i = 0; i < b->nprob; i++)
+        if (!W[i] || !Z[i] || !NQ[i] || !NS[i] || !NT[i]) return false;
+    return true;
+}
+
 int configure(rd_batch *b, int jb, int wg_per_cu)
 {
-    if ((jb != 2 && jb != 4) || wg_per_cu < 1 || wg_per_cu > 8) return RD_ERR_ARG;
+    const bool okjb = b->uxw ? (jb == 1 || jb == 2) : (jb == 2 || jb == 4);
+    if (!okjb || wg_per_cu < 1 || wg_per_cu > 8) return RD_ERR_ARG;
     drop_graph(b);
     RD_HIP(hipStreamSynchronize(b->stream));
     batch_leg_free(&b->leg);
-    int rc = batch_leg_init(&b->leg, b->plans.data(), b->nprob, b->prec, jb);
+    int rc = batch_leg_init(&b->leg, b->plans.data(), b->nprob, b->prec, jb, b->uxw, b->uxw ? 2 : 1);
     if (rc) return rc;
     b->jb = jb;
     b->wg_per_cu = wg_per_cu;
@@ -178,8 +240,9 @@ void destroy(rd_batch *b)
 
 extern "C" {
 
-int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat, const int *nphi, const int *nz,
-                    int C, int d, const double *dz, const double *rint, const double *dir, int prec)
+static int batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat, const int *nphi,
+                        const int *nz, int C, int d, const double *dz, const double *rint, const double *dir,
+                        int prec, int uxw)
 {
     if (!out || nprob < 1 || !lmax || !nlat || !nphi || !nz || !dz || !rint) return RD_ERR_ARG;
     if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32) return RD_ERR_ARG;
@@ -187,6 +250,7 @@ int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat,
     rd_batch *b = new rd_batch();
     b->nprob = nprob;
     b->prec = prec;
+    b->uxw = uxw;
     b->plans.assign(nprob, nullptr);
     b->ana_out.assign(nprob, nullptr);
     bool ok = hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) == hipSuccess &&
@@ -205,22 +269,156 @@ int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat,
         b->n_cu < 1)
         b->n_cu = 256;
     for (int i = 0; i < nprob; i++) {
-        int rc = plan_create(&b->plans[i], lmax[i], nlat[i], nphi[i], nz[i], C, d, dz[i], rint[i], dir, 1);
+        int rc = plan_create(&b->plans[i], lmax[i], nlat[i], nphi[i], nz[i], C, d, dz[i], rint[i], dir,
+                             uxw ? 6 : 1);
+        if (!rc && uxw && b->plans[i]->nr < 1) rc = RD_ERR_ARG;
         if (rc) {
             destroy(b);
             return rc;
         }
     }
     set_stream(b, b->stream);
-    b->jb = 4;
+    b->jb = uxw ? 1 : 4;
     b->wg_per_cu = 2;
-    int rc = batch_leg_init(&b->leg, b->plans.data(), nprob, prec, b->jb);
+    int rc = batch_leg_init(&b->leg, b->plans.data(), nprob, prec, b->jb, uxw, uxw ? 2 : 1);
     if (rc) {
         destroy(b);
         return rc;
     }
     *out = b;
     return RD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One timed pass of the tuner over a round trip or a nonlinear-term
+// evaluation.
+template <typename Fn>
+int time_choices(rd_batch *b, int reps, Fn &&run, double *ms_fixed, double *ms_tuned, int *choice)
+{
+    const int cand_jb_s[2] = {4, 2}, cand_jb_v[2] = {1, 2}, cand_wg[4] = {2, 1, 3, 4};
+    const int *cand_jb = b->uxw ? cand_jb_v : cand_jb_s;
+    hipEvent_t e0, e1;
+    RD_HIP(hipEventCreate(&e0));
+    RD_HIP(hipEventCreate(&e1));
+    double best = 1e30, fixed = 0.0;
+    int bj = cand_jb[0], bw = 2;
+    int rc = RD_OK;
+    for (int a = 0; a < 2 && !rc; a++)
+        for (int c = 0; c < 4 && !rc; c++) {
+            rc = configure(b, cand_jb[a], cand_wg[c]);
+            // one untimed pass (binds the arguments), then reps timed
+            if (!rc) rc = run();
+            if (rc) break;
+            (void)hipEventRecord(e0, b->stream);
+            for (int r = 0; r < reps && !rc; r++) rc = run();
+            (void)hipEventRecord(e1, b->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double t = ms / reps;
+            if (a == 0 && c == 0) fixed = t;
+            if (t < best) {
+                best = t;
+                bj = cand_jb[a];
+                bw = cand_wg[c];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = configure(b, bj, bw);
+    if (rc) return rc;
+    if (ms_fixed) *ms_fixed = fixed;
+    if (ms_tuned) *ms_tuned = best;
+    if (choice) {
+        choice[0] = bj;
+        choice[1] = bw;
+    }
+    return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_batch_create(rd_batch **out, int nprob, const int *lmax, const int *nlat, const int *nphi, const int *nz,
+                    int C, int d, const double *dz, const double *rint, const double *dir, int prec)
+{
+    return batch_create(out, nprob, lmax, nlat, nphi, nz, C, d, dz, rint, dir, prec, 0);
+}
+
+int rd_batch_create_uxw(rd_batch **out, int nprob, const int *lmax, const int *nlat, const int *nphi,
+                        const int *nz, int C, int d, const double *dz, const double *rint, const double *dir,
+                        int prec)
+{
+    return batch_create(out, nprob, lmax, nlat, nphi, nz, C, d, dz, rint, dir, prec, 1);
+}
+
+int rd_batch_nonlinear_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                           double *const *NS, double *const *NT)
+{
+    if (!uxw_args_ok(b, W, Z, NQ, NS, NT)) return RD_ERR_ARG;
+    int rc = bind_uxw(b, NQ, NS, NT);
+    if (rc) return rc;
+    return enqueue_uxw(b, W, Z, NQ, NS, NT);
+}
+
+int rd_batch_capture_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                         double *const *NS, double *const *NT)
+{
+    if (!uxw_args_ok(b, W, Z, NQ, NS, NT)) return RD_ERR_ARG;
+    drop_graph(b);
+    int rc = bind_uxw(b, NQ, NS, NT);
+    if (rc) return rc;
+    RD_HIP(hipStreamSynchronize(b->stream));
+    const size_t ns = b->leg.arg_bytes_syn * b->nprob * b->leg.nsets, na = b->leg.arg_bytes_ana * b->nprob;
+    RD_HIP(hipMalloc(&b->g_syn_args, ns));
+    RD_HIP(hipMalloc(&b->g_ana_args, na));
+    RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
+    RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
+    RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    hipStream_t user = b->stream;
+    void *sa = b->leg.d_syn_args, *aa = b->leg.d_ana_args;
+    int *ct = b->leg.d_counter;
+    b->leg.d_syn_args = b->g_syn_args;
+    b->leg.d_ana_args = b->g_ana_args;
+    b->leg.d_counter = b->g_counter;
+    set_stream(b, b->own_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = enqueue_uxw(b, W, Z, NQ, NS, NT);
+        e = hipStreamEndCapture(b->own_stream, &g);
+    }
+    set_stream(b, user);
+    b->leg.d_syn_args = sa;
+    b->leg.d_ana_args = aa;
+    b->leg.d_counter = ct;
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : RD_ERR_HIP;
+    }
+    b->graph = g;
+    RD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->captured = true;
+    return RD_OK;
+}
+
+int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
+                      double *const *NS, double *const *NT, int reps, double *ms_fixed, double *ms_tuned,
+                      int *choice)
+{
+    if (!uxw_args_ok(b, W, Z, NQ, NS, NT) || reps < 1) return RD_ERR_ARG;
+    return time_choices(
+        b, reps,
+        [&]() {
+            int rc = bind_uxw(b, NQ, NS, NT);
+            return rc ? rc : enqueue_uxw(b, W, Z, NQ, NS, NT);
+        },
+        ms_fixed, ms_tuned, choice);
 }
 
 int rd_batch_destroy(rd_batch *b)
@@ -260,6 +458,7 @@ int rd_batch_info(const rd_batch *b, int *info)
 int rd_batch_3d_inverse(rd_batch *b, const double *const *spec, double *const *grid)
 {
     if (!b || !spec || !grid) return RD_ERR_ARG;
+    if (b->uxw) return RD_ERR_STATE;
     for (int i = 0; i < b->nprob; i++)
         if (!spec[i] || !grid[i]) return RD_ERR_ARG;
     int rc = bind(b, nullptr);
@@ -270,6 +469,7 @@ int rd_batch_3d_inverse(rd_batch *b, const double *const *spec, double *const *g
 int rd_batch_3d_forward(rd_batch *b, const double *const *grid, double *const *spec)
 {
     if (!b || !grid || !spec) return RD_ERR_ARG;
+    if (b->uxw) return RD_ERR_STATE;
     for (int i = 0; i < b->nprob; i++)
         if (!spec[i] || !grid[i]) return RD_ERR_ARG;
     int rc = bind(b, spec);
@@ -280,6 +480,7 @@ int rd_batch_3d_forward(rd_batch *b, const double *const *grid, double *const *s
 int rd_batch_capture(rd_batch *b, const double *const *spec, double *const *grid, double *const *spec_out)
 {
     if (!b || !spec || !grid || !spec_out) return RD_ERR_ARG;
+    if (b->uxw) return RD_ERR_STATE;
     for (int i = 0; i < b->nprob; i++)
         if (!spec[i] || !grid[i] || !spec_out[i]) return RD_ERR_ARG;
     drop_graph(b);
@@ -341,50 +542,15 @@ int rd_batch_tune(rd_batch *b, const double *const *spec, double *const *grid, d
                   int reps, double *ms_fixed, double *ms_tuned, int *choice)
 {
     if (!b || !spec || !grid || !spec_out || reps < 1) return RD_ERR_ARG;
-    const int cand_jb[2] = {4, 2}, cand_wg[4] = {2, 1, 3, 4};
-    hipEvent_t e0, e1;
-    RD_HIP(hipEventCreate(&e0));
-    RD_HIP(hipEventCreate(&e1));
-    double best = 1e30, fixed = 0.0;
-    int bj = 4, bw = 2;
-    int rc = RD_OK;
-    for (int a = 0; a < 2 && !rc; a++)
-        for (int c = 0; c < 4 && !rc; c++) {
-            rc = configure(b, cand_jb[a], cand_wg[c]);
-            if (!rc) rc = bind(b, spec_out);
-            // one untimed round trip, then reps timed
+    if (b->uxw) return RD_ERR_STATE;
+    return time_choices(
+        b, reps,
+        [&]() {
+            int rc = bind(b, spec_out);
             if (!rc) rc = enqueue_inverse(b, spec, grid);
-            if (!rc) rc = enqueue_forward(b, grid);
-            if (rc) break;
-            (void)hipEventRecord(e0, b->stream);
-            for (int r = 0; r < reps && !rc; r++) {
-                rc = enqueue_inverse(b, spec, grid);
-                if (!rc) rc = enqueue_forward(b, grid);
-            }
-            (void)hipEventRecord(e1, b->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double t = ms / reps;
-            if (a == 0 && c == 0) fixed = t;
-            if (t < best) {
-                best = t;
-                bj = cand_jb[a];
-                bw = cand_wg[c];
-            }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    rc = configure(b, bj, bw);
-    if (rc) return rc;
-    if (ms_fixed) *ms_fixed = fixed;
-    if (ms_tuned) *ms_tuned = best;
-    if (choice) {
-        choice[0] = bj;
-        choice[1] = bw;
-    }
-    return RD_OK;
+            return rc ? rc : enqueue_forward(b, grid);
+        },
+        ms_fixed, ms_tuned, choice);
 }
 
 }  // extern "C"

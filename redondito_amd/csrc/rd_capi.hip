@@ -227,6 +227,56 @@ int rd_load_dirichlet_tables(const char *tdir, int C, int d, double *dir)
     return RD_OK;
 }
 
+// Reads a stream FP64 file of exactly n values (after `skip` leading ones
+// returned in head).
+static int read_table(const char *path, double *head, int skip, double *v, size_t n)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        fprintf(stderr, "redondito: could not find table %s\n", path);
+        return RD_ERR_TABLE;
+    }
+    size_t nh = skip ? fread(head, sizeof(double), (size_t)skip, f) : 0;
+    size_t nv = fread(v, sizeof(double), n, f);
+    int extra = fgetc(f);
+    fclose(f);
+    if (nh != (size_t)skip || nv != n || extra != EOF) {
+        fprintf(stderr, "redondito: FC-Gram table of the wrong size (%s)\n", path);
+        return RD_ERR_TABLE;
+    }
+    return RD_OK;
+}
+
+int rd_load_neumann_tables(const char *tdir, int d, int ord, double dx, double *neu)
+{
+    if (!tdir || d <= 0 || !neu || !(dx > 0.0)) return RD_ERR_ARG;
+    if (ord != 1 && ord != 2) {
+        fprintf(stderr, "[WARNING] load_neu_tables called with undefined order. Skipping...\n");
+        return RD_OK;
+    }
+    char pq[1024], pn[1024];
+    snprintf(pq, sizeof pq, "%s/Q%d.dat", tdir, d);
+    snprintf(pn, sizeof pn, "%s/Q%dn%d.dat", tdir, ord, d);
+    std::vector<double> Q((size_t)d * d), Qn((size_t)d * d);
+    double dxp = 0.0;
+    int rc = read_table(pq, nullptr, 0, Q.data(), Q.size());
+    if (!rc) rc = read_table(pn, &dxp, 1, Qn.data(), Qn.size());
+    if (rc) return rc;
+    if (!(dxp > 0.0)) {
+        fprintf(stderr, "redondito: table spacing %g in %s\n", dxp, pn);
+        return RD_ERR_TABLE;
+    }
+    // neu = Q(d,:) Qn^T, the last weight scaled by (dx/dxp)^ord (column major)
+    for (int j = 0; j < d; j++) {
+        double s = 0.0;
+        for (int k = 0; k < d; k++) s += Q[(d - 1) + (size_t)d * k] * Qn[j + (size_t)d * k];
+        neu[j] = s;
+    }
+    const double f = dx / dxp;
+    neu[d - 1] *= (ord == 2) ? f * f : f;
+    return RD_OK;
+}
+
 // The plan constructor; nwork Fourier and spectral scratch arrays (6 for the
 // nonlinear term, 1 for scalar transforms).
 static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz,
@@ -554,6 +604,29 @@ int rd_prodvec(rd_plan *plan, int nplanes, const double *ar, const double *at, c
     if (nplanes < 0 || nplanes > p->nz) return RD_ERR_ARG;
     const int np = nplanes < p->nr ? nplanes : p->nr;
     return launch_prodvec(p, (long)p->nlat * p->nphi * np, ar, at, ap, br, bt, bp, cr, ct, cp);
+}
+
+static int fc_reconstruct(rd_plan *plan, double *a, int boun, int ord, const double *neu, const double *coef)
+{
+    if (bad_plan(plan) || !a || !neu) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if ((boun != 5 && boun != 6) || (ord != 1 && ord != 2) || p->C == 0 || p->d < 1 || p->d > 8) {
+        fprintf(stderr, "[ERROR] Neumann reconstruction not performed. Unsupported derivative order, "
+                        "boundary surface or unallocated table.\n");
+        return RD_ERR_ARG;
+    }
+    return launch_fc_reconstruct(p, (double2 *)a, boun, neu, coef);
+}
+
+int rd_neumann_reconstruct(rd_plan *plan, double *a, int boun, int ord, const double *neu)
+{
+    return fc_reconstruct(plan, a, boun, ord, neu, nullptr);
+}
+
+int rd_robin_reconstruct(rd_plan *plan, double *a, int boun, const double *neu, const double *coef)
+{
+    if (!coef) return RD_ERR_ARG;
+    return fc_reconstruct(plan, a, boun, 1, neu, coef);
 }
 
 int rd_fc_filter(rd_plan *plan, char esp, double *a)

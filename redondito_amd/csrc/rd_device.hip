@@ -88,6 +88,17 @@ void *rd_dev_lookup(const void *host)
 
 int rd_dev_present(const void *host) { return rd_dev_lookup(host) != nullptr; }
 
+int rd_dev_index(const void *host)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int i = 1;
+    for (const auto &kv : g_reg) {
+        if (kv.first == (uintptr_t)host) return i;
+        i++;
+    }
+    return 0;
+}
+
 static int span(const void *host, size_t nbytes, void **dp)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -162,6 +162,10 @@ int launch_prodvec(Plan *p, long npts, const double *ar, const double *at, const
 int launch_fc_filter_k(Plan *p, double2 *a);
 int launch_dirichlet(Plan *p, double2 *a);
 int launch_zero_rows(Plan *p, double2 *a, int row0, int nrows);
+// Wall value of every lm column from its normal derivative (boun 5: z(1),
+// 6: z(nz - C)), Neumann weights neu (d values), Robin coefficients coef
+// (nlm reals, device) or NULL.
+int launch_fc_reconstruct(Plan *p, double2 *a, int boun, const double *neu, const double *coef);
 
 // Plan life cycle for internal users (the batch): nwork scratch arrays.
 int plan_create(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, double dz, double rint,
@@ -169,9 +173,13 @@ int plan_create(Plan **out, int lmax, int nlat, int nphi, int nz, int C, int d, 
 void plan_destroy(Plan *p);
 
 // Persistent Legendre launches over many problems: work items (problem, m,
-// tile, plane tile) pulled from a counter, largest first.
+// tile, plane tile) pulled from a counter, largest first.  Scalar form: one
+// synthesis and one analysis operand set per problem over its nz planes (the
+// 3D transforms).  Vector form (vec): nsets (Q,S,T) synthesis sets per
+// problem (u and omega of the nonlinear term) and one vector analysis set,
+// over the problem's nr physical planes.
 struct BatchLeg {
-    int nprob, prec, jb;
+    int nprob, prec, jb, vec, nsets;
     void *d_syn_args, *d_ana_args;       // per-problem kernel arguments
     int4 *d_syn_items, *d_ana_items;
     int n_syn_items, n_ana_items;
@@ -179,11 +187,15 @@ struct BatchLeg {
     size_t arg_bytes_syn, arg_bytes_ana;
     void *h_syn_args, *h_ana_args;       // host copies of the uploaded arguments
 };
-int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb);
+int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb, int vec = 0, int nsets = 1);
 // Binds the per-problem operands (spectral and Fourier arrays); uploads the
 // arguments when they changed (synchronous on the stream).
 int batch_leg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in, double2 *const *syn_out,
                    const double2 *const *ana_in, double2 *const *ana_out, hipStream_t s);
+// Vector form: syn_in / syn_out hold 3 arrays per (problem, set), set fastest
+// ((p nsets + s) 3 + f); ana_in / ana_out 3 per problem.
+int batch_vleg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in, double2 *const *syn_out,
+                    const double2 *const *ana_in, double2 *const *ana_out, hipStream_t s);
 int launch_batch_synthesis(BatchLeg *b, hipStream_t s, int nwg);
 int launch_batch_analysis(BatchLeg *b, hipStream_t s, int nwg);
 void batch_leg_free(BatchLeg *b);

@@ -1055,13 +1055,57 @@ __global__ __launch_bounds__(256) void batch_analysis_kernel(const AnaArgs<T> *p
     }
 }
 
-int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb)
+// Vector forms of the persistent launches (the nonlinear term of many
+// problems): the same work-item loop over the (Q,S,T) tiles.
+template <typename T, int JBV>
+__global__ __launch_bounds__(256) void batch_vsynthesis_kernel(const SynArgs<T> *sets, const int4 *items,
+                                                               int nitems, int *counter)
+{
+    using Cfg = SynCfg<true, JBV>;
+    __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) return;
+        const int4 w = items[it];
+        const SynArgs<T> A = sets[w.x];
+        syn_tile<T, true, JBV, false>(A, w.y, w.z, w.w, Bs);
+    }
+}
+
+template <typename T, int JBV>
+__global__ __launch_bounds__(256) void batch_vanalysis_kernel(const AnaArgs<T> *probs, const int4 *items,
+                                                              int nitems, int *counter)
+{
+    using Cfg = AnaCfg<true, JBV>;
+    __shared__ T Gn[Cfg::NF][ATC][Cfg::JTP];
+    __shared__ T Gs[Cfg::NF][ATC][Cfg::JTP];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();
+        if (it >= nitems) return;
+        const int4 w = items[it];
+        const AnaArgs<T> A = probs[w.x];
+        ana_tile<T, true, JBV>(A, w.y, w.z, w.w, Gn, Gs);
+    }
+}
+
+int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb, int vec, int nsets)
 {
     if (prec != RD_PREC_FP64 && prec != RD_PREC_FP32) return RD_ERR_ARG;
-    if (jb != 2 && jb != 4) return RD_ERR_ARG;
+    if (vec ? (jb != 1 && jb != 2) : (jb != 2 && jb != 4)) return RD_ERR_ARG;
+    if (nsets < 1 || (!vec && nsets != 1)) return RD_ERR_ARG;
     b->nprob = nprob;
     b->prec = prec;
     b->jb = jb;
+    b->vec = vec;
+    b->nsets = nsets;
     b->arg_bytes_syn = prec == RD_PREC_FP64 ? sizeof(SynArgs<double>) : sizeof(SynArgs<float>);
     b->arg_bytes_ana = prec == RD_PREC_FP64 ? sizeof(AnaArgs<double>) : sizeof(AnaArgs<float>);
     std::vector<int4> si, ai;
@@ -1073,14 +1117,16 @@ int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb)
             if (rc) return rc;
         }
         const int L = pl->L;
-        const int sjt = (2 * pl->nz + 16 * jb - 1) / (16 * jb), ntt = pl->nthp / 64;
-        const int ajt = (2 * pl->nz + 32 * jb - 1) / (32 * jb);
+        const int np = vec ? pl->nr : pl->nz;
+        const int sjt = (2 * np + 16 * jb - 1) / (16 * jb), ntt = pl->nthp / 64;
+        const int ajt = (2 * np + 32 * jb - 1) / (32 * jb);
         for (int m = 0; m < L; m++) {
-            for (int tt = 0; tt < ntt; tt++)
-                for (int jt = 0; jt < sjt; jt++) {
-                    si.push_back(make_int4(p, m, tt, jt));
-                    sc.push_back((L - m + SKC - 1) / SKC);
-                }
+            for (int set = 0; set < nsets; set++)
+                for (int tt = 0; tt < ntt; tt++)
+                    for (int jt = 0; jt < sjt; jt++) {
+                        si.push_back(make_int4(p * nsets + set, m, tt, jt));
+                        sc.push_back((L - m + SKC - 1) / SKC);
+                    }
             const int ncol = rup((L - m + 1) / 2, 32);
             for (int lt = 0; lt < ncol / 32; lt++)
                 for (int jt = 0; jt < ajt; jt++) {
@@ -1103,12 +1149,12 @@ int batch_leg_init(BatchLeg *b, Plan *const *plans, int nprob, int prec, int jb)
     b->n_ana_items = (int)ai.size();
     RD_HIP(hipMalloc(&b->d_syn_items, sizeof(int4) * si.size()));
     RD_HIP(hipMalloc(&b->d_ana_items, sizeof(int4) * ai.size()));
-    RD_HIP(hipMalloc(&b->d_syn_args, b->arg_bytes_syn * nprob));
+    RD_HIP(hipMalloc(&b->d_syn_args, b->arg_bytes_syn * nprob * nsets));
     RD_HIP(hipMalloc(&b->d_ana_args, b->arg_bytes_ana * nprob));
     RD_HIP(hipMalloc(&b->d_counter, 2 * sizeof(int)));
     RD_HIP(hipMemcpy(b->d_syn_items, si.data(), sizeof(int4) * si.size(), hipMemcpyHostToDevice));
     RD_HIP(hipMemcpy(b->d_ana_items, ai.data(), sizeof(int4) * ai.size(), hipMemcpyHostToDevice));
-    b->h_syn_args = calloc(nprob, b->arg_bytes_syn);
+    b->h_syn_args = calloc((size_t)nprob * nsets, b->arg_bytes_syn);
     b->h_ana_args = calloc(nprob, b->arg_bytes_ana);
     if (!b->h_syn_args || !b->h_ana_args) return RD_ERR_NOMEM;
     return RD_OK;
@@ -1188,19 +1234,98 @@ int batch_leg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in
     return RD_OK;
 }
 
+template <typename T>
+static void fill_vargs(const BatchLeg *b, Plan *const *plans, const double2 *const *syn_in,
+                       double2 *const *syn_out, const double2 *const *ana_in, double2 *const *ana_out,
+                       SynArgs<T> *sa, AnaArgs<T> *aa)
+{
+    const bool f32 = b->prec == RD_PREC_FP32;
+    for (int p = 0; p < b->nprob; p++) {
+        const Plan *pl = plans[p];
+        for (int set = 0; set < b->nsets; set++) {
+            SynArgs<T> &S = sa[p * b->nsets + set];
+            S.tP = (const T *)(f32 ? (const void *)pl->syn32_P : (const void *)pl->syn_P);
+            S.tD = (const T *)(f32 ? (const void *)pl->syn32_D : (const void *)pl->syn_D);
+            S.tM = (const T *)(f32 ? (const void *)pl->syn32_M : (const void *)pl->syn_M);
+            S.off = pl->d_syn_off;
+            S.queue = nullptr;
+            for (int f = 0; f < 3; f++) {
+                S.in[f] = syn_in[(p * b->nsets + set) * 3 + f];
+                S.out[f] = syn_out[(p * b->nsets + set) * 3 + f];
+            }
+            S.ld = pl->nlm;
+            S.lmax = pl->lmax;
+            S.nlat = pl->nlat;
+            S.nth = pl->nth;
+            S.nthp = pl->nthp;
+            S.nplanes = pl->nr;
+            S.n_jt = 0;
+        }
+        AnaArgs<T> &A = aa[p];
+        A.tP = (const T *)(f32 ? (const void *)pl->ana32_P : (const void *)pl->ana_P);
+        A.tD = (const T *)(f32 ? (const void *)pl->ana32_D : (const void *)pl->ana_D);
+        A.tM = (const T *)(f32 ? (const void *)pl->ana32_M : (const void *)pl->ana_M);
+        A.off = pl->d_ana_off;
+        A.ncol = pl->d_ana_ncol;
+        A.queue = nullptr;
+        for (int f = 0; f < 3; f++) {
+            A.in[f] = ana_in[p * 3 + f];
+            A.out[f] = ana_out[p * 3 + f];
+        }
+        A.ld = pl->nlm;
+        A.lmax = pl->lmax;
+        A.nlat = pl->nlat;
+        A.nth = pl->nth;
+        A.nthp16 = pl->nthp16;
+        A.nplanes = pl->nr;
+        A.n_jt = 0;
+    }
+}
+
+int batch_vleg_bind(BatchLeg *b, Plan *const *plans, const double2 *const *syn_in, double2 *const *syn_out,
+                    const double2 *const *ana_in, double2 *const *ana_out, hipStream_t s)
+{
+    if (!b->vec) return RD_ERR_STATE;
+    const int np = b->nprob;
+    std::vector<char> hs(b->arg_bytes_syn * np * b->nsets), ha(b->arg_bytes_ana * np);
+    if (b->prec == RD_PREC_FP64)
+        fill_vargs<double>(b, plans, syn_in, syn_out, ana_in, ana_out, (SynArgs<double> *)hs.data(),
+                           (AnaArgs<double> *)ha.data());
+    else
+        fill_vargs<float>(b, plans, syn_in, syn_out, ana_in, ana_out, (SynArgs<float> *)hs.data(),
+                          (AnaArgs<float> *)ha.data());
+    const bool same = memcmp(hs.data(), b->h_syn_args, hs.size()) == 0 &&
+                      memcmp(ha.data(), b->h_ana_args, ha.size()) == 0;
+    if (same) return RD_OK;
+    RD_HIP(hipStreamSynchronize(s));
+    RD_HIP(hipMemcpy(b->d_syn_args, hs.data(), hs.size(), hipMemcpyHostToDevice));
+    RD_HIP(hipMemcpy(b->d_ana_args, ha.data(), ha.size(), hipMemcpyHostToDevice));
+    memcpy(b->h_syn_args, hs.data(), hs.size());
+    memcpy(b->h_ana_args, ha.data(), ha.size());
+    return RD_OK;
+}
+
 int launch_batch_synthesis(BatchLeg *b, hipStream_t s, int nwg)
 {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counter, 1);
     RD_LAUNCH_CHECK();
-#define RD_BS(T, J)                                                                                        \
-    hipLaunchKernelGGL((batch_synthesis_kernel<T, J>), dim3(nwg), dim3(256), 0, s,                         \
-                       (const SynArgs<T> *)b->d_syn_args, b->d_syn_items, b->n_syn_items, b->d_counter)
-    if (b->prec == RD_PREC_FP64) {
-        if (b->jb == 2) RD_BS(double, 2);
-        else RD_BS(double, 4);
+#define RD_BS(K, T, J)                                                                                     \
+    hipLaunchKernelGGL((K<T, J>), dim3(nwg), dim3(256), 0, s, (const SynArgs<T> *)b->d_syn_args,          \
+                       b->d_syn_items, b->n_syn_items, b->d_counter)
+    if (b->vec) {
+        if (b->prec == RD_PREC_FP64) {
+            if (b->jb == 1) RD_BS(batch_vsynthesis_kernel, double, 1);
+            else RD_BS(batch_vsynthesis_kernel, double, 2);
+        } else {
+            if (b->jb == 1) RD_BS(batch_vsynthesis_kernel, float, 1);
+            else RD_BS(batch_vsynthesis_kernel, float, 2);
+        }
+    } else if (b->prec == RD_PREC_FP64) {
+        if (b->jb == 2) RD_BS(batch_synthesis_kernel, double, 2);
+        else RD_BS(batch_synthesis_kernel, double, 4);
     } else {
-        if (b->jb == 2) RD_BS(float, 2);
-        else RD_BS(float, 4);
+        if (b->jb == 2) RD_BS(batch_synthesis_kernel, float, 2);
+        else RD_BS(batch_synthesis_kernel, float, 4);
     }
 #undef RD_BS
     RD_LAUNCH_CHECK();
@@ -1211,15 +1336,23 @@ int launch_batch_analysis(BatchLeg *b, hipStream_t s, int nwg)
 {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counter + 1, 1);
     RD_LAUNCH_CHECK();
-#define RD_BA(T, J)                                                                                        \
-    hipLaunchKernelGGL((batch_analysis_kernel<T, J>), dim3(nwg), dim3(256), 0, s,                          \
-                       (const AnaArgs<T> *)b->d_ana_args, b->d_ana_items, b->n_ana_items, b->d_counter + 1)
-    if (b->prec == RD_PREC_FP64) {
-        if (b->jb == 2) RD_BA(double, 2);
-        else RD_BA(double, 4);
+#define RD_BA(K, T, J)                                                                                     \
+    hipLaunchKernelGGL((K<T, J>), dim3(nwg), dim3(256), 0, s, (const AnaArgs<T> *)b->d_ana_args,          \
+                       b->d_ana_items, b->n_ana_items, b->d_counter + 1)
+    if (b->vec) {
+        if (b->prec == RD_PREC_FP64) {
+            if (b->jb == 1) RD_BA(batch_vanalysis_kernel, double, 1);
+            else RD_BA(batch_vanalysis_kernel, double, 2);
+        } else {
+            if (b->jb == 1) RD_BA(batch_vanalysis_kernel, float, 1);
+            else RD_BA(batch_vanalysis_kernel, float, 2);
+        }
+    } else if (b->prec == RD_PREC_FP64) {
+        if (b->jb == 2) RD_BA(batch_analysis_kernel, double, 2);
+        else RD_BA(batch_analysis_kernel, double, 4);
     } else {
-        if (b->jb == 2) RD_BA(float, 2);
-        else RD_BA(float, 4);
+        if (b->jb == 2) RD_BA(batch_analysis_kernel, float, 2);
+        else RD_BA(batch_analysis_kernel, float, 4);
     }
 #undef RD_BA
     RD_LAUNCH_CHECK();

@@ -104,6 +104,34 @@ __global__ void dirichlet_kernel(double2 *a, long nlm, int last)
     }
 }
 
+// Wall value from its normal derivative along z (neumann_reconstruct /
+// robin_reconstruct, fcgram_mod.f90:466-517, 612-650, on the (lm, nz) layout):
+// row rb holds g on entry; f_b = w[d-1] g + sum_{k<d} w[k-1] a(rows[k-1]),
+// divided by coef[lm] w[d-1] + 1 when coef is given (Robin).
+struct FcRecon {
+    double w[8];
+    int rows[8];
+    int d, rb;
+};
+
+__global__ void fc_reconstruct_kernel(double2 *a, long nlm, FcRecon R, const double *coef)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nlm; i += (long)gridDim.x * blockDim.x) {
+        const double2 g = a[(long)R.rb * nlm + i];
+        double2 v = make_double2(R.w[R.d - 1] * g.x, R.w[R.d - 1] * g.y);
+        for (int k = 0; k < R.d - 1; k++) {
+            const double2 x = a[(long)R.rows[k] * nlm + i];
+            v.x += R.w[k] * x.x;
+            v.y += R.w[k] * x.y;
+        }
+        if (coef) {
+            const double den = coef[i] * R.w[R.d - 1] + 1.0;
+            v = make_double2(v.x / den, v.y / den);
+        }
+        a[(long)R.rb * nlm + i] = v;
+    }
+}
+
 __global__ void zero_kernel(double2 *a, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -165,6 +193,22 @@ int launch_dirichlet(Plan *p, double2 *a)
 {
     hipLaunchKernelGGL(dirichlet_kernel, grid_for(p->nlm), dim3(256), 0, p->stream, a, (long)p->nlm,
                        p->nz - p->C - 1);
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+int launch_fc_reconstruct(Plan *p, double2 *a, int boun, const double *neu, const double *coef)
+{
+    FcRecon R = {};
+    R.d = p->d;
+    if (R.d < 1 || R.d > 8) return RD_ERR_ARG;
+    R.rb = (boun == 5) ? 0 : p->nr - 1;
+    for (int k = 1; k <= R.d; k++) {
+        R.w[k - 1] = neu[k - 1];
+        if (k < R.d) R.rows[k - 1] = (boun == 5) ? R.d - k : p->nr - 1 - R.d + k;
+    }
+    hipLaunchKernelGGL(fc_reconstruct_kernel, grid_for(p->nlm), dim3(256), 0, p->stream, a, (long)p->nlm, R,
+                       coef);
     RD_LAUNCH_CHECK();
     return RD_OK;
 }

@@ -21,6 +21,9 @@
       CALL caso_uxw()
       CALL caso_coupled()
       CALL caso_boundary()
+      CALL caso_box()
+      CALL caso_neumann()
+      CALL caso_redist()
       IF (nfail .gt. 0) THEN
          PRINT '(A,I0,A)', 'prueba_rd: ', nfail, ' failed'
          STOP 1
@@ -450,5 +453,156 @@
       END DO
       CALL fcgram_destroy_plan(planfc)
       END SUBROUTINE caso_boundary
+
+! grid_init and box_init from a namelist file, as a reference program
+! starts (pseudospec_mod.f90:41-69, 126-236): the 3/2 rule, the Gauss
+! nodes of the plan, the radial grid ending at rext on z(nz-Cz), kz
+! in FFT order, (l,m) in the m-major order.
+      SUBROUTINE caso_box()
+      USE grid
+      USE kes
+      USE boxsize
+      TYPE(FCPLAN) :: plan
+      DOUBLE PRECISION, TARGET :: ct(24)
+      DOUBLE PRECISION :: e1, e2, e3
+      INTEGER :: i, l, m, bad
+      OPEN(7,file='prueba_rd.nml',action='write')
+      WRITE(7,'(A)') '&grid'
+      WRITE(7,'(A)') ' lmax = 15, nz = 89, Cz = 25, oz = 5'
+      WRITE(7,'(A)') '/'
+      WRITE(7,'(A)') '&boxparams'
+      WRITE(7,'(A)') ' rint = 0.35, rext = 1.0'
+      WRITE(7,'(A)') '/'
+      CLOSE(7)
+      CALL grid_init('prueba_rd.nml')
+      CALL box_init('prueba_rd.nml')
+      bad = 0
+      IF (nlm .ne. 136 .OR. nlat .ne. 24 .OR. nphi .ne. 48 .OR. Cz .ne. 25 .OR. oz .ne. 5) bad = 1
+      CALL fcgram_create_plan(plan,(/nlat,nphi,nz/),(/0,0,0/),(/0,0,0/), &
+           (/1.0D0,1.0D0,1.0D0/),'.',0,lmax)
+      CALL rd_check(rd_plan_grid(plan%h,C_LOC(ct),C_NULL_PTR,C_NULL_PTR,C_NULL_PTR, &
+           C_NULL_PTR,C_NULL_PTR,C_NULL_PTR),'grid')
+      e1 = MAXVAL(ABS(COS(theta)-ct))
+      e2 = ABS(z(nz-Cz)-rext) + ABS(z(1)-rint) + ABS(kz(2)-Dkz) + ABS(kz(nz)+Dkz) + &
+           ABS(kz(45)-44*Dkz) + ABS(kz(46)+44*Dkz) + ABS(phi(2)-2*pi/nphi)
+      DO m = 0,lmax
+         DO l = m,lmax
+            i = lm(lmax,l,m)
+            IF (ell(i) .ne. l .OR. emm(i) .ne. m) bad = bad + 1
+         END DO
+      END DO
+      e3 = ABS(Dkz - 2*pi/((rext-rint)/(nz-Cz-1)*nz))
+      CALL report('box_init theta vs plan nodes',e1,1.0D-15)
+      CALL report('box_init z, kz, phi',e2,1.0D-13)
+      CALL report('box_init Dkz',e3,1.0D-12)
+      CALL report('grid_init sizes, box_init ell/emm (bad)',DBLE(bad),0.0D0)
+      CALL fcgram_destroy_plan(plan)
+      END SUBROUTINE caso_box
+
+! load_neumann_tables, neumann_reconstruct and robin_reconstruct
+! (fcgram_mod.f90:267-650) on the radial rows: the wall value of a
+! degree-4 profile from its outward normal derivative.
+      SUBROUTINE caso_neumann()
+      INTEGER, PARAMETER :: lmax = 7, nr = 40, C = 25, od = 5, nz = nr + C
+      INTEGER, PARAMETER :: nlat = 12, nphi = 24, nlm = (lmax+1)*(lmax+2)/2
+      DOUBLE PRECISION, PARAMETER :: ri = 0.5D0, ro = 1.5D0
+      DOUBLE COMPLEX, TARGET :: f(nlm,nz), g(nlm,nz)
+      DOUBLE PRECISION, TARGET :: a(nlm,1)
+      DOUBLE PRECISION :: dz, r, e5, e6, e2, er
+      DOUBLE COMPLEX :: c0, c1, c2, c3, c4
+      INTEGER :: j, k
+      CHARACTER(len=256) :: tdir
+      CALL GET_ENVIRONMENT_VARIABLE('RD_TABLE_DIR',tdir)
+      dz = (ro-ri)/(nr-1)
+      CALL fcgram_create_plan(planfc,(/nlat,nphi,nz/),(/0,0,C/),(/0,0,od/), &
+           (/1.0D0,1.0D0,dz/),TRIM(tdir),0,lmax,ri)
+      CALL load_neumann_tables(planfc%z,TRIM(tdir),1)
+      CALL load_neumann_tables(planfc%z,TRIM(tdir),2)
+      CALL load_neumann_tables(planfc%z,TRIM(tdir),1)     ! warns, keeps the table
+      c0 = (0.3D0,0.1D0); c1 = (-1.2D0,0.4D0); c2 = (0.7D0,-0.2D0); c3 = (0.25D0,0.5D0); c4 = (-0.4D0,0.1D0)
+      DO k = 1,nz
+         r = ri+(k-1)*dz
+         DO j = 1,nlm
+            f(j,k) = j*(c0+c1*r+c2*r**2+c3*r**3+c4*r**4)
+         END DO
+      END DO
+      DO j = 1,nlm
+         a(j,1) = 0.5D0+0.01D0*j
+      END DO
+      CALL gdev_alloc(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      e5 = 0; e6 = 0; e2 = 0; er = 0
+      ! boundary 5, first derivative: g = -df/dr at r = ri
+      g = f
+      r = ri
+      g(:,1) = -(/(j*(c1+2*c2*r+3*c3*r**2+4*c4*r**3), j=1,nlm)/)
+      CALL gdev_update_to(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      CALL neumann_reconstruct(planfc,g,5,1)
+      CALL gdev_update_from(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      e5 = MAXVAL(ABS(g(:,1)-f(:,1)))/MAXVAL(ABS(f(:,1)))
+      ! boundary 6, first derivative: g = df/dr at r = ro
+      g = f
+      r = ro
+      g(:,nr) = (/(j*(c1+2*c2*r+3*c3*r**2+4*c4*r**3), j=1,nlm)/)
+      CALL gdev_update_to(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      CALL neumann_reconstruct(planfc,g,6,1)
+      CALL gdev_update_from(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      e6 = MAXVAL(ABS(g(:,nr)-f(:,nr)))/MAXVAL(ABS(f(:,nr)))
+      ! boundary 6, second derivative
+      g = f
+      g(:,nr) = (/(j*(2*c2+6*c3*r+12*c4*r**2), j=1,nlm)/)
+      CALL gdev_update_to(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      CALL neumann_reconstruct(planfc,g,6,2)
+      CALL gdev_update_from(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      e2 = MAXVAL(ABS(g(:,nr)-f(:,nr)))/MAXVAL(ABS(f(:,nr)))
+      ! Robin at boundary 6: g = f' + a f
+      g = f
+      g(:,nr) = (/(j*(c1+2*c2*r+3*c3*r**2+4*c4*r**3), j=1,nlm)/) + a(:,1)*f(:,nr)
+      CALL gdev_update_to(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      CALL robin_reconstruct(planfc,g,6,a)
+      CALL gdev_update_from(C_LOC(g),INT(16*SIZE(g),C_SIZE_T))
+      er = MAXVAL(ABS(g(:,nr)-f(:,nr)))/MAXVAL(ABS(f(:,nr)))
+      CALL report('neumann_reconstruct z=rint order 1',e5,1.0D-12)
+      CALL report('neumann_reconstruct z=rext order 1',e6,1.0D-12)
+      CALL report('neumann_reconstruct z=rext order 2',e2,1.0D-12)
+      CALL report('robin_reconstruct z=rext',er,1.0D-12)
+      CALL gdev_free(C_LOC(g))
+      CALL fcgram_destroy_plan(planfc)
+      END SUBROUTINE caso_neumann
+
+! fftp3d_redist_k_to_lm / _lm_to_k (sphp_sca.F90:629-760): the one-rank
+! exchange copies the block; gdev_lookup finds registered arrays only.
+      SUBROUTINE caso_redist()
+      INTEGER, PARAMETER :: lmax = 7, nz = 6, nlat = 12, nphi = 24, nlm = (lmax+1)*(lmax+2)/2
+      TYPE(FCPLAN) :: plan
+      DOUBLE COMPLEX, TARGET :: a(nlm,nz), b(nlm,nz), c(nlm,nz)
+      DOUBLE PRECISION, TARGET :: e, unreg(4)
+      INTEGER :: j, k, ia
+      CALL fcgram_create_plan(plan,(/nlat,nphi,nz/),(/0,0,0/),(/0,0,0/), &
+           (/1.0D0,1.0D0,1.0D0/),'.',0,lmax)
+      DO k = 1,nz
+         DO j = 1,nlm
+            a(j,k) = DCMPLX(j+0.5D0*k,-k*0.25D0*j)
+         END DO
+      END DO
+      b = 0; c = 0
+      CALL gdev_alloc(C_LOC(a),INT(16*SIZE(a),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(b),INT(16*SIZE(b),C_SIZE_T))
+      CALL gdev_alloc(C_LOC(c),INT(16*SIZE(c),C_SIZE_T))
+      CALL gdev_update_to(C_LOC(a),INT(16*SIZE(a),C_SIZE_T))
+      CALL fftp3d_redist_k_to_lm(plan,a,b,0)
+      CALL fftp3d_redist_lm_to_k(plan,b,c,0)
+      CALL gdev_update_from(C_LOC(b),INT(16*SIZE(b),C_SIZE_T))
+      CALL gdev_update_from(C_LOC(c),INT(16*SIZE(c),C_SIZE_T))
+      e = MAXVAL(ABS(b-a)) + MAXVAL(ABS(c-a))
+      CALL report('fftp3d_redist k->lm->k (copy)',e,0.0D0)
+      ia = gdev_lookup(C_LOC(a))
+      e = 0
+      IF (ia .lt. 1 .OR. gdev_lookup(C_LOC(unreg)) .ne. 0) e = 1
+      CALL report('gdev_lookup registered / unregistered',e,0.0D0)
+      CALL gdev_free(C_LOC(a)); CALL gdev_free(C_LOC(b)); CALL gdev_free(C_LOC(c))
+      IF (gdev_lookup(C_LOC(a)) .ne. 0) e = 1
+      CALL report('gdev_lookup after gdev_free',e,0.0D0)
+      CALL fcgram_destroy_plan(plan)
+      END SUBROUTINE caso_redist
 
       END PROGRAM prueba_rd

@@ -110,6 +110,33 @@
       CALL fc_sync(plan)
       END SUBROUTINE fftp3d_complex_to_real
 
+! src/sphp/sphp_sca.F90:629-693: planes -> lm pencils; one rank holds
+! every plane and lm, so the exchange is a copy of the (nlm,nz) block
+      SUBROUTINE fftp3d_redist_k_to_lm(plan,in,out,comm)
+      USE fcgram
+      IMPLICIT NONE
+      TYPE(FCPLAN), INTENT(IN) :: plan
+      DOUBLE COMPLEX, INTENT(IN), TARGET :: in(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET :: out(*)
+      TYPE(*), INTENT(IN) :: comm
+      CALL rd_check(rd_redist_k_to_lm(plan%h,INT(plan%z%n,C_INT),rd_dptr(C_LOC(in),'in'), &
+           rd_dptr(C_LOC(out),'out')),'fftp3d_redist_k_to_lm')
+      CALL fc_sync(plan)
+      END SUBROUTINE fftp3d_redist_k_to_lm
+
+! src/sphp/sphp_sca.F90:696-760: lm pencils -> planes (a copy on one rank)
+      SUBROUTINE fftp3d_redist_lm_to_k(plan,in,out,comm)
+      USE fcgram
+      IMPLICIT NONE
+      TYPE(FCPLAN), INTENT(IN) :: plan
+      DOUBLE COMPLEX, INTENT(IN), TARGET :: in(*)
+      DOUBLE COMPLEX, INTENT(OUT), TARGET :: out(*)
+      TYPE(*), INTENT(IN) :: comm
+      CALL rd_check(rd_redist_lm_to_k(plan%h,INT(plan%z%n,C_INT),rd_dptr(C_LOC(in),'in'), &
+           rd_dptr(C_LOC(out),'out')),'fftp3d_redist_lm_to_k')
+      CALL fc_sync(plan)
+      END SUBROUTINE fftp3d_redist_lm_to_k
+
 !-----------------------------------------------------------------
 ! src/pseudo/pseudospec_hd.f90, on the global plan planfc (module fft)
 !-----------------------------------------------------------------

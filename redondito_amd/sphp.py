@@ -63,6 +63,9 @@ def lib():
         L.rd_saxpby_r.argtypes = [vp, i, vp, vp, d, vp, d]
         L.rd_prodvec.argtypes = [vp, i] + [vp] * 9
         L.rd_fc_filter.argtypes = [vp, ctypes.c_char, vp]
+        L.rd_load_neumann_tables.argtypes = [ctypes.c_char_p, i, i, d, ctypes.POINTER(d)]
+        L.rd_neumann_reconstruct.argtypes = [vp, vp, i, i, ctypes.POINTER(d)]
+        L.rd_robin_reconstruct.argtypes = [vp, vp, i, ctypes.POINTER(d), vp]
         L.rd_nonlinear_uxw.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
@@ -91,6 +94,17 @@ def load_dirichlet_tables(tdir, C, d):
                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
            "load_dirichlet_tables")
     return out.reshape(d, C).T.copy()
+
+
+def load_neumann_tables(tdir, d, ord_, dx):
+    """neu (d values) of load_neumann_tables (src/sphp/fcgram_mod.f90:267-371)
+    for matching points at spacing dx: Q{d}.dat and Q{ord}n{d}.dat."""
+    import numpy as np
+    out = np.full(d, np.nan)
+    _check(lib().rd_load_neumann_tables(str(tdir).encode(), d, ord_, float(dx),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+           "load_neumann_tables")
+    return out
 
 
 def grid_nlat(lmax):
@@ -281,6 +295,26 @@ def prodvec(plan, ar, at, ap, br, bt, bp, cr, ct, cp):
     _check(lib().rd_prodvec(plan.h, n, *args), "prodvec")
 
 
+def neumann_reconstruct(plan, a, boun, ord_, neu):
+    """Wall value of a (nz, nlm) from its normal derivative in row 0 (boun 5)
+    or nz - C - 1 (boun 6) (neumann_reconstruct, fcgram_mod.f90:374-517)."""
+    import numpy as np
+    w = np.ascontiguousarray(neu, np.float64)
+    _check(lib().rd_neumann_reconstruct(plan.h, plan._spec(a), boun, ord_,
+                                        w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "neumann_reconstruct")
+
+
+def robin_reconstruct(plan, a, boun, neu, coef):
+    """As neumann_reconstruct for f' + coef f = g, coef a float64 CUDA tensor
+    of nlm values (robin_reconstruct, fcgram_mod.f90:520-650)."""
+    import numpy as np
+    w = np.ascontiguousarray(neu, np.float64)
+    if coef.dtype != torch.float64 or not coef.is_cuda or coef.numel() != plan.nlm or not coef.is_contiguous():
+        raise RdError("robin coefficients must be a contiguous float64 CUDA tensor of nlm values")
+    _check(lib().rd_robin_reconstruct(plan.h, plan._spec(a), boun, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      ctypes.c_void_p(coef.data_ptr())), "robin_reconstruct")
+
+
 def fc_filter(plan, a, esp):
     if esp not in ("r", "k"):
         raise RdError(f"Unknown space {esp} in fc_filter")
@@ -388,9 +422,12 @@ class Batch:
     float64 (nz, nphi, nlat) tensors, one per problem.
     """
 
-    def __init__(self, problems, C=25, d=5, prec=RD_PREC_FP32, tdir=TABLE_DIR, device=None):
+    def __init__(self, problems, C=25, d=5, prec=RD_PREC_FP32, tdir=TABLE_DIR, device=None, form="3d"):
         import numpy as np
         L = lib()
+        if form not in ("3d", "uxw"):
+            raise RdError(f"unknown batch form {form}")
+        self.form = form
         i, vp, dd = ctypes.c_int, ctypes.c_void_p, ctypes.c_double
         if not getattr(L, "_batch_bound", False):
             L.rd_batch_create.argtypes = [ctypes.POINTER(vp), i] + [ctypes.POINTER(i)] * 4 + [
@@ -405,6 +442,11 @@ class Batch:
             L.rd_batch_config.argtypes = [vp, i, i]
             L.rd_batch_tune.argtypes = [vp] + [ctypes.POINTER(vp)] * 3 + [i, ctypes.POINTER(dd),
                                                                            ctypes.POINTER(dd), ctypes.POINTER(i)]
+            L.rd_batch_create_uxw.argtypes = L.rd_batch_create.argtypes
+            for nm in ("rd_batch_nonlinear_uxw", "rd_batch_capture_uxw"):
+                getattr(L, nm).argtypes = [vp] + [ctypes.POINTER(vp)] * 5
+            L.rd_batch_tune_uxw.argtypes = [vp] + [ctypes.POINTER(vp)] * 5 + [i, ctypes.POINTER(dd),
+                                                                               ctypes.POINTER(dd), ctypes.POINTER(i)]
             L._batch_bound = True
         self.problems = [tuple(p) for p in problems]
         self.device = torch.device(device if device is not None else "cuda")
@@ -417,8 +459,9 @@ class Batch:
             dirp = dv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _check(L.rd_batch_create(ctypes.byref(h), n, arr(0, i), arr(1, i), arr(2, i), arr(3, i), C, d,
-                                     arr(4, dd), arr(5, dd), dirp, prec), "batch_create")
+            create = L.rd_batch_create_uxw if form == "uxw" else L.rd_batch_create
+            _check(create(ctypes.byref(h), n, arr(0, i), arr(1, i), arr(2, i), arr(3, i), C, d,
+                          arr(4, dd), arr(5, dd), dirp, prec), "batch_create")
         self._h = h
         self.nlm = [(p[0] + 1) * (p[0] + 2) // 2 for p in self.problems]
 
@@ -473,6 +516,23 @@ class Batch:
 
     def config(self, jb, wg_per_cu):
         _check(lib().rd_batch_config(self.h, jb, wg_per_cu), "batch_config")
+
+    def nonlinear_uxw(self, W, Z, NQ, NS, NT):
+        """u x omega of every problem (form "uxw"): lists of (nz, nlm) complex128 tensors."""
+        _check(lib().rd_batch_nonlinear_uxw(self.h, *[self._ptrs(x, "spec") for x in (W, Z, NQ, NS, NT)]),
+               "batch_nonlinear_uxw")
+
+    def capture_uxw(self, W, Z, NQ, NS, NT):
+        _check(lib().rd_batch_capture_uxw(self.h, *[self._ptrs(x, "spec") for x in (W, Z, NQ, NS, NT)]),
+               "batch_capture_uxw")
+
+    def tune_uxw(self, W, Z, NQ, NS, NT, reps=3):
+        """Returns (ms_fixed, ms_tuned, (jb, wg_per_cu)) per evaluation of every problem."""
+        f, t = ctypes.c_double(), ctypes.c_double()
+        ch = (ctypes.c_int * 2)()
+        _check(lib().rd_batch_tune_uxw(self.h, *[self._ptrs(x, "spec") for x in (W, Z, NQ, NS, NT)], reps,
+                                       ctypes.byref(f), ctypes.byref(t), ch), "batch_tune_uxw")
+        return f.value, t.value, (ch[0], ch[1])
 
     def tune(self, spec, grid, spec_out, reps=3):
         """Returns (ms_fixed, ms_tuned, (jb, wg_per_cu)) per round trip."""

@@ -21,7 +21,17 @@ least-squares problem is ill-conditioned in FP64:
   * A[ii-1, k] = T_k(d-1+ii), ii = 1..C: the blend-to-zero continuation of
     g_k sampled on the C continuation points.
 
+Neumann reconstructors Q1n{d}.dat and Q2n{d}.dat (read by
+load_neumann_tables, fcgram_mod.f90:267-371: a leading FP64 dxp, then Qn,
+d x d, column major).  The reference reconstructs the wall value as
+f_d = Q(d,:) Qn^T v with v = (f_1, ..., f_{d-1}, g), g the ord-th derivative
+at the wall x_d: so Qn^T maps v to the Gram coefficients of the degree-<d
+polynomial p with p(x_j) = f_j (j < d) and p^(ord)(x_d) = g, i.e.
+Qn = M^-T with M[j, k] = g_k(x_j) for j < d and M[d-1, k] = g_k^(ord)(x_{d-1}).
+The matching points are at unit spacing, so dxp = 1.
+
 Usage:  python make_fcgram_tables.py [C] [d] [outdir]
+        python make_fcgram_tables.py --neumann [d] [outdir]
 """
 import os
 import struct
@@ -111,6 +121,29 @@ def blend_tables(C, d):
     return A, Q, resid
 
 
+def neumann_tables(d, ord_):
+    """Qn (d x d) of the reconstructor of order ord_ (1: first, 2: second
+    derivative at the wall x_{d-1})."""
+    Q, coef = gram_basis(d)
+    t = mp.mpf(d - 1)
+    M = mp.matrix(d, d)
+    for k in range(d):
+        for j in range(d - 1):
+            M[j, k] = Q[j][k]
+        # ord_-th derivative of g_k(t) = sum_i coef[i][k] t^i at the wall
+        M[d - 1, k] = mp.fsum(coef[i][k] * mp.ff(i, ord_) * t ** (i - ord_) for i in range(ord_, d))
+    Qn = (M ** -1).T
+    return [[Qn[r, c] for c in range(d)] for r in range(d)]
+
+
+def write_neumann(path, Qn, d, dxp=1.0):
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<d", dxp))
+        for c in range(d):
+            for r in range(d):
+                fh.write(struct.pack("<d", float(Qn[r][c])))
+
+
 def write_stream(path, mat, nrow, ncol):
     """Fortran unformatted stream, column major, FP64 little endian."""
     with open(path, "wb") as fh:
@@ -120,6 +153,12 @@ def write_stream(path, mat, nrow, ncol):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--neumann":
+        d = int(sys.argv[2]) if len(sys.argv) > 2 else 5
+        out = sys.argv[3] if len(sys.argv) > 3 else os.path.dirname(os.path.abspath(__file__))
+        for o in (1, 2):
+            write_neumann(os.path.join(out, f"Q{o}n{d}.dat"), neumann_tables(d, o), d)
+        return
     C = int(sys.argv[1]) if len(sys.argv) > 1 else 25
     d = int(sys.argv[2]) if len(sys.argv) > 2 else 5
     out = sys.argv[3] if len(sys.argv) > 3 else os.path.dirname(os.path.abspath(__file__))

@@ -80,3 +80,26 @@ def test_python_mirror_rejects_unknown_space():
         S.derivr(None, None, None, 1, "x")
     with pytest.raises(S.RdError, match="Unknown space"):
         S.fc_filter(None, None, "q")
+
+
+@pytest.mark.parametrize("ord_", [1, 2])
+def test_load_neumann_tables_matches_oracle(ord_):
+    """rd_load_neumann_tables (host) against the oracle's restatement of
+    load_neumann_tables (fcgram_mod.f90:267-371)."""
+    from redondito_amd import sphp as S
+    dx = 0.0123
+    neu = S.load_neumann_tables(S.TABLE_DIR, 5, ord_, dx)
+    Q, dxp, Qn = O.load_neumann_files(S.TABLE_DIR, 5, ord_)
+    ref = O.neumann_table(5, ord_, dx, Q, Qn, dxp)
+    assert np.array_equal(neu, ref)
+
+
+def test_neumann_tables_missing_and_undefined_order(tmp_path, capfd):
+    """A missing table is an error (fcgram_mod.f90:331-352 stops); an undefined
+    order prints the reference's warning and leaves the table unwritten."""
+    from redondito_amd import sphp as S
+    with pytest.raises(S.RdError, match="table"):
+        S.load_neumann_tables(str(tmp_path), 5, 1, 0.1)
+    out = S.load_neumann_tables(S.TABLE_DIR, 5, 3, 0.1)
+    assert np.all(np.isnan(out))
+    assert "undefined order" in capfd.readouterr().err

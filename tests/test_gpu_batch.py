@@ -234,3 +234,69 @@ def test_batch_benchmark_draw_48_problems():
     print(f"fp32 batch vs fp64 {e32:.2e}; fp32 round trip {rt:.2e}")
     assert 0 < e32 < 1e-5 and 0 < rt < 1e-5
     b32.close()
+
+
+# ------------------------------------------------ nonlinear-term form
+UXW_PROBLEMS = [(31, 24), (63, 40), (47, 128), (95, 32), (31, 64), (127, 96)]   # (lmax, nr)
+
+
+def _uxw_inputs(probs, seed):
+    out = []
+    for i, (lmax, nlat, nphi, nz, dz, ri) in enumerate(probs):
+        W, Z = H.shell_fields(lmax, nz - 25, nz, ri, RO, seed=seed + i, slope=2.0)
+        out.append((torch.from_numpy(W).to(dev()), torch.from_numpy(Z).to(dev())))
+    return out
+
+
+def test_batch_nonlinear_uxw_matches_single_plans():
+    """The nonlinear-term form of the batch (u x omega of every problem, the
+    Legendre stage of all problems in one persistent launch per direction)
+    equals rd_nonlinear_uxw of each problem's own plan bit for bit (same tile
+    arithmetic, same accumulation order), in FP64 and in the FP32 form; its
+    hipGraph replays equal the eager result, with eager work in between; the
+    tuner changes the schedule, not the result."""
+    from redondito_amd import sphp as S
+    probs = problem_list(UXW_PROBLEMS)
+    ins = _uxw_inputs(probs, 20261205)
+    W = [w for w, _ in ins]
+    Z = [z for _, z in ins]
+    for prec in (S.RD_PREC_FP64, S.RD_PREC_FP32):
+        b = S.Batch(probs, prec=prec, form="uxw")
+        info = b.info()
+        assert info["jb"] == 1 and info["syn_items"] > 0
+        out = [specs(probs) for _ in range(3)]
+        b.nonlinear_uxw(W, Z, *out)
+        torch.cuda.synchronize()
+        for i, p in enumerate(probs):
+            plan = S.FCPlan(p[0], p[1], p[2], p[3], 25, 5, p[4], p[5])
+            ref = [torch.full_like(W[i], 3.0) for _ in range(3)]
+            S.nonlinear_uxw(plan, W[i], Z[i], *ref, prec=prec)
+            torch.cuda.synchronize()
+            d = max((a[i] - r).abs().max().item() for a, r in zip(out, ref))
+            print(f"prec {prec} problem {p[:4]}: max |batch - single| {d:.1e}")
+            assert all(torch.equal(a[i], r) for a, r in zip(out, ref)), (prec, p[:4], d)
+            plan.close()
+        # graph: replays equal the eager result, with eager work between them
+        g = [specs(probs) for _ in range(3)]
+        b.capture_uxw(W, Z, *g)
+        assert b.info()["captured"] == 1
+        other = [specs(probs) for _ in range(3)]
+        for rep in range(3):
+            for t in g:
+                for x in t:
+                    x.zero_()
+            b.replay()
+            torch.cuda.synchronize()
+            assert all(torch.equal(x, y) for a, r in zip(g, out) for x, y in zip(a, r)), rep
+            b.nonlinear_uxw(Z, W, *other)          # eager work on other arrays
+            torch.cuda.synchronize()
+        fixed, tuned, choice = b.tune_uxw(W, Z, *other, reps=2)
+        print(f"prec {prec}: tune fixed {fixed:.3f} ms tuned {tuned:.3f} ms choice {choice}")
+        assert tuned <= fixed and choice[0] in (1, 2)
+        t2 = [specs(probs) for _ in range(3)]
+        b.nonlinear_uxw(W, Z, *t2)
+        torch.cuda.synchronize()
+        assert all(torch.equal(x, y) for a, r in zip(t2, out) for x, y in zip(a, r))
+        with pytest.raises(S.RdError):
+            b.inverse(W, grids(probs))             # the round trip is not this form's
+        b.close()

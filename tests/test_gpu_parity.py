@@ -349,6 +349,52 @@ def test_fc_filter_vs_oracle(nz):
     assert e_k < 1e-14 and e_r < max(TOL, 30 * floor)
 
 
+@pytest.mark.parametrize("boun", [5, 6])
+def test_neumann_and_robin_reconstruct_vs_oracle(boun):
+    """neumann_reconstruct (orders 1, 2) and robin_reconstruct along z
+    (src/sphp/fcgram_mod.f90:374-650) on the device against the oracle, and
+    the wall value of a degree-4 radial polynomial recovered from its outward
+    normal derivative."""
+    S, plan, dz = _pseudo_setup(lmax=15, nz=65)
+    nz, nlm, C, d = plan.nz, plan.nlm, 25, 5
+    nr = nz - C
+    z = plan.grid()[3]
+    rng = np.random.default_rng(boun)
+    c = rng.standard_normal((5, nlm)) + 1j * rng.standard_normal((5, nlm))
+    f = sum(c[i][None] * (z[:, None] ** i) for i in range(5))
+    rb = 0 if boun == 5 else nr - 1
+    for ord_ in (1, 2):
+        neu = S.load_neumann_tables(tables_dir(), d, ord_, dz)
+        dc = [c[i] * np.prod(np.arange(i - ord_ + 1, i + 1)) if i >= ord_ else 0 * c[i] for i in range(5)]
+        g = sum(dc[i] * z[rb] ** (i - ord_) for i in range(ord_, 5))
+        if ord_ == 1 and boun == 5:
+            g = -g
+        a = f.copy()
+        a[rb] = g
+        A = to_dev(a)
+        S.neumann_reconstruct(plan, A, boun, ord_, neu)
+        got = to_host(A)
+        ref = O.fc_reconstruct(a, C, d, boun, neu)
+        assert np.array_equal(np.delete(got, rb, 0), np.delete(a, rb, 0))
+        e_orc = np.max(np.abs(got[rb] - ref[rb])) / np.max(np.abs(ref[rb]))
+        e_poly = np.max(np.abs(got[rb] - f[rb])) / np.max(np.abs(f[rb]))
+        print(f"boun {boun} ord {ord_}: vs oracle {e_orc:.1e}, vs polynomial {e_poly:.1e}")
+        assert e_orc < 1e-15 and e_poly < 1e-12
+    neu = S.load_neumann_tables(tables_dir(), d, 1, dz)
+    coef = rng.uniform(0.5, 2.0, nlm)
+    fp = sum(c[i] * i * z[rb] ** (i - 1) for i in range(1, 5)) * (-1.0 if boun == 5 else 1.0)
+    a = f.copy()
+    a[rb] = fp + coef * f[rb]
+    A = to_dev(a)
+    S.robin_reconstruct(plan, A, boun, neu, to_dev(coef))
+    got = to_host(A)
+    ref = O.fc_reconstruct(a, C, d, boun, neu, coef)
+    assert np.max(np.abs(got[rb] - ref[rb])) / np.max(np.abs(ref[rb])) < 1e-15
+    assert np.max(np.abs(got[rb] - f[rb])) / np.max(np.abs(f[rb])) < 1e-12
+    with pytest.raises(S.RdError):
+        S.neumann_reconstruct(plan, A, 3, 1, neu)
+
+
 def test_redist_is_the_one_rank_copy():
     """fftp3d_redist_k_to_lm / _lm_to_k (src/sphp/sphp_sca.F90:629-760): with one
     task the exchange is a copy of the (nlm, planes) block, both directions,

@@ -275,6 +275,79 @@ def test_kz_conventions():
     assert np.allclose(k7, [0, 1, 2, 3, -3, -2, -1])
 
 
+# ------------------------------------------------ Neumann / Robin tables
+def _neumann_direct(d, ord_, dx):
+    """The reconstructor built without the Gram tables: the degree-<d
+    polynomial through f_1..f_{d-1} at x_j = (j-1) dx with its ord-th
+    derivative g at x_d, evaluated at x_d, as weights on (f_1..f_{d-1}, g)."""
+    x = dx * np.arange(d)
+    V = np.vander(x, d, increasing=True)            # p(x) = sum c_i x^i
+    M = V.copy()
+    M[d - 1] = [0.0 if i < ord_ else np.prod(np.arange(i - ord_ + 1, i + 1)) * x[d - 1] ** (i - ord_)
+                for i in range(d)]
+    return V[d - 1] @ np.linalg.inv(M)
+
+
+@pytest.mark.parametrize("ord_", [1, 2])
+def test_neumann_table_vs_direct_solve(ord_):
+    """load_neumann_tables (fcgram_mod.f90:267-371) from Q5.dat and Q{ord}n5.dat
+    equals the reconstructor solved directly from the polynomial conditions,
+    at a grid spacing other than the tables' (the (dx/dxp)^ord factor)."""
+    d, dx = 5, 0.37
+    Q, dxp, Qn = O.load_neumann_files(tables(), d, ord_)
+    assert dxp == 1.0
+    neu = O.neumann_table(d, ord_, dx, Q, Qn, dxp)
+    ref = _neumann_direct(d, ord_, dx)
+    assert np.max(np.abs(neu - ref)) < 1e-13 * np.max(np.abs(ref))
+    # weights on the values reproduce constants
+    assert abs(neu[: d - 1].sum() - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("boun", [5, 6])
+@pytest.mark.parametrize("ord_", [1, 2])
+def test_neumann_and_robin_reconstruct_polynomials(boun, ord_):
+    """neumann_reconstruct / robin_reconstruct (fcgram_mod.f90:374-650) along
+    z on (nz, nlm) columns: the wall value of any polynomial of degree < d is
+    recovered from its outward normal derivative (the table frame runs toward
+    the wall, so at z(1) the derivative is -d/dz), whatever the rows away
+    from the d matching points hold; a degree-d polynomial is not."""
+    C, d, nr, ncols = 25, 5, 40, 6
+    n = nr + C
+    rint, dz = 0.35, 0.65 / (nr - 1)
+    z = rint + dz * np.arange(n)
+    rng = np.random.default_rng(boun * 10 + ord_)
+    Q, dxp, Qn = O.load_neumann_files(tables(), d, ord_)
+    neu = O.neumann_table(d, ord_, dz, Q, Qn, dxp)
+    rb = 0 if boun == 5 else nr - 1
+    sgn = (-1.0 if boun == 5 else 1.0) if ord_ == 1 else 1.0
+    for deg, ok in ((d - 1, True), (d, False)):
+        c = rng.standard_normal((deg + 1, ncols)) + 1j * rng.standard_normal((deg + 1, ncols))
+        P = [np.polynomial.polynomial.Polynomial(c[:, j].real) + 1j * np.polynomial.polynomial.Polynomial(c[:, j].imag)
+             for j in range(ncols)]
+        f = np.array([[np.polyval(c[::-1, j], zz) for j in range(ncols)] for zz in z])
+        dpoly = [np.polyder(c[::-1, j], ord_) for j in range(ncols)]
+        g = np.array([sgn * np.polyval(dp, z[rb]) for dp in dpoly])
+        a = f.copy()
+        junk = np.ones(n, bool)
+        junk[:d] = junk[nr - d:nr] = False
+        a[junk] = 1e3 * (rng.standard_normal((junk.sum(), ncols)) + 1j)
+        a[rb] = g
+        out = O.fc_reconstruct(a, C, d, boun, neu)
+        err = np.max(np.abs(out[rb] - f[rb])) / np.max(np.abs(f[rb]))
+        assert np.array_equal(np.delete(out, rb, 0), np.delete(a, rb, 0))
+        if ok:
+            assert err < 1e-12, err
+            coef = rng.uniform(0.5, 2.0, ncols)
+            a[rb] = g + coef * f[rb] if ord_ == 1 else a[rb]
+            if ord_ == 1:
+                out = O.fc_reconstruct(a, C, d, boun, neu, coef)
+                assert np.max(np.abs(out[rb] - f[rb])) / np.max(np.abs(f[rb])) < 1e-12
+        else:
+            assert err > 1e-10, err
+    with pytest.raises(ValueError):
+        O.fc_reconstruct(a, C, d, 3, neu)
+
+
 # ------------------------------------------- extended-precision build
 def test_extended_build_nodes_vs_mpmath():
     """liboracle_ld.so (the same source in x87 long double) really computes in
