@@ -15,11 +15,15 @@ per field and per degree band.  The headline value is the fastest form whose
 joint relative L2 error against FP64 is at most 1e-9.
 
 Also reported:
-  * absolute scalar transforms/s and the wall solves at the same size;
-  * the batching form: 48 uneven problems, persistent launches, hipGraph;
+  * each form's evaluation replayed from a hipGraph;
+  * absolute scalar transforms/s, the wall solves at the same size and the
+    HBM bandwidth of a plain device copy;
+  * the batching form: 48 uneven problems, persistent launches, hipGraph,
+    for the scalar 3D round trip and for the nonlinear term of every problem;
   * the large coupled step at lmax = 511, nr = 256 (u x omega, u x B and
-    u . grad T) in all three forms, with per-field and per-band errors;
-  * the CPU oracle timed on the host cores.
+    u . grad T) in all forms, with per-field and per-band errors;
+  * the CPU oracle timed on all host threads and on one, with its 3D
+    transform at the size the reference's CPU timing is quoted on.
 
 Prints one JSON line.  --gpus above 1 is refused: the path is single-GPU.
 """
@@ -58,16 +62,58 @@ def seeded_fields(n):
     return H.shell_fields(LMAX, NR, n, RI, RO, seed=SEED, slope=2.0)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_calibration(O, S):
+    """The oracle's 3D transforms at the size LEEME.md:36 quotes for the
+    reference's CPU path (lmax 63, nlat = nphi = 128, nz = 128, C = 25, d = 5:
+    11 ms per transform on one MPI rank), on one thread and on all threads;
+    ms per transform (the mean of the forward and the inverse)."""
+    lmax, nlat, nphi, nz = 63, 128, 128, 128
+    A, Q = O.load_tables(S.TABLE_DIR, C, D)
+    sht = O.SHT(lmax, nlat, nphi)
+    fc = O.FC(nz, C, D, O.fc_dir(A, Q))
+    rng = np.random.default_rng(SEED + 3)
+    nlm = (lmax + 1) * (lmax + 2) // 2
+    a = rng.standard_normal((nz, nlm)) + 1j * rng.standard_normal((nz, nlm))
+    out = {}
+    for label, threads in (("one_thread_ms", 1), ("all_threads_ms", 0)):
+        O.set_threads(threads)
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            g = sht.synth(fc.inverse(a.copy()))
+            t1 = time.perf_counter()
+            fc.forward(sht.anal(g))
+            t2 = time.perf_counter()
+            best = min(best, 0.5 * (t2 - t0))
+        out[label] = round(best * 1e3, 2)
+    O.set_threads(0)
+    out["reference_ms_one_rank"] = 11.0
+    out["config"] = "3D transform, lmax 63, nlat = nphi = 128, nz = 128, C 25, d 5 (LEEME.md:36)"
+    return out
+
+
 def cpu_baseline(W, Z, nlat, nphi, n, dz, gpu_out, budget_s=15.0):
     """The oracle (CPU FP64 restatement, OpenMP) on a bounded sample of shells of
-    the same workload; also checks the GPU FP64 result on those shells."""
+    the same workload, on all host threads and on one; also checks the GPU
+    FP64 result on those shells."""
     from oracle import oracle as O
     from tests import helpers as H
     import redondito_amd.sphp as S
     A, Q = O.load_tables(S.TABLE_DIR, C, D)
     sht = O.SHT(LMAX, nlat, nphi)
     fc = O.FC(n, C, D, O.fc_dir(A, Q))
-    cores = os.cpu_count() or 1
+    cores = O.set_threads(0)
     # the radial part is done in full on every call; time it separately
     t0 = time.perf_counter()
     O.nonlinear_uxw(sht, fc, RI, dz, W, Z, 0, 0)
@@ -84,6 +130,16 @@ def cpu_baseline(W, Z, nlat, nphi, n, dz, gpu_out, budget_s=15.0):
     t_eval = t_rad + t_ang * NR
     errs = [H.rell2_joint([g[:nsample] for g in gpu_out], [r[:nsample] for r in ref])]
     errs += [H.rell2(g[:nsample], r[:nsample]) for g, r in zip(gpu_out, ref)]
+    # one thread: the radial step on every shell plus two shells
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.nonlinear_uxw(sht, fc, RI, dz, W, Z, 0, 0)
+    t1_rad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.nonlinear_uxw(sht, fc, RI, dz, W, Z, 0, 2)
+    t1 = time.perf_counter() - t0
+    O.set_threads(0)
+    t1_eval = t1_rad + max(t1 - t1_rad, 1e-9) / 2 * NR
     return {
         "value": 1.0 / t_eval,
         "unit": "nonlinear-term evals/s",
@@ -91,6 +147,10 @@ def cpu_baseline(W, Z, nlat, nphi, n, dz, gpu_out, budget_s=15.0):
         "kind": "port",
         "sample": (f"oracle u x omega on shells 0..{nsample - 1} of {NR} (radial step on all shells) "
                    f"= {t:.2f} s, extrapolated to all {NR} shells; OpenMP threads = {cores}"),
+        "cpu_model": cpu_model(),
+        "one_thread": {"value": 1.0 / t1_eval, "sample": (f"oracle u x omega on shells 0..1 (radial step on all "
+                                                          f"shells) = {t1:.2f} s on 1 thread, extrapolated")},
+        "calibration": cpu_calibration(O, S),
     }, {"joint": errs[0], "NQ": errs[1], "NS": errs[2], "NT": errs[3], "shells": nsample}
 
 
@@ -104,6 +164,25 @@ def copy_calibration(nbytes=1 << 30, reps=3):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """HBM bandwidth of a plain device copy (read + write bytes per second),
+    to set beside the 8 TB/s peak."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
 
 
 def measured_traffic(kernel):
@@ -135,6 +214,21 @@ def time_form(S, plan, prec, Wd, Zd, out, steps, warmup):
     stages = plan.stage_times()
     plan.profile(False)
     return (t1 - t0) * 1e3 / steps, {k: v[0] / max(v[1], 1) for k, v in stages.items()}
+
+
+def time_graph(S, plan, prec, Wd, Zd, out, steps):
+    """One evaluation captured into a hipGraph (after rd_plan_prepare), then
+    `steps` replays timed like the eager steps; returns ms per step."""
+    import torch
+    plan.prepare(prec)
+    plan.capture_uxw(Wd, Zd, *out, prec=prec)
+    plan.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def time_calls(fn, reps=10):
@@ -240,6 +334,55 @@ def batch_bench(reps):
         "round_trip_rel_err": float(f"{err:.3e}"),
         "work_items": {"synthesis": info["syn_items"], "analysis": info["ana_items"]},
     }
+
+
+def batch_uxw_bench(reps):
+    """The nonlinear-term form of the batch on the same 48-problem draw: u x
+    omega of every problem (velocity potentials seeded as in the headline
+    workload), the Legendre stage of all problems as one persistent launch per
+    direction, FP64 and FP32 Legendre, eager and hipGraph-replayed, tuned
+    against fixed tiles; one step = one evaluation of all 48 problems."""
+    import torch
+    import redondito_amd.sphp as S
+    from tests import helpers as H
+    probs = batch_problems()
+    dev = torch.device("cuda:0")
+    W, Z = [], []
+    for i, (lmax, nlat, nphi, nz, dz, ri) in enumerate(probs):
+        w, z = H.shell_fields(lmax, nz - C, nz, ri, RO, seed=SEED + 100 + i, slope=2.0)
+        W.append(torch.from_numpy(w).to(dev))
+        Z.append(torch.from_numpy(z).to(dev))
+    res, outs = {}, {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    for name, prec in (("fp64", S.RD_PREC_FP64), ("fp32", S.RD_PREC_FP32)):
+        b = S.Batch(probs, prec=prec, form="uxw")
+        out = [[torch.zeros_like(w) for w in W] for _ in range(3)]
+        fixed, tuned, choice = b.tune_uxw(W, Z, *out, reps=2)
+        eager = timed(lambda: b.nonlinear_uxw(W, Z, *out))
+        b.capture_uxw(W, Z, *out)
+        graph = timed(b.replay)
+        torch.cuda.synchronize()
+        outs[name] = [[x.cpu().numpy() for x in o] for o in out]
+        res[name] = {"ms_eager": round(eager, 3), "ms_graph": round(graph, 3), "ms_fixed_tiles": round(fixed, 3),
+                     "ms_tuned_tiles": round(tuned, 3),
+                     "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1]},
+                     "evals_per_s": round(len(probs) * 1e3 / min(graph, eager), 1)}
+        del out
+        b.close()
+        torch.cuda.empty_cache()
+    num = sum(float(np.linalg.norm(a - r)) ** 2 for k in range(3) for a, r in zip(outs["fp32"][k], outs["fp64"][k]))
+    den = sum(float(np.linalg.norm(r)) ** 2 for k in range(3) for r in outs["fp64"][k])
+    res["fp32"]["rel_err_vs_fp64"] = float(f"{(num / den) ** 0.5:.3e}")
+    return {"problems": len(probs), "step": "u x omega of every problem", **res}
 
 
 CP_LMAX, CP_NLAT, CP_NPHI, CP_NR = 511, 768, 1536, 256
@@ -361,7 +504,9 @@ def main():
         out = [torch.zeros_like(Wd) for _ in range(3)]
         ms, stages = time_form(S, plan, prec, Wd, Zd, out, args.steps, args.warmup)
         host[name] = [o.cpu().numpy() for o in out]
+        gms = time_graph(S, plan, prec, Wd, Zd, [torch.zeros_like(Wd) for _ in range(3)], args.steps)
         forms[name] = {"evals_per_s": round(1e3 / ms, 3), "ms_per_step": round(ms, 4),
+                       "graph_ms_per_step": round(gms, 4),
                        "stage_ms": {k: round(v, 4) for k, v in stages.items()}, "_stages": stages}
     for name in forms:
         rep = H.band_errors(host[name], host["fp64"], LMAX, NR)
@@ -382,7 +527,9 @@ def main():
     t_roof = (8 * Sb / bw + 2 * max(leg, (3 * Sb + 3 * Fm) / bw) + 9 * Fm / bw
               + max(leg, (3 * Fm + 3 * Sb) / bw))
     aux = aux_timings(S, plan, Wd)
+    aux["hbm_copy_gbs"] = copy_bandwidth()
     batch = None if (args.no_batch or args.fp64_only) else batch_bench(max(2, min(args.steps, 10)))
+    batch_uxw = None if (args.no_batch or args.fp64_only) else batch_uxw_bench(max(2, min(args.steps, 10)))
     coupled = None if (args.no_coupled or args.fp64_only) else coupled_bench(max(2, min(args.steps, 5)))
 
     cpu = parity = None
@@ -413,6 +560,7 @@ def main():
         "forms": forms,
         "aux": aux,
         "batch": batch,
+        "batch_uxw": batch_uxw,
         "coupled": coupled,
         "rel_l2_vs_oracle": parity,
         "cpu_baseline": cpu,

@@ -211,6 +211,13 @@ int rd_copy(rd_plan *plan, const double *a, double *b);
 
 /* ---- the composed nonlinear term ------------------------------------ */
 
+/* Builds everything precision form prec needs (the FP32 / BF16 tables, the
+ * int8 digit tables and operand scratch of RD_PREC_OZAKI) now instead of on
+ * the form's first call, so that later calls allocate nothing and can be
+ * captured.  No counterpart in the reference (its plan is complete at
+ * fftp3d_create_plan). */
+int rd_plan_prepare(rd_plan *plan, int prec);
+
 /* N = u x omega with u = curl curl (W r) + curl (Z r):  W, Z (nlm, nz) in
  * (lm, r) -> NQ, NS, NT (nlm, nz) on the physical rows (rows nr..nz-1 are
  * set to zero).  W and Z are not modified.  prec selects RD_PREC_*.
@@ -220,6 +227,14 @@ int rd_copy(rd_plan *plan, const double *a, double *b);
  * own for it. */
 int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ,
                      double *NS, double *NT);
+
+/* Captures one rd_nonlinear_uxw(plan, prec, W, Z, NQ, NS, NT) into a hipGraph
+ * held by the plan (replacing an earlier one); rd_plan_replay launches it on
+ * the bound stream.  The graph reads and writes these arrays on every
+ * replay.  No reference routine. */
+int rd_nonlinear_uxw_capture(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ, double *NS,
+                             double *NT);
+int rd_plan_replay(rd_plan *plan);
 
 /* One nonlinear step of the coupled state: velocity (poloidal W, toroidal Z),
  * magnetic field B = curl curl (Pb r) + curl (Tb r) and a temperature scalar

@@ -58,6 +58,8 @@ def lib():
         L.orc_dlr.argtypes = [_P, _D, _D, ctypes.POINTER(ctypes.c_int), _I, _L, _D, _D]
         L.orc_prodvec.argtypes = [_L] + [_D] * 9
         L.orc_fc_filter_k.argtypes = [_I, _I, _L, _D, ctypes.c_double, _D]
+        L.orc_set_threads.restype = _I
+        L.orc_set_threads.argtypes = [_I]
         L.orc_neumann_table.argtypes = [_I, _I, ctypes.c_double, _D, _D, ctypes.c_double, _D]
         L.orc_fc_reconstruct.restype = _I
         L.orc_fc_reconstruct.argtypes = [_I, _I, _I, _I, _L, _I, _D, _D, _D]
@@ -76,6 +78,11 @@ def _d(a):
     assert a.flags["C_CONTIGUOUS"]
     assert a.dtype in (np.float64, np.complex128)
     return a.ctypes.data_as(_D)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (0: the default); returns the count."""
+    return lib().orc_set_threads(int(n))
 
 
 def nlm_of(lmax):

@@ -27,6 +27,7 @@
  * Layouts follow the reference: spectral a(lm, k) complex, lm fastest;
  * physical f(theta, phi, k) real, theta fastest.
  */
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include <tgmath.h>
@@ -45,6 +46,16 @@ typedef double real;
 typedef double _Complex cplx;
 #endif
 #define ORC_PI ((real)3.14159265358979323846264338327950288L)
+
+/* OpenMP threads of the parallel loops (n < 1: the runtime's default);
+ * returns the count now in effect. */
+int orc_set_threads(int n)
+{
+    static int deflt = 0;
+    if (!deflt) deflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : deflt);
+    return omp_get_max_threads();
+}
 
 /* ------------------------------------------------------------------ */
 /* Complex FFT: recursive mixed radix, generic radix-p butterflies.     */

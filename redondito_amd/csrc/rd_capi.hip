@@ -120,9 +120,18 @@ int setup_zfft(Plan *p)
     return upload_twiddles(p, n1, &p->tw_z);
 }
 
+void drop_plan_graph(Plan *p)
+{
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+}
+
 void free_plan(Plan *p)
 {
     if (!p) return;
+    drop_plan_graph(p);
     free_bc(p);
     free_precision_tables(p);
     free_oz_tables(p);
@@ -756,6 +765,60 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     MARK(6);
 #undef MARK
     return rc;
+}
+
+int rd_plan_prepare(rd_plan *plan, int prec)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    if (prec < RD_PREC_FP64 || prec > RD_PREC_OZAKI) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    int rc = RD_OK;
+    if (prec == RD_PREC_OZAKI) rc = ensure_oz_tables(p);
+    else if (prec != RD_PREC_FP64) rc = ensure_precision_tables(p, prec);
+    if (rc) return rc;
+    RD_HIP(hipStreamSynchronize(p->stream));
+    return RD_OK;
+}
+
+int rd_nonlinear_uxw_capture(rd_plan *plan, int prec, const double *W, const double *Z, double *NQ, double *NS,
+                             double *NT)
+{
+    if (bad_plan(plan) || !W || !Z || !NQ || !NS || !NT) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->nwork < 6) return RD_ERR_STATE;
+    int rc = rd_plan_prepare(plan, prec);
+    if (rc) return rc;
+    drop_plan_graph(p);
+    // recorded on the plan's own stream without the per-stage events, then
+    // launched on whichever stream the plan is bound to
+    const hipStream_t user = p->stream;
+    const int prof = p->profiling;
+    p->stream = p->own_stream;
+    p->profiling = 0;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(p->own_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = rd_nonlinear_uxw(plan, prec, W, Z, NQ, NS, NT);
+        e = hipStreamEndCapture(p->own_stream, &g);
+    }
+    p->stream = user;
+    p->profiling = prof;
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : RD_ERR_HIP;
+    }
+    p->graph = g;
+    RD_HIP(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+    return RD_OK;
+}
+
+int rd_plan_replay(rd_plan *plan)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (!p->gexec) return RD_ERR_STATE;
+    RD_HIP(hipGraphLaunch(p->gexec, p->stream));
+    return RD_OK;
 }
 
 // One nonlinear step of the coupled state on the plan's six Fourier and six

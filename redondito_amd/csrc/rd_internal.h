@@ -94,6 +94,9 @@ struct Plan {
     double2 *F[6];                 // Fourier coefficients [k][m][theta], nz planes
     double2 *S[6];                 // spectral scratch (nlm, nz)
     long fsize, ssize;             // in elements of the type
+    // one captured evaluation of rd_nonlinear_uxw (rd_nonlinear_uxw_capture)
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
     // per-stage event timing of rd_nonlinear_uxw: a ring of event slots,
     // harvested lazily so that timing adds no host synchronisation
     int profiling;

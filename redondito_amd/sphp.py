@@ -67,6 +67,9 @@ def lib():
         L.rd_neumann_reconstruct.argtypes = [vp, vp, i, i, ctypes.POINTER(d)]
         L.rd_robin_reconstruct.argtypes = [vp, vp, i, ctypes.POINTER(d), vp]
         L.rd_nonlinear_uxw.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.rd_nonlinear_uxw_capture.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.rd_plan_prepare.argtypes = [vp, i]
+        L.rd_plan_replay.argtypes = [vp]
         L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
@@ -159,6 +162,18 @@ class FCPlan:
         return self._h
 
     STAGES = ("radial", "synthesis_u", "synthesis_w", "phi_cross", "analysis", "tail")
+
+    def prepare(self, prec):
+        """Build the tables precision form prec needs now (no allocation on its later calls)."""
+        _check(lib().rd_plan_prepare(self._h, prec), "plan_prepare")
+
+    def capture_uxw(self, W, Z, NQ, NS, NT, prec=RD_PREC_FP64):
+        """Capture one nonlinear_uxw into the plan's hipGraph (replay())."""
+        _check(lib().rd_nonlinear_uxw_capture(self.h, prec, self._spec(W), self._spec(Z), self._spec(NQ),
+                                              self._spec(NS), self._spec(NT)), "nonlinear_uxw_capture")
+
+    def replay(self):
+        _check(lib().rd_plan_replay(self.h), "plan_replay")
 
     def profile(self, enable=True):
         """Switch per-stage event timing of nonlinear_uxw on or off (resets it)."""

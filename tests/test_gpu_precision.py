@@ -96,3 +96,33 @@ def test_bad_precision_rejected():
     for bad in (-1, 4):
         with pytest.raises(S.RdError):
             S.nonlinear_uxw(plan, W, W, *o, prec=bad)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3])
+def test_nonlinear_uxw_graph_replays_equal_eager(prec):
+    """rd_plan_prepare builds the form's tables, after which one evaluation is
+    captured into a hipGraph; every replay equals the eager evaluation, with
+    eager evaluations on other arrays between the replays."""
+    from redondito_amd import sphp as S
+    lmax, nr, C, d, ri, ro = 63, 40, 25, 5, 0.35, 1.0
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    plan = S.FCPlan(lmax, nlat, 2 * nlat, n, C, d, dz, ri)
+    W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=11, slope=2.0)
+    Wd, Zd = (torch.from_numpy(x).to("cuda:0") for x in (W, Z))
+    ref = [torch.zeros_like(Wd) for _ in range(3)]
+    plan.prepare(prec)
+    S.nonlinear_uxw(plan, Wd, Zd, *ref, prec=prec)
+    g = [torch.zeros_like(Wd) for _ in range(3)]
+    plan.capture_uxw(Wd, Zd, *g, prec=prec)
+    other = [torch.zeros_like(Wd) for _ in range(3)]
+    for rep in range(3):
+        for t in g:
+            t.zero_()
+        plan.replay()
+        torch.cuda.synchronize()
+        assert all(torch.equal(a, b) for a, b in zip(g, ref)), rep
+        S.nonlinear_uxw(plan, Zd, Wd, *other, prec=prec)
+        torch.cuda.synchronize()
+    plan.close()
