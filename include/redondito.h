@@ -236,6 +236,26 @@ int rd_nonlinear_uxw_capture(rd_plan *plan, int prec, const double *W, const dou
                              double *NT);
 int rd_plan_replay(rd_plan *plan);
 
+/* One stage of the explicit low-storage Runge-Kutta scheme of order ord
+ * around the path (stage o = ord, ..., 1: a = a0 + dt/o rhs(a); the stage
+ * the boundary step closes, vboundary.f90:59-69), for the incompressible
+ * velocity u = curl curl (W r) + curl (Z r) with prognostic Z and
+ * phi = D_l W:
+ *   1. N = u x omega from (W, Z) (rd_nonlinear_uxw, form prec);
+ *   2. Z   = Z0   + dt/o (r N_T + nu D_l Z),
+ *      phi = phi0 + dt/o (d(r N_S)/dr - N_Q + nu D_l phi)
+ *      (the radial and the radial-curl-curl parts of the curl of the
+ *      momentum equation, pressure free);
+ *   3. filter != 0: fc_filter(Z, 'r'), fc_filter(phi, 'r');
+ *   4. v_imposebc_fast(Z, phi, W): no slip on both walls, W = M_l phi.
+ * Z0, phi0: the values at the start of the step (may be Z, phi themselves
+ * for the first stage).  Needs rd_bc_setup.  Uses the plan's spectral scratch.
+ * No single reference routine: the composition of rd_nonlinear_uxw,
+ * derivr/dlr/multr/saxpby_c (pseudospec_hd.f90), fc_filter
+ * (pseudospec_hd.f90:369-467) and v_imposebc_fast (vboundary.f90:91-130). */
+int rd_stage_uxw(rd_plan *plan, int prec, int o, double dt, double nu, int filter, const double *Z0,
+                 const double *phi0, double *Z, double *phi, double *W);
+
 /* One nonlinear step of the coupled state: velocity (poloidal W, toroidal Z),
  * magnetic field B = curl curl (Pb r) + curl (Tb r) and a temperature scalar
  * T, all (nlm, nz) in (lm, r).  Outputs (nlm, nz), rows nr..nz-1 zero:

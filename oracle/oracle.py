@@ -404,6 +404,30 @@ def v_imposebc_fast(fc, kz, z, ell, binv, d2b, lmax, tor, phi):
     return tor, phi, pol
 
 
+def fc_filter_r(fc, a, kzv, dkz):
+    """fc_filter_r (pseudospec_hd.f90:443-467): forward z, fc_filter_k, inverse z."""
+    return fc.inverse(fc_filter_k(fc.forward(np.array(a, np.complex128, copy=True)), kzv, dkz))
+
+
+def stage_uxw(sht, fc, rint, dz, kzv, z, ell, binv, d2b, lmax, o, dt, nu, filt, Z0, phi0, Z, phi, W):
+    """One stage a = a0 + dt/o rhs(a) of the explicit Runge-Kutta scheme for
+    the toroidal potential Z and phi = D_l W of u = curl curl (W r) + curl (Z r):
+      rhs_Z   = r N_T + nu D_l Z,
+      rhs_phi = d(r N_S)/dr - N_Q + nu D_l phi,
+    N = u x omega (nonlinear_uxw) from (W, Z); then fc_filter along r (filt)
+    and v_imposebc_fast (vboundary.f90:91-130).  Returns (Z, phi, W)."""
+    NQ, NS, NT = nonlinear_uxw(sht, fc, rint, dz, W, Z)
+    c = dt / o
+    zz = z[:, None]
+    Zn = Z0 + c * (zz * NT + nu * fc.dlr(Z, kzv, z, ell))
+    phin = phi0 + c * (fc.derivr_r(np.ascontiguousarray(zz * NS), kzv, 1) - NQ + nu * fc.dlr(phi, kzv, z, ell))
+    if filt:
+        dkz = 2 * np.pi / (dz * fc.n)
+        Zn = fc_filter_r(fc, Zn, kzv, dkz)
+        phin = fc_filter_r(fc, phin, kzv, dkz)
+    return v_imposebc_fast(fc, kzv, z, ell, binv, d2b, lmax, Zn, phin)
+
+
 # ---------------------------------------------------------------------------
 # Extended-precision build of the same source (liboracle_ld.so): every step in
 # x87 long double (64-bit significand) on the same FP64 inputs and FC-Gram

@@ -944,6 +944,49 @@ int rd_v_imposebc_fast(rd_plan *plan, double *tor, double *phi, double *pol)
     return rc;
 }
 
+// b = D_l a = d2a/dr2 - l(l+1) a / r^2 with the scratch t1, t2 (dlr,
+// pseudospec_hd.f90:254-288, on chosen buffers).
+static int dl_into(Plan *p, const double2 *a, double2 *b, double2 *t1, double2 *t2)
+{
+    int rc = launch_derivr_r(p, 2, a, b);
+    if (!rc) rc = launch_laplang(p, a, t1);
+    if (!rc) rc = launch_multr(p, -2, t1, t2);
+    if (!rc) rc = launch_saxpby(p, 2L * p->nlm * p->nz, (double *)b, (double *)b, 1.0, (const double *)t2, 1.0);
+    return rc;
+}
+
+int rd_stage_uxw(rd_plan *plan, int prec, int o, double dt, double nu, int filter, const double *Z0,
+                 const double *phi0, double *Z, double *phi, double *W)
+{
+    if (bad_plan(plan) || !Z0 || !phi0 || !Z || !phi || !W || o < 1) return RD_ERR_ARG;
+    if (Z == phi || Z == W || phi == W || Z0 == phi || phi0 == Z || Z0 == W || phi0 == W) return RD_ERR_ARG;
+    if (prec < RD_PREC_FP64 || prec > RD_PREC_OZAKI) return RD_ERR_ARG;
+    Plan *p = P(plan);
+    if (p->nwork < 6 || !p->bc_ready) return RD_ERR_STATE;
+    double2 **S = p->S;
+    const long nd = 2L * p->nlm * p->nz;
+    const double c = dt / o;
+    // N = u x omega into S0..S2 (the evaluation is done with S0..S5 when its
+    // analysis writes them)
+    int rc = rd_nonlinear_uxw(plan, prec, W, Z, (double *)S[0], (double *)S[1], (double *)S[2]);
+    // toroidal: Z = Z0 + c (r NT + nu D_l Z)
+    if (!rc) rc = dl_into(p, (const double2 *)Z, S[3], S[4], S[5]);
+    if (!rc) rc = launch_multr(p, 1, S[2], S[4]);
+    if (!rc) rc = launch_saxpby(p, nd, Z, Z0, 1.0, (const double *)S[4], c);
+    if (!rc) rc = launch_saxpby(p, nd, Z, Z, 1.0, (const double *)S[3], c * nu);
+    // poloidal: phi = phi0 + c (d(r NS)/dr - NQ + nu D_l phi)
+    if (!rc) rc = dl_into(p, (const double2 *)phi, S[3], S[4], S[5]);
+    if (!rc) rc = launch_multr(p, 1, S[1], S[4]);
+    if (!rc) rc = launch_derivr_r(p, 1, S[4], S[5]);
+    if (!rc) rc = launch_saxpby(p, nd, (double *)S[5], (const double *)S[5], 1.0, (const double *)S[0], -1.0);
+    if (!rc) rc = launch_saxpby(p, nd, phi, phi0, 1.0, (const double *)S[5], c);
+    if (!rc) rc = launch_saxpby(p, nd, phi, phi, 1.0, (const double *)S[3], c * nu);
+    if (!rc && filter) rc = rd_fc_filter(plan, 'r', Z);
+    if (!rc && filter) rc = rd_fc_filter(plan, 'r', phi);
+    if (!rc) rc = rd_v_imposebc_fast(plan, Z, phi, W);
+    return rc;
+}
+
 }  // extern "C"
 
 namespace rd {

@@ -70,6 +70,7 @@ def lib():
         L.rd_nonlinear_uxw_capture.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.rd_plan_prepare.argtypes = [vp, i]
         L.rd_plan_replay.argtypes = [vp]
+        L.rd_stage_uxw.argtypes = [vp, i, i, d, d, i, vp, vp, vp, vp, vp]
         L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
@@ -348,6 +349,14 @@ def nonlinear_uxw(plan, W, Z, NQ, NS, NT, prec=RD_PREC_FP64):
     """N = u x omega, u = curl curl (W r) + curl (Z r), as (Q,S,T) in (lm, r)."""
     _check(lib().rd_nonlinear_uxw(plan.h, prec, plan._spec(W), plan._spec(Z), plan._spec(NQ),
                                   plan._spec(NS), plan._spec(NT)), "nonlinear_uxw")
+
+
+def stage_uxw(plan, Z0, phi0, Z, phi, W, o, dt, nu, filter=True, prec=RD_PREC_FP64):
+    """One stage Z, phi = (Z0, phi0) + dt/o rhs of the explicit Runge-Kutta
+    scheme (rhs from u x omega and viscosity nu), radial filter, no-slip walls
+    (v_imposebc_fast); Z, phi, W updated in place.  Needs setup_bc."""
+    _check(lib().rd_stage_uxw(plan.h, prec, int(o), float(dt), float(nu), 1 if filter else 0,
+                              *[plan._spec(x) for x in (Z0, phi0, Z, phi, W)]), "stage_uxw")
 
 
 def nonlinear_coupled(plan, W, Z, Pb, Tb, T, NQ, NS, NT, MQ, MS, MT, NH, prec=RD_PREC_FP64):

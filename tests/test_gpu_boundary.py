@@ -146,3 +146,98 @@ def test_bc_errors():
         S.invDl(plan, W, W.clone())          # no setup yet
     with pytest.raises(S.RdError):
         S.setup_bc(S.BCPlan(), plan, ("noslip", "freeslip"), "v")
+
+
+def _stage_setup(lmax, nr):
+    from redondito_amd import sphp as S
+    C, d, rint, rext = 25, 5, 0.35, 1.0
+    nz = nr + C
+    dz = (rext - rint) / (nr - 1)
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    plan = S.FCPlan(lmax, nlat, 2 * nlat, nz, C, d, dz, rint)
+    bcp = S.BCPlan()
+    S.setup_bc(bcp, plan, ("noslip", "noslip"), "v")
+    binv, d2b = (to_host(x) for x in S.bc_tables(plan))
+    d2b = d2b.T.copy()
+    A, Q = O.load_tables(S.TABLE_DIR, C, d)
+    fc = O.FC(nz, C, d, O.fc_dir(A, Q))
+    sht = O.SHT(lmax, nlat, 2 * nlat)
+    z, kz = plan.grid()[3], plan.grid()[4]
+    return S, plan, fc, sht, z, kz, H.ell_of(lmax), binv, d2b, dz, rint
+
+
+@pytest.mark.parametrize("lmax,nr,filt", [(15, 24, True), (31, 40, False)])
+def test_stage_uxw_vs_oracle(lmax, nr, filt):
+    """One Runge-Kutta stage (nonlinear term, explicit update with viscosity,
+    radial filter, no-slip walls) against the oracle's composition of the same
+    reference operators, from a state that already satisfies the walls."""
+    S, plan, fc, sht, z, kz, ell, binv, d2b, dz, rint = _stage_setup(lmax, nr)
+    nz = nr + 25
+    W, Zt = H.shell_fields(lmax, nr, nz, rint, 1.0, seed=lmax, slope=2.0)
+    # a wall-consistent start: phi = D_l W, then the wall step
+    Zs, phis, Ws = O.v_imposebc_fast(fc, kz, z, ell, binv, d2b, lmax, Zt, fc.dlr(W, kz, z, ell))
+    o, dt, nu = 2, 1e-3, 0.05
+    Z0, phi0 = Zs * 1.01, phis * 0.99
+    ref = O.stage_uxw(sht, fc, rint, dz, kz, z, ell, binv, d2b, lmax, o, dt, nu, filt, Z0, phi0, Zs, phis, Ws)
+    Zd, phid, Wd = to_dev(Zs), to_dev(phis), to_dev(Ws)
+    S.stage_uxw(plan, to_dev(Z0), to_dev(phi0), Zd, phid, Wd, o, dt, nu, filter=filt)
+    got = [to_host(x) for x in (Zd, phid, Wd)]
+    errs = [H.rell2(g[:nr], r[:nr]) for g, r in zip(got, ref)]
+    floor = H.noise_floor(lambda zz, pp: [x[:nr] for x in O.stage_uxw(sht, fc, rint, dz, kz, z, ell, binv, d2b, lmax,
+                                                                       o, dt, nu, filt, Z0, phi0, zz, pp, Ws)],
+                          [Zs, phis])
+    print(f"lmax={lmax} nr={nr} filter={filt}: Z {errs[0]:.2e} phi {errs[1]:.2e} W {errs[2]:.2e}; "
+          f"floor {floor[0]:.2e} {floor[1]:.2e} {floor[2]:.2e}")
+    assert all(e < max(1e-12, 30 * f) for e, f in zip(errs, floor))
+    # the walls hold after the stage: Z = W = dW/dr = 0
+    N = nr
+    assert np.max(np.abs(got[0][[0, N - 1]])) == 0 and np.max(np.abs(got[2][[0, N - 1]])) < 1e-13 * np.abs(got[2]).max()
+
+
+def test_stage_uxw_solid_body_rotation_is_steady():
+    """u = Omega z x r: u x omega is a gradient, so its curl vanishes and the
+    inviscid stage leaves Z unchanged off the walls (the wall step zeroes it on
+    them) and phi = W = 0."""
+    S, plan, fc, sht, z, kz, ell, binv, d2b, dz, rint = _stage_setup(15, 40)
+    nz, nlm = plan.nz, plan.nlm
+    nr = nz - 25
+    Zs = np.zeros((nz, nlm), complex)
+    Zs[:nr, H.lm_index(15, 1, 0)] = 0.7 * z[:nr] ** 2 * np.sqrt(4 * np.pi / 3)
+    zero = np.zeros_like(Zs)
+    Zd, phid, Wd = to_dev(Zs), to_dev(zero), to_dev(zero)
+    S.stage_uxw(plan, to_dev(Zs), to_dev(zero), Zd, phid, Wd, 1, 0.1, 0.0, filter=False)
+    Zg, pg, Wg = (to_host(x) for x in (Zd, phid, Wd))
+    scale = np.abs(Zs).max()
+    dZ = np.abs(Zg[1:nr - 1] - Zs[1:nr - 1]).max() / scale
+    print(f"solid body: interior Z change {dZ:.2e}, |phi| {np.abs(pg[:nr]).max():.2e}, |W| {np.abs(Wg[:nr]).max():.2e}")
+    assert dZ < 1e-11 and np.abs(pg[:nr]).max() < 1e-9 and np.abs(Wg[:nr]).max() < 1e-10
+    assert np.all(Zg[[0, nr - 1]] == 0)
+
+
+def test_stage_uxw_toroidal_viscous_term():
+    """An axisymmetric azimuthal flow (Z only in l = 1, m = 0, Z = a f(r) with
+    f = sin(pi (r - ri)/(ro - ri))): u x omega lies in the meridional planes,
+    so its toroidal part N_T vanishes and the stage changes Z by exactly
+    dt/o nu D_1 Z = dt/o nu a (f'' - 2 f / r^2), to the FC-Gram derivative
+    accuracy; a sign or operator slip in the viscous term fails."""
+    S, plan, fc, sht, z, kz, ell, binv, d2b, dz, rint = _stage_setup(15, 64)
+    nz, nlm = plan.nz, plan.nlm
+    nr = nz - 25
+    r = z[:nr]
+    k = np.pi / (1.0 - rint)
+    f = np.sin(k * (r - rint))
+    fpp = -k * k * f
+    i10 = H.lm_index(15, 1, 0)
+    Zs = np.zeros((nz, nlm), complex)
+    Zs[:nr, i10] = 0.3 * f
+    zero = np.zeros_like(Zs)
+    o, dt, nu = 2, 0.2, 0.05
+    Zd, phid, Wd = to_dev(Zs), to_dev(zero), to_dev(zero)
+    S.stage_uxw(plan, to_dev(Zs), to_dev(zero), Zd, phid, Wd, o, dt, nu, filter=False)
+    Zg = to_host(Zd)
+    got = (Zg[1:nr - 1, i10] - Zs[1:nr - 1, i10]) / (dt / o * nu)
+    exact = 0.3 * (fpp - 2 * f / r ** 2)[1:nr - 1]
+    e = np.max(np.abs(got - exact)) / np.max(np.abs(exact))
+    other = np.max(np.abs(np.delete(Zg[:nr], i10, 1)))
+    print(f"toroidal viscous term vs D_1 f: {e:.2e}; other modes {other:.1e}")
+    assert e < 1e-4 and other < 1e-14
