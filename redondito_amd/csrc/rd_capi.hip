@@ -335,6 +335,9 @@ static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, in
     } while (0)
 
     TRYH(hipGetDevice(&p->device));
+    if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+        p->n_cu < 1)
+        p->n_cu = 256;
     TRYH(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
     p->stream = p->own_stream;
 

@@ -627,34 +627,38 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Coefficient m = min(tid, lmax) of the six fields of row (k, th), loaded
+// unconditionally: the zeros above lmax and the dropped imaginary part of
+// m = 0 are applied where the values are used, so that nothing here (no
+// branch, no select) makes the compiler wait for the loads.
+__device__ __forceinline__ void cross768_load(const PhiPtrs &P, int lmax, int nlat, int k, int th,
+                                              double2 (&v)[6])
+{
+    const int L = lmax + 1, m = min((int)threadIdx.x, lmax);
+#pragma unroll
+    for (int f = 0; f < 6; f++) v[f] = P.F[f][((long)k * L + m) * nlat + th];
+}
+
+// The phases of one row from its loaded coefficients v (see above); ends
+// with the stores, the LDS buffer still read by FIN (a barrier is needed
+// before the buffer is written again).
 template <typename V>
-__global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
-                                                              const double2 *tw)
+__device__ __forceinline__ void cross768_row(const PhiPtrs &P, int lmax, int nlat, int k, int th,
+                                             const double2 (&vin)[6], const V *T16, V *B, double (*red)[2],
+                                             double *ia_sp, double2 g1, double2 g2, int rtid)
 {
     using S = typename Cx<V>::S;
     constexpr int N = 768;
-    __shared__ V T16[15 * 16];   // T16[(q-1) 16 + b] = e^{-2 pi i b q/256}, q = 1..15
-    __shared__ V B[PHI_BUF];
-    __shared__ double red[4][2];
-    __shared__ double ia_s;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int k = t / nlat, th = t % nlat;
-    if (k >= nplanes) return;
+    double &ia_s = *ia_sp;
     const int L = lmax + 1;
     const int tid = threadIdx.x;
-    const int rtid = rotated_tid<2>(tid);   // wave 0 works all four passes, wave 3 none
-    fill_t16(T16, tw, 3, tid);
-    // e^{-2 pi i b/768} and e^{-2 pi i 2b/768} for b = tid: R3 takes their
-    // conjugates, FIN (m = tid) the values themselves
-    const double2 g1 = tw[tid], g2 = tw[2 * tid];
     // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax),
     // and the sums of squares of u and of w
     double s[2] = {0.0, 0.0};
+    const bool live = tid <= lmax;
 #pragma unroll
     for (int f = 0; f < 6; f++) {
-        double2 v = make_double2(0.0, 0.0);
-        if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
-        if (tid == 0) v.y = 0.0;
+        const double2 v = make_double2(live ? vin[f].x : 0.0, (live && tid > 0) ? vin[f].y : 0.0);
         B[f * 256 + tid] = Cx<V>::make(v.x, v.y);
         s[f / 3] += v.x * v.x + v.y * v.y;
     }
@@ -820,6 +824,29 @@ __global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lma
     }
 }
 
+// One row per workgroup (the grid covers every row).
+template <typename V>
+__global__ __launch_bounds__(256, 4) void phi_cross768_kernel(PhiPtrs P, int lmax, int nlat, int nplanes,
+                                                              const double2 *tw)
+{
+    __shared__ V T16[15 * 16];   // T16[(q-1) 16 + b] = e^{-2 pi i b q/256}, q = 1..15
+    __shared__ V B[PHI_BUF];
+    __shared__ double red[4][2];
+    __shared__ double ia_s;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = t / nlat, th = t % nlat;
+    if (k >= nplanes) return;
+    const int tid = threadIdx.x;
+    const int rtid = rotated_tid<2>(tid);   // wave 0 works all four passes, wave 3 none
+    fill_t16(T16, tw, 3, tid);
+    // e^{-2 pi i b/768} and e^{-2 pi i 2b/768} for b = tid: R3 takes their
+    // conjugates, FIN (m = tid) the values themselves
+    const double2 g1 = tw[tid], g2 = tw[2 * tid];
+    double2 v[6];
+    cross768_load(P, lmax, nlat, k, th, v);
+    cross768_row<V>(P, lmax, nlat, k, th, v, T16, B, red, &ia_s, g1, g2, rtid);
+}
+
 // --------------------------------------- scalar phi transforms, nphi = 768
 // Four theta rows of one plane per workgroup (the tiling of phi_inverse_kernel
 // and phi_forward_kernel), packed in pairs (row 2p + i row 2p+1) into two
@@ -846,13 +873,19 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
     const int rtid = rotated_tid<2>(tid);
     fill_t16(T16, tw, 3, tid);
     const double2 g1 = tw[tid], g2 = tw[2 * tid];
-    // coefficient m = tid of rows th0 .. th0+3 at B[row 256 + m]
+    // coefficient m = tid of rows th0 .. th0+3 at B[row 256 + m]; loaded from
+    // clamped addresses without branches, the zeros applied afterwards, so
+    // that the four loads are in flight together
+    {
+        const int mc = min(tid, lmax);
+        double2 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        double2 v = make_double2(0.0, 0.0);
-        if (tid <= lmax && th0 + r < nlat) v = F[((long)k * L + tid) * nlat + th0 + r];
-        if (tid == 0) v.y = 0.0;
-        B[r * 256 + tid] = v;
+        for (int r = 0; r < 4; r++) v[r] = F[((long)k * L + mc) * nlat + min(th0 + r, nlat - 1)];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const bool live = tid <= lmax && th0 + r < nlat;
+            B[r * 256 + tid] = make_double2(live ? v[r].x : 0.0, (live && tid > 0) ? v[r].y : 0.0);
+        }
     }
     __syncthreads();
     // ---- R3 -> Y[3p+r][b]
@@ -949,15 +982,23 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
     fill_t16(T16, tw, 3, tid);
     const double2 c1 = tw[tid], c2 = tw[2 * tid];   // e^{-2 pi i m/768}, e^{-2 pi i 2m/768}, m = tid
     // packed rows z_p[j] = row th0+2p + i row th0+2p+1 at B[p N + j]
+    // (clamped loads without branches, the zeros applied afterwards)
+    {
+        double v[3][4];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int j = tid + 256 * i;
-        const long o = ((long)k * N + j) * nlat + th0;
-        double v[4];
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) v[r] = th0 + r < nlat ? f[o + r] : 0.0;
-        B[j] = make_double2(v[0], v[1]);
-        B[N + j] = make_double2(v[2], v[3]);
+            for (int r = 0; r < 4; r++)
+                v[i][r] = f[((long)k * N + tid + 256 * i) * nlat + min(th0 + r, nlat - 1)];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int j = tid + 256 * i;
+            double w[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) w[r] = th0 + r < nlat ? v[i][r] : 0.0;
+            B[j] = make_double2(w[0], w[1]);
+            B[N + j] = make_double2(w[2], w[3]);
+        }
     }
     __syncthreads();
     // ---- F1 (packed sequence s on lanes 0..47 of wave s)
@@ -1091,14 +1132,21 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
     const double2 g1 = tw[tid & 255];   // e^{-2 pi i b/1536}, b = tid mod 256
     // the row's six coefficient vectors at B[f 512 + m], m = 0..511 (zero
     // above lmax), and the sums of squares of u and of the second factor
+    // (clamped loads without branches, the zeros applied afterwards, as in
+    // phi_cross768_kernel)
     double s[2] = {0.0, 0.0};
+    {
+        const int mc = min(tid, lmax);
+        const bool live = tid <= lmax;
+        double2 vin[6];
 #pragma unroll
-    for (int f = 0; f < 6; f++) {
-        double2 v = make_double2(0.0, 0.0);
-        if (tid <= lmax) v = P.F[f][((long)k * L + tid) * nlat + th];
-        if (tid == 0) v.y = 0.0;
-        B[f * 512 + tid] = Cx<V>::make(v.x, v.y);
-        s[f / 3] += v.x * v.x + v.y * v.y;
+        for (int f = 0; f < 6; f++) vin[f] = P.F[f][((long)k * L + mc) * nlat + th];
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            const double2 v = make_double2(live ? vin[f].x : 0.0, (live && tid > 0) ? vin[f].y : 0.0);
+            B[f * 512 + tid] = Cx<V>::make(v.x, v.y);
+            s[f / 3] += v.x * v.x + v.y * v.y;
+        }
     }
     block_sums<2, 8>(s, red);
     const double a = wave_uniform(pack_scale(s[0], s[1]));
@@ -1781,14 +1829,64 @@ struct RadFact {
     static constexpr int CB = 256 / (N1 > N2 ? N1 : N2);
 };
 
-template <int N1, int N2>
+// Pass-A rows n2 + N2 n1 of column col of src, for the factorised radial
+// kernels: every row (clamped to the physical ones) and the 2 D wall rows are
+// loaded first, without branches, then the continuation rows past nr are
+// formed from the walls with the FC-Gram operator dirl (sphp_sca.F90:369-381);
+// ok = false gives zeros.
+template <int N1, int N2, int D>
+__device__ __forceinline__ void fact_load_rows(const double2 *src, long ld, int col, bool ok, int n2, int nr,
+                                               int C, const double *dirl, double2 (&x)[N1])
+{
+    double2 wr[D > 0 ? D : 1], wl[D > 0 ? D : 1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) x[n1] = src[(long)min(n2 + N2 * n1, nr - 1) * ld + col];
+#pragma unroll
+    for (int jj = 0; jj < D; jj++) {
+        wr[jj] = src[(long)(nr - D + jj) * ld + col];   // row nr - d + jj - 1, jj 1-based
+        wl[jj] = src[(long)(D - 1 - jj) * ld + col];    // row d - jj
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) {
+        const int r = n2 + N2 * n1;
+        double2 v = x[n1];
+        if (D > 0 && r >= nr) {
+            const int ii = r - nr + 1;
+            v = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int jj = 0; jj < D; jj++) {
+                const double a1 = dirl[(ii - 1) + C * jj];
+                const double a2 = dirl[(C - ii) + C * jj];
+                v.x += a1 * wr[jj].x + a2 * wl[jj].x;
+                v.y += a1 * wr[jj].y + a2 * wl[jj].y;
+            }
+        }
+        x[n1] = ok ? v : make_double2(0.0, 0.0);
+    }
+}
+
+// D: the FC-Gram matching points (the continuation rows are formed from
+// D rows at each wall, D a compile-time count so that every load of a column
+// is issued at once, without branches); D = 0 for a periodic radius (no
+// continuation).  z, kz and dir are staged in LDS.
+constexpr int kRadMaxC = 32;
+
+template <int N1, int N2, int D>
 __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
 {
     constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
     __shared__ double2 buf[CB * N];
     __shared__ double2 twl[N];   // e^{-2 pi i t / n}
+    __shared__ double zl[N], kzl[N];
+    __shared__ double dirl[D > 0 ? kRadMaxC * D : 1];
     const int tid = threadIdx.x;
-    for (int t = tid; t < N; t += 256) twl[t] = tw[t];
+    for (int t = tid; t < N; t += 256) {
+        twl[t] = tw[t];
+        zl[t] = A.z[t];
+        kzl[t] = A.kz[t];
+    }
+    if (D > 0)
+        for (int t = tid; t < A.C * D; t += 256) dirl[t] = A.dir[t];
     const int nr = N - A.C;
     // consecutive column blocks on one XCD: a 128-byte line shared by two
     // blocks' row segments (CB x 16 bytes) is then fetched into one L2 only
@@ -1798,34 +1896,15 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
     const int cb = tid % CB, k1 = tid / CB;   // pass B
     const int lma = c0 + ca;
     const bool okA = inA && lma < A.ncols;
+    const int lmc = okA ? lma : 0;           // a valid column for the loads of idle lanes
     const double inv_n = 1.0 / N;
-    double ll = 0.0;
-    if (okA) ll = (double)A.ell[lma] * (A.ell[lma] + 1);
+    const int ell = A.ell[lmc];
+    const double ll = okA ? (double)ell * (ell + 1) : 0.0;
     __syncthreads();
 
     // rows n2 + N2 n1 of column lma of src, the continuation rows from the table
     auto load_rows = [&](const double2 *src, double2 (&x)[N1]) {
-#pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) {
-            const int r = n2 + N2 * n1;
-            double2 v = make_double2(0.0, 0.0);
-            if (okA) {
-                if (r < nr) {
-                    v = src[(long)r * A.ld + lma];
-                } else {
-                    const int ii = r - nr + 1;
-                    for (int jj = 1; jj <= A.d; jj++) {
-                        const double a1 = A.dir[(ii - 1) + A.C * (jj - 1)];
-                        const double a2 = A.dir[(A.C - ii) + A.C * (jj - 1)];
-                        const double2 xr = src[(long)(nr - A.d + jj - 1) * A.ld + lma];
-                        const double2 xl = src[(long)(A.d - jj) * A.ld + lma];
-                        v.x += a1 * xr.x + a2 * xl.x;
-                        v.y += a1 * xr.y + a2 * xl.y;
-                    }
-                }
-            }
-            x[n1] = v;
-        }
+        fact_load_rows<N1, N2, D>(src, A.ld, lmc, okA, n2, nr, A.C, dirl, x);
     };
     // pass A forward: N1-point DFTs, twiddles, to LDS as [column][k1][n2]
     auto fwd_a = [&](double2 (&x)[N1]) {
@@ -1861,7 +1940,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
             for (int n1 = 0; n1 < N1; n1++) {
                 const int r = n2 + N2 * n1;
                 if (r < nr) {
-                    const double rr = A.z[r];
+                    const double rr = zl[r];
                     A.out[0][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
                 }
             }
@@ -1878,7 +1957,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
             for (int n1 = 0; n1 < N1; n1++) {
                 const int r = n2 + N2 * n1;
                 if (r < nr) {
-                    const double rr = A.z[r];
+                    const double rr = zl[r];
                     A.out[2][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / rr);
                     if (A.nout == 6) A.out[3][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
                 }
@@ -1894,7 +1973,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
     if (inB) {
 #pragma unroll
         for (int q = 0; q < N2; q++) {
-            const double k = A.kz[k1 + N1 * q] * inv_n;
+            const double k = kzl[k1 + N1 * q] * inv_n;
             y[q] = make_double2(-k * XW[q].y, k * XW[q].x);
         }
         inv_b(y);
@@ -1906,7 +1985,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
 #pragma unroll
             for (int n1 = 0; n1 < N1; n1++) {
                 const int r = n2 + N2 * n1;
-                if (r < nr) A.out[1][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+                if (r < nr) A.out[1][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / zl[r]);
             }
     }
     __syncthreads();
@@ -1915,13 +1994,18 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
         if (inB) {
 #pragma unroll
             for (int q = 0; q < N2; q++) {
-                const double k = A.kz[k1 + N1 * q];
+                const double k = kzl[k1 + N1 * q];
                 y[q] = cscale(XW[q], -k * k * inv_n);
             }
             inv_b(y);
         }
         __syncthreads();
         if (inA) {
+            // W at this thread's rows again (clamped, issued before the pass
+            // so that the loads overlap it)
+            double2 wv[N1];
+#pragma unroll
+            for (int n1 = 0; n1 < N1; n1++) wv[n1] = A.W[(long)min(n2 + N2 * n1, nr - 1) * A.ld + lmc];
             inv_a(x);
             if (okA)
 #pragma unroll
@@ -1929,8 +2013,8 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
                     const int r = n2 + N2 * n1;
                     if (r < nr) {
                         const long o = (long)r * A.ld + lma;
-                        const double rr = A.z[r], r2 = 1.0 / (rr * rr);
-                        const double2 w = A.W[o];
+                        const double rr = zl[r], r2 = 1.0 / (rr * rr);
+                        const double2 w = wv[n1];
                         A.out[5][o] = cscale(make_double2(x[n1].x - ll * w.x * r2, x[n1].y - ll * w.y * r2), -1.0 / rr);
                     }
                 }
@@ -1940,7 +2024,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
         if (inB) {
 #pragma unroll
             for (int q = 0; q < N2; q++) {
-                const double k = A.kz[k1 + N1 * q] * inv_n;
+                const double k = kzl[k1 + N1 * q] * inv_n;
                 y[q] = make_double2(-k * XZ[q].y, k * XZ[q].x);
             }
             inv_b(y);
@@ -1952,7 +2036,7 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
 #pragma unroll
                 for (int n1 = 0; n1 < N1; n1++) {
                     const int r = n2 + N2 * n1;
-                    if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / A.z[r]);
+                    if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / zl[r]);
                 }
         }
     }
@@ -1971,20 +2055,28 @@ static int radial_factorised(int n)
 // from A.ain into A.a.  Every one of the n rows is stored.  A workgroup
 // reads all rows of its CB columns before it writes any, so a and ain may be
 // the same array.
-template <int N1, int N2>
+template <int N1, int N2, int D>
 __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const double2 *tw)
 {
     constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
     __shared__ double2 buf[CB * N];
     __shared__ double2 twl[N];   // e^{-2 pi i t / n}
+    __shared__ double kzl[N];
+    __shared__ double dirl[D > 0 ? kRadMaxC * D : 1];
     const int tid = threadIdx.x;
-    for (int t = tid; t < N; t += 256) twl[t] = tw[t];
+    for (int t = tid; t < N; t += 256) {
+        twl[t] = tw[t];
+        kzl[t] = A.kz[t];
+    }
+    if (D > 0)
+        for (int t = tid; t < A.C * D; t += 256) dirl[t] = A.dir[t];
     const int nr = N - A.C;
     const int c0 = xcd_tile(blockIdx.x, gridDim.x) * CB;
     const bool inA = tid < CB * N2, inB = tid < CB * N1;
     const int ca = tid % CB, n2 = tid / CB;   // pass A
     const int cb = tid % CB, k1 = tid / CB;   // pass B
     const bool okA = inA && c0 + ca < A.ncols, okB = inB && c0 + cb < A.ncols;
+    const int cola = okA ? c0 + ca : 0, colb = okB ? c0 + cb : 0;   // valid columns for idle lanes' loads
     const double2 *src = (mode == 2) ? A.ain : A.a;
     __syncthreads();
     double2 x[N1], y[N2];
@@ -1992,8 +2084,9 @@ __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const do
         // pass-B thread (column, k1) holds the spectrum rows k1 + N1 q
         if (inB) {
 #pragma unroll
-            for (int q = 0; q < N2; q++)
-                y[q] = okB ? src[(long)(k1 + N1 * q) * A.ld + c0 + cb] : make_double2(0.0, 0.0);
+            for (int q = 0; q < N2; q++) y[q] = src[(long)(k1 + N1 * q) * A.ld + colb];
+#pragma unroll
+            for (int q = 0; q < N2; q++) y[q] = okB ? y[q] : make_double2(0.0, 0.0);
             dft_odd<N2>(y, +1);
 #pragma unroll
             for (int q = 0; q < N2; q++) buf[cb * N + q * N1 + k1] = cmul(y[q], cconj(twl[(q * k1) % N]));
@@ -2001,27 +2094,7 @@ __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const do
     } else {
         // pass A: rows n2 + N2 n1, continuation rows from the wall rows
         if (inA) {
-#pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                double2 v = make_double2(0.0, 0.0);
-                if (okA) {
-                    if (r < nr) {
-                        v = src[(long)r * A.ld + c0 + ca];
-                    } else {
-                        const int ii = r - nr + 1;
-                        for (int jj = 1; jj <= A.d; jj++) {
-                            const double a1 = A.dir[(ii - 1) + A.C * (jj - 1)];
-                            const double a2 = A.dir[(A.C - ii) + A.C * (jj - 1)];
-                            const double2 xr = src[(long)(nr - A.d + jj - 1) * A.ld + c0 + ca];
-                            const double2 xl = src[(long)(A.d - jj) * A.ld + c0 + ca];
-                            v.x += a1 * xr.x + a2 * xl.x;
-                            v.y += a1 * xr.y + a2 * xl.y;
-                        }
-                    }
-                }
-                x[n1] = v;
-            }
+            fact_load_rows<N1, N2, D>(src, A.ld, cola, okA, n2, nr, A.C, dirl, x);
             dft_odd<N1>(x, -1);
 #pragma unroll
             for (int q = 0; q < N1; q++) buf[ca * N + q * N2 + n2] = cmul(x[q], twl[(n2 * q) % N]);
@@ -2046,7 +2119,7 @@ __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const do
 #pragma unroll
             for (int q = 0; q < N2; q++) {
                 double kp = 1.0;
-                for (int e = 0; e < A.order; e++) kp *= A.kz[k1 + N1 * q];
+                for (int e = 0; e < A.order; e++) kp *= kzl[k1 + N1 * q];
                 double2 v = cscale(y[q], kp / N);
                 if (pm == 1) v = make_double2(-v.y, v.x);
                 else if (pm == 2) v = make_double2(-v.x, -v.y);
@@ -2074,12 +2147,19 @@ __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const do
 // pairs above; returns 1 (nothing launched) otherwise.
 int z_fact_launch(Plan *p, const ZArgs &A, int mode)
 {
-    switch (radial_factorised(p->nz)) {
+    // the standard FC-Gram matching (d = 5) and the periodic radius; other
+    // (C, d) take the Stockham kernel
+    const int fd = p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
+    switch (fd < 0 ? 0 : radial_factorised(p->nz)) {
 #define RD_ZF_CASE(NV, F1, F2)                                                                  \
     case NV: {                                                                                  \
         constexpr int CB = RadFact<F1, F2>::CB;                                                 \
-        hipLaunchKernelGGL((z_fact_kernel<F1, F2>), dim3((A.ncols + CB - 1) / CB), dim3(256), 0, p->stream, A, \
-                           mode, p->tw_z);                                                      \
+        if (fd == 5)                                                                            \
+            hipLaunchKernelGGL((z_fact_kernel<F1, F2, 5>), dim3((A.ncols + CB - 1) / CB), dim3(256), 0,      \
+                               p->stream, A, mode, p->tw_z);                                    \
+        else                                                                                    \
+            hipLaunchKernelGGL((z_fact_kernel<F1, F2, 0>), dim3((A.ncols + CB - 1) / CB), dim3(256), 0,      \
+                               p->stream, A, mode, p->tw_z);                                    \
         RD_LAUNCH_CHECK();                                                                      \
         return RD_OK;                                                                           \
     }
@@ -2108,12 +2188,19 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     A.n = p->nz;
     A.C = p->C;
     A.d = p->d;
-    switch (radial_factorised(p->nz)) {
+    // the factorised kernel for the standard FC-Gram matching (d = 5) and the
+    // periodic radius; other (C, d) take the generic kernel
+    const int fd = p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
+    switch (fd < 0 ? 0 : radial_factorised(p->nz)) {
 #define RD_F_CASE(NV, F1, F2)                                                                   \
     case NV: {                                                                                  \
         constexpr int CB = RadFact<F1, F2>::CB;                                                 \
-        hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2>), dim3((p->nlm + CB - 1) / CB), dim3(256), 0, \
-                           p->stream, A, p->tw_z);                                              \
+        if (fd == 5)                                                                            \
+            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 5>), dim3((p->nlm + CB - 1) / CB), dim3(256), \
+                               0, p->stream, A, p->tw_z);                                       \
+        else                                                                                    \
+            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 0>), dim3((p->nlm + CB - 1) / CB), dim3(256), \
+                               0, p->stream, A, p->tw_z);                                       \
         RD_LAUNCH_CHECK();                                                                      \
         return RD_OK;                                                                           \
     }

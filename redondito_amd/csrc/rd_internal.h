@@ -52,6 +52,7 @@ struct Plan {
     int nz, C, d, nr;
     double dz, rint;
     int device;                    // the device current at creation (the plan's tables live there)
+    int n_cu;                      // compute units of that device (persistent grids)
     hipStream_t stream;
     hipStream_t own_stream;
     // grid (host copies)
