@@ -360,31 +360,36 @@ __device__ __forceinline__ void syn_tile(
     const long lm0 = lm_index(A.lmax, m, m);
 
     // a chunk is NF x (JT/2 planes) x SKC l's, l fastest across threads;
-    // each thread moves NLD double2 of it.  The next chunk is loaded into
-    // registers while the current one is multiplied out of LDS.
+    // each thread moves NLD double2 of it: element tid + 256 i, whose field
+    // (256 i) / PER is a compile-time constant because PER is a multiple of
+    // 256, so the field's base pointer is a scalar kernel argument.  The
+    // loads go to clamped addresses, unconditionally, so that they issue
+    // back to back with no wait between them; entries beyond lmax or the
+    // last plane are zeroed when the chunk is stored to LDS, which happens
+    // after the MFMAs that the loads overlap.
     constexpr int PER = (JT / 2) * SKC;
     constexpr int NLD = NF * PER / 256;
-    static_assert(NF * PER == 256 * NLD, "chunk must split evenly over the workgroup");
-    struct { T x, y; } r[NLD];   // held in T: the FP32 form keeps half the registers
+    static_assert(PER % 256 == 0, "a thread's chunk elements must each lie in one field");
+    double2 r[NLD];
+    unsigned rlive = 0;   // bit i: element i lies inside the spectrum
     auto load = [&](int lbase) {
+        rlive = 0;
 #pragma unroll
         for (int i = 0; i < NLD; i++) {
-            const int e = tid + 256 * i;
-            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
+            const int f = (256 * i) / PER, q = tid + (256 * i) % PER, pl = q / SKC, ll = q % SKC;
             const int l = lbase + ll, k = k0 + pl;
-            double2 v = make_double2(0.0, 0.0);
-            if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-            r[i].x = (T)v.x;
-            r[i].y = (T)v.y;
+            const int lc = l < A.lmax ? l : A.lmax, kc = k < A.nplanes ? k : A.nplanes - 1;
+            r[i] = A.in[f][lm0 + (lc - m) + kc * A.ld];
+            rlive |= (unsigned)(l <= A.lmax && k < A.nplanes) << i;
         }
     };
     auto store = [&]() {
 #pragma unroll
         for (int i = 0; i < NLD; i++) {
-            const int e = tid + 256 * i;
-            const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
-            Bs[f][ll][2 * pl] = r[i].x;
-            Bs[f][ll][2 * pl + 1] = r[i].y;
+            const int f = (256 * i) / PER, q = tid + (256 * i) % PER, pl = q / SKC, ll = q % SKC;
+            const bool ok = (rlive >> i) & 1u;
+            Bs[f][ll][2 * pl] = ok ? (T)r[i].x : T(0);
+            Bs[f][ll][2 * pl + 1] = ok ? (T)r[i].y : T(0);
         }
     };
     // The table values of step g (rows 4 g + kk of both parities) are
@@ -426,14 +431,8 @@ __device__ __forceinline__ void syn_tile(
             __syncthreads();
             if (lbase + SKC <= A.lmax) load(lbase + SKC);
         } else {
-            for (int e = tid; e < NF * PER; e += 256) {
-                const int f = e / PER, q = e % PER, pl = q / SKC, ll = q % SKC;
-                const int l = lbase + ll, k = k0 + pl;
-                double2 v = make_double2(0.0, 0.0);
-                if (l <= A.lmax && k < A.nplanes) v = A.in[f][lm0 + (l - m) + k * A.ld];
-                Bs[f][ll][2 * pl] = (T)v.x;
-                Bs[f][ll][2 * pl + 1] = (T)v.y;
-            }
+            load(lbase);
+            store();
             __syncthreads();
         }
 #pragma unroll
