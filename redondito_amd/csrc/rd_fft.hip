@@ -1812,6 +1812,41 @@ __device__ __forceinline__ void dft_odd(double2 (&x)[R], int sign)
     for (int k = 0; k < R; k++) x[k] = y[k];
 }
 
+// The same DFT with each output handed to emit(k, X[k]) as soon as it is
+// formed (k = 0, then the pairs k, R - k), so that only the sums a_j and
+// differences b_j stay live, not an output array beside them: the form of
+// the passes whose outputs go straight to LDS or to global memory.
+template <int R, typename E>
+__device__ __forceinline__ void dft_odd_emit(const double2 (&x)[R], int sign, E &&emit)
+{
+    constexpr int H = (R - 1) / 2;
+    double2 a[H], b[H];
+    const double2 x0 = x[0];
+    double2 s0 = x0;
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        a[j - 1] = cadd(x[j], x[R - j]);
+        b[j - 1] = csub(x[j], x[R - j]);
+        s0 = cadd(s0, a[j - 1]);
+    }
+    const double sg = sign > 0 ? 1.0 : -1.0;
+    emit(0, s0);
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        double2 C = x0, S = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            const int m = (j * k) % R;
+            C.x += a[j - 1].x * DftConst<R>::c[m];
+            C.y += a[j - 1].y * DftConst<R>::c[m];
+            S.x += b[j - 1].x * DftConst<R>::s[m];
+            S.y += b[j - 1].y * DftConst<R>::s[m];
+        }
+        emit(k, make_double2(C.x - sg * S.y, C.y + sg * S.x));
+        emit(R - k, make_double2(C.x + sg * S.y, C.y - sg * S.x));
+    }
+}
+
 // The radial step of the nonlinear term for n = N1 N2 (n = 153 = 17 x 9 at
 // nr = 128, 121 = 11 x 11, 49 = 7 x 7), CB lm columns per workgroup.  With
 // r = n2 + N2 n1 and k = k1 + N1 k2:
@@ -1872,7 +1907,7 @@ __device__ __forceinline__ void fact_load_rows(const double2 *src, long ld, int 
 constexpr int kRadMaxC = 32;
 
 template <int N1, int N2, int D>
-__global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
+__global__ __launch_bounds__(256, 3) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
 {
     constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
     __shared__ double2 buf[CB * N];
@@ -1907,10 +1942,8 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
         fact_load_rows<N1, N2, D>(src, A.ld, lmc, okA, n2, nr, A.C, dirl, x);
     };
     // pass A forward: N1-point DFTs, twiddles, to LDS as [column][k1][n2]
-    auto fwd_a = [&](double2 (&x)[N1]) {
-        dft_odd<N1>(x, -1);
-#pragma unroll
-        for (int q = 0; q < N1; q++) buf[ca * N + q * N2 + n2] = cmul(x[q], twl[(n2 * q) % N]);
+    auto fwd_a = [&](const double2 (&x)[N1]) {
+        dft_odd_emit<N1>(x, -1, [&](int q, double2 v) { buf[ca * N + q * N2 + n2] = cmul(v, twl[(n2 * q) % N]); });
     };
     // pass B forward: N2-point DFTs -> X[k1 + N1 k2]
     auto fwd_b = [&](double2 (&y)[N2]) {
@@ -1919,19 +1952,19 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
         dft_odd<N2>(y, -1);
     };
     // pass B inverse: N2-point DFTs of a spectrum, twiddles, to LDS as [column][n2][k1]
-    auto inv_b = [&](double2 (&y)[N2]) {
-        dft_odd<N2>(y, +1);
-#pragma unroll
-        for (int q = 0; q < N2; q++) buf[cb * N + q * N1 + k1] = cmul(y[q], cconj(twl[(q * k1) % N]));
+    auto inv_b = [&](const double2 (&y)[N2]) {
+        dft_odd_emit<N2>(y, +1, [&](int q, double2 v) { buf[cb * N + q * N1 + k1] = cmul(v, cconj(twl[(q * k1) % N])); });
     };
-    // pass A inverse: N1-point DFTs -> values at rows n2 + N2 n1
-    auto inv_a = [&](double2 (&x)[N1]) {
+    // pass A inverse: N1-point DFTs; emit(n1, value at row n2 + N2 n1)
+    auto inv_a = [&](double2 (&x)[N1], auto &&emit) {
 #pragma unroll
         for (int q = 0; q < N1; q++) x[q] = buf[ca * N + n2 * N1 + q];
-        dft_odd<N1>(x, +1);
+        dft_odd_emit<N1>(x, +1, emit);
     };
 
-    double2 XW[N2], XZ[N2], x[N1], y[N2];
+    // One radial spectrum is held in registers at a time: W's (for u_S and
+    // the d2W of w_T), then Z's (for w_S).
+    double2 X[N2], x[N1], y[N2];
     // ---- W: u_Q = l(l+1) W / r^2, forward transform
     if (inA) {
         load_rows(A.W, x);
@@ -1947,8 +1980,50 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
         fwd_a(x);
     }
     __syncthreads();
-    if (inB) fwd_b(XW);
+    if (inB) fwd_b(X);
     __syncthreads();
+    // ---- u_S = dW / r
+    if (inB) {
+#pragma unroll
+        for (int q = 0; q < N2; q++) {
+            const double k = kzl[k1 + N1 * q] * inv_n;
+            y[q] = make_double2(-k * X[q].y, k * X[q].x);
+        }
+        inv_b(y);
+    }
+    __syncthreads();
+    if (inA)
+        inv_a(x, [&](int n1, double2 v) {
+            const int r = n2 + N2 * n1;
+            if (okA && r < nr) A.out[1][(long)r * A.ld + lma] = cscale(v, 1.0 / zl[r]);
+        });
+    __syncthreads();
+    if (A.nout == 6) {
+        // ---- w_T = -(d2W - l(l+1) W / r^2) / r
+        if (inB) {
+#pragma unroll
+            for (int q = 0; q < N2; q++) {
+                const double k = kzl[k1 + N1 * q];
+                y[q] = cscale(X[q], -k * k * inv_n);
+            }
+            inv_b(y);
+        }
+        __syncthreads();
+        if (inA) {
+            // W at this thread's rows again, each row loaded where its
+            // output is formed (clamped: no branch before the load)
+            inv_a(x, [&](int n1, double2 v) {
+                const int r = n2 + N2 * n1;
+                const double2 w = A.W[(long)min(r, nr - 1) * A.ld + lmc];
+                if (okA && r < nr) {
+                    const long o = (long)r * A.ld + lma;
+                    const double rr = zl[r], r2 = 1.0 / (rr * rr);
+                    A.out[5][o] = cscale(make_double2(v.x - ll * w.x * r2, v.y - ll * w.y * r2), -1.0 / rr);
+                }
+            });
+        }
+        __syncthreads();
+    }
     // ---- Z: u_T = Z / r, w_Q = l(l+1) Z / r^2, forward transform
     if (inA) {
         load_rows(A.Z, x);
@@ -1964,81 +2039,25 @@ __global__ __launch_bounds__(256, 2) void uxw_radial_fact_kernel(UxwRadArgs A, c
             }
         if (A.nout == 6) fwd_a(x);
     }
-    __syncthreads();
     if (A.nout == 6) {   // the spectrum of Z only feeds the curl outputs
-        if (inB) fwd_b(XZ);
         __syncthreads();
-    }
-    // ---- u_S = dW / r
-    if (inB) {
-#pragma unroll
-        for (int q = 0; q < N2; q++) {
-            const double k = kzl[k1 + N1 * q] * inv_n;
-            y[q] = make_double2(-k * XW[q].y, k * XW[q].x);
-        }
-        inv_b(y);
-    }
-    __syncthreads();
-    if (inA) {
-        inv_a(x);
-        if (okA)
-#pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                if (r < nr) A.out[1][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / zl[r]);
-            }
-    }
-    __syncthreads();
-    if (A.nout == 6) {
-        // ---- w_T = -(d2W - l(l+1) W / r^2) / r
-        if (inB) {
-#pragma unroll
-            for (int q = 0; q < N2; q++) {
-                const double k = kzl[k1 + N1 * q];
-                y[q] = cscale(XW[q], -k * k * inv_n);
-            }
-            inv_b(y);
-        }
-        __syncthreads();
-        if (inA) {
-            // W at this thread's rows again (clamped, issued before the pass
-            // so that the loads overlap it)
-            double2 wv[N1];
-#pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) wv[n1] = A.W[(long)min(n2 + N2 * n1, nr - 1) * A.ld + lmc];
-            inv_a(x);
-            if (okA)
-#pragma unroll
-                for (int n1 = 0; n1 < N1; n1++) {
-                    const int r = n2 + N2 * n1;
-                    if (r < nr) {
-                        const long o = (long)r * A.ld + lma;
-                        const double rr = zl[r], r2 = 1.0 / (rr * rr);
-                        const double2 w = wv[n1];
-                        A.out[5][o] = cscale(make_double2(x[n1].x - ll * w.x * r2, x[n1].y - ll * w.y * r2), -1.0 / rr);
-                    }
-                }
-        }
+        if (inB) fwd_b(X);
         __syncthreads();
         // ---- w_S = dZ / r
         if (inB) {
 #pragma unroll
             for (int q = 0; q < N2; q++) {
                 const double k = kzl[k1 + N1 * q] * inv_n;
-                y[q] = make_double2(-k * XZ[q].y, k * XZ[q].x);
+                y[q] = make_double2(-k * X[q].y, k * X[q].x);
             }
             inv_b(y);
         }
         __syncthreads();
-        if (inA) {
-            inv_a(x);
-            if (okA)
-#pragma unroll
-                for (int n1 = 0; n1 < N1; n1++) {
-                    const int r = n2 + N2 * n1;
-                    if (r < nr) A.out[4][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / zl[r]);
-                }
-        }
+        if (inA)
+            inv_a(x, [&](int n1, double2 v) {
+                const int r = n2 + N2 * n1;
+                if (okA && r < nr) A.out[4][(long)r * A.ld + lma] = cscale(v, 1.0 / zl[r]);
+            });
     }
 }
 
