@@ -474,12 +474,20 @@ def main():
                     help="only run the lmax = 511 coupled-step measurement (for profiler runs) and print it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--fp64-ceilings", action="store_true",
+                    help="only measure the FP64 MFMA and VALU ceilings of the device and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this path runs on one GPU only; --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.fp64_ceilings:
+        from redondito_amd import sphp as S
+        mf, vf = S.fp64_ceilings()
+        print(json.dumps({"fp64_mfma_ceiling_tflops": round(mf, 2), "fp64_valu_ceiling_tflops": round(vf, 2),
+                          "published_fp64_peak_tflops": 78.6}))
         return
     if args.coupled_only:
         print(json.dumps({"coupled": coupled_bench(max(2, args.steps))}))
@@ -528,6 +536,9 @@ def main():
               + max(leg, (3 * Fm + 3 * Sb) / bw))
     aux = aux_timings(S, plan, Wd)
     aux["hbm_copy_gbs"] = copy_bandwidth()
+    mf, vf = S.fp64_ceilings()
+    aux["fp64_mfma_ceiling_tflops"] = round(mf, 2)   # measured, beside the 78.6 published
+    aux["fp64_valu_ceiling_tflops"] = round(vf, 2)
     batch = None if (args.no_batch or args.fp64_only) else batch_bench(max(2, min(args.steps, 10)))
     batch_uxw = None if (args.no_batch or args.fp64_only) else batch_uxw_bench(max(2, min(args.steps, 10)))
     coupled = None if (args.no_coupled or args.fp64_only) else coupled_bench(max(2, min(args.steps, 5)))

@@ -396,6 +396,13 @@ int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *
                       double *const *NS, double *const *NT, int reps, double *ms_fixed, double *ms_tuned,
                       int *choice);
 
+/* Measured FP64 ceilings of the current device in TFLOP/s: a register-only
+ * loop of v_mfma_f64_16x16x4_f64 (the instruction of the FP64 Legendre
+ * kernels) and one of v_fma_f64, each over eight workgroups of four waves per
+ * CU.  A diagnostic for the roofline of the FP64 Legendre kernels; no
+ * reference routine. */
+int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,7 @@ def lib():
         L.rd_plan_prepare.argtypes = [vp, i]
         L.rd_plan_replay.argtypes = [vp]
         L.rd_stage_uxw.argtypes = [vp, i, i, d, d, i, vp, vp, vp, vp, vp]
+        L.rd_fp64_ceilings.argtypes = [ctypes.POINTER(d), ctypes.POINTER(d)]
         L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
@@ -357,6 +358,14 @@ def stage_uxw(plan, Z0, phi0, Z, phi, W, o, dt, nu, filter=True, prec=RD_PREC_FP
     (v_imposebc_fast); Z, phi, W updated in place.  Needs setup_bc."""
     _check(lib().rd_stage_uxw(plan.h, prec, int(o), float(dt), float(nu), 1 if filter else 0,
                               *[plan._spec(x) for x in (Z0, phi0, Z, phi, W)]), "stage_uxw")
+
+
+def fp64_ceilings():
+    """Measured FP64 throughput of the current device in TFLOP/s:
+    (v_mfma_f64_16x16x4 loop, v_fma_f64 loop), register-only."""
+    m, v = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    _check(lib().rd_fp64_ceilings(ctypes.byref(m), ctypes.byref(v)), "fp64_ceilings")
+    return m.value, v.value
 
 
 def nonlinear_coupled(plan, W, Z, Pb, Tb, T, NQ, NS, NT, MQ, MS, MT, NH, prec=RD_PREC_FP64):

@@ -1,0 +1,14 @@
+"""The measured FP64 ceilings that bench.py sets beside the published FP64
+peak: both loops run, and neither exceeds the 78.6 TFLOP/s the device is
+rated at (a timing or flop-count slip in the probe would)."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp64_ceilings_are_plausible():
+    from redondito_amd import sphp as S
+    mfma, valu = S.fp64_ceilings()
+    print(f"FP64 ceilings: MFMA {mfma:.1f} TFLOP/s, VALU {valu:.1f} TFLOP/s")
+    for v in (mfma, valu):
+        assert 5.0 < v < 1.05 * 78.6
