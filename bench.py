@@ -44,8 +44,9 @@ SEED = 20261203
 HBM_PEAK_GBS = 8000.0
 # dense MFMA peaks; the int8 digit form spends 28 int8 products per FP64 product,
 # so its FP64-equivalent peak is the 5.0 POP/s int8 peak over 28
-MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "ozaki": 5000.0 / 28}
-FORMS = (("fp64", 0), ("fp32", 1), ("bf16x3", 2), ("ozaki", 3))
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "ozaki": 5000.0 / 28, "ozaki6": 5000.0 / 21}
+# (name, precision form, Ozaki digits per operand)
+FORMS = (("fp64", 0, 7), ("fp32", 1, 7), ("bf16x3", 2, 7), ("ozaki", 3, 7), ("ozaki6", 3, 6))
 ERR_BAR = 1e-9
 
 
@@ -405,7 +406,8 @@ def coupled_bench(reps):
     plan = S.FCPlan(CP_LMAX, CP_NLAT, CP_NPHI, n, C, D, dz, RI, device=dev)
     ins = [torch.from_numpy(x).to(dev) for x in H.coupled_fields(CP_LMAX, CP_NR, n, RI, RO, seed=SEED + 7)]
     forms, host = {}, {}
-    for name, prec in FORMS:
+    for name, prec, digits in FORMS:
+        plan.set_ozaki_digits(digits)
         out = [torch.zeros_like(ins[0]) for _ in range(7)]
         ms = time_calls(lambda: S.nonlinear_coupled(plan, *ins, *out, prec=prec), reps)
         host[name] = [o.cpu().numpy() for o in out]
@@ -506,9 +508,10 @@ def main():
 
     forms = {}
     host = {}
-    for name, prec in FORMS:
+    for name, prec, digits in FORMS:
         if args.fp64_only and prec != 0:
             continue
+        plan.set_ozaki_digits(digits)
         out = [torch.zeros_like(Wd) for _ in range(3)]
         ms, stages = time_form(S, plan, prec, Wd, Zd, out, args.steps, args.warmup)
         host[name] = [o.cpu().numpy() for o in out]
@@ -562,7 +565,8 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 elsewhere",
                   "bf16x3": "bf16x3 Legendre, f64 elsewhere",
-                  "ozaki": "int8 digit-sliced f64 Legendre (exact int32 accumulation), f64 elsewhere"}[head],
+                  "ozaki": "int8 digit-sliced f64 Legendre (exact int32 accumulation), f64 elsewhere",
+                  "ozaki6": "int8 digit-sliced f64 Legendre, 6 digits (exact int32 accumulation), f64 elsewhere"}[head],
         "data": "synthetic seeded W,Z: complex normal, (l+1)^-2, sin radial profile",
         "config": {"workload": "u x omega nonlinear term", "form": head, "lmax": LMAX, "nlat": nlat,
                    "nphi": nphi, "nr": NR, "fcgram_C": C, "fcgram_d": D, "radial_fft_len": n},

@@ -75,6 +75,18 @@ int rd_plan_destroy(rd_plan *plan);
  * (null) stream.  A new plan runs on a non-blocking stream of its own. */
 int rd_plan_set_stream(rd_plan *plan, void *hip_stream);
 
+/* Digits per operand of the plan's later RD_PREC_OZAKI calls: 7 (the
+ * default), 6 or 5.  The tables are sliced to 7 digits once; a call with D
+ * digits uses their first D digit planes and the digit products with
+ * p + q <= D - 1 (28, 21 or 15 int8 GEMMs), each element exact to
+ * 2^-(7 D + 1) of its row or column maximum.  Measured at lmax = 255,
+ * nr = 128 (joint relative L2 against FP64): u x omega 1.4e-12 with 7,
+ * 2.0e-10 with 6, 2.2e-8 with 5; the u x B part of the lmax = 511 coupled
+ * step needs 7.
+ * RD_ERR_ARG for other counts.  No reference routine: the reference has no
+ * reduced-precision form. */
+int rd_plan_set_ozaki_digits(rd_plan *plan, int digits);
+
 /* Waits for all work queued on the plan's stream (the reference's
  * device_sync, src/memmgt/gdevice_mod.f90). */
 int rd_plan_sync(rd_plan *plan);

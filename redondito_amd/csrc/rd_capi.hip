@@ -301,6 +301,7 @@ static int plan_create_impl(Plan **out, int lmax, int nlat, int nphi, int nz, in
 
     Plan *p = new Plan();
     memset(p, 0, sizeof(Plan));
+    p->oz_digits = 7;
     p->lmax = lmax;
     p->L = lmax + 1;
     p->nlat = nlat;
@@ -427,6 +428,14 @@ int rd_plan_set_stream(rd_plan *plan, void *s)
     if (bad_plan(plan)) return RD_ERR_ARG;
     Plan *p = P(plan);
     p->stream = (hipStream_t)s;   // NULL: the default stream
+    return RD_OK;
+}
+
+int rd_plan_set_ozaki_digits(rd_plan *plan, int digits)
+{
+    if (bad_plan(plan)) return RD_ERR_ARG;
+    if (digits < 5 || digits > 7) return RD_ERR_ARG;
+    P(plan)->oz_digits = digits;
     return RD_OK;
 }
 

@@ -71,6 +71,7 @@ struct Plan {
     float *syn32_P, *syn32_D, *syn32_M, *ana32_P, *ana32_D, *ana32_M;
     uint16_t *synb_hi[3], *synb_lo[3], *anab_hi[3], *anab_lo[3];
     void *oz;                      // int8 digit tables and operand scratch of the Ozaki form
+    int oz_digits;                 // digits per operand of the Ozaki form's calls (5, 6 or 7)
     long *d_syn_off, *d_ana_off;   // per (m,p)
     int *d_ana_ncol;               // per m
     long syn_rows, ana_elems;

@@ -23,7 +23,10 @@
 // are accumulated exactly in int32 per level p + q, and
 //   C = 2^(eA + eB) sum_L 128^-(L+2) acc_L.
 // The truncation of each operand at OZ_S digits bounds the error by
-// 2^-(7 OZ_S + 1) of the row or column maximum per element.
+// 2^-(7 OZ_S + 1) of the row or column maximum per element.  The tables are
+// sliced to OZ_S = 7 digits once; a call uses the plan's digit count D
+// (5, 6 or 7: the first D digit planes of the tables and operands, the
+// products with p + q <= D - 1), for an element error of 2^-(7 D + 1).
 #include "rd_internal.h"
 
 #include <algorithm>
@@ -43,12 +46,13 @@ __device__ __forceinline__ int oz_exponent(double mx)
     return mx > 0.0 ? ilogb(mx) + 2 : 0;
 }
 
-// Digits of x 2^-e (|x 2^-e| < 1/2) into d[0..OZ_S).
-__device__ __forceinline__ void oz_digits(double x, int e, int8_t (&d)[OZ_S])
+// The first D digits of x 2^-e (|x 2^-e| < 1/2) into d[0..D).
+template <int D>
+__device__ __forceinline__ void oz_digits(double x, int e, int8_t (&d)[D])
 {
     double r = ldexp(x, -e);
 #pragma unroll
-    for (int p = 0; p < OZ_S; p++) {
+    for (int p = 0; p < D; p++) {
         r *= 128.0;
         const double q = rint(r);
         r -= q;
@@ -154,29 +158,31 @@ __device__ __forceinline__ double2 mul_i(double2 x) { return make_double2(-x.y, 
 __device__ __forceinline__ double2 mul_mi(double2 x) { return make_double2(x.y, -x.x); }
 __device__ __forceinline__ double2 neg2(double2 x) { return make_double2(-x.x, -x.y); }
 
-// Writes the digits of four consecutive values v[0..3] (complex: columns 2k,
-// 2k+1) of matrix b at K indices kidx..kidx+3, scaled by the column exponents
-// e (re, im); one packed 4-byte store per digit plane and column.
+// Writes the first D digits of four consecutive values v[0..3] (complex:
+// columns 2k, 2k+1) of matrix b at K indices kidx..kidx+3, scaled by the
+// column exponents e (re, im); one packed 4-byte store per digit plane and
+// column.
+template <int D>
 __device__ __forceinline__ void oz_put4(const OzBArgs &A, int m, int b, int kidx, int k, const double2 (&v)[4],
                                         const int *e)
 {
     const OzMat M = A.bm[m * 6 + b];
-    unsigned wr[OZ_S], wi[OZ_S];
+    unsigned wr[D], wi[D];
 #pragma unroll
-    for (int p = 0; p < OZ_S; p++) wr[p] = wi[p] = 0u;
+    for (int p = 0; p < D; p++) wr[p] = wi[p] = 0u;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        int8_t dr[OZ_S], di[OZ_S];
-        oz_digits(v[i].x, e[0], dr);
-        oz_digits(v[i].y, e[1], di);
+        int8_t dr[D], di[D];
+        oz_digits<D>(v[i].x, e[0], dr);
+        oz_digits<D>(v[i].y, e[1], di);
 #pragma unroll
-        for (int p = 0; p < OZ_S; p++) {
+        for (int p = 0; p < D; p++) {
             wr[p] |= (unsigned)(uint8_t)dr[p] << (8 * i);
             wi[p] |= (unsigned)(uint8_t)di[p] << (8 * i);
         }
     }
 #pragma unroll
-    for (int p = 0; p < OZ_S; p++) {
+    for (int p = 0; p < D; p++) {
         *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k) * M.K + kidx) = wr[p];
         *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k + 1) * M.K + kidx) = wi[p];
     }
@@ -193,8 +199,8 @@ __device__ __forceinline__ void oz_max4(double *mx, int b, const double2 (&v)[4]
 }
 
 // synthesis operands: one wave per (m, plane k), groups of four rows t of the
-// even and odd l per lane.
-template <bool VEC>
+// even and odd l per lane; D digit planes.
+template <bool VEC, int D>
 __global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
 {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
@@ -245,8 +251,8 @@ __global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
         double2 qe[4], qo[4];
         load4(0, 0, t0, qe);
         load4(0, 1, t0, qo);
-        oz_put4(A, m, 0, t0, k, qe, es + 0);
-        if (t0 < ro) oz_put4(A, m, 1, t0, k, qo, es + 2);
+        oz_put4<D>(A, m, 0, t0, k, qe, es + 0);
+        if (t0 < ro) oz_put4<D>(A, m, 1, t0, k, qo, es + 2);
         if (VEC) {
             double2 se[4], so[4], te[4], to[4], w[4];
             load4(1, 0, t0, se);
@@ -254,19 +260,19 @@ __global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
             load4(2, 0, t0, te);
             load4(2, 1, t0, to);
             if (t0 < ro) {
-                oz_put4(A, m, 2, t0, k, so, es + 4);
-                oz_put4(A, m, 4, t0, k, to, es + 8);
+                oz_put4<D>(A, m, 2, t0, k, so, es + 4);
+                oz_put4<D>(A, m, 4, t0, k, to, es + 8);
                 rot(to, w, true);
-                oz_put4(A, m, 3, re16 + t0, k, w, es + 6);
+                oz_put4<D>(A, m, 3, re16 + t0, k, w, es + 6);
                 rot(so, w, false);
-                oz_put4(A, m, 5, re16 + t0, k, w, es + 10);
+                oz_put4<D>(A, m, 5, re16 + t0, k, w, es + 10);
             }
             rot(te, w, true);
-            oz_put4(A, m, 2, ro16 + t0, k, w, es + 4);
+            oz_put4<D>(A, m, 2, ro16 + t0, k, w, es + 4);
             rot(se, w, false);
-            oz_put4(A, m, 4, ro16 + t0, k, w, es + 8);
-            oz_put4(A, m, 3, t0, k, se, es + 6);
-            oz_put4(A, m, 5, t0, k, te, es + 10);
+            oz_put4<D>(A, m, 4, ro16 + t0, k, w, es + 8);
+            oz_put4<D>(A, m, 3, t0, k, se, es + 6);
+            oz_put4<D>(A, m, 5, t0, k, te, es + 10);
         }
     }
     if (tid == 0)
@@ -274,8 +280,8 @@ __global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
 }
 
 // analysis operands: one wave per (m, plane k), groups of four northern
-// thetas t per lane.
-template <bool VEC>
+// thetas t per lane; D digit planes.
+template <bool VEC, int D>
 __global__ __launch_bounds__(256) void ana_b_oz_kernel(OzBArgs A)
 {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
@@ -325,26 +331,26 @@ __global__ __launch_bounds__(256) void ana_b_oz_kernel(OzBArgs A)
     for (int t0 = 4 * tid; t0 < A.nth; t0 += 256) {
         double2 rp[4], rm[4];
         load4(0, t0, rp, rm);
-        oz_put4(A, m, 0, t0, k, rp, es + 0);
-        oz_put4(A, m, 1, t0, k, rm, es + 2);
+        oz_put4<D>(A, m, 0, t0, k, rp, es + 0);
+        oz_put4<D>(A, m, 1, t0, k, rm, es + 2);
         if (VEC) {
             double2 tp[4], tm[4], pp[4], pm[4], w[4];
             load4(1, t0, tp, tm);
             load4(2, t0, pp, pm);
-            oz_put4(A, m, 2, t0, k, tm, es + 4);
+            oz_put4<D>(A, m, 2, t0, k, tm, es + 4);
             map(pp, w, 0);
-            oz_put4(A, m, 2, A.kth + t0, k, w, es + 4);
-            oz_put4(A, m, 3, t0, k, tp, es + 6);
+            oz_put4<D>(A, m, 2, A.kth + t0, k, w, es + 4);
+            oz_put4<D>(A, m, 3, t0, k, tp, es + 6);
             map(pm, w, 0);
-            oz_put4(A, m, 3, A.kth + t0, k, w, es + 6);
+            oz_put4<D>(A, m, 3, A.kth + t0, k, w, es + 6);
             map(pm, w, 1);
-            oz_put4(A, m, 4, t0, k, w, es + 8);
+            oz_put4<D>(A, m, 4, t0, k, w, es + 8);
             map(tp, w, 0);
-            oz_put4(A, m, 4, A.kth + t0, k, w, es + 8);
+            oz_put4<D>(A, m, 4, A.kth + t0, k, w, es + 8);
             map(pp, w, 1);
-            oz_put4(A, m, 5, t0, k, w, es + 10);
+            oz_put4<D>(A, m, 5, t0, k, w, es + 10);
             map(tm, w, 0);
-            oz_put4(A, m, 5, A.kth + t0, k, w, es + 10);
+            oz_put4<D>(A, m, 5, A.kth + t0, k, w, es + 10);
         }
     }
     if (tid == 0)
@@ -371,36 +377,38 @@ struct OzGArgs {
 constexpr int OZ_PITCH = 80;
 constexpr int OZ_TILE = OZ_S * 64 * OZ_PITCH;     // bytes of one operand tile
 
+template <int D>
 __device__ __forceinline__ void oz_stage(const int8_t *src, int nrows_total, int r0, int K, int kb,
                                          int8_t *dst)
 {
-    // OZ_S planes x 64 rows x 64 bytes, 16 bytes per thread and step
-    for (int c = threadIdx.x; c < OZ_S * 64 * 4; c += 256) {
+    // D planes x 64 rows x 64 bytes, 16 bytes per thread and step
+    for (int c = threadIdx.x; c < D * 64 * 4; c += 256) {
         const int p = c >> 8, rem = c & 255, row = rem >> 2, q = rem & 3;
         const v4i v = *(const v4i *)(src + ((long)p * nrows_total + r0 + row) * K + kb + 16 * q);
         *(v4i *)(dst + (p * 64 + row) * OZ_PITCH + 16 * q) = v;
     }
 }
 
+template <int D>
 __device__ __forceinline__ void oz_gemm_lds(const int8_t *Ab, int arows, int arow0, const int8_t *Bb,
                                             int ncols, int col0, int K, int8_t *As, int8_t *Bs,
-                                            v4i (&acc)[OZ_S][4])
+                                            v4i (&acc)[D][4])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     for (int kb = 0; kb < K; kb += 64) {
-        oz_stage(Ab, arows, arow0, K, kb, As);
-        oz_stage(Bb, ncols, col0, K, kb, Bs);
+        oz_stage<D>(Ab, arows, arow0, K, kb, As);
+        oz_stage<D>(Bb, ncols, col0, K, kb, Bs);
         __syncthreads();
-        v4i a[OZ_S];
+        v4i a[D];
 #pragma unroll
-        for (int p = 0; p < OZ_S; p++) a[p] = *(const v4i *)(As + (p * 64 + wave * 16 + r) * OZ_PITCH + 16 * g);
+        for (int p = 0; p < D; p++) a[p] = *(const v4i *)(As + (p * 64 + wave * 16 + r) * OZ_PITCH + 16 * g);
 #pragma unroll
-        for (int q = 0; q < OZ_S; q++) {
+        for (int q = 0; q < D; q++) {
             v4i bq[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) bq[j] = *(const v4i *)(Bs + (q * 64 + 16 * j + r) * OZ_PITCH + 16 * g);
 #pragma unroll
-            for (int p = 0; p < OZ_S - q; p++)
+            for (int p = 0; p < D - q; p++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], bq[j], acc[p + q][j], 0, 0, 0);
@@ -410,18 +418,20 @@ __device__ __forceinline__ void oz_gemm_lds(const int8_t *Ab, int arows, int aro
 }
 
 // FP64 value of output element (v, j): the levels from the smallest weight up.
-__device__ __forceinline__ double oz_value(const v4i (&acc)[OZ_S][4], int j, int v, int e)
+template <int D>
+__device__ __forceinline__ double oz_value(const v4i (&acc)[D][4], int j, int v, int e)
 {
     double s = 0.0;
 #pragma unroll
-    for (int L = OZ_S - 1; L >= 0; L--) s += ldexp((double)acc[L][j][v], -7 * (L + 2));
+    for (int L = D - 1; L >= 0; L--) s += ldexp((double)acc[L][j][v], -7 * (L + 2));
     return ldexp(s, e);
 }
 
-__device__ __forceinline__ void oz_zero(v4i (&acc)[OZ_S][4])
+template <int D>
+__device__ __forceinline__ void oz_zero(v4i (&acc)[D][4])
 {
 #pragma unroll
-    for (int L = 0; L < OZ_S; L++)
+    for (int L = 0; L < D; L++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[L][j] = (v4i){0, 0, 0, 0};
 }
@@ -431,24 +441,25 @@ __device__ __forceinline__ void oz_zero(v4i (&acc)[OZ_S][4])
 // multiplies 16 rows by 32 columns.
 constexpr int OZ_SJB = 2;
 
+template <int D>
 __device__ __forceinline__ void oz_gemm_reg(const int8_t *Ab, int arows, int row0, const int8_t *Bb, int ncols,
-                                            int col0, int K, v4i (&acc)[OZ_S][4])
+                                            int col0, int K, v4i (&acc)[D][4])
 {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     for (int kb = 0; kb < K; kb += 64) {
-        v4i a[OZ_S], b[OZ_S][OZ_SJB];
+        v4i a[D], b[D][OZ_SJB];
 #pragma unroll
-        for (int p = 0; p < OZ_S; p++)
+        for (int p = 0; p < D; p++)
             a[p] = *(const v4i *)(Ab + ((long)p * arows + row0 + r) * K + kb + 16 * g);
 #pragma unroll
-        for (int q = 0; q < OZ_S; q++)
+        for (int q = 0; q < D; q++)
 #pragma unroll
             for (int j = 0; j < OZ_SJB; j++)
                 b[q][j] = *(const v4i *)(Bb + ((long)q * ncols + col0 + 16 * j + r) * K + kb + 16 * g);
 #pragma unroll
-        for (int p = 0; p < OZ_S; p++)
+        for (int p = 0; p < D; p++)
 #pragma unroll
-            for (int q = 0; q < OZ_S - p; q++)
+            for (int q = 0; q < D - p; q++)
 #pragma unroll
                 for (int j = 0; j < OZ_SJB; j++)
                     acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[q][j], acc[p + q][j], 0, 0, 0);
@@ -456,7 +467,8 @@ __device__ __forceinline__ void oz_gemm_reg(const int8_t *Ab, int arows, int row
 }
 
 // synthesis: workgroup = 64 thetas (16 per wave) x 32 columns, one field,
-// both parities
+// both parities; D digits per operand
+template <int D>
 __global__ __launch_bounds__(256) void syn_oz_kernel(OzGArgs A)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
@@ -471,16 +483,16 @@ __global__ __launch_bounds__(256) void syn_oz_kernel(OzGArgs A)
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const OzMat Ma = A.am[m * 4 + am[s]], Mb = A.bm[m * 6 + bmx[s]];
-        v4i acc[OZ_S][4];
-        oz_zero(acc);
-        if (Ma.K > 0) oz_gemm_reg(A.A + Ma.off, Ma.rows, row0, A.B + Mb.off, A.ncols, col0, Ma.K, acc);
+        v4i acc[D][4];
+        oz_zero<D>(acc);
+        if (Ma.K > 0) oz_gemm_reg<D>(A.A + Ma.off, Ma.rows, row0, A.B + Mb.off, A.ncols, col0, Ma.K, acc);
 #pragma unroll
         for (int j = 0; j < OZ_SJB; j++) {
             const int col = col0 + 16 * j + r;
             const int eb = A.eB[(m * 6 + bmx[s]) * A.ncols + col];
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                res[s][j][v] = oz_value(acc, j, v, A.eA[(m * 4 + am[s]) * A.ra + row0 + 4 * g + v] + eb);
+                res[s][j][v] = oz_value<D>(acc, j, v, A.eA[(m * 4 + am[s]) * A.ra + row0 + 4 * g + v] + eb);
         }
     }
     const double sg = f == 2 ? -1.0 : 1.0;
@@ -501,7 +513,8 @@ __global__ __launch_bounds__(256) void syn_oz_kernel(OzGArgs A)
 }
 
 // analysis: workgroup = 64 rows l_p (16 per wave) x 64 columns, one field,
-// both parities
+// both parities; D digits per operand
+template <int D>
 __global__ __launch_bounds__(256) void ana_oz_kernel(OzGArgs A)
 {
     extern __shared__ __align__(16) int8_t oz_sm[];
@@ -519,9 +532,9 @@ __global__ __launch_bounds__(256) void ana_oz_kernel(OzGArgs A)
         const int b = f == 0 ? par : (f == 1 ? 2 + par : 4 + par);
         const OzMat Ma = A.am[m * 4 + a], Mb = A.bm[m * 6 + b];
         if (arow0 >= Ma.rows) continue;       // uniform over the workgroup
-        v4i acc[OZ_S][4];
-        oz_zero(acc);
-        oz_gemm_lds(A.A + Ma.off, Ma.rows, arow0, A.B + Mb.off, A.ncols, col0, Ma.K, As, Bs, acc);
+        v4i acc[D][4];
+        oz_zero<D>(acc);
+        oz_gemm_lds<D>(A.A + Ma.off, Ma.rows, arow0, A.B + Mb.off, A.ncols, col0, Ma.K, As, Bs, acc);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int col = col0 + 16 * j + r, k = col >> 1, c = col & 1;
@@ -532,7 +545,7 @@ __global__ __launch_bounds__(256) void ana_oz_kernel(OzGArgs A)
                 const int lp = row0 + 4 * g + v;
                 const int l = m + par + 2 * lp;
                 if (l > A.lmax) continue;
-                double val = oz_value(acc, j, v, A.eA[(m * 4 + a) * A.ra + lp] + eb);
+                double val = oz_value<D>(acc, j, v, A.eA[(m * 4 + a) * A.ra + lp] + eb);
                 if (m == 0 && c == 1) val = 0.0;
                 out[2 * (lm0 + (l - m) + k * A.ld) + c] = val;
             }
@@ -683,10 +696,15 @@ int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const 
     B.nplanes = nplanes;
     B.ncols = S->ncols;
     B.nmat = vec ? 6 : 2;
-    if (vec)
-        hipLaunchKernelGGL(syn_b_oz_kernel<true>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
-    else
-        hipLaunchKernelGGL(syn_b_oz_kernel<false>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    const int D = p->oz_digits;
+    const dim3 bgrid((nplanes + 3) / 4, p->L);
+#define RD_SYN_B(DD)                                                                                    \
+    if (vec)                                                                                            \
+        hipLaunchKernelGGL((syn_b_oz_kernel<true, DD>), bgrid, dim3(256), 0, p->stream, B);             \
+    else                                                                                                \
+        hipLaunchKernelGGL((syn_b_oz_kernel<false, DD>), bgrid, dim3(256), 0, p->stream, B);
+    if (D == 5) { RD_SYN_B(5) } else if (D == 6) { RD_SYN_B(6) } else { RD_SYN_B(7) }
+#undef RD_SYN_B
     RD_LAUNCH_CHECK();
     OzGArgs G = {};
     G.A = S->syn.A;
@@ -707,7 +725,13 @@ int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const 
     G.nlat = p->nlat;
     G.nth = p->nth;
     G.nplanes = nplanes;
-    hipLaunchKernelGGL(syn_oz_kernel, dim3(S->syn.n_items * G.n_jt, vec ? 3 : 1), dim3(256), 0, p->stream, G);
+    const dim3 ggrid(S->syn.n_items * G.n_jt, vec ? 3 : 1);
+    if (D == 5)
+        hipLaunchKernelGGL(syn_oz_kernel<5>, ggrid, dim3(256), 0, p->stream, G);
+    else if (D == 6)
+        hipLaunchKernelGGL(syn_oz_kernel<6>, ggrid, dim3(256), 0, p->stream, G);
+    else
+        hipLaunchKernelGGL(syn_oz_kernel<7>, ggrid, dim3(256), 0, p->stream, G);
     RD_LAUNCH_CHECK();
     return RD_OK;
 }
@@ -734,10 +758,15 @@ int launch_analysis_oz(Plan *p, bool vec, int nplanes, const double2 *g0, const 
     B.nplanes = nplanes;
     B.ncols = S->ncols;
     B.nmat = vec ? 6 : 2;
-    if (vec)
-        hipLaunchKernelGGL(ana_b_oz_kernel<true>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
-    else
-        hipLaunchKernelGGL(ana_b_oz_kernel<false>, dim3((nplanes + 3) / 4, p->L), dim3(256), 0, p->stream, B);
+    const int D = p->oz_digits;
+    const dim3 bgrid((nplanes + 3) / 4, p->L);
+#define RD_ANA_B(DD)                                                                                    \
+    if (vec)                                                                                            \
+        hipLaunchKernelGGL((ana_b_oz_kernel<true, DD>), bgrid, dim3(256), 0, p->stream, B);             \
+    else                                                                                                \
+        hipLaunchKernelGGL((ana_b_oz_kernel<false, DD>), bgrid, dim3(256), 0, p->stream, B);
+    if (D == 5) { RD_ANA_B(5) } else if (D == 6) { RD_ANA_B(6) } else { RD_ANA_B(7) }
+#undef RD_ANA_B
     RD_LAUNCH_CHECK();
     OzGArgs G = {};
     G.A = S->ana.A;
@@ -758,8 +787,13 @@ int launch_analysis_oz(Plan *p, bool vec, int nplanes, const double2 *g0, const 
     G.nlat = p->nlat;
     G.nth = p->nth;
     G.nplanes = nplanes;
-    hipLaunchKernelGGL(ana_oz_kernel, dim3(S->ana.n_items * G.n_jt, vec ? 3 : 1), dim3(256), 2 * OZ_TILE,
-                       p->stream, G);
+    const dim3 ggrid(S->ana.n_items * G.n_jt, vec ? 3 : 1);
+    if (D == 5)
+        hipLaunchKernelGGL(ana_oz_kernel<5>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
+    else if (D == 6)
+        hipLaunchKernelGGL(ana_oz_kernel<6>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
+    else
+        hipLaunchKernelGGL(ana_oz_kernel<7>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
     RD_LAUNCH_CHECK();
     return RD_OK;
 }

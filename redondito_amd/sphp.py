@@ -45,6 +45,7 @@ def lib():
         L.rd_plan_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, i, i, d, d, ctypes.POINTER(d)]
         L.rd_plan_destroy.argtypes = [vp]
         L.rd_plan_set_stream.argtypes = [vp, vp]
+        L.rd_plan_set_ozaki_digits.argtypes = [vp, i]
         L.rd_plan_sync.argtypes = [vp]
         L.rd_plan_info.argtypes = [vp, ctypes.POINTER(i)]
         L.rd_plan_grid.argtypes = [vp] + [ctypes.POINTER(d)] * 5 + [ctypes.POINTER(i)] * 2
@@ -164,6 +165,10 @@ class FCPlan:
         return self._h
 
     STAGES = ("radial", "synthesis_u", "synthesis_w", "phi_cross", "analysis", "tail")
+
+    def set_ozaki_digits(self, digits):
+        """Digits per operand (7, 6 or 5) of this plan's later RD_PREC_OZAKI calls."""
+        _check(lib().rd_plan_set_ozaki_digits(self._h, int(digits)), "plan_set_ozaki_digits")
 
     def prepare(self, prec):
         """Build the tables precision form prec needs now (no allocation on its later calls)."""

@@ -96,6 +96,45 @@ def test_bad_precision_rejected():
     for bad in (-1, 4):
         with pytest.raises(S.RdError):
             S.nonlinear_uxw(plan, W, W, *o, prec=bad)
+    for bad in (0, 4, 8):
+        with pytest.raises(S.RdError):
+            plan.set_ozaki_digits(bad)
+
+
+# Joint relative L2 of the Ozaki form against FP64 per digit count at the
+# benchmark size.  Measured: 7 digits 1.4e-12 to 1.8e-12, 6 digits 2.0e-10
+# (inside the 1e-9 bar), 5 digits 2.2e-8 (outside it: reported, held to 1e-7).
+OZ_BOUND = {7: 1e-11, 6: 1e-9, 5: 1e-7}
+
+
+def test_ozaki_digit_counts_vs_fp64():
+    from redondito_amd import sphp as S
+    lmax, nr, C, d, ri, ro = 255, 128, 25, 5, 0.35, 1.0
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    n = nr + C
+    dz = (ro - ri) / (nr - 1)
+    plan = S.FCPlan(lmax, nlat, 2 * nlat, n, C, d, dz, ri)
+    W, Z = H.shell_fields(lmax, nr, n, ri, ro, seed=20261204, slope=2.0)
+    Wd, Zd = (torch.from_numpy(x).to("cuda:0") for x in (W, Z))
+    ref = [torch.zeros_like(Wd) for _ in range(3)]
+    S.nonlinear_uxw(plan, Wd, Zd, *ref, prec=0)
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy() for x in ref]
+    errs = {}
+    for digits in (7, 6, 5):
+        plan.set_ozaki_digits(digits)
+        o = [torch.zeros_like(Wd) for _ in range(3)]
+        S.nonlinear_uxw(plan, Wd, Zd, *o, prec=3)
+        torch.cuda.synchronize()
+        rep = H.band_errors([x.cpu().numpy() for x in o], ref, lmax, nr)
+        errs[digits] = rep["joint"]
+        print(f"[ozaki int8x{digits} vs fp64 lmax={lmax} nr={nr}] joint {rep['joint']:.2e} "
+              f"NQ {rep['NQ']['all']:.2e} NS {rep['NS']['all']:.2e} NT {rep['NT']['all']:.2e}")
+        assert rep["joint"] < OZ_BOUND[digits], (digits, rep["joint"])
+    # each digit dropped costs accuracy: the counts really differ
+    assert errs[5] > errs[6] > errs[7] > 0
+    plan.set_ozaki_digits(7)
+    plan.close()
 
 
 @pytest.mark.parametrize("prec", [0, 1, 3])
