@@ -199,7 +199,9 @@ __device__ __forceinline__ void oz_max4(double *mx, int b, const double2 (&v)[4]
 }
 
 // synthesis operands: one wave per (m, plane k), groups of four rows t of the
-// even and odd l per lane; D digit planes.
+// even and odd l per lane, so that a wave's stores of one digit plane and
+// column are one contiguous run (several planes per wave for the short
+// columns of large m measured slower: 0.355 -> 0.491 ms).  D digit planes.
 template <bool VEC, int D>
 __global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
 {
@@ -369,36 +371,128 @@ struct OzGArgs {
     int lmax, nlat, nth, nplanes;
 };
 
-// LDS-staged digit GEMM.  A workgroup owns a 64-row x 64-column output tile;
-// per K step of 64 it copies the OZ_S digit planes of the A rows and of the B
-// columns to LDS (rows at a pitch of 80 bytes: the 16-byte fragment reads of
-// 16 lanes then cover all 64 banks), and each wave multiplies its 16 rows by
-// the 64 columns: acc[p + q][j] += A_p B_q[j] for p + q <= OZ_S - 1.
+// LDS-staged digit GEMMs of one workgroup item.  The item is a 64-row slab of
+// the A matrices of a pair of GEMMs (s = 0, 1: Ma[s] x Mb[s]) times the
+// 64-column tiles jt0 <= jt < jt0 + njt of the B matrices.  Per K step of 64
+// the D digit planes of the A rows and B columns go to LDS (rows at a pitch
+// of 80 bytes: the 16-byte fragment reads of 16 lanes then cover all 64
+// banks), and each wave multiplies its 16 rows by the 64 columns:
+// acc[p + q][j] += A_p B_q[j] for p + q <= D - 1.  Thread t stages row t / 4,
+// 16-byte chunk t % 4 of every plane of both operands.  The GEMMs (jt, s) run
+// as one sequence of K steps, each step's staging loads issued during the
+// previous step's MFMAs across GEMM boundaries, so that only the item's first
+// step waits for them in full; epi(jt, s, acc) runs after each GEMM's last
+// step.  A pair with live[s] false is skipped (at least one is live); every
+// live pair has K > 0, a multiple of 64.
 constexpr int OZ_PITCH = 80;
 constexpr int OZ_TILE = OZ_S * 64 * OZ_PITCH;     // bytes of one operand tile
 
 template <int D>
-__device__ __forceinline__ void oz_stage(const int8_t *src, int nrows_total, int r0, int K, int kb,
-                                         int8_t *dst)
+__device__ __forceinline__ void oz_zero(v4i (&acc)[D][4])
 {
-    // D planes x 64 rows x 64 bytes, 16 bytes per thread and step
-    for (int c = threadIdx.x; c < D * 64 * 4; c += 256) {
-        const int p = c >> 8, rem = c & 255, row = rem >> 2, q = rem & 3;
-        const v4i v = *(const v4i *)(src + ((long)p * nrows_total + r0 + row) * K + kb + 16 * q);
-        *(v4i *)(dst + (p * 64 + row) * OZ_PITCH + 16 * q) = v;
+#pragma unroll
+    for (int L = 0; L < D; L++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[L][j] = (v4i){0, 0, 0, 0};
+}
+
+template <int D, typename Epi>
+__device__ __forceinline__ void oz_item(const int8_t *Aall, const int8_t *Ball, int ncols, const OzMat (&Ma)[2],
+                                        const OzMat (&Mb)[2], const bool (&live)[2], int arow0, int jt0,
+                                        int njt, int8_t *As, int8_t *Bs, Epi &&epi)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, g = lane >> 4;
+    const int srow = tid >> 2, sq = tid & 3;
+    v4i ra[D], rb[D];
+    auto gload = [&](int jt, int s, int kb) {
+        const int K = Ma[s].K;
+        const int8_t *Ab = Aall + Ma[s].off, *Bb = Ball + Mb[s].off;
+#pragma unroll
+        for (int p = 0; p < D; p++) {
+            ra[p] = *(const v4i *)(Ab + ((long)p * Ma[s].rows + arow0 + srow) * K + kb + 16 * sq);
+            rb[p] = *(const v4i *)(Bb + ((long)p * ncols + 64 * jt + srow) * K + kb + 16 * sq);
+        }
+    };
+    // the step after (jt, s, kb)
+    auto advance = [&](int &jt, int &s, int &kb) {
+        kb += 64;
+        if (kb < Ma[s].K) return;
+        kb = 0;
+        if (s == 0 && live[1]) {
+            s = 1;
+            return;
+        }
+        s = live[0] ? 0 : 1;
+        jt++;
+    };
+    v4i acc[D][4];
+    oz_zero<D>(acc);
+    const int jend = jt0 + njt;
+    int jt = jt0, s = live[0] ? 0 : 1, kb = 0;
+    gload(jt, s, kb);
+    while (jt < jend) {
+#pragma unroll
+        for (int p = 0; p < D; p++) {
+            *(v4i *)(As + (p * 64 + srow) * OZ_PITCH + 16 * sq) = ra[p];
+            *(v4i *)(Bs + (p * 64 + srow) * OZ_PITCH + 16 * sq) = rb[p];
+        }
+        __syncthreads();
+        int jn = jt, sn = s, kn = kb;
+        advance(jn, sn, kn);
+        if (jn < jend) gload(jn, sn, kn);
+        v4i a[D];
+#pragma unroll
+        for (int p = 0; p < D; p++) a[p] = *(const v4i *)(As + (p * 64 + wave * 16 + r) * OZ_PITCH + 16 * g);
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            v4i bq[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bq[j] = *(const v4i *)(Bs + (q * 64 + 16 * j + r) * OZ_PITCH + 16 * g);
+#pragma unroll
+            for (int p = 0; p < D - q; p++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], bq[j], acc[p + q][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (kn == 0) {   // (jt, s) has had its last step
+            epi(jt, s, acc);
+            oz_zero<D>(acc);
+        }
+        jt = jn;
+        s = sn;
+        kb = kn;
     }
 }
 
+// One digit GEMM of a 64-row x 64-column tile with the staging of oz_item:
+// the next K step's chunks are loaded into registers during the current
+// step's MFMAs (the analysis tiles, whose K of 3 or 6 steps is pipelined
+// within the GEMM; a sequence across both parities measured slower for them).
 template <int D>
 __device__ __forceinline__ void oz_gemm_lds(const int8_t *Ab, int arows, int arow0, const int8_t *Bb,
                                             int ncols, int col0, int K, int8_t *As, int8_t *Bs,
                                             v4i (&acc)[D][4])
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, g = lane >> 4;
+    const int srow = tid >> 2, sq = tid & 3;
+    v4i ra[D], rb[D];
+    auto gload = [&](int kb) {
+#pragma unroll
+        for (int p = 0; p < D; p++) {
+            ra[p] = *(const v4i *)(Ab + ((long)p * arows + arow0 + srow) * K + kb + 16 * sq);
+            rb[p] = *(const v4i *)(Bb + ((long)p * ncols + col0 + srow) * K + kb + 16 * sq);
+        }
+    };
+    gload(0);
     for (int kb = 0; kb < K; kb += 64) {
-        oz_stage<D>(Ab, arows, arow0, K, kb, As);
-        oz_stage<D>(Bb, ncols, col0, K, kb, Bs);
+#pragma unroll
+        for (int p = 0; p < D; p++) {
+            *(v4i *)(As + (p * 64 + srow) * OZ_PITCH + 16 * sq) = ra[p];
+            *(v4i *)(Bs + (p * 64 + srow) * OZ_PITCH + 16 * sq) = rb[p];
+        }
         __syncthreads();
+        if (kb + 64 < K) gload(kb + 64);
         v4i a[D];
 #pragma unroll
         for (int p = 0; p < D; p++) a[p] = *(const v4i *)(As + (p * 64 + wave * 16 + r) * OZ_PITCH + 16 * g);
@@ -427,95 +521,62 @@ __device__ __forceinline__ double oz_value(const v4i (&acc)[D][4], int j, int v,
     return ldexp(s, e);
 }
 
+// synthesis: workgroup = 64 thetas (16 per wave) of one m, one field, every
+// column tile, both parities (the symmetric part of a column tile is held as
+// FP64 while its antisymmetric GEMM runs); D digits per operand
 template <int D>
-__device__ __forceinline__ void oz_zero(v4i (&acc)[D][4])
+__global__ __launch_bounds__(256, 2) void syn_oz_kernel(OzGArgs A)
 {
-#pragma unroll
-    for (int L = 0; L < D; L++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[L][j] = (v4i){0, 0, 0, 0};
-}
-
-// Register digit GEMM for the short-K synthesis tiles: each wave loads its
-// A rows and B columns straight from L2 (no staging barrier per K step) and
-// multiplies 16 rows by 32 columns.
-constexpr int OZ_SJB = 2;
-
-template <int D>
-__device__ __forceinline__ void oz_gemm_reg(const int8_t *Ab, int arows, int row0, const int8_t *Bb, int ncols,
-                                            int col0, int K, v4i (&acc)[D][4])
-{
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    for (int kb = 0; kb < K; kb += 64) {
-        v4i a[D], b[D][OZ_SJB];
-#pragma unroll
-        for (int p = 0; p < D; p++)
-            a[p] = *(const v4i *)(Ab + ((long)p * arows + row0 + r) * K + kb + 16 * g);
-#pragma unroll
-        for (int q = 0; q < D; q++)
-#pragma unroll
-            for (int j = 0; j < OZ_SJB; j++)
-                b[q][j] = *(const v4i *)(Bb + ((long)q * ncols + col0 + 16 * j + r) * K + kb + 16 * g);
-#pragma unroll
-        for (int p = 0; p < D; p++)
-#pragma unroll
-            for (int q = 0; q < D - p; q++)
-#pragma unroll
-                for (int j = 0; j < OZ_SJB; j++)
-                    acc[p + q][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[q][j], acc[p + q][j], 0, 0, 0);
-    }
-}
-
-// synthesis: workgroup = 64 thetas (16 per wave) x 32 columns, one field,
-// both parities; D digits per operand
-template <int D>
-__global__ __launch_bounds__(256) void syn_oz_kernel(OzGArgs A)
-{
+    extern __shared__ __align__(16) int8_t oz_sm[];
+    int8_t *As = oz_sm, *Bs = oz_sm + OZ_TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const int jt = blockIdx.x % A.n_jt;
-    const int2 it = A.items[blockIdx.x / A.n_jt];
-    const int m = it.x, row0 = it.y * 64 + wave * 16, col0 = jt * 16 * OZ_SJB;
+    const int2 it = A.items[blockIdx.x];
+    const int m = it.x, arow0 = it.y * 64, row0 = arow0 + wave * 16;
     const int f = blockIdx.y;
     const int L = A.lmax + 1;
     const int am[2] = {f == 0 ? 0 : 2, f == 0 ? 1 : 3};
     const int bmx[2] = {f == 0 ? 0 : (f == 1 ? 2 : 4), f == 0 ? 1 : (f == 1 ? 3 : 5)};
-    double res[2][OZ_SJB][4];
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-        const OzMat Ma = A.am[m * 4 + am[s]], Mb = A.bm[m * 6 + bmx[s]];
-        v4i acc[D][4];
-        oz_zero<D>(acc);
-        if (Ma.K > 0) oz_gemm_reg<D>(A.A + Ma.off, Ma.rows, row0, A.B + Mb.off, A.ncols, col0, Ma.K, acc);
-#pragma unroll
-        for (int j = 0; j < OZ_SJB; j++) {
-            const int col = col0 + 16 * j + r;
-            const int eb = A.eB[(m * 6 + bmx[s]) * A.ncols + col];
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                res[s][j][v] = oz_value<D>(acc, j, v, A.eA[(m * 4 + am[s]) * A.ra + row0 + 4 * g + v] + eb);
-        }
-    }
+    const OzMat Ma[2] = {A.am[m * 4 + am[0]], A.am[m * 4 + am[1]]};
+    const OzMat Mb[2] = {A.bm[m * 6 + bmx[0]], A.bm[m * 6 + bmx[1]]};
+    // K = 0 only for the odd-l matrix of m = lmax; the even-l part is never empty
+    const bool live[2] = {true, Ma[1].K > 0};
     const double sg = f == 2 ? -1.0 : 1.0;
     double *out = (double *)A.out[f];
+    double sym[4][4];
+    oz_item<D>(A.A, A.B, A.ncols, Ma, Mb, live, arow0, 0, A.n_jt, As, Bs, [&](int jt, int s, const v4i (&acc)[D][4]) {
 #pragma unroll
-    for (int j = 0; j < OZ_SJB; j++) {
-        const int col = col0 + 16 * j + r, k = col >> 1, c = col & 1;
-        if (k >= A.nplanes) continue;
-        const long base = ((long)k * L + m) * A.nlat;
+        for (int j = 0; j < 4; j++) {
+            const int col = 64 * jt + 16 * j + r;
+            const int eb = A.eB[(m * 6 + bmx[s]) * A.ncols + col];
+            double res[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int th = row0 + 4 * g + v;
-            if (th >= A.nth) continue;
-            out[2 * (base + th) + c] = sg * (res[0][j][v] + res[1][j][v]);
-            out[2 * (base + A.nlat - 1 - th) + c] = sg * (res[0][j][v] - res[1][j][v]);
+            for (int v = 0; v < 4; v++)
+                res[v] = oz_value<D>(acc, j, v, A.eA[(m * 4 + am[s]) * A.ra + row0 + 4 * g + v] + eb);
+            if (s == 0 && live[1]) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) sym[j][v] = res[v];
+                continue;
+            }
+            const int k = col >> 1, c = col & 1;
+            if (k >= A.nplanes) continue;
+            const long base = ((long)k * L + m) * A.nlat;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int th = row0 + 4 * g + v;
+                if (th >= A.nth) continue;
+                // s = 1: sym held from s = 0; s = 0 without an odd part: anti = 0
+                const double sy = s == 1 ? sym[j][v] : res[v], an = s == 1 ? res[v] : 0.0;
+                out[2 * (base + th) + c] = sg * (sy + an);
+                out[2 * (base + A.nlat - 1 - th) + c] = sg * (sy - an);
+            }
         }
-    }
+    });
 }
 
 // analysis: workgroup = 64 rows l_p (16 per wave) x 64 columns, one field,
 // both parities; D digits per operand
 template <int D>
-__global__ __launch_bounds__(256) void ana_oz_kernel(OzGArgs A)
+__global__ __launch_bounds__(256, 2) void ana_oz_kernel(OzGArgs A)
 {
     extern __shared__ __align__(16) int8_t oz_sm[];
     int8_t *As = oz_sm, *Bs = oz_sm + OZ_TILE;
@@ -720,18 +781,18 @@ int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const 
     G.ld = ld;
     G.ra = S->syn.ra;
     G.ncols = S->ncols;
-    G.n_jt = (2 * nplanes + 16 * OZ_SJB - 1) / (16 * OZ_SJB);
+    G.n_jt = (2 * nplanes + OZ_CT - 1) / OZ_CT;
     G.lmax = p->lmax;
     G.nlat = p->nlat;
     G.nth = p->nth;
     G.nplanes = nplanes;
-    const dim3 ggrid(S->syn.n_items * G.n_jt, vec ? 3 : 1);
+    const dim3 ggrid(S->syn.n_items, vec ? 3 : 1);
     if (D == 5)
-        hipLaunchKernelGGL(syn_oz_kernel<5>, ggrid, dim3(256), 0, p->stream, G);
+        hipLaunchKernelGGL(syn_oz_kernel<5>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
     else if (D == 6)
-        hipLaunchKernelGGL(syn_oz_kernel<6>, ggrid, dim3(256), 0, p->stream, G);
+        hipLaunchKernelGGL(syn_oz_kernel<6>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
     else
-        hipLaunchKernelGGL(syn_oz_kernel<7>, ggrid, dim3(256), 0, p->stream, G);
+        hipLaunchKernelGGL(syn_oz_kernel<7>, ggrid, dim3(256), 2 * OZ_TILE, p->stream, G);
     RD_LAUNCH_CHECK();
     return RD_OK;
 }
