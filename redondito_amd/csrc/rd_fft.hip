@@ -584,6 +584,28 @@ __device__ __forceinline__ void fill_t16(V *T16, const double2 *tw, int st, int 
     }
 }
 
+// The twiddles e^{-2 pi i b q/256}, q = 1..15, of lane b of a 16 x 16 pass:
+// T16 holds them all, but only q = 1, 2, 3, 4, 8, 12 are read, and the
+// others are formed as one complex product w_{4a} w_r (q = 4a + r), one
+// rounding beyond the table's: six LDS reads per lane instead of fifteen.
+template <typename V>
+struct Tw16 {
+    V t1, t2, t3, t4, t8, t12;
+    __device__ __forceinline__ Tw16(const V *T16, int b)
+        : t1(T16[b]), t2(T16[16 + b]), t3(T16[32 + b]), t4(T16[48 + b]), t8(T16[112 + b]), t12(T16[176 + b])
+    {
+    }
+    __device__ __forceinline__ V get(int q) const
+    {
+        const int a = q >> 2, r = q & 3;
+        const V hi = a == 1 ? t4 : (a == 2 ? t8 : t12);
+        const V lo = r == 1 ? t1 : (r == 2 ? t2 : t3);
+        if (r == 0) return hi;
+        if (a == 0) return lo;
+        return cmul(hi, lo);
+    }
+};
+
 // One (plane, theta) row per workgroup, in place in a single LDS buffer so
 // that four rows run per CU.  Every phase reads its inputs into registers,
 // waits at a barrier, then writes its outputs over the same buffer:
@@ -654,12 +676,16 @@ __device__ __forceinline__ void cross768_row(const PhiPtrs &P, int lmax, int nla
     const int tid = threadIdx.x;
     // the row's six coefficient vectors [f][m], m = 0..255 (zero above lmax),
     // and the sums of squares of u and of w
+    // (the thread's own coefficients stay in registers for R3, which reads
+    // only the mirrored ones back)
     double s[2] = {0.0, 0.0};
     const bool live = tid <= lmax;
+    V own[6];
 #pragma unroll
     for (int f = 0; f < 6; f++) {
         const double2 v = make_double2(live ? vin[f].x : 0.0, (live && tid > 0) ? vin[f].y : 0.0);
-        B[f * 256 + tid] = Cx<V>::make(v.x, v.y);
+        own[f] = Cx<V>::make(v.x, v.y);
+        B[f * 256 + tid] = own[f];
         s[f / 3] += v.x * v.x + v.y * v.y;
     }
     block_sums<2, 4>(s, red);
@@ -675,7 +701,7 @@ __device__ __forceinline__ void cross768_row(const PhiPtrs &P, int lmax, int nla
         V zb[3], zc[3];
 #pragma unroll
         for (int x = 0; x < 3; x++) {
-            const V u = B[x * 256 + b], w = cscale(B[(3 + x) * 256 + b], as);
+            const V u = own[x], w = cscale(own[3 + x], as);   // b = tid
             const V uc = B[x * 256 + mm], wc = cscale(B[(3 + x) * 256 + mm], as);
             zb[x] = V{u.x - w.y, u.y + w.x};
             zc[x] = b == 0 ? V{S(0), S(0)} : V{uc.x + wc.y, -uc.y + wc.x};
@@ -709,10 +735,10 @@ __device__ __forceinline__ void cross768_row(const PhiPtrs &P, int lmax, int nla
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            const Tw16<V> tw(T16, b1);
             B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 1; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
+            for (int q1 = 1; q1 < 16; q1++) B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(tw.get(q1)));
         }
     }
     wave_lds_sync();
@@ -769,10 +795,10 @@ __device__ __forceinline__ void cross768_row(const PhiPtrs &P, int lmax, int nla
         wave_lds_sync();
         if (act) {
             dft16(v, -1);
+            const Tw16<V> tw(T16, a1);
             B[((s * 3 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 1; k1 < 16; k1++)
-                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
+            for (int k1 = 1; k1 < 16; k1++) B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], tw.get(k1));
         }
     }
     wave_lds_sync();
@@ -923,10 +949,10 @@ __global__ __launch_bounds__(256) void phi_inverse768_kernel(PhiPtrs P, int lmax
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            const Tw16<double2> tw(T16, b1);
             B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 1; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
+            for (int q1 = 1; q1 < 16; q1++) B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(tw.get(q1)));
         }
     }
     wave_lds_sync();
@@ -1013,10 +1039,10 @@ __global__ __launch_bounds__(256) void phi_forward768_kernel(PhiPtrs P, int lmax
         wave_lds_sync();
         if (act) {
             dft16(v, -1);
+            const Tw16<double2> tw(T16, a1);
             B[((s * 3 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 1; k1 < 16; k1++)
-                B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
+            for (int k1 = 1; k1 < 16; k1++) B[((s * 3 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], tw.get(k1));
         }
     }
     wave_lds_sync();
@@ -1207,10 +1233,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         wave_lds_sync();
         if (act) {
             dft16(v, +1);
+            const Tw16<V> tw(T16, b1);
             B[(xr * 16 + b1) * 16 + b1] = v[0];
 #pragma unroll
-            for (int q1 = 1; q1 < 16; q1++)
-                B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(T16[(q1 - 1) * 16 + b1]));
+            for (int q1 = 1; q1 < 16; q1++) B[(xr * 16 + b1) * 16 + (q1 ^ b1)] = cmul(v[q1], cconj(tw.get(q1)));
         }
     }
     wave_lds_sync();
@@ -1262,10 +1288,10 @@ __global__ __launch_bounds__(512, 2) void phi1536_kernel(PhiPtrs P, int lmax, in
         __syncthreads();
         if (act) {
             dft16(v, -1);
+            const Tw16<V> tw(T16, a1);
             B[((s * 6 + n2) * 16 + a1) * 16 + a1] = v[0];
 #pragma unroll
-            for (int k1 = 1; k1 < 16; k1++)
-                B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], T16[(k1 - 1) * 16 + a1]);
+            for (int k1 = 1; k1 < 16; k1++) B[((s * 6 + n2) * 16 + a1) * 16 + (k1 ^ a1)] = cmul(v[k1], tw.get(k1));
         }
     }
     wave_lds_sync();
