@@ -1932,158 +1932,290 @@ __device__ __forceinline__ void fact_load_rows(const double2 *src, long ld, int 
 // continuation).  z, kz and dir are staged in LDS.
 constexpr int kRadMaxC = 32;
 
-template <int N1, int N2, int D>
-__global__ __launch_bounds__(256, 3) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
+// Streaming store of one complex value (the outputs of the radial step are
+// read by the next kernel, not by this one: the nontemporal hint keeps them
+// from displacing the input rows it reads a second time).
+__device__ __forceinline__ void st_stream(double2 *p, double2 v)
 {
-    constexpr int N = N1 * N2, CB = RadFact<N1, N2>::CB;
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    const vd2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (vd2 *)p);
+}
+
+// Tiling of uxw_radial_fact_kernel: CB lm columns per workgroup, 16 (two whole
+// 128-byte lines of each row) or 32 for the short lengths, one thread per
+// (column, n2) in pass A and per (column, k1) in pass B on NT threads; the
+// pass-B items past NT (k1 = 16 at n = 153) are second items of the first
+// threads, with their spectra held in LDS.
+template <int N1, int N2>
+struct RadLine {
+    static constexpr int NM = N1 > N2 ? N1 : N2;
+    static constexpr int CB = NM > 8 ? 16 : 32;
+    static constexpr int NT = CB * NM > 256 ? 256 : CB * NM;
+    static constexpr int NB2 = CB * N1 > NT ? CB * N1 - NT : 0;   // second pass-B items
+};
+
+template <int N1, int N2, int D>
+__global__ __launch_bounds__((RadLine<N1, N2>::NT), 3) void uxw_radial_fact_kernel(UxwRadArgs A, const double2 *tw)
+{
+    constexpr int N = N1 * N2, CB = RadLine<N1, N2>::CB, NT = RadLine<N1, N2>::NT;
+    constexpr int NB2 = RadLine<N1, N2>::NB2;
+    constexpr int DW = D > 0 ? D : 1;
     __shared__ double2 buf[CB * N];
     __shared__ double2 twl[N];   // e^{-2 pi i t / n}
     __shared__ double zl[N], kzl[N];
     __shared__ double dirl[D > 0 ? kRadMaxC * D : 1];
+    // wall rows of the workgroup's columns, [j][column]: j < D row nr - D + j,
+    // j >= D row 2 D - 1 - j (the D matching rows at each wall)
+    __shared__ double2 wall[2 * DW][CB];
+    // spectra of the second pass-B items, [item][k2]
+    __shared__ double2 xs2[NB2 > 0 ? NB2 : 1][N2];
     const int tid = threadIdx.x;
-    for (int t = tid; t < N; t += 256) {
+    const int nr = N - A.C;
+    // blockIdx.y: the field, 0 for W (u_Q, u_S, w_T), 1 for Z (u_T, w_Q, w_S)
+    const int fld = blockIdx.y;
+    const double2 *src = fld ? A.Z : A.W;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * CB;
+    // byte offsets of (row, column) in the (nlm, n) arrays: below 2^32 (the
+    // launch takes the generic kernel otherwise), so that the loads and
+    // stores take a uniform base and a 32-bit offset
+    const unsigned ldb = (unsigned)A.ld * 16u;
+    for (int t = tid; t < N; t += NT) {
         twl[t] = tw[t];
         zl[t] = A.z[t];
         kzl[t] = A.kz[t];
     }
-    if (D > 0)
-        for (int t = tid; t < A.C * D; t += 256) dirl[t] = A.dir[t];
-    const int nr = N - A.C;
-    // consecutive column blocks on one XCD: a 128-byte line shared by two
-    // blocks' row segments (CB x 16 bytes) is then fetched into one L2 only
-    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * CB;
-    const bool inA = tid < CB * N2, inB = tid < CB * N1;
-    const int ca = tid % CB, n2 = tid / CB;   // pass A
+    if (D > 0) {
+        for (int t = tid; t < A.C * D; t += NT) dirl[t] = A.dir[t];
+        for (int t = tid; t < 2 * D * CB; t += NT) {
+            const int c = t % CB, j = t / CB;
+            const int row = j < D ? nr - D + j : 2 * D - 1 - j;
+            wall[j][c] = src[(long)row * A.ld + min(c0 + c, A.ncols - 1)];
+        }
+    }
+    // every thread has a pass-B item when CB N1 >= NT; threads t < NB2 have a
+    // second one, (column t % CB, k1 = (NT + t) / CB)
+    const bool inA = tid < CB * N2, inB = CB * N1 >= NT || tid < CB * N1;
     const int cb = tid % CB, k1 = tid / CB;   // pass B
-    const int lma = c0 + ca;
-    const bool okA = inA && lma < A.ncols;
-    const int lmc = okA ? lma : 0;           // a valid column for the loads of idle lanes
+    const bool inB2 = tid < NB2;
+    const int cb2 = (NT + tid) % CB, k12 = (NT + tid) / CB;
     const double inv_n = 1.0 / N;
-    const int ell = A.ell[lmc];
-    const double ll = okA ? (double)ell * (ell + 1) : 0.0;
-    __syncthreads();
+    // l(l+1) of the thread's pass-A column (0 for idle lanes)
+    double ll;
+    {
+        const int lma = c0 + tid % CB;
+        const bool okA = inA && lma < A.ncols;
+        const int ell = A.ell[okA ? lma : 0];
+        ll = okA ? (double)ell * (ell + 1) : 0.0;
+    }
 
-    // rows n2 + N2 n1 of column lma of src, the continuation rows from the table
-    auto load_rows = [&](const double2 *src, double2 (&x)[N1]) {
-        fact_load_rows<N1, N2, D>(src, A.ld, lmc, okA, n2, nr, A.C, dirl, x);
+    // Pass-A indices of the thread, recomputed in every phase from an opaque
+    // copy of tid: shared between phases they (and the row offsets formed
+    // from them) would stay live across the barriers and crowd out the
+    // spectrum held by pass B.
+    struct PA {
+        int ca, n2;
+        bool ok;           // a real column
+        unsigned o0;       // byte offset of row n2 of the column (a valid column for idle lanes)
+    };
+    auto pa = [&]() {
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        PA p;
+        p.ca = t % CB;
+        p.n2 = t / CB;
+        const int lma = c0 + p.ca;
+        p.ok = t < CB * N2 && lma < A.ncols;
+        p.o0 = (unsigned)p.n2 * ldb + (unsigned)(p.ok ? lma : 0) * 16u;
+        return p;
+    };
+    auto at = [](const double2 *base, unsigned off) { return *(const double2 *)((const char *)base + off); };
+    auto put = [](double2 *base, unsigned off, double2 v) { st_stream((double2 *)((char *)base + off), v); };
+
+    // Rows n2 + N2 n1 of the thread's column of the field: the continuation
+    // rows past nr are formed first from the staged walls with the FC-Gram
+    // operator (sphp_sca.F90:369-381), then every row clamped to the physical
+    // ones is loaded, without branches, so that the wall reads and the loads
+    // in flight do not hold registers together.  The continuation rows lie in
+    // the last NCR values of n1, since C <= kRadMaxC.
+    constexpr int NCR = (kRadMaxC + N2 - 1) / N2 + 1 < N1 ? (kRadMaxC + N2 - 1) / N2 + 1 : N1;
+    auto load_rows = [&](const PA &p, double2 (&x)[N1]) {
+        double2 cv[NCR];
+#pragma unroll
+        for (int i = 0; i < NCR; i++) {
+            const int r = p.n2 + N2 * (N1 - NCR + i);
+            const int ii = min(max(r - nr + 1, 1), A.C > 0 ? A.C : 1);
+            double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int jj = 0; jj < D; jj++) {
+                const double a1 = dirl[(ii - 1) + A.C * jj];
+                const double a2 = dirl[(A.C - ii) + A.C * jj];
+                const double2 wr = wall[jj][p.ca], wl = wall[D + jj][p.ca];
+                v.x += a1 * wr.x + a2 * wl.x;
+                v.y += a1 * wr.y + a2 * wl.y;
+            }
+            cv[i] = v;
+        }
+        const unsigned olast = p.o0 + (unsigned)(nr - 1 - p.n2) * ldb;   // row nr - 1
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const int r = p.n2 + N2 * n1;
+            x[n1] = at(src, r < nr ? p.o0 + (unsigned)(N2 * n1) * ldb : olast);
+        }
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const int r = p.n2 + N2 * n1;
+            double2 v = x[n1];
+            if (D > 0 && n1 >= N1 - NCR && r >= nr) v = cv[n1 - (N1 - NCR)];
+            x[n1] = p.ok ? v : make_double2(0.0, 0.0);
+        }
     };
     // pass A forward: N1-point DFTs, twiddles, to LDS as [column][k1][n2]
-    auto fwd_a = [&](const double2 (&x)[N1]) {
-        dft_odd_emit<N1>(x, -1, [&](int q, double2 v) { buf[ca * N + q * N2 + n2] = cmul(v, twl[(n2 * q) % N]); });
+    auto fwd_a = [&](const PA &p, const double2 (&x)[N1]) {
+        dft_odd_emit<N1>(x, -1, [&](int q, double2 v) { buf[p.ca * N + q * N2 + p.n2] = cmul(v, twl[(p.n2 * q) % N]); });
     };
-    // pass B forward: N2-point DFTs -> X[k1 + N1 k2]
-    auto fwd_b = [&](double2 (&y)[N2]) {
+    // pass B forward of item (column c, kk): N2-point DFTs -> X[kk + N1 k2]
+    auto fwd_b = [&](int c, int kk, double2 (&y)[N2]) {
 #pragma unroll
-        for (int q = 0; q < N2; q++) y[q] = buf[cb * N + k1 * N2 + q];
+        for (int q = 0; q < N2; q++) y[q] = buf[c * N + kk * N2 + q];
         dft_odd<N2>(y, -1);
     };
     // pass B inverse: N2-point DFTs of a spectrum, twiddles, to LDS as [column][n2][k1]
-    auto inv_b = [&](const double2 (&y)[N2]) {
-        dft_odd_emit<N2>(y, +1, [&](int q, double2 v) { buf[cb * N + q * N1 + k1] = cmul(v, cconj(twl[(q * k1) % N])); });
+    auto inv_b = [&](int c, int kk, const double2 (&y)[N2]) {
+        dft_odd_emit<N2>(y, +1, [&](int q, double2 v) { buf[c * N + q * N1 + kk] = cmul(v, cconj(twl[(q * kk) % N])); });
+    };
+    // the forward pass B of the second items, their spectra kept in xs2
+    auto fwd_b2 = [&]() {
+        if (inB2) {
+            double2 z[N2];
+            fwd_b(cb2, k12, z);
+#pragma unroll
+            for (int q = 0; q < N2; q++) xs2[tid][q] = z[q];
+        }
+    };
+    // the inverse pass B of the second items, of the spectrum times f(k, X)
+    auto inv_b2 = [&](auto &&f) {
+        if (inB2) {
+            double2 z[N2];
+#pragma unroll
+            for (int q = 0; q < N2; q++) z[q] = f(kzl[k12 + N1 * q], xs2[tid][q]);
+            inv_b(cb2, k12, z);
+        }
     };
     // pass A inverse: N1-point DFTs; emit(n1, value at row n2 + N2 n1)
-    auto inv_a = [&](double2 (&x)[N1], auto &&emit) {
+    auto inv_a = [&](const PA &p, double2 (&x)[N1], auto &&emit) {
 #pragma unroll
-        for (int q = 0; q < N1; q++) x[q] = buf[ca * N + n2 * N1 + q];
+        for (int q = 0; q < N1; q++) x[q] = buf[p.ca * N + p.n2 * N1 + q];
         dft_odd_emit<N1>(x, +1, emit);
     };
 
-    // One radial spectrum is held in registers at a time: W's (for u_S and
-    // the d2W of w_T), then Z's (for w_S).
-    double2 X[N2], x[N1], y[N2];
-    // ---- W: u_Q = l(l+1) W / r^2, forward transform
-    if (inA) {
-        load_rows(A.W, x);
-        if (okA)
+    // i kz / n times a spectrum value (the first radial derivative)
+    auto ddr = [&](double kz, double2 v) {
+        const double k = kz * inv_n;
+        return make_double2(-k * v.y, k * v.x);
+    };
+    double2 x[N1], y[N2], X[N2];
+    __syncthreads();
+    if (fld == 0) {
+        // ---- W: u_Q = l(l+1) W / r^2, forward transform
+        if (inA) {
+            const PA p = pa();
+            load_rows(p, x);
+            if (p.ok)
 #pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                if (r < nr) {
-                    const double rr = zl[r];
-                    A.out[0][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
+                for (int n1 = 0; n1 < N1; n1++) {
+                    const int r = p.n2 + N2 * n1;
+                    if (r < nr) {
+                        const double rr = zl[r];
+                        put(A.out[0], p.o0 + (unsigned)(N2 * n1) * ldb, cscale(x[n1], ll / (rr * rr)));
+                    }
                 }
-            }
-        fwd_a(x);
-    }
-    __syncthreads();
-    if (inB) fwd_b(X);
-    __syncthreads();
-    // ---- u_S = dW / r
-    if (inB) {
-#pragma unroll
-        for (int q = 0; q < N2; q++) {
-            const double k = kzl[k1 + N1 * q] * inv_n;
-            y[q] = make_double2(-k * X[q].y, k * X[q].x);
+            fwd_a(p, x);
         }
-        inv_b(y);
-    }
-    __syncthreads();
-    if (inA)
-        inv_a(x, [&](int n1, double2 v) {
-            const int r = n2 + N2 * n1;
-            if (okA && r < nr) A.out[1][(long)r * A.ld + lma] = cscale(v, 1.0 / zl[r]);
-        });
-    __syncthreads();
-    if (A.nout == 6) {
-        // ---- w_T = -(d2W - l(l+1) W / r^2) / r
+        __syncthreads();
+        if (inB) fwd_b(cb, k1, X);
+        fwd_b2();
+        __syncthreads();
+        // ---- u_S = dW / r
         if (inB) {
 #pragma unroll
-            for (int q = 0; q < N2; q++) {
-                const double k = kzl[k1 + N1 * q];
-                y[q] = cscale(X[q], -k * k * inv_n);
-            }
-            inv_b(y);
+            for (int q = 0; q < N2; q++) y[q] = ddr(kzl[k1 + N1 * q], X[q]);
+            inv_b(cb, k1, y);
         }
+        inv_b2(ddr);
         __syncthreads();
         if (inA) {
-            // W at this thread's rows again, each row loaded where its
-            // output is formed (clamped: no branch before the load)
-            inv_a(x, [&](int n1, double2 v) {
-                const int r = n2 + N2 * n1;
-                const double2 w = A.W[(long)min(r, nr - 1) * A.ld + lmc];
-                if (okA && r < nr) {
-                    const long o = (long)r * A.ld + lma;
-                    const double rr = zl[r], r2 = 1.0 / (rr * rr);
-                    A.out[5][o] = cscale(make_double2(v.x - ll * w.x * r2, v.y - ll * w.y * r2), -1.0 / rr);
-                }
+            const PA p = pa();
+            inv_a(p, x, [&](int n1, double2 v) {
+                const int r = p.n2 + N2 * n1;
+                if (p.ok && r < nr) put(A.out[1], p.o0 + (unsigned)(N2 * n1) * ldb, cscale(v, 1.0 / zl[r]));
             });
         }
-        __syncthreads();
-    }
-    // ---- Z: u_T = Z / r, w_Q = l(l+1) Z / r^2, forward transform
-    if (inA) {
-        load_rows(A.Z, x);
-        if (okA)
+        if (A.nout == 6) {
+            __syncthreads();
+            // ---- w_T = -(d2W - l(l+1) W / r^2) / r
+            auto d2dr2 = [&](double k, double2 v) { return cscale(v, -k * k * inv_n); };
+            if (inB) {
 #pragma unroll
-            for (int n1 = 0; n1 < N1; n1++) {
-                const int r = n2 + N2 * n1;
-                if (r < nr) {
-                    const double rr = zl[r];
-                    A.out[2][(long)r * A.ld + lma] = cscale(x[n1], 1.0 / rr);
-                    if (A.nout == 6) A.out[3][(long)r * A.ld + lma] = cscale(x[n1], ll / (rr * rr));
-                }
+                for (int q = 0; q < N2; q++) y[q] = d2dr2(kzl[k1 + N1 * q], X[q]);
+                inv_b(cb, k1, y);
             }
-        if (A.nout == 6) fwd_a(x);
-    }
-    if (A.nout == 6) {   // the spectrum of Z only feeds the curl outputs
-        __syncthreads();
-        if (inB) fwd_b(X);
-        __syncthreads();
-        // ---- w_S = dZ / r
-        if (inB) {
-#pragma unroll
-            for (int q = 0; q < N2; q++) {
-                const double k = kzl[k1 + N1 * q] * inv_n;
-                y[q] = make_double2(-k * X[q].y, k * X[q].x);
+            inv_b2(d2dr2);
+            __syncthreads();
+            if (inA) {
+                const PA p = pa();
+                // W at this thread's rows again, each row loaded where its
+                // output is formed (clamped: no branch before the load)
+                inv_a(p, x, [&](int n1, double2 v) {
+                    const int r = p.n2 + N2 * n1;
+                    const unsigned o = p.o0 + (unsigned)(N2 * min(n1, (nr - 1 - p.n2) / N2)) * ldb;
+                    const double2 w = at(A.W, o);
+                    if (p.ok && r < nr) {
+                        const double rr = zl[r], r2 = 1.0 / (rr * rr);
+                        put(A.out[5], o, cscale(make_double2(v.x - ll * w.x * r2, v.y - ll * w.y * r2), -1.0 / rr));
+                    }
+                });
             }
-            inv_b(y);
         }
-        __syncthreads();
-        if (inA)
-            inv_a(x, [&](int n1, double2 v) {
-                const int r = n2 + N2 * n1;
-                if (okA && r < nr) A.out[4][(long)r * A.ld + lma] = cscale(v, 1.0 / zl[r]);
-            });
+    } else {
+        // ---- Z: u_T = Z / r, w_Q = l(l+1) Z / r^2, forward transform
+        if (inA) {
+            const PA p = pa();
+            load_rows(p, x);
+            if (p.ok)
+#pragma unroll
+                for (int n1 = 0; n1 < N1; n1++) {
+                    const int r = p.n2 + N2 * n1;
+                    if (r < nr) {
+                        const double rr = zl[r];
+                        const unsigned o = p.o0 + (unsigned)(N2 * n1) * ldb;
+                        put(A.out[2], o, cscale(x[n1], 1.0 / rr));
+                        if (A.nout == 6) put(A.out[3], o, cscale(x[n1], ll / (rr * rr)));
+                    }
+                }
+            if (A.nout == 6) fwd_a(p, x);
+        }
+        if (A.nout == 6) {   // the spectrum of Z only feeds the curl outputs
+            __syncthreads();
+            if (inB) fwd_b(cb, k1, X);
+            fwd_b2();
+            __syncthreads();
+            // ---- w_S = dZ / r
+            if (inB) {
+#pragma unroll
+                for (int q = 0; q < N2; q++) y[q] = ddr(kzl[k1 + N1 * q], X[q]);
+                inv_b(cb, k1, y);
+            }
+            inv_b2(ddr);
+            __syncthreads();
+            if (inA) {
+                const PA p = pa();
+                inv_a(p, x, [&](int n1, double2 v) {
+                    const int r = p.n2 + N2 * n1;
+                    if (p.ok && r < nr) put(A.out[4], p.o0 + (unsigned)(N2 * n1) * ldb, cscale(v, 1.0 / zl[r]));
+                });
+            }
+        }
     }
 }
 
@@ -2234,17 +2366,19 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     A.C = p->C;
     A.d = p->d;
     // the factorised kernel for the standard FC-Gram matching (d = 5) and the
-    // periodic radius; other (C, d) take the generic kernel
-    const int fd = p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
+    // periodic radius, when byte offsets into the (nlm, nz) arrays fit in 32
+    // bits; other plans take the generic kernel
+    const bool off32 = (long)p->nz * p->nlm * (long)sizeof(double2) < (1L << 32);
+    const int fd = !off32 ? -1 : p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
     switch (fd < 0 ? 0 : radial_factorised(p->nz)) {
 #define RD_F_CASE(NV, F1, F2)                                                                   \
     case NV: {                                                                                  \
-        constexpr int CB = RadFact<F1, F2>::CB;                                                 \
+        constexpr int CB = RadLine<F1, F2>::CB, NT = RadLine<F1, F2>::NT;                       \
         if (fd == 5)                                                                            \
-            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 5>), dim3((p->nlm + CB - 1) / CB), dim3(256), \
+            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 5>), dim3((p->nlm + CB - 1) / CB, 2), dim3(NT), \
                                0, p->stream, A, p->tw_z);                                       \
         else                                                                                    \
-            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 0>), dim3((p->nlm + CB - 1) / CB), dim3(256), \
+            hipLaunchKernelGGL((uxw_radial_fact_kernel<F1, F2, 0>), dim3((p->nlm + CB - 1) / CB, 2), dim3(NT), \
                                0, p->stream, A, p->tw_z);                                       \
         RD_LAUNCH_CHECK();                                                                      \
         return RD_OK;                                                                           \
