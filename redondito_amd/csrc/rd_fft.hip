@@ -63,6 +63,16 @@ __device__ __forceinline__ float2 cmuli(float2 a, int sign)
 {
     return sign > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
+// Streaming (nontemporal) store of one complex value: for outputs read by a
+// later kernel, so that they do not displace lines the kernel (or its
+// neighbours on the XCD) read again.
+__device__ __forceinline__ void st_stream(double2 *p, double2 v)
+{
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    const vd2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (vd2 *)p);
+}
+
 // scalar type and constructor of a complex type
 template <typename V> struct Cx;
 template <> struct Cx<double2> {
@@ -1931,16 +1941,6 @@ __device__ __forceinline__ void fact_load_rows(const double2 *src, long ld, int 
 // is issued at once, without branches); D = 0 for a periodic radius (no
 // continuation).  z, kz and dir are staged in LDS.
 constexpr int kRadMaxC = 32;
-
-// Streaming store of one complex value (the outputs of the radial step are
-// read by the next kernel, not by this one: the nontemporal hint keeps them
-// from displacing the input rows it reads a second time).
-__device__ __forceinline__ void st_stream(double2 *p, double2 v)
-{
-    typedef double vd2 __attribute__((ext_vector_type(2)));
-    const vd2 t = {v.x, v.y};
-    __builtin_nontemporal_store(t, (vd2 *)p);
-}
 
 // Tiling of uxw_radial_fact_kernel: CB lm columns per workgroup, 16 (two whole
 // 128-byte lines of each row) or 32 for the short lengths, one thread per
