@@ -136,15 +136,6 @@ __global__ void ana_tab_oz_kernel(const double *tP, const double *tD, const doub
 }
 
 // ------------------------------------------------------------ operands
-// Wave maxima of 12 values (6 matrices x {re, im}); every lane gets them.
-__device__ __forceinline__ void wave_max12(double (&v)[12])
-{
-#pragma unroll
-    for (int i = 0; i < 12; i++)
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v[i] = fmax(v[i], __shfl_xor(v[i], off));
-}
-
 struct OzBArgs {
     const double2 *in[3];
     int8_t *B;
@@ -154,41 +145,74 @@ struct OzBArgs {
     int lmax, nlat, nth, kth, nplanes, ncols, nmat;
 };
 
-__device__ __forceinline__ double2 mul_i(double2 x) { return make_double2(-x.y, x.x); }
-__device__ __forceinline__ double2 mul_mi(double2 x) { return make_double2(x.y, -x.x); }
-__device__ __forceinline__ double2 neg2(double2 x) { return make_double2(-x.x, -x.y); }
-
-// Writes the first D digits of four consecutive values v[0..3] (complex:
-// columns 2k, 2k+1) of matrix b at K indices kidx..kidx+3, scaled by the
-// column exponents e (re, im); one packed 4-byte store per digit plane and
-// column.
+// The first D digits of four values x[i] 2^-e, packed per digit plane:
+// byte i of w[p] is digit p of value i.  The digits are those of oz_digits
+// (d_p = rint(128 r), r <- 128 r - d_p, ties to even): t = 128 r + 1.5 2^52
+// rounds 128 r to an integer in its low bits, whose low byte is the int8
+// digit, and 128 r - d_p = fma(r, 128, 1.5 2^52 - t) is exact.
 template <int D>
-__device__ __forceinline__ void oz_put4(const OzBArgs &A, int m, int b, int kidx, int k, const double2 (&v)[4],
-                                        const int *e)
+__device__ __forceinline__ void oz_pack4(const double (&x)[4], int e, unsigned (&w)[D])
 {
-    const OzMat M = A.bm[m * 6 + b];
-    unsigned wr[D], wi[D];
+    constexpr double M = 6755399441055744.0;   // 1.5 2^52
 #pragma unroll
-    for (int p = 0; p < D; p++) wr[p] = wi[p] = 0u;
+    for (int p = 0; p < D; p++) w[p] = 0u;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        int8_t dr[D], di[D];
-        oz_digits<D>(v[i].x, e[0], dr);
-        oz_digits<D>(v[i].y, e[1], di);
+        double r = ldexp(x[i], -e);
 #pragma unroll
         for (int p = 0; p < D; p++) {
-            wr[p] |= (unsigned)(uint8_t)dr[p] << (8 * i);
-            wi[p] |= (unsigned)(uint8_t)di[p] << (8 * i);
+            const double t = fma(r, 128.0, M);
+            const unsigned lo = (unsigned)__double2loint(t);
+            r = fma(r, 128.0, M - t);
+            w[p] |= (lo & 0xffu) << (8 * i);
         }
-    }
-#pragma unroll
-    for (int p = 0; p < D; p++) {
-        *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k) * M.K + kidx) = wr[p];
-        *(unsigned *)(A.B + M.off + ((long)p * A.ncols + 2 * k + 1) * M.K + kidx) = wi[p];
     }
 }
 
-__device__ __forceinline__ void oz_max4(double *mx, int b, const double2 (&v)[4], bool swap)
+// Stores the D digit planes of four consecutive K values (kidx..kidx+3) of
+// column col of matrix b.
+template <int D>
+__device__ __forceinline__ void oz_store4(const OzBArgs &A, const OzMat &M, int col, int kidx, const double (&x)[4],
+                                          int e)
+{
+    unsigned w[D];
+    oz_pack4<D>(x, e, w);
+    int8_t *dst = A.B + M.off + (long)col * M.K + kidx;
+#pragma unroll
+    for (int p = 0; p < D; p++) *(unsigned *)(dst + (long)p * A.ncols * M.K) = w[p];
+}
+
+// Operand slicing kernels: lane item (plane k, word w) holds K indices
+// 4w .. 4w+3 of every operand matrix of one order m and emits all their
+// digits from one set of complex loads.  The G lanes of one plane (G = 1 ..
+// 64, the smallest power of two holding its words; more words than 64 are
+// taken in turns) are consecutive, so a wave covers 64 / G planes and every
+// lane has work; the column maxima of a plane are reduced over its G lanes.
+// Digits of a column are written only where a matrix has data; the rest of
+// the [p][column][K] layout (K padding) stays zero from its allocation.
+constexpr int OZ_WPB = 4;   // waves per workgroup
+
+// Group width (lanes per plane) for nw words per column.
+__host__ __device__ inline int oz_group_any(int nw)
+{
+    int g = 1;
+    while (g < nw && g < 64) g <<= 1;
+    return g;
+}
+__device__ __forceinline__ int oz_group(int nw) { return oz_group_any(nw); }
+inline int oz_group_host(int nw) { return oz_group_any(nw); }
+
+// Maxima of the 12 columns (6 matrices x {re, im}) of a plane, over its G lanes.
+__device__ __forceinline__ void oz_group_max12(double (&v)[12], int G)
+{
+#pragma unroll
+    for (int i = 0; i < 12; i++)
+        for (int off = 1; off < G; off <<= 1) v[i] = fmax(v[i], __shfl_xor(v[i], off));
+}
+
+// Running maxima of column pair (2b, 2b + 1) over four complex values; swap
+// for a rotated part (i x or -i x: the re column holds Im x, the im column Re x).
+__device__ __forceinline__ void oz_max4c(double *mx, int b, const double2 (&v)[4], bool swap)
 {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -198,164 +222,193 @@ __device__ __forceinline__ void oz_max4(double *mx, int b, const double2 (&v)[4]
     }
 }
 
-// synthesis operands: one wave per (m, plane k), groups of four rows t of the
-// even and odd l per lane, so that a wave's stores of one digit plane and
-// column are one contiguous run (several planes per wave for the short
-// columns of large m measured slower: 0.355 -> 0.491 ms).  D digit planes.
-template <bool VEC, int D>
-__global__ __launch_bounds__(256) void syn_b_oz_kernel(OzBArgs A)
+// re and im columns of a part of matrix b at kidx: the values v, or i v
+// (rot = +1: re -Im, im Re) or -i v (rot = -1: re Im, im -Re), or -v (rot = 2).
+template <int D>
+__device__ __forceinline__ void oz_put_part(const OzBArgs &A, int m, int b, int k, int kidx, const double2 (&v)[4],
+                                            int rot, const int *es)
 {
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
-    if (k >= A.nplanes) return;
+    const OzMat M = A.bm[m * 6 + b];
+    double re[4], im[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        re[i] = rot == 0 ? v[i].x : rot == 1 ? -v[i].y : rot == -1 ? v[i].y : -v[i].x;
+        im[i] = rot == 0 ? v[i].y : rot == 1 ? v[i].x : rot == -1 ? -v[i].x : -v[i].y;
+    }
+    oz_store4<D>(A, M, 2 * k, kidx, re, es[2 * b]);
+    oz_store4<D>(A, M, 2 * k + 1, kidx, im, es[2 * b + 1]);
+}
+
+// synthesis operands: words w over the row index t of the even and odd l of
+// order m (l = m + 2t + parity).  Matrix b is [part 1; part 2] over K, part 2
+// from K index split on:
+//   b  part 1       part 2
+//   0  Q_e          -
+//   1  Q_o          -
+//   2  S_o          i T_e    (split ro16)
+//   3  S_e          i T_o    (split re16)
+//   4  T_o          -i S_e   (split ro16)
+//   5  T_e          -i S_o   (split re16)
+// One workgroup per (block of OZ_WPB * 64 / G planes, m).
+template <bool VEC, int D>
+__global__ __launch_bounds__(64 * OZ_WPB) void syn_b_oz_kernel(OzBArgs A)
+{
+    const int m = blockIdx.y;
     const int L = A.lmax + 1, re = (L - m + 1) / 2, ro = (L - m) / 2;
     const int re16 = rup(re, 16), ro16 = rup(ro, 16);
-    const long lm0 = (long)m * (2 * A.lmax + 3 - m) / 2 + k * A.ld;   // lm_index(m, m) + plane offset
+    const int nw = (re + 3) / 4, G = oz_group(nw);
+    const int lane = threadIdx.x & 63, g = lane % G;
+    const int k = (blockIdx.x * OZ_WPB + (threadIdx.x >> 6)) * (64 / G) + lane / G;
+    if ((blockIdx.x * OZ_WPB + (threadIdx.x >> 6)) * (64 / G) >= A.nplanes) return;   // whole wave idle
+    const bool kin = k < A.nplanes;
+    const int kc = kin ? k : 0;
+    const long lm0 = (long)m * (2 * A.lmax + 3 - m) / 2 + kc * A.ld;   // (m, m) of plane kc
+    // four rows t0..t0+3 of field f, parity par (zero past their count or off-plane)
     auto load4 = [&](int f, int par, int t0, double2 (&v)[4]) {
         const int nrow = par ? ro : re;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            v[i] = make_double2(0.0, 0.0);
-            if (t0 + i < nrow) v[i] = A.in[f][lm0 + 2 * (t0 + i) + par];
-            if (m == 0) v[i].y = 0.0;
+            const int t = t0 + i;
+            // (clamped to the last row of the parity, or to (m, m) when it has none)
+            double2 x = A.in[f][nrow > 0 ? lm0 + 2 * min(t, nrow - 1) + par : lm0];
+            if (m == 0) x.y = 0.0;
+            v[i] = (kin && t < nrow) ? x : make_double2(0.0, 0.0);
         }
-    };
-    auto rot = [](const double2 (&v)[4], double2 (&o)[4], bool plus_i) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) o[i] = plus_i ? mul_i(v[i]) : mul_mi(v[i]);
     };
     double mx[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) mx[i] = 0.0;
-    for (int t0 = 4 * tid; t0 < re; t0 += 256) {
+    for (int w = g; w < nw; w += G) {
+        double2 qe[4], qo[4];
+        load4(0, 0, 4 * w, qe);
+        load4(0, 1, 4 * w, qo);
+        oz_max4c(mx, 0, qe, false);
+        oz_max4c(mx, 1, qo, false);
+        if (VEC) {
+            double2 se[4], so[4], te[4], to[4];
+            load4(1, 0, 4 * w, se);
+            load4(1, 1, 4 * w, so);
+            load4(2, 0, 4 * w, te);
+            load4(2, 1, 4 * w, to);
+            oz_max4c(mx, 2, so, false); oz_max4c(mx, 2, te, true);
+            oz_max4c(mx, 3, se, false); oz_max4c(mx, 3, to, true);
+            oz_max4c(mx, 4, to, false); oz_max4c(mx, 4, se, true);
+            oz_max4c(mx, 5, te, false); oz_max4c(mx, 5, so, true);
+        }
+    }
+    oz_group_max12(mx, G);
+    int es[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
+    if (!kin) return;
+    for (int w = g; w < nw; w += G) {
+        const int t0 = 4 * w;
         double2 qe[4], qo[4];
         load4(0, 0, t0, qe);
         load4(0, 1, t0, qo);
-        oz_max4(mx, 0, qe, false);
-        oz_max4(mx, 1, qo, false);
+        oz_put_part<D>(A, m, 0, k, t0, qe, 0, es);
+        if (t0 < ro) oz_put_part<D>(A, m, 1, k, t0, qo, 0, es);
         if (VEC) {
             double2 se[4], so[4], te[4], to[4];
             load4(1, 0, t0, se);
             load4(1, 1, t0, so);
             load4(2, 0, t0, te);
             load4(2, 1, t0, to);
-            // |(iX).re| = |X.im|, |(iX).im| = |X.re|, likewise for -iX
-            oz_max4(mx, 2, so, false); oz_max4(mx, 2, te, true);
-            oz_max4(mx, 3, se, false); oz_max4(mx, 3, to, true);
-            oz_max4(mx, 4, to, false); oz_max4(mx, 4, se, true);
-            oz_max4(mx, 5, te, false); oz_max4(mx, 5, so, true);
-        }
-    }
-    wave_max12(mx);
-    int es[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
-    for (int t0 = 4 * tid; t0 < re; t0 += 256) {
-        double2 qe[4], qo[4];
-        load4(0, 0, t0, qe);
-        load4(0, 1, t0, qo);
-        oz_put4<D>(A, m, 0, t0, k, qe, es + 0);
-        if (t0 < ro) oz_put4<D>(A, m, 1, t0, k, qo, es + 2);
-        if (VEC) {
-            double2 se[4], so[4], te[4], to[4], w[4];
-            load4(1, 0, t0, se);
-            load4(1, 1, t0, so);
-            load4(2, 0, t0, te);
-            load4(2, 1, t0, to);
             if (t0 < ro) {
-                oz_put4<D>(A, m, 2, t0, k, so, es + 4);
-                oz_put4<D>(A, m, 4, t0, k, to, es + 8);
-                rot(to, w, true);
-                oz_put4<D>(A, m, 3, re16 + t0, k, w, es + 6);
-                rot(so, w, false);
-                oz_put4<D>(A, m, 5, re16 + t0, k, w, es + 10);
+                oz_put_part<D>(A, m, 2, k, t0, so, 0, es);
+                oz_put_part<D>(A, m, 4, k, t0, to, 0, es);
+                oz_put_part<D>(A, m, 3, k, re16 + t0, to, 1, es);
+                oz_put_part<D>(A, m, 5, k, re16 + t0, so, -1, es);
             }
-            rot(te, w, true);
-            oz_put4<D>(A, m, 2, ro16 + t0, k, w, es + 4);
-            rot(se, w, false);
-            oz_put4<D>(A, m, 4, ro16 + t0, k, w, es + 8);
-            oz_put4<D>(A, m, 3, t0, k, se, es + 6);
-            oz_put4<D>(A, m, 5, t0, k, te, es + 10);
+            oz_put_part<D>(A, m, 2, k, ro16 + t0, te, 1, es);
+            oz_put_part<D>(A, m, 4, k, ro16 + t0, se, -1, es);
+            oz_put_part<D>(A, m, 3, k, t0, se, 0, es);
+            oz_put_part<D>(A, m, 5, k, t0, te, 0, es);
         }
     }
-    if (tid == 0)
+    if (g == 0)
         for (int i = 0; i < 2 * A.nmat; i++) A.eB[(m * 6 + i / 2) * A.ncols + 2 * k + (i & 1)] = es[i];
 }
 
-// analysis operands: one wave per (m, plane k), groups of four northern
-// thetas t per lane; D digit planes.
+// analysis operands: words w over the northern thetas t; x+- = x_north +-
+// x_south of the Fourier coefficients of field x (r, theta, phi):
+//   b  part 1   part 2 (from kth)
+//   0  r+       -
+//   1  r-       -
+//   2  t-       -i p+
+//   3  t+       -i p-
+//   4  -p-      -i t+
+//   5  -p+      -i t-
 template <bool VEC, int D>
-__global__ __launch_bounds__(256) void ana_b_oz_kernel(OzBArgs A)
+__global__ __launch_bounds__(64 * OZ_WPB) void ana_b_oz_kernel(OzBArgs A)
 {
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), m = blockIdx.y, tid = threadIdx.x & 63;
-    if (k >= A.nplanes) return;
+    const int m = blockIdx.y;
     const int L = A.lmax + 1;
-    const long base = ((long)k * L + m) * A.nlat;
-    // x+ and x- of field f at thetas t0..t0+3
+    const int nw = (A.nth + 3) / 4, G = oz_group(nw);
+    const int lane = threadIdx.x & 63, g = lane % G;
+    const int k = (blockIdx.x * OZ_WPB + (threadIdx.x >> 6)) * (64 / G) + lane / G;
+    if ((blockIdx.x * OZ_WPB + (threadIdx.x >> 6)) * (64 / G) >= A.nplanes) return;   // whole wave idle
+    const bool kin = k < A.nplanes;
+    const long base = ((long)(kin ? k : 0) * L + m) * A.nlat;
+    // x+ and x- of field f at thetas t0..t0+3 (zero past nth or off-plane)
     auto load4 = [&](int f, int t0, double2 (&sp)[4], double2 (&sm)[4]) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            double2 n = make_double2(0.0, 0.0), s = n;
-            if (t0 + i < A.nth) {
-                n = A.in[f][base + t0 + i];
-                s = A.in[f][base + A.nlat - 1 - (t0 + i)];
+            const int t = t0 + i, tc = min(t, A.nth - 1);
+            double2 n = A.in[f][base + tc], s = A.in[f][base + A.nlat - 1 - tc];
+            if (m == 0) {
+                n.y = 0.0;
+                s.y = 0.0;
             }
-            if (m == 0) { n.y = 0.0; s.y = 0.0; }
-            sp[i] = make_double2(n.x + s.x, n.y + s.y);
-            sm[i] = make_double2(n.x - s.x, n.y - s.y);
+            const bool ok = kin && t < A.nth;
+            sp[i] = ok ? make_double2(n.x + s.x, n.y + s.y) : make_double2(0.0, 0.0);
+            sm[i] = ok ? make_double2(n.x - s.x, n.y - s.y) : make_double2(0.0, 0.0);
         }
-    };
-    auto map = [](const double2 (&v)[4], double2 (&o)[4], int how) {   // 0: -i x, 1: -x
-#pragma unroll
-        for (int i = 0; i < 4; i++) o[i] = how == 0 ? mul_mi(v[i]) : neg2(v[i]);
     };
     double mx[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) mx[i] = 0.0;
-    for (int t0 = 4 * tid; t0 < A.nth; t0 += 256) {
+    for (int w = g; w < nw; w += G) {
+        double2 rp[4], rm[4];
+        load4(0, 4 * w, rp, rm);
+        oz_max4c(mx, 0, rp, false);
+        oz_max4c(mx, 1, rm, false);
+        if (VEC) {
+            double2 tp[4], tm[4], pp[4], pm[4];
+            load4(1, 4 * w, tp, tm);
+            load4(2, 4 * w, pp, pm);
+            oz_max4c(mx, 2, tm, false); oz_max4c(mx, 2, pp, true);
+            oz_max4c(mx, 3, tp, false); oz_max4c(mx, 3, pm, true);
+            oz_max4c(mx, 4, pm, false); oz_max4c(mx, 4, tp, true);
+            oz_max4c(mx, 5, pp, false); oz_max4c(mx, 5, tm, true);
+        }
+    }
+    oz_group_max12(mx, G);
+    int es[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
+    if (!kin) return;
+    for (int w = g; w < nw; w += G) {
+        const int t0 = 4 * w;
         double2 rp[4], rm[4];
         load4(0, t0, rp, rm);
-        oz_max4(mx, 0, rp, false);
-        oz_max4(mx, 1, rm, false);
+        oz_put_part<D>(A, m, 0, k, t0, rp, 0, es);
+        oz_put_part<D>(A, m, 1, k, t0, rm, 0, es);
         if (VEC) {
             double2 tp[4], tm[4], pp[4], pm[4];
             load4(1, t0, tp, tm);
             load4(2, t0, pp, pm);
-            oz_max4(mx, 2, tm, false); oz_max4(mx, 2, pp, true);
-            oz_max4(mx, 3, tp, false); oz_max4(mx, 3, pm, true);
-            oz_max4(mx, 4, pm, false); oz_max4(mx, 4, tp, true);
-            oz_max4(mx, 5, pp, false); oz_max4(mx, 5, tm, true);
+            oz_put_part<D>(A, m, 2, k, t0, tm, 0, es);
+            oz_put_part<D>(A, m, 2, k, A.kth + t0, pp, -1, es);
+            oz_put_part<D>(A, m, 3, k, t0, tp, 0, es);
+            oz_put_part<D>(A, m, 3, k, A.kth + t0, pm, -1, es);
+            oz_put_part<D>(A, m, 4, k, t0, pm, 2, es);
+            oz_put_part<D>(A, m, 4, k, A.kth + t0, tp, -1, es);
+            oz_put_part<D>(A, m, 5, k, t0, pp, 2, es);
+            oz_put_part<D>(A, m, 5, k, A.kth + t0, tm, -1, es);
         }
     }
-    wave_max12(mx);
-    int es[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) es[i] = oz_exponent(mx[i]);
-    for (int t0 = 4 * tid; t0 < A.nth; t0 += 256) {
-        double2 rp[4], rm[4];
-        load4(0, t0, rp, rm);
-        oz_put4<D>(A, m, 0, t0, k, rp, es + 0);
-        oz_put4<D>(A, m, 1, t0, k, rm, es + 2);
-        if (VEC) {
-            double2 tp[4], tm[4], pp[4], pm[4], w[4];
-            load4(1, t0, tp, tm);
-            load4(2, t0, pp, pm);
-            oz_put4<D>(A, m, 2, t0, k, tm, es + 4);
-            map(pp, w, 0);
-            oz_put4<D>(A, m, 2, A.kth + t0, k, w, es + 4);
-            oz_put4<D>(A, m, 3, t0, k, tp, es + 6);
-            map(pm, w, 0);
-            oz_put4<D>(A, m, 3, A.kth + t0, k, w, es + 6);
-            map(pm, w, 1);
-            oz_put4<D>(A, m, 4, t0, k, w, es + 8);
-            map(tp, w, 0);
-            oz_put4<D>(A, m, 4, A.kth + t0, k, w, es + 8);
-            map(pp, w, 1);
-            oz_put4<D>(A, m, 5, t0, k, w, es + 10);
-            map(tm, w, 0);
-            oz_put4<D>(A, m, 5, A.kth + t0, k, w, es + 10);
-        }
-    }
-    if (tid == 0)
+    if (g == 0)
         for (int i = 0; i < 2 * A.nmat; i++) A.eB[(m * 6 + i / 2) * A.ncols + 2 * k + (i & 1)] = es[i];
 }
 
@@ -758,12 +811,14 @@ int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const 
     B.ncols = S->ncols;
     B.nmat = vec ? 6 : 2;
     const int D = p->oz_digits;
-    const dim3 bgrid((nplanes + 3) / 4, p->L);
+    // planes per workgroup for the longest columns (m = 0)
+    const int ppw = OZ_WPB * 64 / oz_group_host(((p->L + 1) / 2 + 3) / 4);
+    const dim3 bgrid((nplanes + ppw - 1) / ppw, p->L);
 #define RD_SYN_B(DD)                                                                                    \
     if (vec)                                                                                            \
-        hipLaunchKernelGGL((syn_b_oz_kernel<true, DD>), bgrid, dim3(256), 0, p->stream, B);             \
+        hipLaunchKernelGGL((syn_b_oz_kernel<true, DD>), bgrid, dim3(64 * OZ_WPB), 0, p->stream, B);     \
     else                                                                                                \
-        hipLaunchKernelGGL((syn_b_oz_kernel<false, DD>), bgrid, dim3(256), 0, p->stream, B);
+        hipLaunchKernelGGL((syn_b_oz_kernel<false, DD>), bgrid, dim3(64 * OZ_WPB), 0, p->stream, B);
     if (D == 5) { RD_SYN_B(5) } else if (D == 6) { RD_SYN_B(6) } else { RD_SYN_B(7) }
 #undef RD_SYN_B
     RD_LAUNCH_CHECK();
@@ -820,12 +875,13 @@ int launch_analysis_oz(Plan *p, bool vec, int nplanes, const double2 *g0, const 
     B.ncols = S->ncols;
     B.nmat = vec ? 6 : 2;
     const int D = p->oz_digits;
-    const dim3 bgrid((nplanes + 3) / 4, p->L);
+    const int ppw = OZ_WPB * 64 / oz_group_host((p->nth + 3) / 4);
+    const dim3 bgrid((nplanes + ppw - 1) / ppw, p->L);
 #define RD_ANA_B(DD)                                                                                    \
     if (vec)                                                                                            \
-        hipLaunchKernelGGL((ana_b_oz_kernel<true, DD>), bgrid, dim3(256), 0, p->stream, B);             \
+        hipLaunchKernelGGL((ana_b_oz_kernel<true, DD>), bgrid, dim3(64 * OZ_WPB), 0, p->stream, B);     \
     else                                                                                                \
-        hipLaunchKernelGGL((ana_b_oz_kernel<false, DD>), bgrid, dim3(256), 0, p->stream, B);
+        hipLaunchKernelGGL((ana_b_oz_kernel<false, DD>), bgrid, dim3(64 * OZ_WPB), 0, p->stream, B);
     if (D == 5) { RD_ANA_B(5) } else if (D == 6) { RD_ANA_B(6) } else { RD_ANA_B(7) }
 #undef RD_ANA_B
     RD_LAUNCH_CHECK();
