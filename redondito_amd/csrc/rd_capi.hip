@@ -754,27 +754,27 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     }
 #define MARK(i) \
     if (prof) RD_HIP(hipEventRecord(ev[i], p->stream))
+    // the radial step also clears the output rows nr..nz-1
+    double2 *outs[3] = {(double2 *)NQ, (double2 *)NS, (double2 *)NT};
     MARK(0);
-    int rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, p->S);
+    int rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, p->S, 6, outs);
     MARK(1);
-    if (!rc) rc = launch_synthesis(p, prec, true, nr, p->S[0], p->S[1], p->S[2], p->nlm, p->F[0], p->F[1], p->F[2]);
+    if (!rc) {
+        const double2 *su[3] = {p->S[0], p->S[1], p->S[2]}, *sw[3] = {p->S[3], p->S[4], p->S[5]};
+        double2 *fu[3] = {p->F[0], p->F[1], p->F[2]}, *fw[3] = {p->F[3], p->F[4], p->F[5]};
+        rc = launch_synthesis2(p, prec, nr, su, fu, sw, fw, p->nlm);
+    }
     MARK(2);
-    if (!rc) rc = launch_synthesis(p, prec, true, nr, p->S[3], p->S[4], p->S[5], p->nlm, p->F[3], p->F[4], p->F[5]);
-    MARK(3);
     if (!rc) {
         const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
         double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
         rc = launch_phi_cross(p, nr, Fin, Fout, prec);
     }
-    MARK(4);
+    MARK(3);
     if (!rc)
         rc = launch_analysis(p, prec, true, nr, p->F[0], p->F[1], p->F[2], (double2 *)NQ, (double2 *)NS,
                              (double2 *)NT, p->nlm);
-    MARK(5);
-    if (!rc) rc = launch_zero_rows(p, (double2 *)NQ, nr, p->nz - nr);
-    if (!rc) rc = launch_zero_rows(p, (double2 *)NS, nr, p->nz - nr);
-    if (!rc) rc = launch_zero_rows(p, (double2 *)NT, nr, p->nz - nr);
-    MARK(6);
+    MARK(4);
 #undef MARK
     return rc;
 }

@@ -1687,6 +1687,9 @@ struct UxwRadArgs {
     long ld;
     int ncols, n, C, d;
     int nout;   // 6: u and omega; 3: u only (Q, S, T of W, Z; no curl)
+    // arrays whose rows nr..n-1 are cleared (the outputs of the nonlinear
+    // term), or null; the factorised kernel's W workgroups clear their columns
+    double2 *zero[3];
 };
 
 // lm columns per workgroup in the fused radial step.  One column keeps the
@@ -1995,6 +1998,13 @@ __global__ __launch_bounds__((RadLine<N1, N2>::NT), 3) void uxw_radial_fact_kern
     }
     // every thread has a pass-B item when CB N1 >= NT; threads t < NB2 have a
     // second one, (column t % CB, k1 = (NT + t) / CB)
+    if (fld == 0 && A.zero[0])
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            for (int t = tid; t < A.C * CB; t += NT) {
+                const int col = c0 + t % CB;
+                if (col < A.ncols) st_stream(&A.zero[a][(long)(nr + t / CB) * A.ld + col], make_double2(0.0, 0.0));
+            }
     const bool inA = tid < CB * N2, inB = CB * N1 >= NT || tid < CB * N1;
     const int cb = tid % CB, k1 = tid / CB;   // pass B
     const bool inB2 = tid < NB2;
@@ -2348,10 +2358,12 @@ int z_fact_launch(Plan *p, const ZArgs &A, int mode)
     }
 }
 
-int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout)
+int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout,
+                      double2 *const *zero3)
 {
     if (nout != 6 && nout != 3) return RD_ERR_ARG;
     UxwRadArgs A;
+    for (int i = 0; i < 3; i++) A.zero[i] = zero3 ? zero3[i] : nullptr;
     A.W = W;
     A.Z = Z;
     for (int i = 0; i < 6; i++) A.out[i] = i < nout ? qst[i] : nullptr;
@@ -2410,6 +2422,10 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
     }
 #undef RD_U_CASE
     RD_LAUNCH_CHECK();
+    for (int i = 0; zero3 && i < 3; i++) {
+        const int rc = launch_zero_rows(p, zero3[i], p->nz - p->C, p->C);
+        if (rc) return rc;
+    }
     return RD_OK;
 }
 

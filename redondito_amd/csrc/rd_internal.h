@@ -112,13 +112,19 @@ struct Plan {
 constexpr int kProfSlots = 512;
 
 // Stages of rd_nonlinear_uxw, in order.
-enum UxwStage { ST_RADIAL = 0, ST_SYN_U, ST_SYN_W, ST_PHI_CROSS, ST_ANALYSIS, ST_TAIL, ST_COUNT };
+// (the u and omega syntheses are one launch; the radial step also clears
+// the output rows past nr)
+enum UxwStage { ST_RADIAL = 0, ST_SYN, ST_PHI_CROSS, ST_ANALYSIS, ST_COUNT };
 
 // ---------------------------------------------------------------- kernels
 // Legendre contractions over all planes (Fourier layout F[k][m][theta]);
 // prec is RD_PREC_FP64, RD_PREC_FP32, RD_PREC_BF16X3 or RD_PREC_OZAKI.
 int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q, const double2 *s,
                      const double2 *t, long ld, double2 *f0, double2 *f1, double2 *f2);
+// Two vector syntheses (field sets a and b) in one launch where the form has
+// the merged kernel (FP64, FP32), else one after the other.
+int launch_synthesis2(Plan *p, int prec, int nplanes, const double2 *const *in_a, double2 *const *out_a,
+                      const double2 *const *in_b, double2 *const *out_b, long ld);
 int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0, const double2 *g1,
                     const double2 *g2, double2 *q, double2 *s, double2 *t, long ld);
 int build_legendre_tables(Plan *p);
@@ -152,7 +158,10 @@ int launch_z_inverse(Plan *p, double2 *a);
 int launch_derivr_r(Plan *p, int order, const double2 *a, double2 *b);
 // Fused radial step of the nonlinear term: W, Z -> the six (Q,S,T) arrays of
 // u and omega (nout = 6), or the three of u only (nout = 3).
-int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout = 6);
+// zero3: three (nlm, nz) arrays whose rows nr..nz-1 the kernel clears
+// (the continuation rows of the nonlinear term's outputs), or null.
+int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *const *qst, int nout = 6,
+                      double2 *const *zero3 = nullptr);
 // Radial derivative of order p of ncols columns with plane stride ld.
 int launch_derivr_cols(Plan *p, int order, const double2 *a, double2 *b, int ncols, long ld);
 

@@ -321,8 +321,13 @@ struct SynArgs {
     const int *queue;
     const double2 *in[3];
     double2 *out[3];
+    // a second field set transformed by the same launch (nset = 2): the
+    // one-problem kernel runs both sets' plane tiles of one (m, theta tile)
+    // entry back to back, so the entry's table rows serve both from L2
+    const double2 *in2[3];
+    double2 *out2[3];
     long ld;
-    int lmax, nlat, nth, nthp, nplanes, n_jt;
+    int lmax, nlat, nth, nthp, nplanes, n_jt, nset;
 };
 
 // One synthesis tile (m, theta tile tt, plane tile jt) of one problem.
@@ -513,21 +518,37 @@ __global__ __launch_bounds__(256, 4) void synthesis_kernel(SynArgs<T> A)
 {
     using Cfg = SynCfg<VEC, syn_jb<T, VEC>()>;
     __shared__ T Bs[Cfg::NF][SKC][Cfg::JTP];
-    // per-XCD queue entry (m, theta tile); the plane tiles of one entry run
-    // back to back on one XCD, so its table rows are read from HBM once
+    // per-XCD queue entry (m, theta tile); the plane tiles of one entry (of
+    // every field set) run back to back on one XCD, so its table rows are
+    // read from HBM once
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
-    const int jt = i % A.n_jt;
-    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    const int nj = A.n_jt * A.nset, jts = i % nj;
+    const int ent = A.queue[(i / nj) * kXcds + xcd];
     if (ent < 0) return;
-    syn_tile<T, VEC, Cfg::JB>(A, ent & 0xffff, ent >> 16, jt, Bs);
+    SynArgs<T> S = A;
+    if (jts >= A.n_jt)
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            S.in[f] = A.in2[f];
+            S.out[f] = A.out2[f];
+        }
+    syn_tile<T, VEC, Cfg::JB>(S, ent & 0xffff, ent >> 16, jts % A.n_jt, Bs);
 }
 
+// Synthesis of the field set (q, s, t) -> (f0, f1, f2), and of the second
+// set (in2 -> out2) in the same launch when in2 is given (vector form only).
 template <typename T>
 static int launch_synthesis_t(Plan *p, const T *tP, const T *tD, const T *tM, bool vec, int nplanes,
                               const double2 *q, const double2 *s, const double2 *t, long ld, double2 *f0,
-                              double2 *f1, double2 *f2)
+                              double2 *f1, double2 *f2, const double2 *const *in2 = nullptr,
+                              double2 *const *out2 = nullptr)
 {
-    SynArgs<T> A;
+    SynArgs<T> A = {};
+    A.nset = (vec && in2) ? 2 : 1;
+    for (int f = 0; f < 3; f++) {
+        A.in2[f] = A.nset == 2 ? in2[f] : nullptr;
+        A.out2[f] = A.nset == 2 ? out2[f] : nullptr;
+    }
     A.tP = tP;
     A.tD = tD;
     A.tM = tM;
@@ -547,7 +568,7 @@ static int launch_synthesis_t(Plan *p, const T *tP, const T *tD, const T *tM, bo
     A.queue = p->d_syn_q;
     const int JT = vec ? SynCfg<true, syn_jb<T, true>()>::JT : SynCfg<false, syn_jb<T, false>()>::JT;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
-    dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
+    dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt * A.nset));
     if (vec)
         hipLaunchKernelGGL((synthesis_kernel<T, true>), grid, dim3(256), 0, p->stream, A);
     else
@@ -569,6 +590,23 @@ int launch_synthesis(Plan *p, int prec, bool vec, int nplanes, const double2 *q,
         return launch_synthesis_t<float>(p, p->syn32_P, p->syn32_D, p->syn32_M, vec, nplanes, q, s, t, ld, f0,
                                          f1, f2);
     return launch_synthesis_bf3(p, vec, nplanes, q, s, t, ld, f0, f1, f2);
+}
+
+int launch_synthesis2(Plan *p, int prec, int nplanes, const double2 *const *in_a, double2 *const *out_a,
+                      const double2 *const *in_b, double2 *const *out_b, long ld)
+{
+    if (prec == RD_PREC_FP64)
+        return launch_synthesis_t<double>(p, p->syn_P, p->syn_D, p->syn_M, true, nplanes, in_a[0], in_a[1], in_a[2],
+                                          ld, out_a[0], out_a[1], out_a[2], in_b, out_b);
+    if (prec == RD_PREC_FP32) {
+        int rc = ensure_precision_tables(p, prec);
+        if (rc) return rc;
+        return launch_synthesis_t<float>(p, p->syn32_P, p->syn32_D, p->syn32_M, true, nplanes, in_a[0], in_a[1],
+                                         in_a[2], ld, out_a[0], out_a[1], out_a[2], in_b, out_b);
+    }
+    int rc = launch_synthesis(p, prec, true, nplanes, in_a[0], in_a[1], in_a[2], ld, out_a[0], out_a[1], out_a[2]);
+    if (!rc) rc = launch_synthesis(p, prec, true, nplanes, in_b[0], in_b[1], in_b[2], ld, out_b[0], out_b[1], out_b[2]);
+    return rc;
 }
 
 // -------------------------------------------------------------- analysis

@@ -164,7 +164,9 @@ class FCPlan:
         lib().rd_plan_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
         return self._h
 
-    STAGES = ("radial", "synthesis_u", "synthesis_w", "phi_cross", "analysis", "tail")
+    # (the u and omega syntheses are one launch; the radial step also clears
+    # the output rows past nr)
+    STAGES = ("radial", "synthesis", "phi_cross", "analysis")
 
     def set_ozaki_digits(self, digits):
         """Digits per operand (7, 6 or 5) of this plan's later RD_PREC_OZAKI calls."""
