@@ -447,7 +447,8 @@ def roofline(form, per_stage, nlat, nlm, kernel=None):
     Fm = 16.0 * nlat * L * NR               # one Fourier field (m <= lmax)
     leg_flops = 10.0 * nlat * nlm * NR      # one vector synthesis or analysis
     alg = {
-        "radial": ("hbm", 8 * Sb),
+        # reads W, Z, writes six (Q,S,T) arrays, clears the outputs' rows past nr
+        "radial": ("hbm", 8 * Sb + 3 * 16.0 * nlm * C),
         "synthesis": ("mfma", 2 * leg_flops),   # u and omega in one launch
         "phi_cross": ("hbm", 9 * Fm),
         "analysis": ("mfma", leg_flops),
@@ -539,7 +540,7 @@ def main():
     L = LMAX + 1
     Sb, Fm = 16.0 * nlm * NR, 16.0 * nlat * L * NR
     bw, leg = HBM_PEAK_GBS * 1e9, 10.0 * nlat * nlm * NR / (MFMA_PEAK_TFLOPS[head] * 1e12)
-    t_roof = (8 * Sb / bw + 2 * max(leg, (3 * Sb + 3 * Fm) / bw) + 9 * Fm / bw
+    t_roof = ((8 * Sb + 48.0 * nlm * C) / bw + 2 * max(leg, (3 * Sb + 3 * Fm) / bw) + 9 * Fm / bw
               + max(leg, (3 * Fm + 3 * Sb) / bw))
     aux = aux_timings(S, plan, Wd)
     aux["hbm_copy_gbs"] = copy_bandwidth()
