@@ -94,8 +94,9 @@ int rd_plan_sync(rd_plan *plan);
 /* info[0..8] = lmax, nlat, nphi, nlm, nz, C, d, nr, 0. */
 int rd_plan_info(const rd_plan *plan, int *info);
 
-/* Per-stage HIP-event timing of rd_nonlinear_uxw (stages: radial, u
- * synthesis, omega synthesis, phi/cross step, analysis, tail), recorded on
+/* Per-stage HIP-event timing of rd_nonlinear_uxw (stages: radial step,
+ * synthesis of u and omega (one launch in the FP64 and FP32 forms),
+ * phi/cross step, analysis), recorded on
  * the plan's stream without host synchronisation and read back by
  * rd_plan_stage_times (which waits).  No counterpart in the reference, whose
  * gtimer handles (src/utils/gtimer_mod.f90) time host wall clock only.
