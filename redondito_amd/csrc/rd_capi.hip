@@ -9,6 +9,32 @@
 
 using namespace rd;
 
+namespace rd {
+// Smallest primitive root of a prime n: g^((n-1)/f) != 1 for every prime f | n-1.
+int primitive_root(int n)
+{
+    std::vector<int> fs;
+    for (int m = n - 1, f = 2; m > 1; f++)
+        if (m % f == 0) {
+            fs.push_back(f);
+            while (m % f == 0) m /= f;
+        }
+    auto powmod = [n](long b, long e) {
+        long r = 1;
+        b %= n;
+        for (; e; e >>= 1, b = b * b % n)
+            if (e & 1) r = r * b % n;
+        return r;
+    };
+    for (int g = 2; g < n; g++) {
+        bool ok = true;
+        for (int f : fs) ok = ok && powmod(g, (n - 1) / f) != 1;
+        if (ok) return g;
+    }
+    return 0;
+}
+}  // namespace rd
+
 namespace {
 
 inline bool bad_plan(const rd_plan *p) { return p == nullptr; }
@@ -45,30 +71,6 @@ int largest_prime_factor(int n)
             n /= f;
         }
     return lp;
-}
-
-// Smallest primitive root of a prime n: g^((n-1)/f) != 1 for every prime f | n-1.
-int primitive_root(int n)
-{
-    std::vector<int> fs;
-    for (int m = n - 1, f = 2; m > 1; f++)
-        if (m % f == 0) {
-            fs.push_back(f);
-            while (m % f == 0) m /= f;
-        }
-    auto powmod = [n](long b, long e) {
-        long r = 1;
-        b %= n;
-        for (; e; e >>= 1, b = b * b % n)
-            if (e & 1) r = r * b % n;
-        return r;
-    };
-    for (int g = 2; g < n; g++) {
-        bool ok = true;
-        for (int f : fs) ok = ok && powmod(g, (n - 1) / f) != 1;
-        if (ok) return g;
-    }
-    return 0;
 }
 
 // The radial FFT of length nz: direct Stockham stages, or the Rader form for

@@ -1814,6 +1814,13 @@ template <> struct DftConst<17> {
                                      -0.3612416661871529};
 };
 
+template <> struct DftConst<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494745, -0.8090169943749475, -0.8090169943749475,
+                                    0.30901699437494745};
+    static constexpr double s[5] = {0.0, 0.9510565162951535, 0.5877852522924731, -0.5877852522924731,
+                                    -0.9510565162951535};
+};
+
 // In-register DFT of odd length R, X[k] = sum_n x[n] e^{sign 2 pi i n k / R},
 // pairing n with R - n (a_j = x_j + x_{R-j}, b_j = x_j - x_{R-j}):
 //   X[k]   = x_0 + sum_j a_j cos(2 pi j k / R) + sign i sum_j b_j sin(2 pi j k / R)
@@ -2330,10 +2337,519 @@ __global__ __launch_bounds__(256) void z_fact_kernel(ZArgs A, int mode, const do
     }
 }
 
+// ------------------------------ Rader radial step, prime n = RA RB + 1 (281)
+// For a prime radial length the transform is Rader's: with g a primitive root
+// of n, X[g^-q] = x_0 + conv[q], conv the cyclic convolution of length n - 1
+// of a_p = x[g^p] with c_m = e^{s 2 pi i g^-m / n} (the generic kernels do
+// the same in LDS Stockham stages).  Here every convolution runs as register
+// passes over n - 1 = RA RB, and the Rader orderings are folded into the HBM
+// loads and stores, so that LDS carries only the two transposes of each
+// convolution:
+//   forward (s = -1): a thread holds a_p = x[g^p] for p = n2 + RB n1 and ends
+//     with X at the rows g^-p;
+//   inverse (s = +1): from b_p = S[g^-p], the same thread ends with
+//     Y[g^p] = S_0 + conv'[p], conv' = b * h, h_d = e^{+2 pi i g^d / n}, whose
+//     kernel spectrum is cf_+ read backwards;
+// so the inverse of a spectrum formed in place lands on the rows the thread
+// loaded, and row 0 (X[0] = x_0 + sum a, Y[0] = S_0 + sum b) comes from the
+// DC term of the convolution's own transform.
+
+// Prime-factor (Good-Thomas) register DFTs of R = P Q points, P and Q
+// coprime: input x[(Q a + P b) mod R], output X[(Q AL c + P BE d) mod R] with
+// AL = Q^-1 mod P and BE = P^-1 mod Q, so the Q-point DFTs over b and the
+// P-point DFTs over a need no twiddles.  X[k] = sum_n x[n] e^{sign 2 pi i n k / R}.
+template <int R> struct Pfa;
+template <> struct Pfa<14> {
+    static constexpr int P = 2, Q = 7, AL = 1, BE = 4;
+};
+template <> struct Pfa<20> {
+    static constexpr int P = 4, Q = 5, AL = 1, BE = 4;
+};
+
+template <int R, typename E>
+__device__ __forceinline__ void dft_pfa_emit(const double2 (&x)[R], int sign, E &&emit)
+{
+    constexpr int P = Pfa<R>::P, Q = Pfa<R>::Q, AL = Pfa<R>::AL, BE = Pfa<R>::BE;
+    static_assert(P * Q == R && (Q * AL) % P == 1 && (P * BE) % Q == 1, "prime-factor map");
+    double2 u[P][Q];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+#pragma unroll
+        for (int b = 0; b < Q; b++) u[a][b] = x[(Q * a + P * b) % R];
+        dft_odd<Q>(u[a], sign);
+    }
+    // each output handed to emit(k, X[k]) as soon as its P-point DFT is done
+#pragma unroll
+    for (int d = 0; d < Q; d++) {
+        if constexpr (P == 2) {
+            const double2 s0 = cadd(u[0][d], u[1][d]), s1 = csub(u[0][d], u[1][d]);
+            u[0][d] = s0;
+            u[1][d] = s1;
+        } else {
+            dft4_inplace(u[0][d], u[1][d], u[2][d], u[3][d], sign);
+        }
+#pragma unroll
+        for (int c = 0; c < P; c++) emit((Q * AL * c + P * BE * d) % R, u[c][d]);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_pfa(double2 (&x)[R], int sign)
+{
+    dft_pfa_emit<R>(x, sign, [&](int k, double2 y) { x[k] = y; });
+}
+
+// An index that the compiler cannot compute before the value y exists: LDS
+// reads addressed by it are issued where y is used, not hoisted above the
+// DFT that forms y (where they would hold their registers beside its values).
+__device__ __forceinline__ int after(int i, double2 y)
+{
+    asm volatile("" : "+v"(i) : "v"(y.x));
+    return i;
+}
+
+constexpr int rd_powmod(int b, int e, int m)
+{
+    long r = 1;
+    for (int i = 0; i < e; i++) r = r * b % m;
+    return (int)r;
+}
+// smallest primitive root of a prime n (the rule of the host's tables)
+constexpr int rd_primroot(int n)
+{
+    for (int g = 2; g < n; g++) {
+        bool ok = true;
+        int m = n - 1;
+        for (int f = 2; f <= m; f++)
+            if (m % f == 0) {
+                if (rd_powmod(g, (n - 1) / f, n) == 1) ok = false;
+                while (m % f == 0) m /= f;
+            }
+        if (ok) return g;
+    }
+    return 0;
+}
+
+// Tiling of the Rader kernels: CB lm columns per workgroup (one 128-byte line
+// of each row), one pass-A item (column, n2) per thread, pass-B items
+// (column, k1) on the first CB RA threads; 53 KB of LDS, three workgroups per
+// CU.
+template <int RA, int RB>
+struct RaderLine {
+    static constexpr int N1 = RA * RB, N = N1 + 1;
+    static constexpr int CB = 8;
+    static constexpr int NT = CB * RB;
+    static constexpr int G = rd_primroot(N), GI = rd_powmod(G, N - 2, N);
+    static constexpr int GR = rd_powmod(G, RB, N), GIR = rd_powmod(GI, RB, N);
+    static constexpr int NTW = (RA - 1) * (RB - 1) + 1;   // twiddles e^{-2 pi i t / N1}, t = n2 k1
+    static_assert(RB <= 32, "rows() takes n2 < 32");
+};
+
+// LDS of the Rader kernels: the transpose buffer [RA][RB][CB], the two kernel
+// spectra, the twiddles, the wall rows [2D][CB], the radii and the FC-Gram
+// operator.
+template <int RA, int RB, int D>
+static size_t rader_lds(int C)
+{
+    using L = RaderLine<RA, RB>;
+    const int DW = D > 0 ? D : 1;
+    return sizeof(double2) * (L::N1 * L::CB + 2 * L::N1 + L::NTW + 2 * DW * L::CB) + sizeof(double) * (L::N + C * DW);
+}
+
+struct RaderTabs {
+    const double2 *kf;   // forward: cf_-[k] = DFT of c_m (s = -1) / (n - 1)
+    const double2 *kp;   // inverse: cf_+[k], read at (n - 1 - k) mod (n - 1)
+    const double2 *tw;   // e^{-2 pi i t / (n - 1)}
+};
+
+// One cyclic convolution of length N1 = RA RB, in place on the values
+// v[n1] = x[n2 + RB n1] of the thread's pass-A item (column tid % CB,
+// n2 = tid / CB): conv = IDFT(DFT(x) K), K (in LDS) holding the 1 / N1.
+//   pass A: RA-point DFTs over n1, twiddles e^{-2 pi i n2 k1 / N1}, to
+//     T[k1][n2][column];
+//   pass B (threads < CB RA, item (column, k1 = tid / CB)): RB-point DFTs over
+//     n2 -> k2, times K[k1 + RA k2], RB-point inverse DFTs over k2 -> n2,
+//     twiddles e^{+2 pi i n2 k1 / N1}, back to the slots it read;
+//   pass A': RA-point inverse DFTs over k1 -> conv[n2 + RB n1].
+// Pass A of a following convolution writes only the slots pass A' of the same
+// thread read: two barriers per convolution.  dc: on pass-B items with k1 = 0,
+// the sum of the inputs.
+template <int RA, int RB, int CB>
+__device__ __forceinline__ void rader_conv(double2 (&v)[RA], const double2 *K, double2 *T, const double2 *twl,
+                                           int tid, double2 &dc)
+{
+    const int c = tid % CB, j = tid / CB;
+    dft_pfa_emit<RA>(v, -1, [&](int k1, double2 y) {
+        T[(k1 * RB + j) * CB + c] = k1 == 0 ? y : cmul(y, twl[after(j, y) * k1]);
+    });
+    __syncthreads();
+    if (tid < CB * RA) {
+        double2 w[RB];
+#pragma unroll
+        for (int q = 0; q < RB; q++) w[q] = T[(j * RB + q) * CB + c];
+        dft_pfa_emit<RB>(w, -1, [&](int k2, double2 y) {
+            if (k2 == 0) dc = y;
+            w[k2] = cmul(y, K[after(j, y) + RA * k2]);
+        });
+        dft_pfa_emit<RB>(w, +1, [&](int q, double2 y) {
+            T[(j * RB + q) * CB + c] = q == 0 ? y : cmul(y, cconj(twl[q * after(j, y)]));
+        });
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < RA; k1++) v[k1] = T[(k1 * RB + j) * CB + c];
+    dft_pfa<RA>(v, +1);
+}
+
+// State and helpers shared by the Rader kernels: LDS staging, the rows of a
+// thread's pass-A item, the FC-Gram continuation and the wavenumbers.
+template <int RA, int RB, int D>
+struct RaderCtx {
+    using L = RaderLine<RA, RB>;
+    static constexpr int N = L::N, N1 = L::N1, CB = L::CB, NT = L::NT, DW = D > 0 ? D : 1;
+    double2 *T, *Kf, *Ki, *twl, *wall;   // Kf forward, Ki inverse kernel spectrum
+    double *zl, *dirl;
+    int nr, C, c0, ncols;
+    unsigned ldb, oc;   // bytes per row; byte offset of the thread's column (a valid one for idle lanes)
+    bool ok;            // the thread's column is a real one
+    double dk;          // kz[k] = k' dk, k' = k or k - n
+
+    __device__ RaderCtx(double2 *sm, int tile, const RaderTabs &R, const double2 *src, const double *z,
+                        const double *kz, const double *dir, long ld, int ncols_, int C_)
+    {
+        T = sm;
+        Kf = T + N1 * CB;
+        Ki = Kf + N1;
+        twl = Ki + N1;
+        wall = twl + L::NTW;
+        zl = (double *)(wall + 2 * DW * CB);
+        dirl = zl + N;
+        C = C_;
+        nr = N - C;
+        ncols = ncols_;
+        c0 = tile * CB;
+        ldb = (unsigned)ld * 16u;
+        const int tid = threadIdx.x;
+        for (int t = tid; t < L::NTW; t += NT) twl[t] = R.tw[t];
+        for (int t = tid; t < N1; t += NT) {
+            Kf[t] = R.kf[t];
+            Ki[t] = R.kp[t == 0 ? 0 : N1 - t];
+        }
+        if (z)
+            for (int t = tid; t < N; t += NT) zl[t] = z[t];
+        if (D > 0 && src) {
+            for (int t = tid; t < C * D; t += NT) dirl[t] = dir[t];
+            // [j][column]: j < D row nr - D + j, j >= D row 2 D - 1 - j
+            for (int t = tid; t < 2 * D * CB; t += NT) {
+                const int c = t % CB, j = t / CB;
+                const int row = j < D ? nr - D + j : 2 * D - 1 - j;
+                wall[j * CB + c] = src[(long)row * ld + min(c0 + c, ncols - 1)];
+            }
+        }
+        const int lm = c0 + tid % CB;
+        ok = lm < ncols;
+        oc = (unsigned)(ok ? lm : ncols - 1) * 16u;
+        dk = kz[1];
+    }
+    // r[n1] = base^(n2 + RB n1) mod n for the thread's n2, from an opaque copy
+    // of the thread index (recomputed where used rather than held)
+    template <int BASE, int STEP>
+    __device__ __forceinline__ void rows(int (&r)[RA]) const
+    {
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int j = t / CB;
+        int x = 1, b = BASE;
+#pragma unroll
+        for (int bit = 0; bit < 5; bit++) {
+            if ((j >> bit) & 1) x = x * b % N;
+            b = b * b % N;
+        }
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) {
+            r[n1] = x;
+            x = x * STEP % N;
+        }
+    }
+    __device__ __forceinline__ double kzof(int k) const { return (double)(k <= (N - 1) / 2 ? k : k - N) * dk; }
+    __device__ __forceinline__ unsigned off(int row) const { return (unsigned)row * ldb + oc; }
+    static __device__ __forceinline__ double2 at(const double2 *base, unsigned o)
+    {
+        return *(const double2 *)((const char *)base + o);
+    }
+    static __device__ __forceinline__ void put(double2 *base, unsigned o, double2 v)
+    {
+        st_stream((double2 *)((char *)base + o), v);
+    }
+    // x[g^p] at p = n2 + RB n1 and x_0 of the thread's column: every load from
+    // a row clamped to the physical ones, issued at once, then the rows past
+    // nr replaced by the FC-Gram continuation of the staged walls
+    // (sphp_sca.F90:369-381); zeros for idle lanes
+    __device__ __forceinline__ void load(const double2 *src, double2 (&v)[RA], double2 &x0) const
+    {
+        int r[RA];
+        rows<L::G, L::GR>(r);
+        x0 = at(src, oc);
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) v[n1] = at(src, off(min(r[n1], nr - 1)));
+        const int ca = threadIdx.x % CB;
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) {
+            if (D > 0 && r[n1] >= nr) {
+                const int ii = r[n1] - nr + 1;
+                double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int jj = 0; jj < D; jj++) {
+                    const double a1 = dirl[(ii - 1) + C * jj], a2 = dirl[(C - ii) + C * jj];
+                    const double2 wr = wall[jj * CB + ca], wl = wall[(D + jj) * CB + ca];
+                    s.x += a1 * wr.x + a2 * wl.x;
+                    s.y += a1 * wr.y + a2 * wl.y;
+                }
+                v[n1] = s;
+            }
+            if (!ok) v[n1] = make_double2(0.0, 0.0);
+        }
+        if (!ok) x0 = make_double2(0.0, 0.0);
+    }
+};
+
+// i kz / n times a spectrum value (the first radial derivative)
+__device__ __forceinline__ double2 rader_ddr(double kz, double inv_n, double2 v)
+{
+    const double k = kz * inv_n;
+    return make_double2(-k * v.y, k * v.x);
+}
+
+// The radial step of the nonlinear term (the outputs of uxw_radial_kernel) for
+// a prime n = RA RB + 1, as three kinds of workgroup, each with one forward and
+// one inverse transform of its column tile:
+//   0: W -> u_Q = l(l+1) W / r^2, u_S = dW / r
+//   1: Z -> u_T = Z / r, w_Q = l(l+1) Z / r^2, w_S = dZ / r
+//   2: W -> w_T = -(d2W - l(l+1) W / r^2) / r
+// (with nout = 3 only kinds 0 and 1, and kind 1 without transforms).  Holding
+// W's spectrum for both of its inverse transforms would spill; the kinds of a
+// tile are consecutive workgroups on one XCD, so kind 2 reads W from L2.
+// kz[0] = 0, so the derivative spectra have no row-0 term.
+template <int RA, int RB, int D>
+__global__ __launch_bounds__((RaderLine<RA, RB>::NT), 3) void uxw_radial_rader_kernel(UxwRadArgs A, RaderTabs R)
+{
+    using X = RaderCtx<RA, RB, D>;
+    constexpr int N = X::N, CB = X::CB, NT = X::NT;
+    constexpr int G = X::L::G, GR = X::L::GR, GI = X::L::GI, GIR = X::L::GIR;
+    extern __shared__ double2 sm[];
+    const int nk = A.nout == 6 ? 3 : 2;
+    const int t = xcd_tile(blockIdx.x, gridDim.x), kind = t % nk;
+    const double2 *src = kind == 1 ? A.Z : A.W;
+    const X c(sm, t / nk, R, src, A.z, A.kz, A.dir, A.ld, A.ncols, A.C);
+    const int tid = threadIdx.x;
+    const int nr = c.nr;
+    if (kind == 0 && A.zero[0])
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            for (int e = tid; e < A.C * CB; e += NT) {
+                const int col = c.c0 + e % CB;
+                if (col < A.ncols) st_stream(&A.zero[a][(long)(nr + e / CB) * A.ld + col], make_double2(0.0, 0.0));
+            }
+    double ll;
+    {
+        const int lm = c.c0 + tid % CB;
+        const int ell = A.ell[c.ok ? lm : 0];
+        ll = c.ok ? (double)ell * (ell + 1) : 0.0;
+    }
+    double2 v[RA], x0, dc;
+    int r[RA];
+    __syncthreads();
+    c.load(src, v, x0);
+    // ---- the outputs formed from the loads, at the rows read and at row 0
+    if (kind != 2 && c.ok) {
+        c.template rows<G, GR>(r);
+        auto emit = [&](unsigned o, double2 x, double rr) {
+            if (kind == 0) {
+                X::put(A.out[0], o, cscale(x, ll / (rr * rr)));
+            } else {
+                X::put(A.out[2], o, cscale(x, 1.0 / rr));
+                if (A.nout == 6) X::put(A.out[3], o, cscale(x, ll / (rr * rr)));
+            }
+        };
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++)
+            if (r[n1] < nr) emit(c.off(r[n1]), v[n1], c.zl[r[n1]]);
+        if (tid < CB) emit(c.oc, x0, c.zl[0]);
+    }
+    if (kind == 1 && A.nout != 6) return;
+    // ---- forward transform; the spectrum at the rows g^-p times i kz / n
+    //      (kinds 0, 1) or -kz^2 / n (kind 2); inverse transform
+    rader_conv<RA, RB, CB>(v, c.Kf, c.T, c.twl, tid, dc);
+    c.template rows<GI, GIR>(r);
+    const double inv_n = 1.0 / N;
+#pragma unroll
+    for (int n1 = 0; n1 < RA; n1++) {
+        const double k = c.kzof(r[n1]);
+        const double2 x = cadd(x0, v[n1]);
+        v[n1] = kind == 2 ? cscale(x, -k * k * inv_n) : rader_ddr(k, inv_n, x);
+    }
+    rader_conv<RA, RB, CB>(v, c.Ki, c.T, c.twl, tid, dc);
+    // ---- out = (y - s l(l+1) W / r^2) f: s = 0, f = 1 / r for u_S and w_S;
+    //      s = 1, f = -1 / r for w_T, with W read again at each row
+    double2 *dst = kind == 0 ? A.out[1] : kind == 1 ? A.out[4] : A.out[5];
+    const double sw = kind == 2 ? ll : 0.0, sf = kind == 2 ? -1.0 : 1.0;
+    auto fin = [&](unsigned o, double2 y, double2 w, double rr) {
+        const double r2 = 1.0 / (rr * rr);
+        X::put(dst, o, cscale(make_double2(y.x - sw * w.x * r2, y.y - sw * w.y * r2), sf / rr));
+    };
+    if (tid < CB && c.ok) fin(c.oc, dc, x0, c.zl[0]);
+    c.template rows<G, GR>(r);
+    if (kind == 2) {
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) {
+            const unsigned o = c.off(min(r[n1], nr - 1));
+            const double2 w = X::at(A.W, o);
+            if (c.ok && r[n1] < nr) fin(o, v[n1], w, c.zl[r[n1]]);
+        }
+    } else if (c.ok) {
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++)
+            if (r[n1] < nr) fin(c.off(r[n1]), v[n1], make_double2(0.0, 0.0), c.zl[r[n1]]);
+    }
+}
+
+// Single-field radial transforms of prime length n = RA RB + 1, with the modes
+// and stored rows of z_kernel: 0 forward (continuation, FFT -1, / n), 1
+// inverse (FFT +1), 2 derivative of order A.order >= 1 (forward, (i kz)^p / n,
+// inverse) from A.ain into A.a.  Every one of the n rows is stored; a
+// workgroup reads all rows of its columns before it writes any, so a and ain
+// may be the same array.
+template <int RA, int RB, int D, int MODE>
+__global__ __launch_bounds__((RaderLine<RA, RB>::NT), 3) void z_rader_kernel(ZArgs A, RaderTabs R)
+{
+    constexpr int mode = MODE;
+    using X = RaderCtx<RA, RB, D>;
+    constexpr int N = X::N, CB = X::CB;
+    constexpr int G = X::L::G, GR = X::L::GR, GI = X::L::GI, GIR = X::L::GIR;
+    extern __shared__ double2 sm[];
+    const double2 *src = (mode == 2) ? A.ain : A.a;
+    const X c(sm, xcd_tile(blockIdx.x, gridDim.x), R, mode == 1 ? nullptr : src, nullptr, A.kz, A.dir, A.ld, A.ncols,
+              A.C);
+    const int tid = threadIdx.x;
+    double2 v[RA], x0, dc;
+    int r[RA];
+    __syncthreads();
+    if (mode == 1) {
+        // the spectrum at the rows g^-p, and S_0
+        c.template rows<GI, GIR>(r);
+        x0 = X::at(src, c.oc);
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) v[n1] = X::at(src, c.off(r[n1]));
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) v[n1] = c.ok ? v[n1] : make_double2(0.0, 0.0);
+        rader_conv<RA, RB, CB>(v, c.Ki, c.T, c.twl, tid, dc);
+        c.template rows<G, GR>(r);
+        if (c.ok) {
+#pragma unroll
+            for (int n1 = 0; n1 < RA; n1++) X::put(A.a, c.off(r[n1]), cadd(x0, v[n1]));
+            if (tid < CB) X::put(A.a, c.oc, cadd(x0, dc));
+        }
+        return;
+    }
+    c.load(src, v, x0);
+    rader_conv<RA, RB, CB>(v, c.Kf, c.T, c.twl, tid, dc);
+    const double inv_n = 1.0 / N;
+    c.template rows<GI, GIR>(r);
+    if (mode == 0) {
+        if (c.ok) {
+#pragma unroll
+            for (int n1 = 0; n1 < RA; n1++) X::put(A.a, c.off(r[n1]), cscale(cadd(x0, v[n1]), inv_n));
+            if (tid < CB) X::put(A.a, c.oc, cscale(cadd(x0, dc), inv_n));
+        }
+        return;
+    }
+    // (i kz)^p / n at the rows g^-p
+    const int pm = A.order & 3;
+#pragma unroll
+    for (int n1 = 0; n1 < RA; n1++) {
+        const double k = c.kzof(r[n1]);
+        double kp = inv_n;
+        for (int e = 0; e < A.order; e++) kp *= k;
+        double2 w = cscale(cadd(x0, v[n1]), kp);
+        if (pm == 1) w = make_double2(-w.y, w.x);
+        else if (pm == 2) w = make_double2(-w.x, -w.y);
+        else if (pm == 3) w = make_double2(w.y, -w.x);
+        v[n1] = w;
+    }
+    rader_conv<RA, RB, CB>(v, c.Ki, c.T, c.twl, tid, dc);
+    c.template rows<G, GR>(r);
+    if (c.ok) {
+#pragma unroll
+        for (int n1 = 0; n1 < RA; n1++) X::put(A.a, c.off(r[n1]), v[n1]);
+        if (tid < CB) X::put(A.a, c.oc, dc);
+    }
+}
+
+// The Rader register-pass kernels take n = 281 (n - 1 = 14 x 20) with the
+// host's primitive root and Rader tables, the standard FC-Gram matching
+// (d = 5) or the periodic radius, and byte offsets below 2^32.  Returns the
+// D to launch with, or -1.
+static int rader_fits(const Plan *p, long ncols, long ld)
+{
+    using L = RaderLine<14, 20>;
+    if (p->nz != L::N || !p->zfft.rader || !p->zfft.cf[0] || primitive_root(p->nz) != L::G) return -1;
+    if ((long)p->nz * (ld > ncols ? ld : ncols) * (long)sizeof(double2) >= (1L << 32)) return -1;
+    return p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
+}
+
+static RaderTabs rader_tabs(const Plan *p)
+{
+    RaderTabs R;
+    R.kf = p->zfft.cf[0];
+    R.kp = p->zfft.cf[1];
+    R.tw = p->tw_z;
+    return R;
+}
+
+template <int D, int MODE>
+static int launch_z_rader_m(Plan *p, const ZArgs &A)
+{
+    using L = RaderLine<14, 20>;
+    const size_t shm = rader_lds<14, 20, D>(p->C);
+    const int lrc = allow_lds(z_rader_kernel<14, 20, D, MODE>, shm);
+    if (lrc) return lrc;
+    hipLaunchKernelGGL((z_rader_kernel<14, 20, D, MODE>), dim3((A.ncols + L::CB - 1) / L::CB), dim3(L::NT), shm,
+                       p->stream, A, rader_tabs(p));
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
+template <int D>
+static int launch_z_rader(Plan *p, const ZArgs &A, int mode)
+{
+    return mode == 0 ? launch_z_rader_m<D, 0>(p, A) : mode == 1 ? launch_z_rader_m<D, 1>(p, A)
+                                                                : launch_z_rader_m<D, 2>(p, A);
+}
+
+template <int D>
+static int launch_uxw_rader(Plan *p, const UxwRadArgs &A)
+{
+    using L = RaderLine<14, 20>;
+    const size_t shm = rader_lds<14, 20, D>(p->C);
+    const int lrc = allow_lds(uxw_radial_rader_kernel<14, 20, D>, shm);
+    if (lrc) return lrc;
+    const int nk = A.nout == 6 ? 3 : 2;
+    hipLaunchKernelGGL((uxw_radial_rader_kernel<14, 20, D>), dim3(nk * ((A.ncols + L::CB - 1) / L::CB)), dim3(L::NT),
+                       shm, p->stream, A, rader_tabs(p));
+    RD_LAUNCH_CHECK();
+    return RD_OK;
+}
+
 // The factorised single-field radial transform when n is one of the factor
 // pairs above; returns 1 (nothing launched) otherwise.
 int z_fact_launch(Plan *p, const ZArgs &A, int mode)
 {
+    // prime n = 281: the Rader register passes (a derivative of order 0 keeps
+    // a row-0 term they do not form)
+    {
+        const int rd = rader_fits(p, A.ncols, A.ld);
+        if (rd >= 0 && !(mode == 2 && A.order < 1)) return rd == 5 ? launch_z_rader<5>(p, A, mode) : launch_z_rader<0>(p, A, mode);
+    }
     // the standard FC-Gram matching (d = 5) and the periodic radius; other
     // (C, d) take the Stockham kernel
     const int fd = p->C == 0 ? 0 : (p->d == 5 && p->C <= kRadMaxC ? 5 : -1);
@@ -2400,6 +2916,10 @@ int launch_uxw_radial(Plan *p, const double2 *W, const double2 *Z, double2 *cons
         RD_F_CASE(49, 7, 7)
 #undef RD_F_CASE
     default: break;
+    }
+    {
+        const int rd = off32 ? rader_fits(p, p->nlm, p->nlm) : -1;
+        if (rd >= 0) return rd == 5 ? launch_uxw_rader<5>(p, A) : launch_uxw_rader<0>(p, A);
     }
     size_t shm = sizeof(double2) * ((6 * UTC + 1) * p->nz + kZScalars);
     const dim3 grid((p->nlm + UTC - 1) / UTC), block(256);

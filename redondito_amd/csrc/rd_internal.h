@@ -19,6 +19,8 @@ struct FftSeq {
 };
 
 FftSeq make_fft_seq(int n);
+// smallest primitive root of a prime n (the generator of the Rader tables)
+int primitive_root(int n);
 
 // The radial FFT of length n as the kernels run it.  A prime n whose n - 1
 // has no prime factor above 13 runs in Rader form: X[0] = sum_j x_j and

@@ -4,6 +4,7 @@ FP64 runs against the CPU oracle at sizes the oracle finishes in seconds. These 
 phi-kernel path:
   * radial lengths 49 and 153 take the factorised kernel;
   * radial length 89 (prime) takes the generic Stockham kernel;
+  * radial length 281 (prime, 280 = 14 x 20) takes the Rader register-pass kernels;
   * nphi = 768 takes the fused in-place cross kernel;
   * nphi = 1536 (lmax 300 on a 452 x 1536 grid) takes the fused 1536-point cross and dot kernels.
 Each output group is held to 1e-12 relative L2. Each field is held to 1e-12 relative L2 as well, or to 30x its
@@ -92,6 +93,43 @@ def test_coupled_vs_oracle(lmax, nr, grid):
               for b in H.field_bounds(rr, ff, TOL)]
     assert all(e < bd for e, bd in zip(errs, bounds)), bounds
     assert all(float(np.max(np.abs(g[nr:]))) == 0.0 for g in got)
+
+
+def test_coupled_prime_281():
+    """The coupled step at nr = 256 (radial length 281, prime: the Rader
+    register-pass kernels for the u/omega, B and T radial steps), lmax 13.
+    u x B and u . grad T against the FP64 oracle at 1e-12 per group; u x omega,
+    whose second radial derivative leaves the FP64 oracle itself ~8e-12 from
+    the truth at this radius, against the x87 extended-precision composition
+    (1e-12 jointly, per field at least as close as the oracle within 1.5x,
+    plus 1e-13)."""
+    S = sphp()
+    lmax, nr = 13, 256
+    n = nr + C
+    dz = (RO - RI) / (nr - 1)
+    nlat = 4 * ((3 * (lmax + 1) + 7) // 8)
+    nphi = 2 * nlat
+    plan = S.FCPlan(lmax, nlat, nphi, n, C, D, dz, RI)
+    ins = H.coupled_fields(lmax, nr, n, RI, RO, seed=20261203 + lmax)
+    A, Q = O.load_tables(S.TABLE_DIR, C, D)
+    ref = O.nonlinear_coupled(O.SHT(lmax, nlat, nphi), O.FC(n, C, D, O.fc_dir(A, Q)), RI, dz, *ins)
+    got = run_gpu(plan, ins)
+    assert all(float(np.max(np.abs(g[nr:]))) == 0.0 for g in got)
+    got = [g[:nr] for g in got]
+    ref = [r[:nr] for r in ref]
+    (gu, gb, gt), (ru, rb, rt) = groups(got), groups(ref)
+    j_b, j_t = H.rell2_joint(gb, rb), H.rell2_joint(gt, rt)
+    tru = [t[:nr].astype(np.complex128) for t in
+           O.nonlinear_uxw_ld(lmax, nlat, nphi, A, Q, RI, dz, np.ascontiguousarray(ins[0]),
+                              np.ascontiguousarray(ins[1]))]
+    e_gpu = [H.rell2(g, t) for g, t in zip(gu, tru)]
+    e_orc = [H.rell2(r, t) for r, t in zip(ru, tru)]
+    j_u = H.rell2_joint(gu, tru)
+    print(f"uxB vs oracle {j_b:.2e}, u.gT vs oracle {j_t:.2e}, uxw vs truth {j_u:.2e} "
+          f"(fields {', '.join(f'{a:.2e}' for a in e_gpu)}; oracle {', '.join(f'{b:.2e}' for b in e_orc)})")
+    assert j_b < TOL and j_t < TOL and j_u < TOL
+    assert all(a <= 1.5 * b + 1e-13 for a, b in zip(e_gpu, e_orc)), (e_gpu, e_orc)
+    plan.close()
 
 
 @pytest.fixture(scope="module")
