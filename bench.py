@@ -364,10 +364,29 @@ def batch_uxw_bench(reps):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / reps
 
+    # the baseline the batch replaces: one plan per problem, the problems'
+    # evaluations one after another on one stream
+    single = {}
+    plans = [S.FCPlan(lmax, nlat, nphi, nz, C, D, dz, ri, device=dev) for lmax, nlat, nphi, nz, dz, ri in probs]
+    for name, prec in (("fp64", S.RD_PREC_FP64), ("fp32", S.RD_PREC_FP32)):
+        out1 = [[torch.zeros_like(w) for w in W] for _ in range(3)]
+
+        def run_all():
+            for i, pl in enumerate(plans):
+                S.nonlinear_uxw(pl, W[i], Z[i], out1[0][i], out1[1][i], out1[2][i], prec=prec)
+
+        single[name] = timed(run_all)
+        del out1
+    for pl in plans:
+        pl.close()
+    del plans
+    torch.cuda.empty_cache()
+
     for name, prec in (("fp64", S.RD_PREC_FP64), ("fp32", S.RD_PREC_FP32)):
         b = S.Batch(probs, prec=prec, form="uxw")
         out = [[torch.zeros_like(w) for w in W] for _ in range(3)]
         fixed, tuned, choice = b.tune_uxw(W, Z, *out, reps=2)
+        split = b.uxw_split()
         eager = timed(lambda: b.nonlinear_uxw(W, Z, *out))
         b.capture_uxw(W, Z, *out)
         graph = timed(b.replay)
@@ -375,8 +394,11 @@ def batch_uxw_bench(reps):
         outs[name] = [[x.cpu().numpy() for x in o] for o in out]
         res[name] = {"ms_eager": round(eager, 3), "ms_graph": round(graph, 3), "ms_fixed_tiles": round(fixed, 3),
                      "ms_tuned_tiles": round(tuned, 3),
-                     "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1]},
-                     "evals_per_s": round(len(probs) * 1e3 / min(graph, eager), 1)}
+                     "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1],
+                                      "one_problem_lmax_min": split[0], "one_problem_count": split[1]},
+                     "evals_per_s": round(len(probs) * 1e3 / min(graph, eager), 1),
+                     "ms_single_plans": round(single[name], 3),
+                     "speedup_vs_single_plans": round(single[name] / min(graph, eager), 3)}
         del out
         b.close()
         torch.cuda.empty_cache()

@@ -9,6 +9,13 @@
 // them as parallel branches).  The whole round trip
 // can be captured once into a hipGraph and replayed, and a tuner times the
 // tile / grid choices of the persistent launches and keeps the fastest.
+// In the nonlinear-term form the problems at or above a degree threshold run
+// their own one-problem evaluation instead (the fused kernels of
+// rd_nonlinear_uxw, spread over the lanes), and the persistent launches take
+// the rest: a large problem fills the device on its own and its one-problem
+// kernels (prefetching, both syntheses in one launch) are faster than the
+// persistent tiles, while many small problems gain from sharing launches.  The
+// tuner picks the threshold.
 //
 // Per problem p (the reference's fftp3d_complex_to_real and
 // fftp3d_real_to_complex, src/sphp/sphp_sca.F90:237-270,400-433, applied to
@@ -20,16 +27,29 @@
 //            radial forward FFT (1/nz) -> spec
 #include "rd_internal.h"
 
+#include <algorithm>
+#include <climits>
 #include <vector>
 
 using namespace rd;
 
 constexpr int kLanes = 4;   // streams the per-problem stages are spread over
+// default degree threshold of the one-problem evaluations in the nonlinear-term
+// form: every problem on its own (on the 48-problem benchmark draw, lmax 31 to
+// 255, this is the fastest threshold: 25.8 ms against 30.8 ms with every
+// problem in the persistent launches); the tuner chooses among the batch's own
+// degrees
+constexpr int kSplitDefault = 0;
 
 struct rd_batch {
     int nprob, prec;
     int uxw;                          // the nonlinear-term form (rd_batch_create_uxw)
     std::vector<Plan *> plans;
+    // nonlinear-term form: problems with lmax >= split run one-problem
+    // evaluations (indices `large`), the rest the persistent launches (`small`;
+    // every problem in the round-trip form)
+    int split;
+    std::vector<int> small, large, all;
     hipStream_t stream, own_stream;
     hipStream_t side[kLanes - 1];   // default priority, like every stream here
     hipEvent_t ev_fork, ev_join[kLanes - 1];
@@ -70,6 +90,24 @@ void drop_graph(rd_batch *b)
 
 int nwg(const rd_batch *b) { return b->n_cu * b->wg_per_cu; }
 
+void partition(rd_batch *b)
+{
+    b->small.clear();
+    b->large.clear();
+    b->all.clear();
+    for (int i = 0; i < b->nprob; i++) {
+        b->all.push_back(i);
+        (b->uxw && b->plans[i]->lmax >= b->split ? b->large : b->small).push_back(i);
+    }
+}
+
+std::vector<Plan *> plans_of(const rd_batch *b, const std::vector<int> &ids)
+{
+    std::vector<Plan *> v;
+    for (int i : ids) v.push_back(b->plans[i]);
+    return v;
+}
+
 int bind(rd_batch *b, double *const *spec_out)
 {
     const int np = b->nprob;
@@ -85,17 +123,18 @@ int bind(rd_batch *b, double *const *spec_out)
     return batch_leg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
 }
 
-// Enqueues f(i) for every problem i with plan i bound to lane i % kLanes:
-// the side streams wait for the batch stream's work so far, and the batch
-// stream waits for all of them afterwards.
+// Enqueues f(i) for every problem i of ids, the j-th of them with its plan
+// bound to lane j % kLanes: the side streams wait for the batch stream's work
+// so far, and the batch stream waits for all of them afterwards.
 template <typename Fn>
-int per_problem(rd_batch *b, Fn &&f)
+int per_problem(rd_batch *b, const std::vector<int> &ids, Fn &&f)
 {
+    if (ids.empty()) return RD_OK;
     RD_HIP(hipEventRecord(b->ev_fork, b->stream));
     for (int l = 0; l < kLanes - 1; l++) RD_HIP(hipStreamWaitEvent(b->side[l], b->ev_fork, 0));
     int rc = RD_OK;
-    for (int i = 0; i < b->nprob && !rc; i++) {
-        const int lane = i % kLanes;
+    for (size_t j = 0; j < ids.size() && !rc; j++) {
+        const int i = ids[j], lane = (int)(j % kLanes);
         Plan *p = b->plans[i];
         p->stream = lane == 0 ? b->stream : b->side[lane - 1];
         rc = f(i);
@@ -112,7 +151,7 @@ int per_problem(rd_batch *b, Fn &&f)
 // synchronisation)
 int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
 {
-    int rc = per_problem(b, [&](int i) {
+    int rc = per_problem(b, b->all, [&](int i) {
         Plan *p = b->plans[i];
         RD_HIP(hipMemcpyAsync(p->S[0], spec[i], sizeof(double2) * p->ssize, hipMemcpyDeviceToDevice, p->stream));
         return launch_z_inverse(p, p->S[0]);
@@ -120,7 +159,7 @@ int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
     if (rc) return rc;
     rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
     if (rc) return rc;
-    return per_problem(b, [&](int i) {
+    return per_problem(b, b->all, [&](int i) {
         Plan *p = b->plans[i];
         const double2 *F[1] = {p->F[0]};
         double *f[1] = {grid[i]};
@@ -130,7 +169,7 @@ int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
 
 int enqueue_forward(rd_batch *b, const double *const *grid)
 {
-    int rc = per_problem(b, [&](int i) {
+    int rc = per_problem(b, b->all, [&](int i) {
         Plan *p = b->plans[i];
         const double *f[1] = {grid[i]};
         double2 *F[1] = {p->F[0]};
@@ -139,58 +178,61 @@ int enqueue_forward(rd_batch *b, const double *const *grid)
     if (rc) return rc;
     rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
     if (rc) return rc;
-    return per_problem(b, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
+    return per_problem(b, b->all, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
 }
 
-// The nonlinear term of every problem (rd_nonlinear_uxw per problem): the
-// radial steps, then one persistent vector synthesis over the u and omega
-// sets of all problems, the fused phi transforms and cross products, one
-// persistent vector analysis, and the zeroed continuation rows.
+// The nonlinear term of every problem (rd_nonlinear_uxw per problem).  The
+// problems of `large` run their one-problem evaluation over the lanes; those
+// of `small` run the radial steps (which also clear the output rows
+// nr..nz-1), one persistent vector synthesis over their u and omega sets, the
+// fused phi transforms and cross products, and one persistent vector
+// analysis.
 int bind_uxw(rd_batch *b, double *const *NQ, double *const *NS, double *const *NT)
 {
-    const int np = b->nprob;
+    const int np = (int)b->small.size();
+    if (np == 0) return RD_OK;
     std::vector<const double2 *> sin(np * 6), ain(np * 3);
     std::vector<double2 *> sout(np * 6), aout(np * 3);
-    for (int p = 0; p < np; p++) {
+    for (int j = 0; j < np; j++) {
+        const int p = b->small[j];
         Plan *pl = b->plans[p];
         for (int f = 0; f < 6; f++) {
-            sin[p * 6 + f] = pl->S[f];
-            sout[p * 6 + f] = pl->F[f];
+            sin[j * 6 + f] = pl->S[f];
+            sout[j * 6 + f] = pl->F[f];
         }
-        for (int f = 0; f < 3; f++) ain[p * 3 + f] = pl->F[f];
-        aout[p * 3 + 0] = (double2 *)NQ[p];
-        aout[p * 3 + 1] = (double2 *)NS[p];
-        aout[p * 3 + 2] = (double2 *)NT[p];
+        for (int f = 0; f < 3; f++) ain[j * 3 + f] = pl->F[f];
+        aout[j * 3 + 0] = (double2 *)NQ[p];
+        aout[j * 3 + 1] = (double2 *)NS[p];
+        aout[j * 3 + 2] = (double2 *)NT[p];
     }
-    return batch_vleg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
+    const std::vector<Plan *> pls = plans_of(b, b->small);
+    return batch_vleg_bind(&b->leg, pls.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
 }
 
 int enqueue_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
                 double *const *NS, double *const *NT)
 {
-    int rc = per_problem(b, [&](int i) {
+    int rc = per_problem(b, b->large, [&](int i) {
+        return uxw_evaluate(b->plans[i], b->prec, (const double2 *)W[i], (const double2 *)Z[i], (double2 *)NQ[i],
+                            (double2 *)NS[i], (double2 *)NT[i], nullptr);
+    });
+    if (rc || b->small.empty()) return rc;
+    rc = per_problem(b, b->small, [&](int i) {
         Plan *p = b->plans[i];
-        return launch_uxw_radial(p, (const double2 *)W[i], (const double2 *)Z[i], p->S);
+        double2 *outs[3] = {(double2 *)NQ[i], (double2 *)NS[i], (double2 *)NT[i]};
+        return launch_uxw_radial(p, (const double2 *)W[i], (const double2 *)Z[i], p->S, 6, outs);
     });
     if (rc) return rc;
     rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
     if (rc) return rc;
-    rc = per_problem(b, [&](int i) {
+    rc = per_problem(b, b->small, [&](int i) {
         Plan *p = b->plans[i];
         const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
         double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
         return launch_phi_cross(p, p->nr, Fin, Fout, b->prec);
     });
     if (rc) return rc;
-    rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
-    if (rc) return rc;
-    return per_problem(b, [&](int i) {
-        Plan *p = b->plans[i];
-        int r = launch_zero_rows(p, (double2 *)NQ[i], p->nr, p->nz - p->nr);
-        if (!r) r = launch_zero_rows(p, (double2 *)NS[i], p->nr, p->nz - p->nr);
-        if (!r) r = launch_zero_rows(p, (double2 *)NT[i], p->nr, p->nz - p->nr);
-        return r;
-    });
+    return launch_batch_analysis(&b->leg, b->stream, nwg(b));
 }
 
 bool uxw_args_ok(const rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
@@ -209,8 +251,13 @@ int configure(rd_batch *b, int jb, int wg_per_cu)
     drop_graph(b);
     RD_HIP(hipStreamSynchronize(b->stream));
     batch_leg_free(&b->leg);
-    int rc = batch_leg_init(&b->leg, b->plans.data(), b->nprob, b->prec, jb, b->uxw, b->uxw ? 2 : 1);
-    if (rc) return rc;
+    b->leg = BatchLeg{};
+    partition(b);
+    if (!b->small.empty()) {
+        const std::vector<Plan *> pls = plans_of(b, b->small);
+        int rc = batch_leg_init(&b->leg, pls.data(), (int)pls.size(), b->prec, jb, b->uxw, b->uxw ? 2 : 1);
+        if (rc) return rc;
+    }
     b->jb = jb;
     b->wg_per_cu = wg_per_cu;
     return RD_OK;
@@ -278,9 +325,8 @@ static int batch_create(rd_batch **out, int nprob, const int *lmax, const int *n
         }
     }
     set_stream(b, b->stream);
-    b->jb = uxw ? 1 : 4;
-    b->wg_per_cu = 2;
-    int rc = batch_leg_init(&b->leg, b->plans.data(), nprob, prec, b->jb, uxw, uxw ? 2 : 1);
+    b->split = uxw ? kSplitDefault : INT_MAX;
+    int rc = configure(b, uxw ? 1 : 4, 2);
     if (rc) {
         destroy(b);
         return rc;
@@ -374,18 +420,23 @@ int rd_batch_capture_uxw(rd_batch *b, const double *const *W, const double *cons
     int rc = bind_uxw(b, NQ, NS, NT);
     if (rc) return rc;
     RD_HIP(hipStreamSynchronize(b->stream));
-    const size_t ns = b->leg.arg_bytes_syn * b->nprob * b->leg.nsets, na = b->leg.arg_bytes_ana * b->nprob;
-    RD_HIP(hipMalloc(&b->g_syn_args, ns));
-    RD_HIP(hipMalloc(&b->g_ana_args, na));
-    RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
-    RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
-    RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    const bool leg = !b->small.empty();
+    const size_t ns = b->leg.arg_bytes_syn * b->leg.nprob * b->leg.nsets, na = b->leg.arg_bytes_ana * b->leg.nprob;
+    if (leg) {
+        RD_HIP(hipMalloc(&b->g_syn_args, ns));
+        RD_HIP(hipMalloc(&b->g_ana_args, na));
+        RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
+        RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
+        RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    }
     hipStream_t user = b->stream;
     void *sa = b->leg.d_syn_args, *aa = b->leg.d_ana_args;
     int *ct = b->leg.d_counter;
-    b->leg.d_syn_args = b->g_syn_args;
-    b->leg.d_ana_args = b->g_ana_args;
-    b->leg.d_counter = b->g_counter;
+    if (leg) {
+        b->leg.d_syn_args = b->g_syn_args;
+        b->leg.d_ana_args = b->g_ana_args;
+        b->leg.d_counter = b->g_counter;
+    }
     set_stream(b, b->own_stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeThreadLocal);
@@ -412,13 +463,64 @@ int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *
                       int *choice)
 {
     if (!uxw_args_ok(b, W, Z, NQ, NS, NT) || reps < 1) return RD_ERR_ARG;
-    return time_choices(
-        b, reps,
-        [&]() {
-            int rc = bind_uxw(b, NQ, NS, NT);
-            return rc ? rc : enqueue_uxw(b, W, Z, NQ, NS, NT);
-        },
-        ms_fixed, ms_tuned, choice);
+    auto run = [&]() {
+        int rc = bind_uxw(b, NQ, NS, NT);
+        return rc ? rc : enqueue_uxw(b, W, Z, NQ, NS, NT);
+    };
+    // the degree threshold of the one-problem evaluations, at the default
+    // tiles: none, or every distinct degree of the batch
+    std::vector<int> cand = {INT_MAX};
+    for (Plan *p : b->plans)
+        if (std::find(cand.begin(), cand.end(), p->lmax) == cand.end()) cand.push_back(p->lmax);
+    hipEvent_t e0, e1;
+    RD_HIP(hipEventCreate(&e0));
+    RD_HIP(hipEventCreate(&e1));
+    double best = 1e30, fixed = 0.0;
+    int bs = kSplitDefault, rc = RD_OK;
+    const int cand_def = kSplitDefault;
+    // the default threshold first: its time is the untuned one
+    std::stable_partition(cand.begin(), cand.end(), [&](int v) { return v == cand_def; });
+    if (cand[0] != cand_def) cand.insert(cand.begin(), cand_def);
+    for (size_t c = 0; c < cand.size() && !rc; c++) {
+        b->split = cand[c];
+        rc = configure(b, 1, 2);
+        if (!rc) rc = run();
+        if (rc) break;
+        (void)hipEventRecord(e0, b->stream);
+        for (int r = 0; r < reps && !rc; r++) rc = run();
+        (void)hipEventRecord(e1, b->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (c == 0) fixed = ms / reps;
+        if (ms / reps < best) {
+            best = ms / reps;
+            bs = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->split = bs;
+    // then the tiles of the persistent launches for that threshold
+    rc = time_choices(b, reps, run, nullptr, ms_tuned, choice);
+    if (ms_fixed) *ms_fixed = fixed;
+    return rc;
+}
+
+int rd_batch_split_uxw(rd_batch *b, int lmax_min)
+{
+    if (!b || !b->uxw || lmax_min < 0) return RD_ERR_ARG;
+    b->split = lmax_min;
+    return configure(b, b->jb, b->wg_per_cu);
+}
+
+int rd_batch_get_split_uxw(const rd_batch *b, int *lmax_min, int *n_one)
+{
+    if (!b || !b->uxw || !lmax_min || !n_one) return RD_ERR_ARG;
+    *lmax_min = b->split;
+    *n_one = (int)b->large.size();
+    return RD_OK;
 }
 
 int rd_batch_destroy(rd_batch *b)

@@ -33,6 +33,38 @@ int primitive_root(int n)
     }
     return 0;
 }
+
+// One evaluation of u x omega on the plan's stream: the radial step (which
+// also clears the output rows nr..nz-1), the u and omega syntheses, the phi
+// transforms with the cross product, the analysis.  ev: ST_COUNT + 1 events
+// recorded around the stages, or null.
+int uxw_evaluate(Plan *p, int prec, const double2 *W, const double2 *Z, double2 *NQ, double2 *NS, double2 *NT,
+                 hipEvent_t *ev)
+{
+    const int nr = p->nr;
+#define MARK(i) \
+    if (ev) RD_HIP(hipEventRecord(ev[i], p->stream))
+    double2 *outs[3] = {NQ, NS, NT};
+    MARK(0);
+    int rc = launch_uxw_radial(p, W, Z, p->S, 6, outs);
+    MARK(1);
+    if (!rc) {
+        const double2 *su[3] = {p->S[0], p->S[1], p->S[2]}, *sw[3] = {p->S[3], p->S[4], p->S[5]};
+        double2 *fu[3] = {p->F[0], p->F[1], p->F[2]}, *fw[3] = {p->F[3], p->F[4], p->F[5]};
+        rc = launch_synthesis2(p, prec, nr, su, fu, sw, fw, p->nlm);
+    }
+    MARK(2);
+    if (!rc) {
+        const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
+        double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
+        rc = launch_phi_cross(p, nr, Fin, Fout, prec);
+    }
+    MARK(3);
+    if (!rc) rc = launch_analysis(p, prec, true, nr, p->F[0], p->F[1], p->F[2], NQ, NS, NT, p->nlm);
+    MARK(4);
+#undef MARK
+    return rc;
+}
 }  // namespace rd
 
 namespace {
@@ -743,10 +775,8 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
     if (prec < RD_PREC_FP64 || prec > RD_PREC_OZAKI) return RD_ERR_ARG;
     Plan *p = P(plan);
     if (p->nwork < 6) return RD_ERR_STATE;
-    const int nr = p->nr;
-    const bool prof = p->profiling != 0;
     hipEvent_t *ev = nullptr;
-    if (prof) {
+    if (p->profiling) {
         const int s = p->next_slot;
         p->next_slot = (s + 1) % kProfSlots;
         int hrc = harvest_slot(p, s);
@@ -754,31 +784,8 @@ int rd_nonlinear_uxw(rd_plan *plan, int prec, const double *W, const double *Z, 
         ev = p->ev + (long)s * (ST_COUNT + 1);
         p->slot_used[s] = 1;
     }
-#define MARK(i) \
-    if (prof) RD_HIP(hipEventRecord(ev[i], p->stream))
-    // the radial step also clears the output rows nr..nz-1
-    double2 *outs[3] = {(double2 *)NQ, (double2 *)NS, (double2 *)NT};
-    MARK(0);
-    int rc = launch_uxw_radial(p, (const double2 *)W, (const double2 *)Z, p->S, 6, outs);
-    MARK(1);
-    if (!rc) {
-        const double2 *su[3] = {p->S[0], p->S[1], p->S[2]}, *sw[3] = {p->S[3], p->S[4], p->S[5]};
-        double2 *fu[3] = {p->F[0], p->F[1], p->F[2]}, *fw[3] = {p->F[3], p->F[4], p->F[5]};
-        rc = launch_synthesis2(p, prec, nr, su, fu, sw, fw, p->nlm);
-    }
-    MARK(2);
-    if (!rc) {
-        const double2 *Fin[6] = {p->F[0], p->F[1], p->F[2], p->F[3], p->F[4], p->F[5]};
-        double2 *Fout[3] = {p->F[0], p->F[1], p->F[2]};
-        rc = launch_phi_cross(p, nr, Fin, Fout, prec);
-    }
-    MARK(3);
-    if (!rc)
-        rc = launch_analysis(p, prec, true, nr, p->F[0], p->F[1], p->F[2], (double2 *)NQ, (double2 *)NS,
-                             (double2 *)NT, p->nlm);
-    MARK(4);
-#undef MARK
-    return rc;
+    return uxw_evaluate(p, prec, (const double2 *)W, (const double2 *)Z, (double2 *)NQ, (double2 *)NS,
+                        (double2 *)NT, ev);
 }
 
 int rd_plan_prepare(rd_plan *plan, int prec)

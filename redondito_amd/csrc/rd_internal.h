@@ -178,6 +178,10 @@ int launch_prodvec(Plan *p, long npts, const double *ar, const double *at, const
 int launch_fc_filter_k(Plan *p, double2 *a);
 int launch_dirichlet(Plan *p, double2 *a);
 int launch_zero_rows(Plan *p, double2 *a, int row0, int nrows);
+// One u x omega evaluation on the plan's stream (rd_nonlinear_uxw without its
+// argument checks); ev: ST_COUNT + 1 events recorded around the stages, or null.
+int uxw_evaluate(Plan *p, int prec, const double2 *W, const double2 *Z, double2 *NQ, double2 *NS, double2 *NT,
+                 hipEvent_t *ev);
 // Wall value of every lm column from its normal derivative (boun 5: z(1),
 // 6: z(nz - C)), Neumann weights neu (d values), Robin coefficients coef
 // (nlm reals, device) or NULL.

@@ -249,34 +249,45 @@ def _uxw_inputs(probs, seed):
 
 
 def test_batch_nonlinear_uxw_matches_single_plans():
-    """The nonlinear-term form of the batch (u x omega of every problem, the
-    Legendre stage of all problems in one persistent launch per direction)
+    """The nonlinear-term form of the batch (u x omega of every problem: the
+    problems below the degree threshold share one persistent Legendre launch
+    per direction, those at or above it run their one-problem evaluation)
     equals rd_nonlinear_uxw of each problem's own plan bit for bit (same tile
-    arithmetic, same accumulation order), in FP64 and in the FP32 form; its
-    hipGraph replays equal the eager result, with eager work in between; the
-    tuner changes the schedule, not the result."""
+    arithmetic, same accumulation order), in FP64 and in the FP32 form, with
+    every problem persistent, every problem on its own, and split between
+    the two; its hipGraph replays equal the eager result, with eager work in
+    between; the tuner changes the schedule, not the result."""
     from redondito_amd import sphp as S
     probs = problem_list(UXW_PROBLEMS)
     ins = _uxw_inputs(probs, 20261205)
     W = [w for w, _ in ins]
     Z = [z for _, z in ins]
     for prec in (S.RD_PREC_FP64, S.RD_PREC_FP32):
-        b = S.Batch(probs, prec=prec, form="uxw")
-        info = b.info()
-        assert info["jb"] == 1 and info["syn_items"] > 0
-        out = [specs(probs) for _ in range(3)]
-        b.nonlinear_uxw(W, Z, *out)
-        torch.cuda.synchronize()
+        refs = []
         for i, p in enumerate(probs):
             plan = S.FCPlan(p[0], p[1], p[2], p[3], 25, 5, p[4], p[5])
             ref = [torch.full_like(W[i], 3.0) for _ in range(3)]
             S.nonlinear_uxw(plan, W[i], Z[i], *ref, prec=prec)
             torch.cuda.synchronize()
-            d = max((a[i] - r).abs().max().item() for a, r in zip(out, ref))
-            print(f"prec {prec} problem {p[:4]}: max |batch - single| {d:.1e}")
-            assert all(torch.equal(a[i], r) for a, r in zip(out, ref)), (prec, p[:4], d)
+            refs.append(ref)
             plan.close()
-        # graph: replays equal the eager result, with eager work between them
+        b = S.Batch(probs, prec=prec, form="uxw")
+        assert b.info()["jb"] == 1 and b.uxw_split() == (0, len(probs))
+        # all on their own (the default), none, and lmax >= 63 (3 of 6)
+        for split, n_one in ((0, len(probs)), (2**31 - 1, 0), (63, 3)):
+            b.split_uxw(split)
+            assert b.uxw_split() == (split, n_one)
+            # the persistent launches hold work items only for the others
+            assert (b.info()["syn_items"] > 0) == (n_one < len(probs))
+            out = [specs(probs) for _ in range(3)]
+            b.nonlinear_uxw(W, Z, *out)
+            torch.cuda.synchronize()
+            for i, p in enumerate(probs):
+                d = max((a[i] - r).abs().max().item() for a, r in zip(out, refs[i]))
+                print(f"prec {prec} split {split} problem {p[:4]}: max |batch - single| {d:.1e}")
+                assert all(torch.equal(a[i], r) for a, r in zip(out, refs[i])), (prec, split, p[:4], d)
+        # graph of the split schedule: replays equal the eager result, with
+        # eager work between them
         g = [specs(probs) for _ in range(3)]
         b.capture_uxw(W, Z, *g)
         assert b.info()["captured"] == 1
@@ -287,16 +298,17 @@ def test_batch_nonlinear_uxw_matches_single_plans():
                     x.zero_()
             b.replay()
             torch.cuda.synchronize()
-            assert all(torch.equal(x, y) for a, r in zip(g, out) for x, y in zip(a, r)), rep
+            assert all(torch.equal(x[i], y) for i in range(len(probs)) for x, y in zip(g, refs[i])), rep
             b.nonlinear_uxw(Z, W, *other)          # eager work on other arrays
             torch.cuda.synchronize()
         fixed, tuned, choice = b.tune_uxw(W, Z, *other, reps=2)
-        print(f"prec {prec}: tune fixed {fixed:.3f} ms tuned {tuned:.3f} ms choice {choice}")
-        assert tuned <= fixed and choice[0] in (1, 2)
+        split = b.uxw_split()
+        print(f"prec {prec}: tune fixed {fixed:.3f} ms tuned {tuned:.3f} ms choice {choice} split {split}")
+        assert choice[0] in (1, 2) and split[0] in (2**31 - 1, 0) + tuple(p[0] for p in probs)
         t2 = [specs(probs) for _ in range(3)]
         b.nonlinear_uxw(W, Z, *t2)
         torch.cuda.synchronize()
-        assert all(torch.equal(x, y) for a, r in zip(t2, out) for x, y in zip(a, r))
+        assert all(torch.equal(x[i], y) for i in range(len(probs)) for x, y in zip(t2, refs[i]))
         with pytest.raises(S.RdError):
             b.inverse(W, grids(probs))             # the round trip is not this form's
         b.close()
