@@ -304,6 +304,7 @@ def batch_bench(reps):
         out.append(torch.empty_like(spec[-1]))
     b = S.Batch(probs, prec=S.RD_PREC_FP32)
     ms_fixed, ms_tuned, choice = b.tune(spec, grid, out, reps=2)
+    split = b.get_split()
 
     def timed(fn):
         fn()
@@ -330,7 +331,8 @@ def batch_bench(reps):
         "problems": len(probs), "prec": "fp32 Legendre", "planes_per_direction": nsyn,
         "round_trip_ms_eager": round(eager, 4), "round_trip_ms_graph": round(graph, 4),
         "round_trip_ms_fixed_tiles": round(ms_fixed, 4), "round_trip_ms_tuned_tiles": round(ms_tuned, 4),
-        "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1]},
+        "tuned_choice": {"plane_tile_blocks": choice[0], "workgroups_per_cu": choice[1],
+                         "one_problem_lmax_min": split[0], "one_problem_count": split[1]},
         "transforms_per_s": round(2 * len(probs) * 1e3 / graph, 1),
         "round_trip_rel_err": float(f"{err:.3e}"),
         "work_items": {"synthesis": info["syn_items"], "analysis": info["ana_items"]},
@@ -386,7 +388,7 @@ def batch_uxw_bench(reps):
         b = S.Batch(probs, prec=prec, form="uxw")
         out = [[torch.zeros_like(w) for w in W] for _ in range(3)]
         fixed, tuned, choice = b.tune_uxw(W, Z, *out, reps=2)
-        split = b.uxw_split()
+        split = b.get_split()
         eager = timed(lambda: b.nonlinear_uxw(W, Z, *out))
         b.capture_uxw(W, Z, *out)
         graph = timed(b.replay)

@@ -404,22 +404,23 @@ int rd_batch_nonlinear_uxw(rd_batch *b, const double *const *W, const double *co
  * (rd_batch_replay launches it on the bound stream). */
 int rd_batch_capture_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
                          double *const *NS, double *const *NT);
-/* As rd_batch_tune, timing evaluations: first the degree threshold of the
- * one-problem evaluations (rd_batch_split_uxw; none, or each distinct lmax of
- * the batch) at the default tiles (1, 2), then the tiles for the threshold
- * kept.  ms_fixed: the default threshold and tiles. */
+/* As rd_batch_tune, timing evaluations; the fixed default is (1, 2). */
 int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *Z, double *const *NQ,
                       double *const *NS, double *const *NT, int reps, double *ms_fixed, double *ms_tuned,
                       int *choice);
-/* Problems with lmax >= lmax_min run their own one-problem evaluation (the
- * kernels of rd_nonlinear_uxw, over the batch's lanes) instead of sharing the
- * persistent launches; 0 (the default): all of them, INT_MAX: none.
- * Results are bit-identical either way.  Drops a captured graph.  No reference
- * routine: a schedule of the batching form. */
-int rd_batch_split_uxw(rd_batch *b, int lmax_min);
+/* Problems with lmax >= lmax_min run their own one-problem chain (the kernels
+ * of rd_3d_inverse / rd_3d_forward at the batch's precision, or of
+ * rd_nonlinear_uxw, over the batch's lanes) instead of sharing the persistent
+ * launches; 0 (the default): all of them, INT_MAX: none.  Results are
+ * bit-identical either way.  The tuners (rd_batch_tune, rd_batch_tune_uxw)
+ * first time this threshold (the default, none, and each distinct lmax of the
+ * batch) at the default tiles, whose time is their ms_fixed, then the tiles
+ * for the threshold kept.  Drops a captured graph.  No reference routine: a
+ * schedule of the batching form. */
+int rd_batch_split(rd_batch *b, int lmax_min);
 /* The threshold in force and the number of problems it sends to one-problem
- * evaluations. */
-int rd_batch_get_split_uxw(const rd_batch *b, int *lmax_min, int *n_one);
+ * chains. */
+int rd_batch_get_split(const rd_batch *b, int *lmax_min, int *n_one);
 
 /* Measured FP64 ceilings of the current device in TFLOP/s: a register-only
  * loop of v_mfma_f64_16x16x4_f64 (the instruction of the FP64 Legendre

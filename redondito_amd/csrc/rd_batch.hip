@@ -9,13 +9,13 @@
 // them as parallel branches).  The whole round trip
 // can be captured once into a hipGraph and replayed, and a tuner times the
 // tile / grid choices of the persistent launches and keeps the fastest.
-// In the nonlinear-term form the problems at or above a degree threshold run
-// their own one-problem evaluation instead (the fused kernels of
-// rd_nonlinear_uxw, spread over the lanes), and the persistent launches take
-// the rest: a large problem fills the device on its own and its one-problem
-// kernels (prefetching, both syntheses in one launch) are faster than the
-// persistent tiles, while many small problems gain from sharing launches.  The
-// tuner picks the threshold.
+// The problems at or above a degree threshold run their own one-problem chain
+// instead (the kernels of the single-plan routines, spread over the lanes), and
+// the persistent launches take the rest: a large problem fills the device on
+// its own and its one-problem kernels (prefetching; in the nonlinear-term form
+// both syntheses in one launch) are faster than the persistent tiles, while
+// many small problems gain from sharing launches.  The tuners pick the
+// threshold.
 //
 // Per problem p (the reference's fftp3d_complex_to_real and
 // fftp3d_real_to_complex, src/sphp/sphp_sca.F90:237-270,400-433, applied to
@@ -34,20 +34,18 @@
 using namespace rd;
 
 constexpr int kLanes = 4;   // streams the per-problem stages are spread over
-// default degree threshold of the one-problem evaluations in the nonlinear-term
-// form: every problem on its own (on the 48-problem benchmark draw, lmax 31 to
-// 255, this is the fastest threshold: 25.8 ms against 30.8 ms with every
-// problem in the persistent launches); the tuner chooses among the batch's own
-// degrees
+// default degree threshold of the one-problem chains: every problem on its own
+// (on the 48-problem benchmark draw, lmax 31 to 255, the fastest threshold of
+// the nonlinear-term form: 25.8 ms against 30.8 ms with every problem in the
+// persistent launches); the tuners choose among the batch's own degrees
 constexpr int kSplitDefault = 0;
 
 struct rd_batch {
     int nprob, prec;
     int uxw;                          // the nonlinear-term form (rd_batch_create_uxw)
     std::vector<Plan *> plans;
-    // nonlinear-term form: problems with lmax >= split run one-problem
-    // evaluations (indices `large`), the rest the persistent launches (`small`;
-    // every problem in the round-trip form)
+    // problems with lmax >= split run one-problem chains (indices `large`),
+    // the rest the persistent launches (`small`)
     int split;
     std::vector<int> small, large, all;
     hipStream_t stream, own_stream;
@@ -97,7 +95,7 @@ void partition(rd_batch *b)
     b->all.clear();
     for (int i = 0; i < b->nprob; i++) {
         b->all.push_back(i);
-        (b->uxw && b->plans[i]->lmax >= b->split ? b->large : b->small).push_back(i);
+        (b->plans[i]->lmax >= b->split ? b->large : b->small).push_back(i);
     }
 }
 
@@ -110,17 +108,21 @@ std::vector<Plan *> plans_of(const rd_batch *b, const std::vector<int> &ids)
 
 int bind(rd_batch *b, double *const *spec_out)
 {
-    const int np = b->nprob;
+    for (int p = 0; p < b->nprob; p++)
+        if (spec_out) b->ana_out[p] = (double2 *)spec_out[p];
+    const int np = (int)b->small.size();
+    if (np == 0) return RD_OK;
     std::vector<const double2 *> sin(np), ain(np);
     std::vector<double2 *> sout(np), aout(np);
-    for (int p = 0; p < np; p++) {
-        sin[p] = b->plans[p]->S[0];
-        sout[p] = b->plans[p]->F[0];
-        ain[p] = b->plans[p]->F[0];
-        aout[p] = spec_out ? (double2 *)spec_out[p] : b->ana_out[p];
+    for (int j = 0; j < np; j++) {
+        const int p = b->small[j];
+        sin[j] = b->plans[p]->S[0];
+        sout[j] = b->plans[p]->F[0];
+        ain[j] = b->plans[p]->F[0];
+        aout[j] = b->ana_out[p];
     }
-    b->ana_out = aout;
-    return batch_leg_bind(&b->leg, b->plans.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
+    const std::vector<Plan *> pls = plans_of(b, b->small);
+    return batch_leg_bind(&b->leg, pls.data(), sin.data(), sout.data(), ain.data(), aout.data(), b->stream);
 }
 
 // Enqueues f(i) for every problem i of ids, the j-th of them with its plan
@@ -148,37 +150,58 @@ int per_problem(rd_batch *b, const std::vector<int> &ids, Fn &&f)
 }
 
 // the inverse chain, enqueued on the batch stream and its lanes (no host
-// synchronisation)
+// synchronisation): the one-problem chains of `large` (rd_3d_inverse's
+// kernels, the batch's precision), then `small` through the persistent
+// synthesis
 int enqueue_inverse(rd_batch *b, const double *const *spec, double *const *grid)
 {
-    int rc = per_problem(b, b->all, [&](int i) {
+    auto copy_z = [&](int i) {
         Plan *p = b->plans[i];
         RD_HIP(hipMemcpyAsync(p->S[0], spec[i], sizeof(double2) * p->ssize, hipMemcpyDeviceToDevice, p->stream));
         return launch_z_inverse(p, p->S[0]);
-    });
-    if (rc) return rc;
-    rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
-    if (rc) return rc;
-    return per_problem(b, b->all, [&](int i) {
+    };
+    auto phi = [&](int i) {
         Plan *p = b->plans[i];
         const double2 *F[1] = {p->F[0]};
         double *f[1] = {grid[i]};
         return launch_phi_inverse(p, 1, p->nz, F, f);
+    };
+    int rc = per_problem(b, b->large, [&](int i) {
+        Plan *p = b->plans[i];
+        int r = copy_z(i);
+        if (!r) r = launch_synthesis(p, b->prec, false, p->nz, p->S[0], nullptr, nullptr, p->nlm, p->F[0], nullptr,
+                                     nullptr);
+        return r ? r : phi(i);
     });
+    if (rc || b->small.empty()) return rc;
+    rc = per_problem(b, b->small, copy_z);
+    if (rc) return rc;
+    rc = launch_batch_synthesis(&b->leg, b->stream, nwg(b));
+    if (rc) return rc;
+    return per_problem(b, b->small, phi);
 }
 
 int enqueue_forward(rd_batch *b, const double *const *grid)
 {
-    int rc = per_problem(b, b->all, [&](int i) {
+    auto phi = [&](int i) {
         Plan *p = b->plans[i];
         const double *f[1] = {grid[i]};
         double2 *F[1] = {p->F[0]};
         return launch_phi_forward(p, 1, p->nz, f, F);
+    };
+    int rc = per_problem(b, b->large, [&](int i) {
+        Plan *p = b->plans[i];
+        int r = phi(i);
+        if (!r) r = launch_analysis(p, b->prec, false, p->nz, p->F[0], nullptr, nullptr, b->ana_out[i], nullptr,
+                                    nullptr, p->nlm);
+        return r ? r : launch_z_forward(p, b->ana_out[i]);
     });
+    if (rc || b->small.empty()) return rc;
+    rc = per_problem(b, b->small, phi);
     if (rc) return rc;
     rc = launch_batch_analysis(&b->leg, b->stream, nwg(b));
     if (rc) return rc;
-    return per_problem(b, b->all, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
+    return per_problem(b, b->small, [&](int i) { return launch_z_forward(b->plans[i], b->ana_out[i]); });
 }
 
 // The nonlinear term of every problem (rd_nonlinear_uxw per problem).  The
@@ -253,6 +276,13 @@ int configure(rd_batch *b, int jb, int wg_per_cu)
     batch_leg_free(&b->leg);
     b->leg = BatchLeg{};
     partition(b);
+    // the one-problem chains of a reduced-precision batch read the plans' own
+    // precision tables: built here, before any capture
+    if (b->prec != RD_PREC_FP64)
+        for (int i : b->large) {
+            const int rc = ensure_precision_tables(b->plans[i], b->prec);
+            if (rc) return rc;
+        }
     if (!b->small.empty()) {
         const std::vector<Plan *> pls = plans_of(b, b->small);
         int rc = batch_leg_init(&b->leg, pls.data(), (int)pls.size(), b->prec, jb, b->uxw, b->uxw ? 2 : 1);
@@ -325,7 +355,7 @@ static int batch_create(rd_batch **out, int nprob, const int *lmax, const int *n
         }
     }
     set_stream(b, b->stream);
-    b->split = uxw ? kSplitDefault : INT_MAX;
+    b->split = kSplitDefault;
     int rc = configure(b, uxw ? 1 : 4, 2);
     if (rc) {
         destroy(b);
@@ -384,6 +414,47 @@ int time_choices(rd_batch *b, int reps, Fn &&run, double *ms_fixed, double *ms_t
         choice[1] = bw;
     }
     return RD_OK;
+}
+
+// The degree threshold of the one-problem chains, at the default tiles (none,
+// or every distinct degree of the batch; the default threshold first, its time
+// being the untuned one), then the persistent tiles for the threshold kept.
+template <typename Fn>
+int tune_all(rd_batch *b, int reps, Fn &&run, double *ms_fixed, double *ms_tuned, int *choice)
+{
+    const int jb0 = b->uxw ? 1 : 4;
+    std::vector<int> cand = {kSplitDefault, INT_MAX};
+    for (Plan *p : b->plans)
+        if (std::find(cand.begin(), cand.end(), p->lmax) == cand.end()) cand.push_back(p->lmax);
+    hipEvent_t e0, e1;
+    RD_HIP(hipEventCreate(&e0));
+    RD_HIP(hipEventCreate(&e1));
+    double best = 1e30, fixed = 0.0;
+    int bs = kSplitDefault, rc = RD_OK;
+    for (size_t c = 0; c < cand.size() && !rc; c++) {
+        b->split = cand[c];
+        rc = configure(b, jb0, 2);
+        if (!rc) rc = run();
+        if (rc) break;
+        (void)hipEventRecord(e0, b->stream);
+        for (int r = 0; r < reps && !rc; r++) rc = run();
+        (void)hipEventRecord(e1, b->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (c == 0) fixed = ms / reps;
+        if (ms / reps < best) {
+            best = ms / reps;
+            bs = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->split = bs;
+    rc = time_choices(b, reps, run, nullptr, ms_tuned, choice);
+    if (ms_fixed) *ms_fixed = fixed;
+    return rc;
 }
 
 }  // namespace
@@ -463,61 +534,25 @@ int rd_batch_tune_uxw(rd_batch *b, const double *const *W, const double *const *
                       int *choice)
 {
     if (!uxw_args_ok(b, W, Z, NQ, NS, NT) || reps < 1) return RD_ERR_ARG;
-    auto run = [&]() {
-        int rc = bind_uxw(b, NQ, NS, NT);
-        return rc ? rc : enqueue_uxw(b, W, Z, NQ, NS, NT);
-    };
-    // the degree threshold of the one-problem evaluations, at the default
-    // tiles: none, or every distinct degree of the batch
-    std::vector<int> cand = {INT_MAX};
-    for (Plan *p : b->plans)
-        if (std::find(cand.begin(), cand.end(), p->lmax) == cand.end()) cand.push_back(p->lmax);
-    hipEvent_t e0, e1;
-    RD_HIP(hipEventCreate(&e0));
-    RD_HIP(hipEventCreate(&e1));
-    double best = 1e30, fixed = 0.0;
-    int bs = kSplitDefault, rc = RD_OK;
-    const int cand_def = kSplitDefault;
-    // the default threshold first: its time is the untuned one
-    std::stable_partition(cand.begin(), cand.end(), [&](int v) { return v == cand_def; });
-    if (cand[0] != cand_def) cand.insert(cand.begin(), cand_def);
-    for (size_t c = 0; c < cand.size() && !rc; c++) {
-        b->split = cand[c];
-        rc = configure(b, 1, 2);
-        if (!rc) rc = run();
-        if (rc) break;
-        (void)hipEventRecord(e0, b->stream);
-        for (int r = 0; r < reps && !rc; r++) rc = run();
-        (void)hipEventRecord(e1, b->stream);
-        if (hipEventSynchronize(e1) != hipSuccess) rc = RD_ERR_HIP;
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (c == 0) fixed = ms / reps;
-        if (ms / reps < best) {
-            best = ms / reps;
-            bs = cand[c];
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    b->split = bs;
-    // then the tiles of the persistent launches for that threshold
-    rc = time_choices(b, reps, run, nullptr, ms_tuned, choice);
-    if (ms_fixed) *ms_fixed = fixed;
-    return rc;
+    return tune_all(
+        b, reps,
+        [&]() {
+            int rc = bind_uxw(b, NQ, NS, NT);
+            return rc ? rc : enqueue_uxw(b, W, Z, NQ, NS, NT);
+        },
+        ms_fixed, ms_tuned, choice);
 }
 
-int rd_batch_split_uxw(rd_batch *b, int lmax_min)
+int rd_batch_split(rd_batch *b, int lmax_min)
 {
-    if (!b || !b->uxw || lmax_min < 0) return RD_ERR_ARG;
+    if (!b || lmax_min < 0) return RD_ERR_ARG;
     b->split = lmax_min;
     return configure(b, b->jb, b->wg_per_cu);
 }
 
-int rd_batch_get_split_uxw(const rd_batch *b, int *lmax_min, int *n_one)
+int rd_batch_get_split(const rd_batch *b, int *lmax_min, int *n_one)
 {
-    if (!b || !b->uxw || !lmax_min || !n_one) return RD_ERR_ARG;
+    if (!b || !lmax_min || !n_one) return RD_ERR_ARG;
     *lmax_min = b->split;
     *n_one = (int)b->large.size();
     return RD_OK;
@@ -589,21 +624,26 @@ int rd_batch_capture(rd_batch *b, const double *const *spec, double *const *grid
     int rc = bind(b, spec_out);
     if (rc) return rc;
     RD_HIP(hipStreamSynchronize(b->stream));
-    const size_t ns = b->leg.arg_bytes_syn * b->nprob, na = b->leg.arg_bytes_ana * b->nprob;
-    RD_HIP(hipMalloc(&b->g_syn_args, ns));
-    RD_HIP(hipMalloc(&b->g_ana_args, na));
-    RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
-    RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
-    RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    const bool leg = !b->small.empty();
+    const size_t ns = b->leg.arg_bytes_syn * b->leg.nprob, na = b->leg.arg_bytes_ana * b->leg.nprob;
+    if (leg) {
+        RD_HIP(hipMalloc(&b->g_syn_args, ns));
+        RD_HIP(hipMalloc(&b->g_ana_args, na));
+        RD_HIP(hipMalloc(&b->g_counter, 2 * sizeof(int)));
+        RD_HIP(hipMemcpy(b->g_syn_args, b->leg.d_syn_args, ns, hipMemcpyDeviceToDevice));
+        RD_HIP(hipMemcpy(b->g_ana_args, b->leg.d_ana_args, na, hipMemcpyDeviceToDevice));
+    }
     // record on the batch's own (non-default) stream, with the graph's copies
     // of the arguments and counters; the graph is launched later on whichever
     // stream the batch is bound to
     hipStream_t user = b->stream;
     void *sa = b->leg.d_syn_args, *aa = b->leg.d_ana_args;
     int *ct = b->leg.d_counter;
-    b->leg.d_syn_args = b->g_syn_args;
-    b->leg.d_ana_args = b->g_ana_args;
-    b->leg.d_counter = b->g_counter;
+    if (leg) {
+        b->leg.d_syn_args = b->g_syn_args;
+        b->leg.d_ana_args = b->g_ana_args;
+        b->leg.d_counter = b->g_counter;
+    }
     set_stream(b, b->own_stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeThreadLocal);
@@ -645,7 +685,7 @@ int rd_batch_tune(rd_batch *b, const double *const *spec, double *const *grid, d
 {
     if (!b || !spec || !grid || !spec_out || reps < 1) return RD_ERR_ARG;
     if (b->uxw) return RD_ERR_STATE;
-    return time_choices(
+    return tune_all(
         b, reps,
         [&]() {
             int rc = bind(b, spec_out);

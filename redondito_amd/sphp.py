@@ -485,8 +485,8 @@ class Batch:
             L.rd_batch_create_uxw.argtypes = L.rd_batch_create.argtypes
             for nm in ("rd_batch_nonlinear_uxw", "rd_batch_capture_uxw"):
                 getattr(L, nm).argtypes = [vp] + [ctypes.POINTER(vp)] * 5
-            L.rd_batch_split_uxw.argtypes = [vp, i]
-            L.rd_batch_get_split_uxw.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+            L.rd_batch_split.argtypes = [vp, i]
+            L.rd_batch_get_split.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
             L.rd_batch_tune_uxw.argtypes = [vp] + [ctypes.POINTER(vp)] * 5 + [i, ctypes.POINTER(dd),
                                                                                ctypes.POINTER(dd), ctypes.POINTER(i)]
             L._batch_bound = True
@@ -568,15 +568,15 @@ class Batch:
         _check(lib().rd_batch_capture_uxw(self.h, *[self._ptrs(x, "spec") for x in (W, Z, NQ, NS, NT)]),
                "batch_capture_uxw")
 
-    def split_uxw(self, lmax_min):
-        """Problems with lmax >= lmax_min run one-problem evaluations (0: all,
-        a value above every lmax: none); the results do not change."""
-        _check(lib().rd_batch_split_uxw(self.h, int(min(lmax_min, 2**31 - 1))), "batch_split_uxw")
+    def split(self, lmax_min):
+        """Problems with lmax >= lmax_min run one-problem chains (0: all, a
+        value above every lmax: none); the results do not change."""
+        _check(lib().rd_batch_split(self.h, int(min(lmax_min, 2**31 - 1))), "batch_split")
 
-    def uxw_split(self):
-        """(lmax threshold, number of problems run as one-problem evaluations)."""
+    def get_split(self):
+        """(lmax threshold, number of problems run as one-problem chains)."""
         a, n = ctypes.c_int(), ctypes.c_int()
-        _check(lib().rd_batch_get_split_uxw(self._h, ctypes.byref(a), ctypes.byref(n)), "batch_get_split_uxw")
+        _check(lib().rd_batch_get_split(self._h, ctypes.byref(a), ctypes.byref(n)), "batch_get_split")
         return a.value, n.value
 
     def tune_uxw(self, W, Z, NQ, NS, NT, reps=3):

@@ -56,37 +56,54 @@ def specs(probs):
 
 
 def test_batch_matches_single_plans():
+    """The round trip of every problem equals the single-plan 3D transforms,
+    with every problem on its own chain (the default), every problem in the
+    persistent launches, and split between the two."""
     from redondito_amd import sphp as S
     probs = problem_list()
-    b = S.Batch(probs, prec=S.RD_PREC_FP64)
-    info = b.info()
-    print("batch info", info)
-    assert info["nprob"] == len(probs) and info["syn_items"] > 0
     spec = seeded_spectra(probs, 1)
-    g = grids(probs)
-    b.inverse(spec, g)
-    out = specs(probs)
-    b.forward(g, out)
-    torch.cuda.synchronize()
+    refs = []
     for i, p in enumerate(probs):
         plan = S.FCPlan(p[0], p[1], p[2], p[3], 25, 5, p[4], p[5])
         sc = spec[i].clone()
-        gr = torch.zeros_like(g[i])
+        gr = torch.zeros((p[3], p[2], p[1]), dtype=torch.float64, device=dev())
         S.fftp3d_complex_to_real(plan, sc, gr)
-        so = torch.zeros_like(out[i])
+        so = torch.zeros_like(spec[i])
         S.fftp3d_real_to_complex(plan, gr.clone(), so)
         torch.cuda.synchronize()
-        eg = (g[i] - gr).abs().max().item() / gr.abs().max().item()
-        es = (out[i] - so).abs().max().item() / so.abs().max().item()
-        print(f"problem {p[:4]}: grid {eg:.1e} spec {es:.1e}")
-        assert eg <= 1e-15 and es <= 1e-15
+        refs.append((gr, so))
+        plan.close()
+    b = S.Batch(probs, prec=S.RD_PREC_FP64)
+    print("batch info", b.info())
+    assert b.info()["nprob"] == len(probs) and b.get_split() == (0, len(probs))
+    mid = sorted(p[0] for p in probs)[len(probs) // 2]
+    n_mid = sum(p[0] >= mid for p in probs)
+    for split, n_one in ((0, len(probs)), (2**31 - 1, 0), (mid, n_mid)):
+        b.split(split)
+        assert b.get_split() == (split, n_one)
+        assert (b.info()["syn_items"] > 0) == (n_one < len(probs))
+        g = grids(probs)
+        b.inverse(spec, g)
+        out = specs(probs)
+        b.forward(g, out)
+        torch.cuda.synchronize()
+        for i, p in enumerate(probs):
+            gr, so = refs[i]
+            eg = (g[i] - gr).abs().max().item() / gr.abs().max().item()
+            es = (out[i] - so).abs().max().item() / so.abs().max().item()
+            print(f"split {split} problem {p[:4]}: grid {eg:.1e} spec {es:.1e}")
+            assert eg <= 1e-15 and es <= 1e-15
     b.close()
 
 
 def test_batch_round_trip_and_graph():
+    """Round trip and its graph with the problems split between one-problem
+    chains and the persistent launches."""
     from redondito_amd import sphp as S
     probs = problem_list()
     b = S.Batch(probs, prec=S.RD_PREC_FP64)
+    b.split(sorted(p[0] for p in probs)[len(probs) // 2])
+    assert 0 < b.get_split()[1] < len(probs)
     rng = np.random.default_rng(3)
     g0 = [torch.from_numpy(rng.standard_normal((p[3], p[2], p[1]))).to(dev()) for p in probs]
     s0 = specs(probs)
@@ -144,8 +161,9 @@ def test_batch_tune_keeps_results():
     torch.cuda.synchronize()
     ref = [x.clone() for x in out]
     ms_fixed, ms_tuned, choice = b.tune(spec, grids(probs), specs(probs), reps=2)
-    print(f"tune: fixed {ms_fixed:.3f} ms, tuned {ms_tuned:.3f} ms, choice {choice}")
-    assert ms_tuned <= ms_fixed and choice[0] in (2, 4) and 1 <= choice[1] <= 8
+    print(f"tune: fixed {ms_fixed:.3f} ms, tuned {ms_tuned:.3f} ms, choice {choice}, split {b.get_split()}")
+    # two separate timings (the threshold pass, then the tile pass): within noise
+    assert ms_tuned <= 1.1 * ms_fixed and choice[0] in (2, 4) and 1 <= choice[1] <= 8
     assert b.info()["jb"] == choice[0]
     out2 = specs(probs)
     b.inverse(spec, g)
@@ -272,11 +290,11 @@ def test_batch_nonlinear_uxw_matches_single_plans():
             refs.append(ref)
             plan.close()
         b = S.Batch(probs, prec=prec, form="uxw")
-        assert b.info()["jb"] == 1 and b.uxw_split() == (0, len(probs))
+        assert b.info()["jb"] == 1 and b.get_split() == (0, len(probs))
         # all on their own (the default), none, and lmax >= 63 (3 of 6)
         for split, n_one in ((0, len(probs)), (2**31 - 1, 0), (63, 3)):
-            b.split_uxw(split)
-            assert b.uxw_split() == (split, n_one)
+            b.split(split)
+            assert b.get_split() == (split, n_one)
             # the persistent launches hold work items only for the others
             assert (b.info()["syn_items"] > 0) == (n_one < len(probs))
             out = [specs(probs) for _ in range(3)]
@@ -302,7 +320,7 @@ def test_batch_nonlinear_uxw_matches_single_plans():
             b.nonlinear_uxw(Z, W, *other)          # eager work on other arrays
             torch.cuda.synchronize()
         fixed, tuned, choice = b.tune_uxw(W, Z, *other, reps=2)
-        split = b.uxw_split()
+        split = b.get_split()
         print(f"prec {prec}: tune fixed {fixed:.3f} ms tuned {tuned:.3f} ms choice {choice} split {split}")
         assert choice[0] in (1, 2) and split[0] in (2**31 - 1, 0) + tuple(p[0] for p in probs)
         t2 = [specs(probs) for _ in range(3)]
