@@ -153,8 +153,11 @@ void free_precision_tables(Plan *p)
 // Same tiling as synthesis_kernel: one m, 64 northern nodes (16 per wave),
 // JT real columns; the l loop in chunks of 32 (one K=16 MFMA per parity).
 // LDS holds the chunk as [field][parity][column][l_p], hi and lo.  Vector
-// tiles take one 16-column block per wave: 90 VGPRs and 5 waves per SIMD
-// (0.45 against 0.55 ms with two blocks at 2 waves, lmax 255, 256 planes).
+// tiles take one 16-column block per wave (0.45 against 0.55 ms with two
+// blocks at 2 waves, lmax 255, 256 planes).  With the next chunk's table
+// fragments prefetched into a second register set and unconditional chunk
+// loads the vector tile takes 112 VGPRs (4 waves per SIMD): u and omega
+// syntheses at lmax 255, 128 planes, 0.758 -> 0.656 ms, outputs bit-identical.
 constexpr int BP = 20;   // column pitch in BF16 elements (16 + 4, 8-byte aligned)
 
 struct Bf3Args {
@@ -200,55 +203,75 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void synthesis_bf3_kernel(Bf3Args
     const long lm0 = lm_idx(A.lmax, m, m);
 
     // a chunk is NF x (JT/2 planes) x 32 l's, l fastest across threads; each
-    // thread moves NLD double2 of it, and the next chunk is loaded into
-    // registers while the current one is multiplied out of LDS
+    // thread moves NLD double2 of it: element tid + 256 i, whose field
+    // (256 i) / PER is a compile-time constant.  The next chunk is loaded into
+    // registers while the current one is multiplied out of LDS, from clamped
+    // addresses, unconditionally (a load under a condition makes the loads of
+    // a chunk issue one after another); entries beyond lmax or the last plane
+    // are zeroed where the chunk is split into LDS.
     constexpr int PER = (JT / 2) * 32, NLD = NF * PER / 256;
-    static_assert(NF * PER == 256 * NLD, "chunk must split evenly over the workgroup");
+    static_assert(PER % 256 == 0, "a thread's chunk elements must each lie in one field");
     double2 rv[NLD];
+    unsigned rlive = 0;   // bit i: element i lies inside the spectrum
     auto load = [&](int lbase) {
+        rlive = 0;
 #pragma unroll
         for (int i = 0; i < NLD; i++) {
-            const int e = tid + 256 * i;
-            const int f = e / PER, r = e % PER, pl = r / 32, ll = r % 32;
+            const int f = (256 * i) / PER, r = tid + (256 * i) % PER, pl = r / 32, ll = r % 32;
             const int l = lbase + ll, k = k0 + pl;
-            rv[i] = make_double2(0.0, 0.0);
-            if (l <= A.lmax && k < A.nplanes) rv[i] = A.in[f][lm0 + (l - m) + k * A.ld];
+            const int lc = l < A.lmax ? l : A.lmax, kc = k < A.nplanes ? k : A.nplanes - 1;
+            rv[i] = A.in[f][lm0 + (lc - m) + kc * A.ld];
+            rlive |= (unsigned)(l <= A.lmax && k < A.nplanes) << i;
         }
     };
-    load(m);
-    for (int lbase = m; lbase <= A.lmax; lbase += 32) {
+    // table fragments (4 consecutive rows of both parities of the three
+    // tables) of the chunk at lbase, 32-bit element offsets from the uniform
+    // table bases; the next chunk's are loaded before the current chunk's
+    // MFMAs, into the other of two register sets
+    struct Tab {
+        v4s Peh, Pel, Poh, Pol, Deh, Del, Doh, Dol, Meh, Mel, Moh, Mol;
+    };
+    auto ltab = [&](int lbase, Tab &t) {
+        const int rg = (((lbase - m) >> 1) >> 2) + kk;   // group of 4 rows
+        const unsigned ie = (unsigned)offe + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
+        const unsigned io = (unsigned)offo + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
+        t.Peh = *(const v4s *)(A.th_[0] + ie);
+        t.Pel = *(const v4s *)(A.tl_[0] + ie);
+        t.Poh = *(const v4s *)(A.th_[0] + io);
+        t.Pol = *(const v4s *)(A.tl_[0] + io);
+        if (VEC) {
+            t.Deh = *(const v4s *)(A.th_[1] + ie);
+            t.Del = *(const v4s *)(A.tl_[1] + ie);
+            t.Doh = *(const v4s *)(A.th_[1] + io);
+            t.Dol = *(const v4s *)(A.tl_[1] + io);
+            t.Meh = *(const v4s *)(A.th_[2] + ie);
+            t.Mel = *(const v4s *)(A.tl_[2] + ie);
+            t.Moh = *(const v4s *)(A.th_[2] + io);
+            t.Mol = *(const v4s *)(A.tl_[2] + io);
+        }
+    };
+    int lbase = m;
+    auto chunk = [&](const Tab &cur, Tab &nxt) {
 #pragma unroll
         for (int i = 0; i < NLD; i++) {
-            const int e = tid + 256 * i;
-            const int f = e / PER, r = e % PER, pl = r / 32, ll = r % 32;
+            const int f = (256 * i) / PER, r = tid + (256 * i) % PER, pl = r / 32, ll = r % 32;
             const int par = ll & 1, lp = ll >> 1;
+            const bool ok = (rlive >> i) & 1u;
             unsigned short h, lo;
-            split_bf16(rv[i].x, h, lo);
+            split_bf16(ok ? rv[i].x : 0.0, h, lo);
             Bh[f][par][2 * pl][lp] = h;
             Bl[f][par][2 * pl][lp] = lo;
-            split_bf16(rv[i].y, h, lo);
+            split_bf16(ok ? rv[i].y : 0.0, h, lo);
             Bh[f][par][2 * pl + 1][lp] = h;
             Bl[f][par][2 * pl + 1][lp] = lo;
         }
         __syncthreads();
-        if (lbase + 32 <= A.lmax) load(lbase + 32);
-        // 32-bit element offsets from the uniform table bases
-        const int rg = (((lbase - m) >> 1) >> 2) + kk;   // group of 4 rows
-        const unsigned ie = (unsigned)offe + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
-        const unsigned io = (unsigned)offo + (unsigned)(rg * 4 * A.nthp + 4 * (th0 + col));
-        const v4s Peh = *(const v4s *)(A.th_[0] + ie), Pel = *(const v4s *)(A.tl_[0] + ie);
-        const v4s Poh = *(const v4s *)(A.th_[0] + io), Pol = *(const v4s *)(A.tl_[0] + io);
-        v4s Deh = {}, Del = {}, Doh = {}, Dol = {}, Meh = {}, Mel = {}, Moh = {}, Mol = {};
-        if (VEC) {
-            Deh = *(const v4s *)(A.th_[1] + ie);
-            Del = *(const v4s *)(A.tl_[1] + ie);
-            Doh = *(const v4s *)(A.th_[1] + io);
-            Dol = *(const v4s *)(A.tl_[1] + io);
-            Meh = *(const v4s *)(A.th_[2] + ie);
-            Mel = *(const v4s *)(A.tl_[2] + ie);
-            Moh = *(const v4s *)(A.th_[2] + io);
-            Mol = *(const v4s *)(A.tl_[2] + io);
-        }
+        const bool more = lbase + 32 <= A.lmax;
+        if (more) load(lbase + 32);
+        ltab(more ? lbase + 32 : lbase, nxt);   // (the last chunk reloads its own rows)
+        const v4s Peh = cur.Peh, Pel = cur.Pel, Poh = cur.Poh, Pol = cur.Pol;
+        const v4s Deh = cur.Deh, Del = cur.Del, Doh = cur.Doh, Dol = cur.Dol;
+        const v4s Meh = cur.Meh, Mel = cur.Mel, Moh = cur.Moh, Mol = cur.Mol;
 #pragma unroll
         for (int b = 0; b < JB; b++) {
             const int j = b * 16 + col;
@@ -278,6 +301,18 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void synthesis_bf3_kernel(Bf3Args
 #undef LDB
         }
         __syncthreads();
+        lbase += 32;
+    };
+    // two table register sets, alternating with the chunk parity (no copy of
+    // a set whose loads are in flight)
+    Tab ta = {}, tb = {};
+    load(m);
+    ltab(m, ta);
+    for (;;) {
+        chunk(ta, tb);
+        if (lbase > A.lmax) break;
+        chunk(tb, ta);
+        if (lbase > A.lmax) break;
     }
 
 #pragma unroll
@@ -342,18 +377,63 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void analysis_bf3_kernel(Bf3Args 
 
     const long offe = A.off[2 * m], offo = A.off[2 * m + 1];
 
-    for (int tc = 0; tc < A.nth; tc += 16) {
-        constexpr int PER = (JT / 2) * 16;
-        for (int e = tid; e < NF * PER; e += 256) {
-            const int f = e / PER, r = e % PER;
-            const int pl = r / 16, tl = r % 16;
+    // a chunk is NF x (JT/2 planes) x 16 northern nodes, node fastest across
+    // threads; each thread loads element tid + 256 i of it (field (256 i) /
+    // PER, a compile-time constant), north and south, from clamped addresses,
+    // unconditionally, so that the loads issue back to back (a load under a
+    // condition makes them go out one after another); nodes past nth and
+    // planes past the last are zeroed where the chunk is split into LDS.
+    // (Loading the next chunk during the current one's MFMAs was slower:
+    // 0.352 against 0.290 ms at lmax 255, 128 planes; its registers cost 8
+    // spills at 4 waves per SIMD.)
+    constexpr int PER = (JT / 2) * 16, NLD = NF * PER / 256;
+    static_assert(PER % 256 == 0, "a thread's chunk elements must each lie in one field");
+    double2 rn[NLD], rs[NLD];
+    unsigned rlive = 0;
+    auto load = [&](int tc) {
+        rlive = 0;
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int f = (256 * i) / PER, r = tid + (256 * i) % PER, pl = r / 16, tl = r % 16;
             const int th = tc + tl, k = k0 + pl;
-            double2 vn = make_double2(0.0, 0.0), vs = vn;
-            if (th < A.nth && k < A.nplanes) {
-                const long base = ((long)k * L + m) * A.nlat;
-                vn = A.in[f][base + th];
-                vs = A.in[f][base + A.nlat - 1 - th];
-            }
+            const int thc = th < A.nth ? th : A.nth - 1, kc = k < A.nplanes ? k : A.nplanes - 1;
+            const long base = ((long)kc * L + m) * A.nlat;
+            rn[i] = A.in[f][base + thc];
+            rs[i] = A.in[f][base + A.nlat - 1 - thc];
+            rlive |= (unsigned)(th < A.nth && k < A.nplanes) << i;
+        }
+    };
+    // table fragments (4 consecutive nodes of both parities of the three
+    // tables) of the chunk at tc, loaded after the chunk's barrier
+    struct Tab {
+        v4s Peh, Pel, Poh, Pol, Deh, Del, Doh, Dol, Meh, Mel, Moh, Mol;
+    };
+    auto ltab = [&](int tc, Tab &t) {
+        const long tg = (tc >> 2) + kk;   // group of 4 nodes
+        const long ie = offe + tg * 4L * ncol + 4L * (lp0 + col);
+        const long io = offo + tg * 4L * ncol + 4L * (lp0 + col);
+        t.Peh = *(const v4s *)(A.th_[0] + ie);
+        t.Pel = *(const v4s *)(A.tl_[0] + ie);
+        t.Poh = *(const v4s *)(A.th_[0] + io);
+        t.Pol = *(const v4s *)(A.tl_[0] + io);
+        if (VEC) {
+            t.Deh = *(const v4s *)(A.th_[1] + ie);
+            t.Del = *(const v4s *)(A.tl_[1] + ie);
+            t.Doh = *(const v4s *)(A.th_[1] + io);
+            t.Dol = *(const v4s *)(A.tl_[1] + io);
+            t.Meh = *(const v4s *)(A.th_[2] + ie);
+            t.Mel = *(const v4s *)(A.tl_[2] + ie);
+            t.Moh = *(const v4s *)(A.th_[2] + io);
+            t.Mol = *(const v4s *)(A.tl_[2] + io);
+        }
+    };
+    for (int tc = 0; tc < A.nth; tc += 16) {
+        load(tc);
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int f = (256 * i) / PER, r = tid + (256 * i) % PER, pl = r / 16, tl = r % 16;
+            const bool ok = (rlive >> i) & 1u;
+            const double2 vn = ok ? rn[i] : make_double2(0.0, 0.0), vs = ok ? rs[i] : make_double2(0.0, 0.0);
             unsigned short h, lo;
             split_bf16(vn.x + vs.x, h, lo);
             Gh[f][0][2 * pl][tl] = h;
@@ -370,22 +450,11 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void analysis_bf3_kernel(Bf3Args 
         }
         __syncthreads();
         if (live) {
-            const long tg = (tc >> 2) + kk;   // group of 4 nodes
-            const long ie = offe + tg * 4L * ncol + 4L * (lp0 + col);
-            const long io = offo + tg * 4L * ncol + 4L * (lp0 + col);
-            const v4s Peh = *(const v4s *)(A.th_[0] + ie), Pel = *(const v4s *)(A.tl_[0] + ie);
-            const v4s Poh = *(const v4s *)(A.th_[0] + io), Pol = *(const v4s *)(A.tl_[0] + io);
-            v4s Deh = {}, Del = {}, Doh = {}, Dol = {}, Meh = {}, Mel = {}, Moh = {}, Mol = {};
-            if (VEC) {
-                Deh = *(const v4s *)(A.th_[1] + ie);
-                Del = *(const v4s *)(A.tl_[1] + ie);
-                Doh = *(const v4s *)(A.th_[1] + io);
-                Dol = *(const v4s *)(A.tl_[1] + io);
-                Meh = *(const v4s *)(A.th_[2] + ie);
-                Mel = *(const v4s *)(A.tl_[2] + ie);
-                Moh = *(const v4s *)(A.th_[2] + io);
-                Mol = *(const v4s *)(A.tl_[2] + io);
-            }
+            Tab cur = {};
+            ltab(tc, cur);
+            const v4s Peh = cur.Peh, Pel = cur.Pel, Poh = cur.Poh, Pol = cur.Pol;
+            const v4s Deh = cur.Deh, Del = cur.Del, Doh = cur.Doh, Dol = cur.Dol;
+            const v4s Meh = cur.Meh, Mel = cur.Mel, Moh = cur.Moh, Mol = cur.Mol;
 #pragma unroll
             for (int b = 0; b < JB; b++) {
                 const int j = (wj * JB + b) * 16 + col;
