@@ -635,8 +635,10 @@ struct Tw16 {
 // the 256th roots in LDS (one read at a constant offset per value, lanes of
 // consecutive b on consecutive slots); R3 and FIN read e^{-2 pi i b/768} and
 // e^{-2 pi i 2b/768} straight from the global table of the 768 roots.
-// V = float2 transforms the row in FP32 (20 KB of LDS) for the BF16x3 form;
-// loads, the pack scale and the stores stay FP64.
+// The phases are written for V = double2 (the instance launched) and for
+// V = float2 (the row in FP32, loads, pack scale and stores in FP64); the FP32
+// instance measured slower (0.745 against 0.661 ms at lmax 255, 128 planes)
+// and is not launched.
 constexpr int PHI_BUF = 9 * 256;
 
 // Thread index with the wave index permuted per workgroup (XOR with a hash of
@@ -1455,12 +1457,11 @@ int launch_phi_cross(Plan *p, int nplanes, const double2 *const *Fin, double2 *c
     for (int i = 0; i < 6; i++) P.F[i] = Fin[i];
     for (int i = 0; i < 3; i++) P.Fo[i] = Fout[i];
     if (p->nphi == 768 && p->lmax <= 255) {
-        if (prec == RD_PREC_BF16X3)
-            hipLaunchKernelGGL(phi_cross768_kernel<float2>, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P,
-                               p->lmax, p->nlat, nplanes, p->tw_phi);
-        else
-            hipLaunchKernelGGL(phi_cross768_kernel<double2>, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P,
-                               p->lmax, p->nlat, nplanes, p->tw_phi);
+        // every form transforms these rows in FP64: the FP32 instance of this
+        // kernel (BF16x3 form) took 0.745 against 0.661 ms at lmax 255, 128
+        // planes, and lost accuracy besides
+        hipLaunchKernelGGL(phi_cross768_kernel<double2>, dim3(nplanes * p->nlat), dim3(256), 0, p->stream, P,
+                           p->lmax, p->nlat, nplanes, p->tw_phi);
         RD_LAUNCH_CHECK();
         return RD_OK;
     }
