@@ -225,10 +225,9 @@ __device__ __forceinline__ void oz_max4c(double *mx, int b, const double2 (&v)[4
 // re and im columns of a part of matrix b at kidx: the values v, or i v
 // (rot = +1: re -Im, im Re) or -i v (rot = -1: re Im, im -Re), or -v (rot = 2).
 template <int D>
-__device__ __forceinline__ void oz_put_part(const OzBArgs &A, int m, int b, int k, int kidx, const double2 (&v)[4],
-                                            int rot, const int *es)
+__device__ __forceinline__ void oz_put_part(const OzBArgs &A, const OzMat &M, int b, int k, int kidx,
+                                            const double2 (&v)[4], int rot, const int *es)
 {
-    const OzMat M = A.bm[m * 6 + b];
     double re[4], im[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -263,6 +262,11 @@ __global__ __launch_bounds__(64 * OZ_WPB) void syn_b_oz_kernel(OzBArgs A)
     const bool kin = k < A.nplanes;
     const int kc = kin ? k : 0;
     const long lm0 = (long)m * (2 * A.lmax + 3 - m) / 2 + kc * A.ld;   // (m, m) of plane kc
+    // the operand matrices' descriptors of this m, loaded once here, in
+    // uniform control flow (scalar loads), instead of once per part written
+    OzMat Mb[6];
+#pragma unroll
+    for (int b = 0; b < (VEC ? 6 : 2); b++) Mb[b] = A.bm[m * 6 + b];
     // four rows t0..t0+3 of field f, parity par (zero past their count or off-plane)
     auto load4 = [&](int f, int par, int t0, double2 (&v)[4]) {
         const int nrow = par ? ro : re;
@@ -306,8 +310,8 @@ __global__ __launch_bounds__(64 * OZ_WPB) void syn_b_oz_kernel(OzBArgs A)
         double2 qe[4], qo[4];
         load4(0, 0, t0, qe);
         load4(0, 1, t0, qo);
-        oz_put_part<D>(A, m, 0, k, t0, qe, 0, es);
-        if (t0 < ro) oz_put_part<D>(A, m, 1, k, t0, qo, 0, es);
+        oz_put_part<D>(A, Mb[0], 0, k, t0, qe, 0, es);
+        if (t0 < ro) oz_put_part<D>(A, Mb[1], 1, k, t0, qo, 0, es);
         if (VEC) {
             double2 se[4], so[4], te[4], to[4];
             load4(1, 0, t0, se);
@@ -315,15 +319,15 @@ __global__ __launch_bounds__(64 * OZ_WPB) void syn_b_oz_kernel(OzBArgs A)
             load4(2, 0, t0, te);
             load4(2, 1, t0, to);
             if (t0 < ro) {
-                oz_put_part<D>(A, m, 2, k, t0, so, 0, es);
-                oz_put_part<D>(A, m, 4, k, t0, to, 0, es);
-                oz_put_part<D>(A, m, 3, k, re16 + t0, to, 1, es);
-                oz_put_part<D>(A, m, 5, k, re16 + t0, so, -1, es);
+                oz_put_part<D>(A, Mb[2], 2, k, t0, so, 0, es);
+                oz_put_part<D>(A, Mb[4], 4, k, t0, to, 0, es);
+                oz_put_part<D>(A, Mb[3], 3, k, re16 + t0, to, 1, es);
+                oz_put_part<D>(A, Mb[5], 5, k, re16 + t0, so, -1, es);
             }
-            oz_put_part<D>(A, m, 2, k, ro16 + t0, te, 1, es);
-            oz_put_part<D>(A, m, 4, k, ro16 + t0, se, -1, es);
-            oz_put_part<D>(A, m, 3, k, t0, se, 0, es);
-            oz_put_part<D>(A, m, 5, k, t0, te, 0, es);
+            oz_put_part<D>(A, Mb[2], 2, k, ro16 + t0, te, 1, es);
+            oz_put_part<D>(A, Mb[4], 4, k, ro16 + t0, se, -1, es);
+            oz_put_part<D>(A, Mb[3], 3, k, t0, se, 0, es);
+            oz_put_part<D>(A, Mb[5], 5, k, t0, te, 0, es);
         }
     }
     if (g == 0)
@@ -350,6 +354,9 @@ __global__ __launch_bounds__(64 * OZ_WPB) void ana_b_oz_kernel(OzBArgs A)
     if ((blockIdx.x * OZ_WPB + (threadIdx.x >> 6)) * (64 / G) >= A.nplanes) return;   // whole wave idle
     const bool kin = k < A.nplanes;
     const long base = ((long)(kin ? k : 0) * L + m) * A.nlat;
+    OzMat Mb[6];   // the operand matrices' descriptors of this m, loaded once (as in syn_b_oz_kernel)
+#pragma unroll
+    for (int b = 0; b < (VEC ? 6 : 2); b++) Mb[b] = A.bm[m * 6 + b];
     // x+ and x- of field f at thetas t0..t0+3 (zero past nth or off-plane)
     auto load4 = [&](int f, int t0, double2 (&sp)[4], double2 (&sm)[4]) {
 #pragma unroll
@@ -392,20 +399,20 @@ __global__ __launch_bounds__(64 * OZ_WPB) void ana_b_oz_kernel(OzBArgs A)
         const int t0 = 4 * w;
         double2 rp[4], rm[4];
         load4(0, t0, rp, rm);
-        oz_put_part<D>(A, m, 0, k, t0, rp, 0, es);
-        oz_put_part<D>(A, m, 1, k, t0, rm, 0, es);
+        oz_put_part<D>(A, Mb[0], 0, k, t0, rp, 0, es);
+        oz_put_part<D>(A, Mb[1], 1, k, t0, rm, 0, es);
         if (VEC) {
             double2 tp[4], tm[4], pp[4], pm[4];
             load4(1, t0, tp, tm);
             load4(2, t0, pp, pm);
-            oz_put_part<D>(A, m, 2, k, t0, tm, 0, es);
-            oz_put_part<D>(A, m, 2, k, A.kth + t0, pp, -1, es);
-            oz_put_part<D>(A, m, 3, k, t0, tp, 0, es);
-            oz_put_part<D>(A, m, 3, k, A.kth + t0, pm, -1, es);
-            oz_put_part<D>(A, m, 4, k, t0, pm, 2, es);
-            oz_put_part<D>(A, m, 4, k, A.kth + t0, tp, -1, es);
-            oz_put_part<D>(A, m, 5, k, t0, pp, 2, es);
-            oz_put_part<D>(A, m, 5, k, A.kth + t0, tm, -1, es);
+            oz_put_part<D>(A, Mb[2], 2, k, t0, tm, 0, es);
+            oz_put_part<D>(A, Mb[2], 2, k, A.kth + t0, pp, -1, es);
+            oz_put_part<D>(A, Mb[3], 3, k, t0, tp, 0, es);
+            oz_put_part<D>(A, Mb[3], 3, k, A.kth + t0, pm, -1, es);
+            oz_put_part<D>(A, Mb[4], 4, k, t0, pm, 2, es);
+            oz_put_part<D>(A, Mb[4], 4, k, A.kth + t0, tp, -1, es);
+            oz_put_part<D>(A, Mb[5], 5, k, t0, pp, 2, es);
+            oz_put_part<D>(A, Mb[5], 5, k, A.kth + t0, tm, -1, es);
         }
     }
     if (g == 0)
