@@ -132,8 +132,10 @@ int launch_analysis(Plan *p, int prec, bool vec, int nplanes, const double2 *g0,
 int build_legendre_tables(Plan *p);
 int ensure_precision_tables(Plan *p, int prec);
 void free_precision_tables(Plan *p);
+// in2/out2: a second field set synthesised by the same launch (vector form)
 int launch_synthesis_bf3(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
-                         long ld, double2 *f0, double2 *f1, double2 *f2);
+                         long ld, double2 *f0, double2 *f1, double2 *f2, const double2 *const *in2 = nullptr,
+                         double2 *const *out2 = nullptr);
 int ensure_oz_tables(Plan *p);
 void free_oz_tables(Plan *p);
 int launch_synthesis_oz(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,

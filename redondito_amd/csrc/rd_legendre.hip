@@ -604,6 +604,12 @@ int launch_synthesis2(Plan *p, int prec, int nplanes, const double2 *const *in_a
         return launch_synthesis_t<float>(p, p->syn32_P, p->syn32_D, p->syn32_M, true, nplanes, in_a[0], in_a[1],
                                          in_a[2], ld, out_a[0], out_a[1], out_a[2], in_b, out_b);
     }
+    if (prec == RD_PREC_BF16X3) {
+        int rc = ensure_precision_tables(p, prec);
+        if (rc) return rc;
+        return launch_synthesis_bf3(p, true, nplanes, in_a[0], in_a[1], in_a[2], ld, out_a[0], out_a[1], out_a[2],
+                                    in_b, out_b);
+    }
     int rc = launch_synthesis(p, prec, true, nplanes, in_a[0], in_a[1], in_a[2], ld, out_a[0], out_a[1], out_a[2]);
     if (!rc) rc = launch_synthesis(p, prec, true, nplanes, in_b[0], in_b[1], in_b[2], ld, out_b[0], out_b[1], out_b[2]);
     return rc;

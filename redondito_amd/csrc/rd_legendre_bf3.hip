@@ -167,8 +167,13 @@ struct Bf3Args {
     const int *queue;
     const double2 *in[3];
     double2 *out[3];
+    // a second field set synthesised by the same launch (nset = 2): the plane
+    // tiles of both sets of one (m, theta tile) entry run back to back on one
+    // XCD, so the entry's table fragments serve both from L2
+    const double2 *in2[3];
+    double2 *out2[3];
     long ld;
-    int lmax, nlat, nth, nthp, nthp16, nplanes, n_jt;
+    int lmax, nlat, nth, nthp, nthp16, nplanes, n_jt, nset;
 };
 
 template <bool VEC>
@@ -181,9 +186,15 @@ __global__ __launch_bounds__(256, VEC ? 4 : 1) void synthesis_bf3_kernel(Bf3Args
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int L = A.lmax + 1;
     const int xcd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
-    const int jt = i % A.n_jt;
-    const int ent = A.queue[(i / A.n_jt) * kXcds + xcd];
+    const int nj = A.n_jt * A.nset, jts = i % nj, jt = jts % A.n_jt;
+    const int ent = A.queue[(i / nj) * kXcds + xcd];
     if (ent < 0) return;
+    if (jts >= A.n_jt)
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            A.in[f] = A.in2[f];
+            A.out[f] = A.out2[f];
+        }
     const int m = ent & 0xffff, tt = ent >> 16;
     const int k0 = jt * (JT / 2);
     const int th0 = tt * 64 + wave * 16;
@@ -536,16 +547,29 @@ static Bf3Args bf3_args(Plan *p, bool syn, bool vec, int nplanes, const double2 
     A.nthp = p->nthp;
     A.nthp16 = p->nthp16;
     A.nplanes = nplanes;
+    A.nset = 1;
+    for (int f = 0; f < 3; f++) {
+        A.in2[f] = nullptr;
+        A.out2[f] = nullptr;
+    }
     return A;
 }
 
 int launch_synthesis_bf3(Plan *p, bool vec, int nplanes, const double2 *q, const double2 *s, const double2 *t,
-                         long ld, double2 *f0, double2 *f1, double2 *f2)
+                         long ld, double2 *f0, double2 *f1, double2 *f2, const double2 *const *in2,
+                         double2 *const *out2)
 {
     Bf3Args A = bf3_args(p, true, vec, nplanes, q, s, t, f0, f1, f2, ld);
+    if (vec && in2) {
+        A.nset = 2;
+        for (int f = 0; f < 3; f++) {
+            A.in2[f] = in2[f];
+            A.out2[f] = out2[f];
+        }
+    }
     const int JT = vec ? 16 : 64;
     A.n_jt = (2 * nplanes + JT - 1) / JT;
-    dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt));
+    dim3 grid((unsigned)(kXcds * p->syn_qlen * A.n_jt * A.nset));
     if (vec)
         hipLaunchKernelGGL(synthesis_bf3_kernel<true>, grid, dim3(256), 0, p->stream, A);
     else
