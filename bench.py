@@ -513,8 +513,9 @@ def main():
     if args.fp64_ceilings:
         from redondito_amd import sphp as S
         mf, vf = S.fp64_ceilings()
-        print(json.dumps({"fp64_mfma_ceiling_tflops": round(mf, 2), "fp64_valu_ceiling_tflops": round(vf, 2),
-                          "published_fp64_peak_tflops": 78.6}))
+        m4 = S.fp64_mfma4x4_ceiling()
+        print(json.dumps({"fp64_mfma_ceiling_tflops": round(mf, 2), "fp64_mfma4x4_ceiling_tflops": round(m4, 2),
+                          "fp64_valu_ceiling_tflops": round(vf, 2), "published_fp64_peak_tflops": 78.6}))
         return
     if args.coupled_only:
         print(json.dumps({"coupled": coupled_bench(max(2, args.steps))}))
@@ -570,6 +571,7 @@ def main():
     aux["hbm_copy_gbs"] = copy_bandwidth()
     mf, vf = S.fp64_ceilings()
     aux["fp64_mfma_ceiling_tflops"] = round(mf, 2)   # measured, beside the 78.6 published
+    aux["fp64_mfma4x4_ceiling_tflops"] = round(S.fp64_mfma4x4_ceiling(), 2)
     aux["fp64_valu_ceiling_tflops"] = round(vf, 2)
     batch = None if (args.no_batch or args.fp64_only) else batch_bench(max(2, min(args.steps, 10)))
     batch_uxw = None if (args.no_batch or args.fp64_only) else batch_uxw_bench(max(2, min(args.steps, 10)))

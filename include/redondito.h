@@ -429,6 +429,12 @@ int rd_batch_get_split(const rd_batch *b, int *lmax_min, int *n_one);
  * reference routine. */
 int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops);
 
+/* The same for a register-only loop of v_mfma_f64_4x4x4_4b_f64 (four 4x4x4
+ * blocks per instruction), which sustains more FP64 work on this device than
+ * the 16x16x4 form but needs twice the operands per flop when it stands in
+ * for a 16x16 tile.  A diagnostic; no reference routine. */
+int rd_fp64_mfma4x4_ceiling(double *tflops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,9 @@
 // FP64 throughput ceilings of the current device: register-only loops of
-// v_mfma_f64_16x16x4_f64 (four independent accumulators per wave) and of
-// v_fma_f64 (sixteen independent chains per lane), one double stored per
-// thread at the end.  They set the measured ceiling beside the published
-// FP64 peak for the Legendre kernels, which run on the FP64 MFMA.
+// v_mfma_f64_16x16x4_f64 (four independent accumulators per wave), of
+// v_mfma_f64_4x4x4_4b_f64 (eight) and of v_fma_f64 (sixteen independent
+// chains per lane), one double stored per thread at the end.  They set the
+// measured ceilings beside the published FP64 peak for the Legendre kernels,
+// which run on the 16x16x4 FP64 MFMA.
 #include "rd_internal.h"
 
 namespace rd {
@@ -24,6 +25,24 @@ __global__ __launch_bounds__(256) void probe_mfma_kernel(double *out, int iters,
     double s = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+    out[(long)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// v_mfma_f64_4x4x4_4b_f64: four 4x4x4 blocks per instruction, 512 flops;
+// eight independent accumulators per wave
+__global__ __launch_bounds__(256) void probe_mfma4_kernel(double *out, int iters, double a0)
+{
+    double acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = 0.0;
+    const double a = a0 + threadIdx.x * 1e-9, b = a0 - threadIdx.x * 1e-9;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[i], 0, 0, 0);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += acc[i];
     out[(long)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -51,9 +70,12 @@ __global__ __launch_bounds__(256) void probe_valu_kernel(double *out, int iters,
 
 using namespace rd;
 
-extern "C" int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops)
+namespace {
+
+// Times the register-only loops: kind 0 the 16x16x4 MFMA, 1 the FMA, 2 the
+// 4x4x4 MFMA, each over eight workgroups of four waves per CU.
+int time_probes(const int *kinds, double *const *res, int n)
 {
-    if (!mfma_tflops || !valu_tflops) return RD_ERR_ARG;
     int dev = 0, ncu = 0;
     RD_HIP(hipGetDevice(&dev));
     RD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -73,15 +95,17 @@ extern "C" int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops)
     if (check(hipMalloc(&d, sizeof(double) * grid * 256)) && check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
         check(hipEventCreate(&e0)) && check(hipEventCreate(&e1))) {
         // flops per thread and iteration: 4 MFMAs of 2048 flops over 64 lanes;
-        // 64 FMAs of 2 flops
-        const double fl[2] = {4 * 32.0, 64 * 2.0};
-        double *res[2] = {mfma_tflops, valu_tflops};
-        for (int k = 0; k < 2 && rc == RD_OK; k++) {
+        // 64 FMAs of 2 flops; 8 MFMAs of 512 flops over 64 lanes
+        const double fl[3] = {4 * 32.0, 64 * 2.0, 8 * 8.0};
+        for (int q = 0; q < n && rc == RD_OK; q++) {
+            const int k = kinds[q];
             auto launch = [&]() {
                 if (k == 0)
                     hipLaunchKernelGGL(probe_mfma_kernel, dim3(grid), dim3(256), 0, st, d, iters, 1.0);
-                else
+                else if (k == 1)
                     hipLaunchKernelGGL(probe_valu_kernel, dim3(grid), dim3(256), 0, st, d, iters, 1.0);
+                else
+                    hipLaunchKernelGGL(probe_mfma4_kernel, dim3(grid), dim3(256), 0, st, d, iters, 1.0);
             };
             launch();
             if (!check(hipGetLastError()) || !check(hipStreamSynchronize(st))) break;
@@ -91,7 +115,7 @@ extern "C" int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops)
             if (!check(hipEventSynchronize(e1))) break;
             float ms = 0.0f;
             check(hipEventElapsedTime(&ms, e0, e1));
-            *res[k] = fl[k] * iters * (double)grid * 256.0 / (ms / reps * 1e-3) / 1e12;
+            *res[q] = fl[k] * iters * (double)grid * 256.0 / (ms / reps * 1e-3) / 1e12;
         }
     }
     if (e0) hipEventDestroy(e0);
@@ -99,4 +123,22 @@ extern "C" int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops)
     if (st) hipStreamDestroy(st);
     if (d) hipFree(d);
     return rc;
+}
+
+}  // namespace
+
+extern "C" int rd_fp64_ceilings(double *mfma_tflops, double *valu_tflops)
+{
+    if (!mfma_tflops || !valu_tflops) return RD_ERR_ARG;
+    const int kinds[2] = {0, 1};
+    double *res[2] = {mfma_tflops, valu_tflops};
+    return time_probes(kinds, res, 2);
+}
+
+extern "C" int rd_fp64_mfma4x4_ceiling(double *tflops)
+{
+    if (!tflops) return RD_ERR_ARG;
+    const int kinds[1] = {2};
+    double *res[1] = {tflops};
+    return time_probes(kinds, res, 1);
 }

@@ -73,6 +73,7 @@ def lib():
         L.rd_plan_replay.argtypes = [vp]
         L.rd_stage_uxw.argtypes = [vp, i, i, d, d, i, vp, vp, vp, vp, vp]
         L.rd_fp64_ceilings.argtypes = [ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.rd_fp64_mfma4x4_ceiling.argtypes = [ctypes.POINTER(d)]
         L.rd_nonlinear_coupled.argtypes = [vp, i] + [vp] * 12
         L.rd_plan_profile.argtypes = [vp, i]
         L.rd_plan_stage_times.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_long), i]
@@ -373,6 +374,14 @@ def fp64_ceilings():
     m, v = ctypes.c_double(0.0), ctypes.c_double(0.0)
     _check(lib().rd_fp64_ceilings(ctypes.byref(m), ctypes.byref(v)), "fp64_ceilings")
     return m.value, v.value
+
+
+def fp64_mfma4x4_ceiling():
+    """Measured FP64 throughput in TFLOP/s of a register-only
+    v_mfma_f64_4x4x4_4b_f64 loop."""
+    t = ctypes.c_double(0.0)
+    _check(lib().rd_fp64_mfma4x4_ceiling(ctypes.byref(t)), "fp64_mfma4x4_ceiling")
+    return t.value
 
 
 def nonlinear_coupled(plan, W, Z, Pb, Tb, T, NQ, NS, NT, MQ, MS, MT, NH, prec=RD_PREC_FP64):
